@@ -1,0 +1,171 @@
+/*
+ * vacc.h -- C ABI of libvacc, the MI355X (gfx950) velocity-space acceleration
+ * library.  Drop-in for the per-cell acceleration path of fmihpc/vlasiator:
+ * the Lorentz-force transform and SLICE-3D intersections, the three 1-D
+ * semi-Lagrangian PLM/PPM/PQM column remaps over sparse 4x4x4 velocity blocks
+ * with inline block add/remove, and the sparsity-driven block adjust.
+ *
+ * Plain C types only.  Every function returns 0 on success, a positive status
+ * bitmask for recoverable per-cell conditions (VACC_STATUS_*), or a negative
+ * error code (VACC_E_*).  The library never aborts.
+ *
+ * Reference interfaces replaced (file:line in fmihpc/vlasiator):
+ *   vacc_compute_cell_intersections  <- compute_cell_intersections(SpatialCell*,popID,map_order,dt,timer)
+ *                                       vlasovsolver/cpu_acc_intersections.cpp:44
+ *                                       (with compute_acceleration_transformation,
+ *                                       vlasovsolver/cpu_acc_transform.cpp:66)
+ *   vacc_accelerate_cells            <- cpu_accelerate_cells(mpiGrid,cells,popID,map_order)
+ *                                       vlasovsolver/cpu_acc_semilag.cpp:45 (and its GPU twin
+ *                                       gpu_accelerate_cells, vlasovsolver/gpu_acc_semilag.cpp:52)
+ *                                       and, on a one-cell context,
+ *                                       cpu_accelerate_cell(SpatialCell*,popID,map_order)
+ *                                       vlasovsolver/cpu_acc_semilag.cpp:92
+ *   vacc_map_1d                      <- map_1d(SpatialCell*,popID,I,di,dj,dk,dim)
+ *                                       vlasovsolver/cpu_acc_map.cpp:121
+ *   vacc_adjust_blocks               <- SpatialCell::adjustSingleCellVelocityBlocks(popID,doDeleteEmpty)
+ *                                       spatial_cells/spatial_cell_cpu.cpp:275 (adjust_velocity_blocks :179)
+ *   vacc_moments                     <- calculateMoments_V / blockVelocityFirstMoments
+ *                                       vlasovsolver/arch_moments.cpp:342, arch_moments.h:83
+ *   vacc_max_v_dt                    <- updateAccelerationMaxdt, vlasovsolver/cpu_acc_transform.cpp:41
+ *   vacc_upload_cells/vacc_download_cells
+ *                                    <- the SpatialCell velocity mesh + block container state
+ *                                       (spatial_cells/velocity_mesh_cpu.h:47, velocity_block_container.h:59)
+ */
+#ifndef VACC_H
+#define VACC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VACC_WID 4
+#define VACC_WID3 64
+#define VACC_INVALID_ID 0xFFFFFFFFu
+
+/* reconstruction (Makefile ACC_SEMILAG_PLM / _PPM / _PQM) */
+enum { VACC_PLM = 0, VACC_PPM = 1, VACC_PQM = 2 };
+
+/* storage type of the distribution function in HBM */
+enum { VACC_STORE_F32 = 0 };
+
+/* per-cell status bits (OR-ed over cells in return values) */
+#define VACC_STATUS_WALL 1u        /* target blocks within wall margin (cpu_acc_map.cpp:325-338) */
+#define VACC_STATUS_DK 2u          /* non-positive or too small intersection_dk: cell not remapped */
+#define VACC_STATUS_CAPACITY 4u    /* block pool capacity exceeded: step not performed */
+
+/* errors */
+#define VACC_E_ARG (-1)
+#define VACC_E_HIP (-2)
+#define VACC_E_NOMEM (-3)
+#define VACC_E_STATE (-4)
+
+/* Velocity mesh of one population (velocity_mesh_parameters.h:57-65). */
+typedef struct vacc_mesh {
+   uint32_t grid_length[3];   /* blocks per axis, each <= 65535 */
+   double min_limits[3];
+   double max_limits[3];
+} vacc_mesh;
+
+/* Particle species and sparsity parameters (particle_species.h, object_wrapper.cpp:49-56). */
+typedef struct vacc_species {
+   double mass;
+   double charge;
+   double sparse_min_value;      /* <pop>_sparse.minValue */
+   int32_t sparse_add_width_v;   /* <pop>_sparse.blockAddWidthV */
+   int32_t reserved;
+} vacc_species;
+
+typedef struct vacc_options {
+   int32_t method;            /* VACC_PLM / VACC_PPM / VACC_PQM */
+   int32_t storage;           /* VACC_STORE_F32 */
+   int32_t wall_margin;       /* bailout.velocity_space_wall_block_margin */
+   int32_t ohm_gradpe_term;   /* fieldsolver.ohmGradPeTerm */
+   double hall_minimum_rhoq;  /* P::hallMinimumRhoq */
+   double max_rotation_deg;   /* vlasovsolver.maxSlAccelerationRotation */
+} vacc_options;
+
+/* Per-cell field input of the transform (cpu_acc_transform.cpp:71-115). */
+typedef struct vacc_cell_fields {
+   double B[3];          /* BGB*VOL + PERB*VOL */
+   double dB[6];         /* dPERBXVOLdy, dPERBXVOLdz, dPERBYVOLdx, dPERBYVOLdz, dPERBZVOLdx, dPERBZVOLdy */
+   double V[3];          /* bulk velocity V_V (used when the moments are not taken from f) */
+   double rhoq;          /* RHOQ_V */
+   double EgradPe[3];    /* E*GRADPE */
+   double dt;            /* subcycle dt of the cell (Population::subcycleDt) */
+} vacc_cell_fields;
+
+typedef struct vacc_ctx vacc_ctx;
+
+/* Context: one population's mesh and species; owns device pools and a stream. */
+int vacc_create(const vacc_mesh* mesh, const vacc_species* species, const vacc_options* opts, vacc_ctx** out);
+int vacc_destroy(vacc_ctx* ctx);
+const char* vacc_last_error(void);
+
+/* Upload ncells cells, packed: cell c holds blocks [offsets[c], offsets[c+1]) of gids and
+ * f (64 floats per block, in-block index i + 4*j + 16*k).  Replaces the context's cells.
+ * Blocks of a cell must have distinct valid GIDs. */
+int vacc_upload_cells(vacc_ctx* ctx, uint32_t ncells, const uint64_t* offsets,
+                      const uint32_t* gids, const float* f);
+/* Number of cells and blocks per cell (counts[ncells]) currently held. */
+int vacc_get_counts(vacc_ctx* ctx, uint32_t* ncells, uint32_t* counts);
+/* Download: offsets[ncells+1] written; gids/f sized by the caller from vacc_get_counts.
+ * Blocks of each cell are returned in ascending GID order. */
+int vacc_download_cells(vacc_ctx* ctx, uint64_t* offsets, uint32_t* gids, float* f);
+
+/* Per-cell intersections (12 doubles per cell: x {I,di,dj,dk}, y {...}, z {...}),
+ * as stored in Population::intersection_* (spatial_cell_cpu.hpp:89-91). */
+int vacc_set_intersections(vacc_ctx* ctx, const double* isect);
+int vacc_get_intersections(vacc_ctx* ctx, double* isect);
+/* Transform + intersections for every cell on the device (FP64).  If use_moments != 0 the
+ * pivot bulk velocity is the first moment of the current f (calculateMoments_V), otherwise
+ * fields[c].V.  fields: ncells entries. */
+int vacc_compute_cell_intersections(vacc_ctx* ctx, const vacc_cell_fields* fields, uint32_t map_order,
+                                    int32_t use_moments);
+/* Host FP64 transform + intersections of a single cell (no device work). */
+int vacc_cell_intersections_host(const vacc_mesh* mesh, const vacc_species* species, const vacc_options* opts,
+                                 const vacc_cell_fields* fields, uint32_t map_order, double out[12]);
+/* max_v_dt of a cell (cpu_acc_transform.cpp:41-55). */
+double vacc_max_v_dt(const vacc_species* species, const vacc_options* opts, const double B[3]);
+
+/* One 1-D remap of every cell along dim with the stored intersections. */
+int vacc_map_1d(vacc_ctx* ctx, uint32_t dim);
+/* Three remaps in map_order (0 XYZ, 1 YZX, 2 ZXY) for every cell of the context.  A
+ * context holding many cells is the batched form (every kernel covers all cells); a
+ * context holding one cell is the per-SpatialCell form of cpu_accelerate_cell. */
+int vacc_accelerate_cells(vacc_ctx* ctx, uint32_t map_order);
+/* Single-cell sparsity adjust of every cell (doDeleteEmptyBlocks = true).  rho_loss may be
+ * NULL; otherwise ncells FP64 values of the removed mass (RHOLOSSADJUST increments). */
+int vacc_adjust_blocks(vacc_ctx* ctx, double* rho_loss);
+/* FP64 moments per cell: out[4*c+0] = n, [1..3] = n*V (arch_moments.h:83-108). */
+int vacc_moments(vacc_ctx* ctx, double* out);
+/* Synthetic initial state, the project helper used by tests and benchmarks: mirrors
+ * Project::setVelocitySpace + MultiPeak::fillPhaseSpace (projects/project.cpp:239-263,
+ * projects/MultiPeak/MultiPeak.cpp:127-222) followed by the initial single-cell adjust
+ * (grid.cpp:289).  Every block of the candidate box bbox[6*c..] = {x0,y0,z0,x1,y1,z1}
+ * (inclusive block indices) is sampled at cell centres as the float sum of the cell's
+ * peaks; blocks with content and their sparse_add_width_v neighbours are kept with their
+ * sampled values.  Replaces the context's cells; peaks holds max_peaks entries per cell. */
+typedef struct vacc_peak {
+   int32_t kind;       /* 0: Maxwellian (projects/project.h:66-72), 1: gyrotropic ring */
+   int32_t reserved;
+   double n;           /* number density */
+   double V[3];        /* centre (Maxwellian) or ring centre velocity */
+   double T;           /* temperature (Maxwellian) */
+   double axis[3];     /* unit ring axis */
+   double vperp;       /* ring speed perpendicular to axis */
+   double vth;         /* ring thermal speed */
+} vacc_peak;
+int vacc_init_peaks(vacc_ctx* ctx, uint32_t ncells, uint32_t max_peaks, const vacc_peak* peaks,
+                    const int32_t* npeaks, const int32_t* bbox);
+/* Wait for the context's stream; returns the OR of per-cell status bits since the last call. */
+int vacc_sync(vacc_ctx* ctx);
+/* Per-cell status words since the last vacc_sync (ncells entries). */
+int vacc_get_status(vacc_ctx* ctx, uint32_t* status);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

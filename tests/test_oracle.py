@@ -1,0 +1,248 @@
+"""CPU tests of the oracle: golden vectors of the reference's reconstructions, closed forms,
+invariants and brute force.  No GPU."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from tests import cases
+from vlasiator_amd import workloads as W
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "reconstruction.npz")
+
+
+@pytest.mark.parametrize("method,name", [(O.PLM, "plm"), (O.PPM, "ppm"), (O.PQM, "pqm")])
+def test_reconstruction_matches_reference_golden(method, name):
+    g = np.load(GOLDEN)
+    got = O.coeff_batch(method, g["stencils"], float(g["threshold"]))
+    assert np.array_equal(got.view(np.uint32), g[name].view(np.uint32))
+
+
+@pytest.mark.skipif(O.ref_lib() is None, reason="oracle/_ref (reference headers) not built here")
+@pytest.mark.parametrize("method", [O.PLM, O.PPM, O.PQM])
+def test_reconstruction_matches_reference_random(method):
+    rng = np.random.default_rng(123 + method)
+    s = (10 ** rng.uniform(-17, -9, (20000, 9))).astype(np.float32)
+    s[rng.random(s.shape) < 0.2] = 0
+    a = O.coeff_batch(method, s, 1e-15)
+    b = O.ref_coeff_batch(method, s, 1e-15)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_reconstruction_integrates_to_cell_average():
+    # sum of the integration coefficients equals the cell average (to float rounding)
+    g = np.load(GOLDEN)
+    s = g["stencils"]
+    for m in (O.PLM, O.PPM, O.PQM):
+        a = O.coeff_batch(m, s, 1e-15).astype(np.float64)
+        tot = a.sum(axis=1)
+        v = s[:, 4].astype(np.float64)
+        scale = np.maximum(np.abs(a).max(axis=1), np.abs(v)) + 1e-30
+        assert np.all(np.abs(tot - v) <= 1e-5 * scale)
+
+
+def test_ppm_of_linear_profile():
+    # linear cell averages: h4 faces are exact, no limiting, and the integral coefficients
+    # of F(x) = x*(a0 + x*(a1 + x*a2)) are a0 = left face, a1 = slope/2, a2 = 0
+    x = np.arange(9, dtype=np.float64)
+    s = (1e-12 * (1.0 + 0.05 * x))[None, :].astype(np.float32)
+    a = O.coeff_batch(O.PPM, s, 1e-15)[0].astype(np.float64)
+    assert abs(a[0] - 1e-12 * (1.0 + 0.05 * 3.5)) < 1e-18
+    assert abs(a[1] - 1e-12 * 0.025) < 1e-18
+    assert abs(a[2]) < 1e-18
+    # PQM on the same profile: F(1) = a0+..+a4 = cell average and derivative coefficient a1 = slope/2
+    q = O.coeff_batch(O.PQM, s, 1e-15)[0].astype(np.float64)
+    assert abs(q.sum() - 1e-12 * 1.2) < 1e-18
+    assert abs(q[1] - 1e-12 * 0.025) < 1e-18
+
+
+def _mesh():
+    return O.make_mesh((50, 50, 50), (-2e6,) * 3, (2e6,) * 3)
+
+
+def _maxwellian_cell(T=3e5, V=(1e5, -2e5, 5e4)):
+    return O.init_cell(_mesh(), [O.make_peak(0, 1e6, V, T)], W.MASS_PROTON, 1e-15, 1)
+
+
+def test_sort_columns_brute_force():
+    rng = np.random.default_rng(5)
+    m = O.make_mesh((12, 9, 7), (-1, -1, -1), (1, 1, 1))
+    gids = rng.choice(12 * 9 * 7, 200, replace=False).astype(np.uint32)
+    for dim in range(3):
+        srt, co, cl, so, sn = O.sort_columns(m, gids, dim)
+        x, y, z = gids % 12, (gids // 12) % 9, gids // 108
+        c = [x, y, z]
+        t = [c[d] for d in range(3) if d != dim]
+        # brute force: group by transverse coordinates, runs along dim
+        groups = {}
+        for g, a, b, k in zip(gids, t[0], t[1], c[dim]):
+            groups.setdefault((int(a), int(b)), []).append((int(k), int(g)))
+        nruns = sum(1 + sum(1 for (k0, _), (k1, _) in zip(sorted(v), sorted(v)[1:]) if k1 != k0 + 1)
+                    for v in groups.values())
+        assert len(sn) == len(groups)
+        assert len(cl) == nruns
+        assert cl.sum() == gids.size
+        assert sorted(srt.tolist()) == sorted(gids.tolist())
+
+
+def test_identity_transform_copies_f():
+    g, f = _maxwellian_cell()
+    m = _mesh()
+    T = np.array([1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0], float)
+    for mo in range(3):
+        isx = O.intersections(m, T, mo)
+        for method in (O.PPM, O.PQM):
+            g2, f2, wall = O.accelerate_cell(m, g, f, isx, mo, method, 1e-15)
+            gs, fs = cases.sort_cell(g2, f2)
+            idx = np.searchsorted(gs, g)
+            assert np.array_equal(gs[idx], g)
+            # copy of every source block up to the float rounding of F(1) = sum(a): a few ulp for
+            # PPM, more for PQM whose coefficients are ~10x the cell average with cancellation
+            rtol = 1e-5 if method == O.PPM else 1e-4
+            assert np.allclose(fs[idx], f, rtol=rtol, atol=0)
+            extra = np.setdiff1d(gs, g)
+            assert np.all(fs[np.searchsorted(gs, extra)] == 0)   # added target blocks stay zero
+
+
+def test_integer_cell_shift_is_exact():
+    g, f = _maxwellian_cell()
+    m = _mesh()
+    dv = 4e6 / 200
+    for dim in range(3):
+        for shift in (1, 3, -2):
+            isx = np.zeros(4)
+            # Lagrangian cell gk covers source cell gk+shift
+            isx[0] = -2e6 + shift * dv
+            isx[3] = dv
+            g2, f2, _ = O.map_1d(m, g, f, isx, dim, O.PPM, 1e-15)
+            src = {int(a): b for a, b in zip(g, f)}
+            dst = {int(a): b for a, b in zip(g2, f2)}
+            nx = 50
+            stride = [1, 4, 16][dim]
+            bstride = [1, nx, nx * nx][dim]
+            for gid, vals in src.items():
+                bi = [gid % nx, (gid // nx) % nx, gid // (nx * nx)][dim]
+                for e in range(64):
+                    k = [(e & 3), (e >> 2) & 3, e >> 4][dim]
+                    gk = 4 * bi + k - shift
+                    tb, tk = divmod(gk, 4)
+                    tg = gid + (tb - bi) * bstride
+                    te = e + (tk - k) * stride
+                    if tg in dst:
+                        assert abs(dst[tg][te] - vals[e]) <= 1e-5 * abs(vals[e])
+
+
+def test_mass_conserved_by_rotation_step():
+    g, f = _maxwellian_cell(T=5e5)
+    m = _mesh()
+    B = (3e-9, -4e-9, 1e-8)
+    Tg = W.gyro_period(B)
+    fl = O.make_fields(B, (1e5, -2e5, 5e4))
+    T = O.transform(fl, W.MASS_PROTON, W.CHARGE, Tg * 5 / 360, 1.6e-19)
+    m0 = O.moments(m, g, f)
+    for mo in range(3):
+        isx = O.intersections(m, T, mo)
+        for method in (O.PPM, O.PQM, O.PLM):
+            g2, f2, _ = O.accelerate_cell(m, g, f, isx, mo, method, 1e-15)
+            m1 = O.moments(m, g2, f2)
+            assert abs(m1[0] / m0[0] - 1) < 1e-5
+            # the pivot (bulk velocity) is invariant under the rotation about it
+            assert np.allclose(m1[1:] / m1[0], m0[1:] / m0[0], atol=2e2)
+
+
+def test_transform_closed_form():
+    # B along z, zero pivot: rotation by -2*pi*dt/Tg about z
+    B = (0, 0, 1e-8)
+    Tg = W.gyro_period(B)
+    dt = Tg * 5 / 360
+    T = O.transform(O.make_fields(B, (0, 0, 0)), W.MASS_PROTON, W.CHARGE, dt, 1.6e-19)
+    L = T.reshape(3, 4)[:, :3]
+    th = -2 * math.pi * dt / Tg
+    R = np.array([[math.cos(th), -math.sin(th), 0], [math.sin(th), math.cos(th), 0], [0, 0, 1]])
+    assert np.allclose(L, R, atol=1e-13)
+    assert np.allclose(T.reshape(3, 4)[:, 3], 0, atol=1e-6)
+    assert abs(np.linalg.det(L) - 1) < 1e-13
+    inv = O.affine_inverse(T).reshape(3, 4)
+    assert np.allclose(inv[:, :3] @ L, np.eye(3), atol=1e-13)
+
+
+def test_transform_pivot_fixed_point_and_orthonormal():
+    rng = np.random.default_rng(3)
+    for _ in range(20):
+        B = rng.normal(size=3) * 1e-8
+        V = rng.normal(size=3) * 3e5
+        Tg = W.gyro_period(B)
+        T = O.transform(O.make_fields(B, V), W.MASS_PROTON, W.CHARGE, Tg * rng.uniform(1, 20) / 360, 1.6e-19)
+        A = T.reshape(3, 4)
+        L = A[:, :3]
+        assert np.allclose(L @ L.T, np.eye(3), atol=1e-12)
+        assert np.allclose(L @ V + A[:, 3], V, rtol=0, atol=1e-6 * np.linalg.norm(V))
+        assert np.allclose(L @ (B / np.linalg.norm(B)), B / np.linalg.norm(B), atol=1e-12)
+
+
+def test_intersections_translation_closed_form():
+    # pure translation by u: along the mapped axis d the step is dv (stored at index 1+d:
+    # intersection_x_di, intersection_y_dj, intersection_z_dk), the cross steps are 0
+    m = _mesh()
+    u = np.array([3e4, -7e4, 1.1e5])
+    T = np.array([1, 0, 0, u[0], 0, 1, 0, u[1], 0, 0, 1, u[2]], float)
+    dv = 4e6 / 200
+    for mo in range(3):
+        isx = O.intersections(m, T, mo).reshape(3, 4)
+        for d in range(3):
+            for q in range(3):
+                want = dv if q == d else 0.0
+                assert abs(isx[d, 1 + q] - want) < 1e-6
+            # Lagrangian grid = forward image of the Euclidean grid: base shifted by -u (bwd map)
+            assert abs(isx[d, 0] - (-2e6 - u[d])) < 1e-3
+
+
+def test_adjust_brute_force():
+    rng = np.random.default_rng(11)
+    m = O.make_mesh((10, 10, 10), (-1, -1, -1), (1, 1, 1))
+    gids = rng.choice(1000, 150, replace=False).astype(np.uint32)
+    f = np.zeros((150, 64), np.float32)
+    has = rng.random(150) < 0.3
+    f[has, rng.integers(0, 64, has.sum())] = 1.0
+    f[~has, 0] = 1e-20
+    g2, f2, loss = O.adjust(m, gids, f, 1e-15, 1)
+    keep = set()
+    for g, h in zip(gids.tolist(), has):
+        if not h:
+            continue
+        x, y, z = g % 10, (g // 10) % 10, g // 100
+        for dx in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dz in (-1, 0, 1):
+                    a, b, c = x + dx, y + dy, z + dz
+                    if 0 <= a < 10 and 0 <= b < 10 and 0 <= c < 10:
+                        keep.add(int(a + 10 * b + 100 * c))
+    assert sorted(keep) == g2.tolist()
+    removed = [i for i, g in enumerate(gids) if not has[i] and int(g) not in keep]
+    dv3 = (0.2 / 4) ** 3
+    assert abs(loss - dv3 * float(np.float32(1e-20)) * len(removed)) <= 1e-12 * abs(loss) + 1e-300
+
+
+def test_moments_of_maxwellian():
+    V = (1e5, -2e5, 5e4)
+    g, f = _maxwellian_cell(T=4e5, V=V)
+    mo = O.moments(_mesh(), g, f)
+    assert abs(mo[0] / 1e6 - 1) < 2e-3
+    assert np.allclose(mo[1:] / mo[0], V, atol=200)
+
+
+def test_batched_driver_matches_per_cell():
+    w = W.config(0, ncells=4)
+    cells = cases.build_cells(w)
+    isx = cases.oracle_intersections(w, 1)
+    off = np.zeros(5, np.uint32)
+    off[1:] = np.cumsum([len(c[0]) for c in cells])
+    g = np.concatenate([c[0] for c in cells])
+    f = np.concatenate([c[1] for c in cells])
+    oo, go, fo, wall = O.accelerate_cells(cases.oracle_mesh(w), off, g, f, isx, 1, O.PPM, 1e-15, nthreads=2)
+    for i, (gc, fc) in enumerate(cells):
+        g1, f1, _ = O.accelerate_cell(cases.oracle_mesh(w), gc, fc, isx[i], 1, O.PPM, 1e-15)
+        assert np.array_equal(go[oo[i]:oo[i + 1]], g1)
+        assert np.array_equal(fo[oo[i]:oo[i + 1]], f1)

@@ -1,0 +1,674 @@
+// vacc_api.hip -- host side of libvacc: device pools, pass orchestration and the C ABI
+// declared in include/vacc.h.
+//
+// State of a context between calls (all on the device):
+//   list L[lcur]: per-cell offsets, block GIDs and, per block, the index of its 64 values
+//                 in pool F[fcur] (INVALID = a block created by adjust, all zero);
+//   pool F[fcur]: block values.
+// A remap pass reads L[lcur]/F[fcur] and writes a compact L[1-lcur]/F[1-fcur]; the block
+// adjust writes a new list L[1-lcur] that still points into F[fcur].
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "vacc_common.h"
+#include "vacc_kernels.h"
+
+using namespace vacc;
+
+static thread_local std::string g_err;
+
+static int set_err(int code, const std::string& msg) {
+   g_err = msg;
+   return code;
+}
+
+#define HIPCHK(expr)                                                                          \
+   do {                                                                                       \
+      hipError_t e_ = (expr);                                                                 \
+      if (e_ != hipSuccess) return set_err(VACC_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+   } while (0)
+
+struct vacc_ctx {
+   vacc_mesh mesh;
+   vacc_species sp;
+   vacc_options opt;
+   MeshInfo mi;
+   hipStream_t stream = nullptr;
+   uint32_t ncells = 0;
+   uint32_t cell_cap = 0;      // capacity of per-cell arrays
+   size_t cap = 0;             // block capacity of lists, pools and pass scratch
+   int lcur = 0, fcur = 0;
+   uint64_t nblocks = 0;       // total blocks as of the last synchronisation point
+   bool content_valid = false;
+   // lists and pools
+   uint32_t* off[2] = {nullptr, nullptr};
+   uint32_t* gid[2] = {nullptr, nullptr};
+   uint32_t* src[2] = {nullptr, nullptr};
+   float* f[2] = {nullptr, nullptr};
+   uint8_t* content = nullptr;
+   // pass scratch
+   uint32_t* sorted_src = nullptr;
+   uint4* set_rec = nullptr;
+   uint4* col_rec = nullptr;
+   uint32_t* n_sets = nullptr;
+   uint32_t* n_tgt = nullptr;
+   uint32_t* set_base = nullptr;
+   uint32_t* status = nullptr;
+   uint32_t* flags = nullptr;
+   double* isect = nullptr;
+   TransformIn* fields = nullptr;
+   double* moments = nullptr;
+   double* rho_loss = nullptr;
+};
+
+static void free_blocks(vacc_ctx* c) {
+   for (int i = 0; i < 2; ++i) {
+      if (c->gid[i]) (void)hipFree(c->gid[i]);
+      if (c->src[i]) (void)hipFree(c->src[i]);
+      if (c->f[i]) (void)hipFree(c->f[i]);
+      c->gid[i] = c->src[i] = nullptr;
+      c->f[i] = nullptr;
+   }
+   if (c->content) (void)hipFree(c->content);
+   if (c->sorted_src) (void)hipFree(c->sorted_src);
+   if (c->set_rec) (void)hipFree(c->set_rec);
+   if (c->col_rec) (void)hipFree(c->col_rec);
+   c->content = nullptr;
+   c->sorted_src = nullptr;
+   c->set_rec = c->col_rec = nullptr;
+   c->cap = 0;
+}
+
+static void free_cells(vacc_ctx* c) {
+   for (int i = 0; i < 2; ++i) {
+      if (c->off[i]) (void)hipFree(c->off[i]);
+      c->off[i] = nullptr;
+   }
+   uint32_t** u32s[] = {&c->n_sets, &c->n_tgt, &c->set_base, &c->status};
+   for (auto p : u32s) {
+      if (*p) (void)hipFree(*p);
+      *p = nullptr;
+   }
+   if (c->isect) (void)hipFree(c->isect);
+   if (c->fields) (void)hipFree(c->fields);
+   if (c->moments) (void)hipFree(c->moments);
+   if (c->rho_loss) (void)hipFree(c->rho_loss);
+   c->isect = nullptr;
+   c->fields = nullptr;
+   c->moments = nullptr;
+   c->rho_loss = nullptr;
+   c->cell_cap = 0;
+}
+
+static int alloc_cells(vacc_ctx* c, uint32_t n) {
+   if (n <= c->cell_cap && c->off[0]) return 0;
+   free_cells(c);
+   const size_t m = (size_t)n + 1;
+   for (int i = 0; i < 2; ++i) HIPCHK(hipMalloc(&c->off[i], m * sizeof(uint32_t)));
+   HIPCHK(hipMalloc(&c->n_sets, m * sizeof(uint32_t)));
+   HIPCHK(hipMalloc(&c->n_tgt, m * sizeof(uint32_t)));
+   HIPCHK(hipMalloc(&c->set_base, m * sizeof(uint32_t)));
+   HIPCHK(hipMalloc(&c->status, m * sizeof(uint32_t)));
+   HIPCHK(hipMemset(c->status, 0, m * sizeof(uint32_t)));
+   HIPCHK(hipMalloc(&c->isect, m * 12 * sizeof(double)));
+   HIPCHK(hipMemset(c->isect, 0, m * 12 * sizeof(double)));
+   HIPCHK(hipMalloc(&c->fields, m * sizeof(TransformIn)));
+   HIPCHK(hipMalloc(&c->moments, m * 4 * sizeof(double)));
+   HIPCHK(hipMalloc(&c->rho_loss, m * sizeof(double)));
+   c->cell_cap = n;
+   return 0;
+}
+
+// (Re)allocate block storage for `want` blocks, preserving the current list and pool.
+static int ensure_blocks(vacc_ctx* c, size_t want) {
+   if (want <= c->cap && c->gid[0]) return 0;
+   if (want > 0xF0000000ull) return set_err(VACC_E_NOMEM, "block count exceeds 32-bit list indices");
+   const size_t ncap = std::max<size_t>(want, 1024);
+   uint32_t *g[2], *s[2];
+   float* f[2];
+   uint8_t* ct;
+   uint32_t* ss;
+   uint4 *sr, *cr;
+   for (int i = 0; i < 2; ++i) {
+      HIPCHK(hipMalloc(&g[i], ncap * sizeof(uint32_t)));
+      HIPCHK(hipMalloc(&s[i], ncap * sizeof(uint32_t)));
+      HIPCHK(hipMalloc(&f[i], ncap * WID3 * sizeof(float)));
+   }
+   HIPCHK(hipMalloc(&ct, ncap));
+   HIPCHK(hipMalloc(&ss, ncap * sizeof(uint32_t)));
+   HIPCHK(hipMalloc(&sr, ncap * sizeof(uint4)));
+   HIPCHK(hipMalloc(&cr, ncap * sizeof(uint4)));
+   if (c->cap && c->nblocks) {
+      // carry the current state: list entries and the whole current pool extent
+      const size_t n = c->nblocks;
+      HIPCHK(hipMemcpyAsync(g[c->lcur], c->gid[c->lcur], n * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
+      HIPCHK(hipMemcpyAsync(s[c->lcur], c->src[c->lcur], n * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
+      HIPCHK(hipMemcpyAsync(f[c->fcur], c->f[c->fcur], c->cap * WID3 * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+      HIPCHK(hipMemcpyAsync(ct, c->content, n, hipMemcpyDeviceToDevice, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+   }
+   free_blocks(c);
+   for (int i = 0; i < 2; ++i) {
+      c->gid[i] = g[i];
+      c->src[i] = s[i];
+      c->f[i] = f[i];
+   }
+   c->content = ct;
+   c->sorted_src = ss;
+   c->set_rec = sr;
+   c->col_rec = cr;
+   c->cap = ncap;
+   return 0;
+}
+
+// Read the total block count of the current list (synchronises the stream).
+static int refresh_count(vacc_ctx* c) {
+   uint32_t tot = 0;
+   HIPCHK(hipMemcpyAsync(&tot, c->off[c->lcur] + c->ncells, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+   HIPCHK(hipStreamSynchronize(c->stream));
+   c->nblocks = tot;
+   return 0;
+}
+
+// Keep head-room for the growth of one step (new target blocks and adjust halo blocks).
+static int reserve_for_step(vacc_ctx* c) {
+   const size_t want = c->nblocks + c->nblocks / 2 + 64 * (size_t)c->ncells + 4096;
+   if (want > c->cap) return ensure_blocks(c, want + want / 4);
+   return 0;
+}
+
+static float cell_f(const vacc_ctx* c, int d) { return (float)c->mi.cell[d]; }
+
+static int do_pass(vacc_ctx* c, int dim) {
+   const int li = c->lcur, lo = 1 - c->lcur, fi = c->fcur, fo = 1 - c->fcur;
+   ColumnArgs ca{};
+   ca.cell_off = c->off[li];
+   ca.gid = c->gid[li];
+   ca.src = c->src[li];
+   ca.isect = c->isect;
+   ca.sorted_src = c->sorted_src;
+   ca.set_rec = c->set_rec;
+   ca.col_rec = c->col_rec;
+   ca.n_sets = c->n_sets;
+   ca.n_targets = c->n_tgt;
+   ca.status = c->status;
+   ca.ncells = c->ncells;
+   for (int d = 0; d < 3; ++d) ca.len[d] = c->mi.len[d];
+   ca.dv = cell_f(c, dim);                 // Realf dv = cellSize (cpu_acc_map.cpp:147)
+   ca.vmin = (float)c->mi.vmin[dim];       // Realf v_min (cpu_acc_map.cpp:148)
+   ca.wall_margin = c->opt.wall_margin;
+   launch_columns(dim, ca, c->stream);
+   ScanArgs sa{};
+   sa.a = c->n_tgt;
+   sa.b = c->n_sets;
+   sa.a_off = c->off[lo];
+   sa.b_off = c->set_base;
+   sa.n = c->ncells;
+   sa.cap_a = (uint32_t)std::min<size_t>(c->cap, 0xFFFFFFFFull);
+   sa.flags = c->flags;
+   launch_scan(sa, c->stream);
+   RemapArgs ra{};
+   ra.cell_off = c->off[li];
+   ra.set_base = c->set_base;
+   ra.out_off = c->off[lo];
+   ra.set_rec = c->set_rec;
+   ra.col_rec = c->col_rec;
+   ra.sorted_src = c->sorted_src;
+   ra.isect = c->isect;
+   ra.f_in = c->f[fi];
+   ra.f_out = c->f[fo];
+   ra.gid_out = c->gid[lo];
+   ra.src_out = c->src[lo];
+   ra.content_out = c->content;
+   ra.status = c->status;
+   ra.flags = c->flags;
+   ra.ncells = c->ncells;
+   for (int d = 0; d < 3; ++d) ra.len[d] = c->mi.len[d];
+   ra.dv = ca.dv;
+   ra.vmin = ca.vmin;
+   ra.threshold = (float)c->sp.sparse_min_value;   // Realf threshold argument of compute_*_coeff
+   ra.min_value = c->sp.sparse_min_value;
+   launch_remap(dim, c->opt.method, ra, 0xFFFFFFFFu, c->stream);
+   HIPCHK(hipGetLastError());
+   c->lcur = lo;
+   c->fcur = fo;
+   c->content_valid = true;
+   return 0;
+}
+
+// Rebuild the list: add_width >= 0 is the sparsity adjust, add_width < 0 a re-sort by GID.
+static int do_relist(vacc_ctx* c, int add_width, bool want_loss) {
+   const int li = c->lcur, lo = 1 - c->lcur;
+   if (add_width >= 0 && !c->content_valid) {
+      launch_content(c->src[li], c->f[c->fcur], c->content, (uint32_t)c->nblocks, c->sp.sparse_min_value, c->stream);
+      c->content_valid = true;
+   }
+   AdjustArgs aa{};
+   aa.cell_off = c->off[li];
+   aa.gid = c->gid[li];
+   aa.src = c->src[li];
+   aa.content = c->content;
+   aa.f = c->f[c->fcur];
+   aa.n_keep = c->n_tgt;
+   aa.rho_loss = want_loss ? c->rho_loss : nullptr;
+   aa.ncells = c->ncells;
+   for (int d = 0; d < 3; ++d) aa.len[d] = c->mi.len[d];
+   aa.min_value = c->sp.sparse_min_value;
+   aa.dv3 = c->mi.cell[0] * c->mi.cell[1] * c->mi.cell[2];
+   aa.add_width = add_width;
+   aa.flags = c->flags;
+   launch_adjust(aa, false, c->stream);
+   ScanArgs sa{};
+   sa.a = c->n_tgt;
+   sa.b = nullptr;
+   sa.a_off = c->off[lo];
+   sa.b_off = nullptr;
+   sa.n = c->ncells;
+   sa.cap_a = (uint32_t)std::min<size_t>(c->cap, 0xFFFFFFFFull);
+   sa.flags = c->flags;
+   launch_scan(sa, c->stream);
+   aa.out_off = c->off[lo];
+   aa.gid_out = c->gid[lo];
+   aa.src_out = c->src[lo];
+   launch_adjust(aa, true, c->stream);
+   HIPCHK(hipGetLastError());
+   c->lcur = lo;
+   c->content_valid = false;
+   return 0;
+}
+
+// Compact the pool so that src is the identity (block i of the list at pool index i).
+static int do_compact(vacc_ctx* c) {
+   launch_gather(c->src[c->lcur], c->f[c->fcur], c->f[1 - c->fcur], (uint32_t)c->nblocks, c->stream);
+   launch_iota(c->src[c->lcur], (uint32_t)c->nblocks, c->stream);
+   HIPCHK(hipGetLastError());
+   c->fcur = 1 - c->fcur;
+   return 0;
+}
+
+// OR of the per-cell status words (synchronises the stream).
+static int status_or(vacc_ctx* c) {
+   if (!c->ncells) return 0;
+   std::vector<uint32_t> st(c->ncells);
+   HIPCHK(hipMemcpyAsync(st.data(), c->status, c->ncells * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+   HIPCHK(hipStreamSynchronize(c->stream));
+   uint32_t any = 0;
+   for (uint32_t v : st) any |= v;
+   return (int)(any & 0x7FFFFFFFu);
+}
+
+static int check_flags(vacc_ctx* c) {
+   uint32_t fl = 0;
+   HIPCHK(hipMemcpyAsync(&fl, c->flags, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+   HIPCHK(hipStreamSynchronize(c->stream));
+   if (fl & VACC_STATUS_CAPACITY) return set_err(VACC_E_STATE, "block pool capacity exceeded; state is undefined");
+   return 0;
+}
+
+extern "C" {
+
+const char* vacc_last_error(void) { return g_err.c_str(); }
+
+int vacc_create(const vacc_mesh* mesh, const vacc_species* species, const vacc_options* opts, vacc_ctx** out) {
+   if (!mesh || !species || !opts || !out) return set_err(VACC_E_ARG, "null argument");
+   for (int d = 0; d < 3; ++d) {
+      if (mesh->grid_length[d] == 0 || mesh->grid_length[d] > 65535)
+         return set_err(VACC_E_ARG, "grid_length must be in [1, 65535]");
+      if (!(mesh->max_limits[d] > mesh->min_limits[d])) return set_err(VACC_E_ARG, "empty mesh extent");
+   }
+   if (opts->method < VACC_PLM || opts->method > VACC_PQM) return set_err(VACC_E_ARG, "unknown method");
+   if (opts->storage != VACC_STORE_F32) return set_err(VACC_E_ARG, "unsupported storage");
+   if (species->sparse_add_width_v < 0 || species->sparse_add_width_v > 8)
+      return set_err(VACC_E_ARG, "sparse_add_width_v must be in [0, 8]");
+   int ndev = 0;
+   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return set_err(VACC_E_HIP, "no HIP device");
+   vacc_ctx* c = new vacc_ctx();
+   c->mesh = *mesh;
+   c->sp = *species;
+   c->opt = *opts;
+   for (int d = 0; d < 3; ++d) {
+      c->mi.len[d] = mesh->grid_length[d];
+      c->mi.vmin[d] = mesh->min_limits[d];
+      c->mi.cell[d] = cell_size(mesh->min_limits[d], mesh->max_limits[d], mesh->grid_length[d]);
+   }
+   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+       hipMalloc(&c->flags, sizeof(uint32_t)) != hipSuccess || hipMemset(c->flags, 0, sizeof(uint32_t)) != hipSuccess) {
+      delete c;
+      return set_err(VACC_E_HIP, "stream/flag allocation failed");
+   }
+   *out = c;
+   return 0;
+}
+
+int vacc_destroy(vacc_ctx* c) {
+   if (!c) return 0;
+   if (c->stream) (void)hipStreamSynchronize(c->stream);
+   free_blocks(c);
+   free_cells(c);
+   if (c->flags) (void)hipFree(c->flags);
+   if (c->stream) (void)hipStreamDestroy(c->stream);
+   delete c;
+   return 0;
+}
+
+int vacc_upload_cells(vacc_ctx* c, uint32_t ncells, const uint64_t* offsets, const uint32_t* gids, const float* f) {
+   if (!c || !offsets || (ncells && (!gids || !f))) return set_err(VACC_E_ARG, "null argument");
+   const uint64_t total = offsets[ncells];
+   if (total > 0xF0000000ull) return set_err(VACC_E_ARG, "too many blocks");
+   const uint64_t gmax = (uint64_t)c->mi.len[0] * c->mi.len[1] * c->mi.len[2];
+   std::vector<uint32_t> off32(ncells + 1);
+   for (uint32_t i = 0; i <= ncells; ++i) {
+      if (i && offsets[i] < offsets[i - 1]) return set_err(VACC_E_ARG, "offsets not ascending");
+      off32[i] = (uint32_t)offsets[i];
+   }
+   for (uint64_t i = 0; i < total; ++i)
+      if (gids[i] >= gmax) return set_err(VACC_E_ARG, "GID outside the velocity mesh");
+   int rc = alloc_cells(c, ncells);
+   if (rc) return rc;
+   HIPCHK(hipStreamSynchronize(c->stream));
+   c->ncells = ncells;
+   c->nblocks = 0;
+   c->lcur = 0;
+   c->fcur = 0;
+   const size_t want = total + total / 2 + 64 * (size_t)ncells + 4096;
+   if (want > c->cap) {
+      rc = ensure_blocks(c, want);
+      if (rc) return rc;
+   }
+   HIPCHK(hipMemcpyAsync(c->off[0], off32.data(), (ncells + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+   if (total) {
+      HIPCHK(hipMemcpyAsync(c->gid[0], gids, total * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+      HIPCHK(hipMemcpyAsync(c->f[0], f, total * WID3 * sizeof(float), hipMemcpyHostToDevice, c->stream));
+   }
+   launch_iota(c->src[0], (uint32_t)total, c->stream);
+   HIPCHK(hipMemsetAsync(c->status, 0, (ncells + 1) * sizeof(uint32_t), c->stream));
+   HIPCHK(hipMemsetAsync(c->flags, 0, sizeof(uint32_t), c->stream));
+   HIPCHK(hipStreamSynchronize(c->stream));
+   c->nblocks = total;
+   c->content_valid = false;
+   return 0;
+}
+
+int vacc_get_counts(vacc_ctx* c, uint32_t* ncells, uint32_t* counts) {
+   if (!c) return set_err(VACC_E_ARG, "null context");
+   int rc = refresh_count(c);
+   if (rc) return rc;
+   if (ncells) *ncells = c->ncells;
+   if (counts && c->ncells) {
+      std::vector<uint32_t> off(c->ncells + 1);
+      HIPCHK(hipMemcpy(off.data(), c->off[c->lcur], (c->ncells + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost));
+      for (uint32_t i = 0; i < c->ncells; ++i) counts[i] = off[i + 1] - off[i];
+   }
+   return 0;
+}
+
+int vacc_download_cells(vacc_ctx* c, uint64_t* offsets, uint32_t* gids, float* f) {
+   if (!c || !offsets) return set_err(VACC_E_ARG, "null argument");
+   int rc = refresh_count(c);
+   if (rc) return rc;
+   rc = do_relist(c, -1, false);    // ascending GID order, same blocks
+   if (rc) return rc;
+   rc = do_compact(c);
+   if (rc) return rc;
+   std::vector<uint32_t> off(c->ncells + 1);
+   HIPCHK(hipMemcpyAsync(off.data(), c->off[c->lcur], (c->ncells + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+   HIPCHK(hipStreamSynchronize(c->stream));
+   for (uint32_t i = 0; i <= c->ncells; ++i) offsets[i] = off[i];
+   const size_t n = off[c->ncells];
+   if (n && gids) HIPCHK(hipMemcpy(gids, c->gid[c->lcur], n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+   if (n && f) HIPCHK(hipMemcpy(f, c->f[c->fcur], n * WID3 * sizeof(float), hipMemcpyDeviceToHost));
+   return 0;
+}
+
+int vacc_set_intersections(vacc_ctx* c, const double* isect) {
+   if (!c || (!isect && c->ncells)) return set_err(VACC_E_ARG, "null argument");
+   if (c->ncells)
+      HIPCHK(hipMemcpyAsync(c->isect, isect, (size_t)c->ncells * 12 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+   HIPCHK(hipStreamSynchronize(c->stream));
+   return 0;
+}
+
+int vacc_get_intersections(vacc_ctx* c, double* isect) {
+   if (!c || (!isect && c->ncells)) return set_err(VACC_E_ARG, "null argument");
+   if (c->ncells)
+      HIPCHK(hipMemcpyAsync(isect, c->isect, (size_t)c->ncells * 12 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+   HIPCHK(hipStreamSynchronize(c->stream));
+   return 0;
+}
+
+static void to_tin(const vacc_cell_fields& in, TransformIn& t) {
+   for (int i = 0; i < 3; ++i) {
+      t.B[i] = in.B[i];
+      t.V[i] = in.V[i];
+      t.EgradPe[i] = in.EgradPe[i];
+   }
+   for (int i = 0; i < 6; ++i) t.dB[i] = in.dB[i];
+   t.rhoq = in.rhoq;
+   t.dt = in.dt;
+}
+
+int vacc_compute_cell_intersections(vacc_ctx* c, const vacc_cell_fields* fields, uint32_t map_order, int32_t use_moments) {
+   if (!c || (!fields && c->ncells) || map_order > 2) return set_err(VACC_E_ARG, "bad argument");
+   if (!c->ncells) return 0;
+   std::vector<TransformIn> tin(c->ncells);
+   for (uint32_t i = 0; i < c->ncells; ++i) to_tin(fields[i], tin[i]);
+   HIPCHK(hipMemcpyAsync(c->fields, tin.data(), tin.size() * sizeof(TransformIn), hipMemcpyHostToDevice, c->stream));
+   if (use_moments) {
+      MomentArgs ma{};
+      ma.cell_off = c->off[c->lcur];
+      ma.gid = c->gid[c->lcur];
+      ma.src = c->src[c->lcur];
+      ma.f = c->f[c->fcur];
+      ma.out = c->moments;
+      ma.ncells = c->ncells;
+      for (int d = 0; d < 3; ++d) {
+         ma.len[d] = c->mi.len[d];
+         ma.vmin[d] = c->mi.vmin[d];
+         ma.cell[d] = c->mi.cell[d];
+      }
+      launch_moments(ma, c->stream);
+   }
+   launch_transform(c->mi, c->fields, c->moments, use_moments ? 1 : 0, c->sp.mass, c->sp.charge,
+                    c->opt.hall_minimum_rhoq, c->opt.ohm_gradpe_term, (int)map_order, c->isect, c->ncells, c->stream);
+   HIPCHK(hipGetLastError());
+   HIPCHK(hipStreamSynchronize(c->stream));   // fields buffer is host-owned until here
+   return 0;
+}
+
+int vacc_cell_intersections_host(const vacc_mesh* mesh, const vacc_species* sp, const vacc_options* opt,
+                                 const vacc_cell_fields* fields, uint32_t map_order, double out[12]) {
+   if (!mesh || !sp || !opt || !fields || !out || map_order > 2) return set_err(VACC_E_ARG, "bad argument");
+   MeshInfo mi;
+   for (int d = 0; d < 3; ++d) {
+      mi.len[d] = mesh->grid_length[d];
+      mi.vmin[d] = mesh->min_limits[d];
+      mi.cell[d] = cell_size(mesh->min_limits[d], mesh->max_limits[d], mesh->grid_length[d]);
+   }
+   TransformIn t;
+   to_tin(*fields, t);
+   const Affine fwd = acceleration_transform(t, sp->mass, sp->charge, opt->hall_minimum_rhoq, opt->ohm_gradpe_term);
+   cell_intersections(mi, fwd, (int)map_order, out);
+   return 0;
+}
+
+double vacc_max_v_dt(const vacc_species* sp, const vacc_options* opt, const double B[3]) {
+   const double B_mag = std::sqrt(B[0] * B[0] + B[1] * B[1] + B[2] * B[2]) + 1e-30;
+   const double gyro_period = 2 * PI * sp->mass / (sp->charge * B_mag);
+   return std::fabs(gyro_period) * (opt->max_rotation_deg / 360.0);
+}
+
+int vacc_map_1d(vacc_ctx* c, uint32_t dim) {
+   if (!c || dim > 2) return set_err(VACC_E_ARG, "bad argument");
+   if (!c->ncells) return 0;
+   int rc = reserve_for_step(c);
+   if (rc) return rc;
+   rc = do_pass(c, (int)dim);
+   if (rc) return rc;
+   rc = check_flags(c);
+   if (rc) return rc;
+   rc = refresh_count(c);
+   if (rc) return rc;
+   return status_or(c);
+}
+
+int vacc_accelerate_cells(vacc_ctx* c, uint32_t map_order) {
+   if (!c || map_order > 2) return set_err(VACC_E_ARG, "bad argument");
+   if (!c->ncells) return 0;
+   int rc = reserve_for_step(c);
+   if (rc) return rc;
+   for (int p = 0; p < 3; ++p) {
+      rc = do_pass(c, pass_dim((int)map_order, p));
+      if (rc) return rc;
+   }
+   rc = check_flags(c);
+   if (rc) return rc;
+   rc = refresh_count(c);
+   if (rc) return rc;
+   return status_or(c);
+}
+
+int vacc_adjust_blocks(vacc_ctx* c, double* rho_loss) {
+   if (!c) return set_err(VACC_E_ARG, "null context");
+   if (!c->ncells) return 0;
+   int rc = reserve_for_step(c);
+   if (rc) return rc;
+   rc = do_relist(c, c->sp.sparse_add_width_v, true);
+   if (rc) return rc;
+   rc = check_flags(c);
+   if (rc) return rc;
+   if (rho_loss) HIPCHK(hipMemcpy(rho_loss, c->rho_loss, c->ncells * sizeof(double), hipMemcpyDeviceToHost));
+   return refresh_count(c);
+}
+
+int vacc_moments(vacc_ctx* c, double* out) {
+   if (!c || (!out && c->ncells)) return set_err(VACC_E_ARG, "null argument");
+   if (!c->ncells) return 0;
+   MomentArgs ma{};
+   ma.cell_off = c->off[c->lcur];
+   ma.gid = c->gid[c->lcur];
+   ma.src = c->src[c->lcur];
+   ma.f = c->f[c->fcur];
+   ma.out = c->moments;
+   ma.ncells = c->ncells;
+   for (int d = 0; d < 3; ++d) {
+      ma.len[d] = c->mi.len[d];
+      ma.vmin[d] = c->mi.vmin[d];
+      ma.cell[d] = c->mi.cell[d];
+   }
+   launch_moments(ma, c->stream);
+   HIPCHK(hipGetLastError());
+   HIPCHK(hipMemcpyAsync(out, c->moments, (size_t)c->ncells * 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+   HIPCHK(hipStreamSynchronize(c->stream));
+   return 0;
+}
+
+int vacc_init_peaks(vacc_ctx* c, uint32_t ncells, uint32_t max_peaks, const vacc_peak* peaks, const int32_t* npeaks,
+                    const int32_t* bbox) {
+   if (!c || (ncells && (!peaks || !npeaks || !bbox)) || max_peaks == 0) return set_err(VACC_E_ARG, "bad argument");
+   std::vector<uint32_t> bm_off(ncells + 1, 0);
+   for (uint32_t i = 0; i < ncells; ++i) {
+      const int32_t* b = bbox + 6 * (size_t)i;
+      uint32_t words = 0;
+      if (npeaks[i] > 0) {
+         if (npeaks[i] > (int32_t)max_peaks) return set_err(VACC_E_ARG, "npeaks > max_peaks");
+         for (int d = 0; d < 3; ++d)
+            if (b[d] < 0 || b[d + 3] < b[d] || b[d + 3] >= (int32_t)c->mi.len[d])
+               return set_err(VACC_E_ARG, "candidate box outside the mesh");
+         const uint32_t rx = b[3] - b[0] + 1, ry = b[4] - b[1] + 1, rz = b[5] - b[2] + 1;
+         words = ry * rz * ((rx + 31) / 32);
+         if (words > (uint32_t)init_box_words_max()) return set_err(VACC_E_ARG, "candidate box too large");
+      }
+      bm_off[i + 1] = bm_off[i] + words;
+   }
+   int rc = alloc_cells(c, ncells);
+   if (rc) return rc;
+   HIPCHK(hipStreamSynchronize(c->stream));
+   c->ncells = ncells;
+   c->lcur = 0;
+   c->fcur = 0;
+   c->nblocks = 0;
+   c->content_valid = false;
+   if (!ncells) return 0;
+   vacc_peak* d_pk = nullptr;
+   int32_t *d_np = nullptr, *d_bb = nullptr;
+   uint32_t *d_bmo = nullptr, *d_bm = nullptr;
+   HIPCHK(hipMalloc(&d_pk, (size_t)ncells * max_peaks * sizeof(vacc_peak)));
+   HIPCHK(hipMalloc(&d_np, ncells * sizeof(int32_t)));
+   HIPCHK(hipMalloc(&d_bb, (size_t)ncells * 6 * sizeof(int32_t)));
+   HIPCHK(hipMalloc(&d_bmo, (ncells + 1) * sizeof(uint32_t)));
+   HIPCHK(hipMalloc(&d_bm, std::max<size_t>(bm_off[ncells], 1) * sizeof(uint32_t)));
+   HIPCHK(hipMemcpy(d_pk, peaks, (size_t)ncells * max_peaks * sizeof(vacc_peak), hipMemcpyHostToDevice));
+   HIPCHK(hipMemcpy(d_np, npeaks, ncells * sizeof(int32_t), hipMemcpyHostToDevice));
+   HIPCHK(hipMemcpy(d_bb, bbox, (size_t)ncells * 6 * sizeof(int32_t), hipMemcpyHostToDevice));
+   HIPCHK(hipMemcpy(d_bmo, bm_off.data(), (ncells + 1) * sizeof(uint32_t), hipMemcpyHostToDevice));
+   InitArgs ia{};
+   ia.peaks = d_pk;
+   ia.npeaks = d_np;
+   ia.bbox = d_bb;
+   ia.bm_off = d_bmo;
+   ia.content_bm = d_bm;
+   ia.n_keep = c->n_tgt;
+   ia.ncells = ncells;
+   ia.max_peaks = max_peaks;
+   for (int d = 0; d < 3; ++d) {
+      ia.len[d] = c->mi.len[d];
+      ia.vmin[d] = c->mi.vmin[d];
+      ia.cell[d] = c->mi.cell[d];
+   }
+   ia.mass = c->sp.mass;
+   ia.min_value = c->sp.sparse_min_value;
+   ia.add_width = c->sp.sparse_add_width_v;
+   launch_init(ia, false, c->stream);
+   ScanArgs sa{};
+   sa.a = c->n_tgt;
+   sa.a_off = c->off[0];
+   sa.n = ncells;
+   launch_scan(sa, c->stream);
+   HIPCHK(hipGetLastError());
+   rc = refresh_count(c);
+   if (rc) return rc;
+   const size_t total = c->nblocks;
+   c->nblocks = 0;   // nothing to carry over
+   rc = ensure_blocks(c, total + total / 2 + 64 * (size_t)ncells + 4096);
+   if (rc) return rc;
+   ia.out_off = c->off[0];
+   ia.gid_out = c->gid[0];
+   ia.src_out = c->src[0];
+   ia.f_out = c->f[0];
+   launch_init(ia, true, c->stream);
+   HIPCHK(hipGetLastError());
+   HIPCHK(hipMemsetAsync(c->status, 0, (ncells + 1) * sizeof(uint32_t), c->stream));
+   HIPCHK(hipMemsetAsync(c->flags, 0, sizeof(uint32_t), c->stream));
+   HIPCHK(hipStreamSynchronize(c->stream));
+   c->nblocks = total;
+   (void)hipFree(d_pk);
+   (void)hipFree(d_np);
+   (void)hipFree(d_bb);
+   (void)hipFree(d_bmo);
+   (void)hipFree(d_bm);
+   return 0;
+}
+
+int vacc_sync(vacc_ctx* c) {
+   if (!c) return set_err(VACC_E_ARG, "null context");
+   HIPCHK(hipStreamSynchronize(c->stream));
+   return status_or(c);
+}
+
+int vacc_get_status(vacc_ctx* c, uint32_t* status) {
+   if (!c || (!status && c->ncells)) return set_err(VACC_E_ARG, "null argument");
+   HIPCHK(hipStreamSynchronize(c->stream));
+   if (c->ncells) {
+      HIPCHK(hipMemcpy(status, c->status, c->ncells * sizeof(uint32_t), hipMemcpyDeviceToHost));
+      HIPCHK(hipMemset(c->status, 0, c->ncells * sizeof(uint32_t)));
+   }
+   return 0;
+}
+
+}  // extern "C"

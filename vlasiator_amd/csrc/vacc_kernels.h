@@ -1,0 +1,141 @@
+// vacc_kernels.h -- kernel argument blocks and launch wrappers of libvacc.
+#pragma once
+
+#include "vacc_common.h"
+
+namespace vacc {
+
+// Column-set record (one per transverse position with blocks), 16 bytes.
+//   x: b0 | b1 << 16   transverse block indices in the pass frame
+//   y: first column (cell-local)
+//   z: number of columns | COPY_FLAG
+//   w: unused
+// Column record, 16 bytes.
+//   x: first block index along the pass axis | length << 16
+//   y: position of the first block in the sorted source list (cell-local)
+//   z: first target block | last target block << 16
+//   w: output slot of the first target block (cell-local)
+constexpr uint32_t COPY_FLAG = 0x80000000u;
+
+struct ColumnArgs {
+   const uint32_t* cell_off;   // [ncells+1] input list offsets
+   const uint32_t* gid;        // input list GIDs
+   const uint32_t* src;        // input list -> f pool index (INVALID = zero block)
+   const double* isect;        // [ncells*12]
+   uint32_t* sorted_src;       // [N] at cell_off
+   uint4* set_rec;             // [N] at cell_off
+   uint4* col_rec;             // [N] at cell_off
+   uint32_t* n_sets;           // [ncells]
+   uint32_t* n_targets;        // [ncells]
+   uint32_t* status;           // [ncells] OR-ed status bits
+   uint32_t ncells;
+   uint32_t len[3];
+   float dv;                   // (float) cell size along the pass axis
+   float vmin;                 // (float) mesh minimum along the pass axis
+   int wall_margin;
+};
+
+struct ScanArgs {
+   const uint32_t* a;      // [n] counts
+   const uint32_t* b;      // [n] counts (may be null)
+   uint32_t* a_off;        // [n+1] exclusive scan
+   uint32_t* b_off;        // [n+1]
+   uint32_t n;
+   uint32_t cap_a;         // capacity check on the a total (0 = none)
+   uint32_t* flags;        // device flags word: bit VACC_STATUS_CAPACITY set on overflow
+};
+
+struct RemapArgs {
+   const uint32_t* cell_off;    // input list offsets (record base of each cell)
+   const uint32_t* set_base;    // [ncells+1] global set index base
+   const uint32_t* out_off;     // [ncells+1] output block offsets
+   const uint4* set_rec;
+   const uint4* col_rec;
+   const uint32_t* sorted_src;
+   const double* isect;
+   const float* f_in;
+   float* f_out;
+   uint32_t* gid_out;
+   uint32_t* src_out;
+   uint8_t* content_out;
+   uint32_t* status;
+   const uint32_t* flags;       // global flags (capacity overflow -> skip)
+   uint32_t ncells;
+   uint32_t len[3];
+   float dv, vmin;
+   float threshold;             // (float) sparse min value, reconstruction threshold
+   double min_value;            // content test threshold
+};
+
+struct AdjustArgs {
+   const uint32_t* cell_off;   // input list
+   const uint32_t* gid;
+   const uint32_t* src;
+   const uint8_t* content;     // per input block, or null -> computed from f
+   const float* f;             // pool the src indices point into
+   const uint32_t* out_off;    // [ncells+1] (write phase)
+   uint32_t* n_keep;           // [ncells] (count phase)
+   uint32_t* gid_out;          // (write phase)
+   uint32_t* src_out;          // (write phase)
+   double* rho_loss;           // [ncells] (count phase, may be null)
+   const uint32_t* flags;      // global flags (capacity overflow -> the write phase is skipped)
+   uint32_t ncells;
+   uint32_t len[3];
+   double min_value;
+   double dv3;
+   int add_width;              // < 0: keep exactly the existing blocks (re-sort only)
+};
+
+struct MomentArgs {
+   const uint32_t* cell_off;
+   const uint32_t* gid;
+   const uint32_t* src;
+   const float* f;
+   double* out;                // [ncells*4]
+   uint32_t ncells;
+   uint32_t len[3];
+   double vmin[3];
+   double cell[3];
+};
+
+// synthetic initial state (vacc_init.hip)
+struct InitArgs {
+   const vacc_peak* peaks;
+   const int32_t* npeaks;
+   const int32_t* bbox;
+   const uint32_t* bm_off;   // [ncells+1] word offsets of the per-cell content bitmaps
+   uint32_t* content_bm;
+   uint32_t* n_keep;
+   const uint32_t* out_off;
+   uint32_t* gid_out;
+   uint32_t* src_out;
+   float* f_out;
+   uint32_t ncells;
+   uint32_t max_peaks;
+   uint32_t len[3];
+   double vmin[3];
+   double cell[3];
+   double mass;
+   double min_value;
+   int add_width;
+};
+
+void launch_init(const InitArgs& a, bool write_phase, hipStream_t s);
+int init_box_words_max();
+void launch_content(const uint32_t* src, const float* f, uint8_t* content, uint32_t n, double mv, hipStream_t s);
+
+void launch_columns(int dim, const ColumnArgs& a, hipStream_t s);
+void launch_scan(const ScanArgs& a, hipStream_t s);
+void launch_remap(int dim, int method, const RemapArgs& a, uint32_t max_sets, hipStream_t s);
+void launch_adjust(const AdjustArgs& a, bool write_phase, hipStream_t s);
+void launch_moments(const MomentArgs& a, hipStream_t s);
+void launch_transform(const MeshInfo& m, const TransformIn* fields, const double* moments, int use_moments,
+                      double mass, double charge, double hall_min_rhoq, int ohm_gradpe, int map_order,
+                      double* isect_out, uint32_t ncells, hipStream_t s);
+void launch_gather(const uint32_t* src, const float* f_in, float* f_out, uint32_t n, hipStream_t s);
+void launch_iota(uint32_t* dst, uint32_t n, hipStream_t s);
+void launch_fill_u32(uint32_t* dst, uint32_t v, uint32_t n, hipStream_t s);
+
+int remap_grid_size();
+
+}  // namespace vacc

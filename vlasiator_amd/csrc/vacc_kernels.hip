@@ -1,0 +1,1144 @@
+// vacc_kernels.hip -- gfx950 kernels of the velocity-space acceleration step.
+//
+// One 1-D remap pass along a dimension is three launches over all cells:
+//   k_columns  one workgroup per cell: LDS bitmap counting sort of the cell's
+//              blocks into column sets / columns along the pass axis, target
+//              extents of every column (cpu_acc_map.cpp:250-353) and the
+//              output slot of every target block.
+//   k_scan2    one workgroup: exclusive scans of per-cell set and target
+//              counts (global work list and output offsets).
+//   k_remap    persistent: 16 lanes per column set, one lane per transverse
+//              pencil, walks the columns of the set sourcing the blocks from
+//              HBM once and writing every target block once, with the PLM,
+//              PPM or PQM reconstruction and the piecewise integration of
+//              cpu_acc_map.cpp:393-631 in registers and a per-lane target
+//              ring in LDS.
+// Block adjust (spatial_cell_cpu.cpp:179-273) is k_adjust<false> (count) +
+// k_scan2 + k_adjust<true> (write), again with an LDS bitmap per cell.
+//
+// Floating point follows the reference CPU path built with the portable
+// vector class (VEC_FALLBACK_GENERIC): every float operation is rounded to
+// float, a float combined with a double literal is evaluated in double and
+// rounded.  The file is compiled with -ffp-contract=off.
+#include "vacc_kernels.h"
+
+#include <float.h>
+#include <limits.h>
+
+namespace vacc {
+
+constexpr int WG = 256;
+constexpr int BM_WORDS = 7168;   // LDS bitmap words per chunk of rows (LDS per workgroup <= 64 KiB)
+constexpr int ROW_MAX = 2048;    // rows per chunk (k_columns)
+
+// ---------------------------------------------------------------------------
+// small helpers
+// ---------------------------------------------------------------------------
+
+__device__ inline float dmul(double c, float x) { return (float)(c * (double)x); }
+
+template <int DIM>
+__device__ inline void pass_coords(uint32_t gid, const uint32_t* len, int& b0, int& b1, int& k) {
+   const uint32_t x = gid % len[0];
+   const uint32_t y = (gid / len[0]) % len[1];
+   const uint32_t z = gid / (len[0] * len[1]);
+   // frame of map_1d after swapBlockIndices (cpu_acc_map.cpp:85-106)
+   if (DIM == 0) { b0 = (int)z; b1 = (int)y; k = (int)x; }
+   else if (DIM == 1) { b0 = (int)x; b1 = (int)z; k = (int)y; }
+   else { b0 = (int)x; b1 = (int)y; k = (int)z; }
+}
+
+template <int DIM>
+__device__ inline uint32_t pass_gid(int b0, int b1, int k, const uint32_t* len) {
+   // block_indices_to_id of cpu_acc_map.cpp:151-199
+   const uint32_t nx = len[0], nxy = len[0] * len[1];
+   if (DIM == 0) return (uint32_t)b0 * nxy + (uint32_t)b1 * nx + (uint32_t)k;
+   if (DIM == 1) return (uint32_t)b0 + (uint32_t)b1 * nxy + (uint32_t)k * nx;
+   return (uint32_t)b0 + (uint32_t)b1 * nx + (uint32_t)k * nxy;
+}
+
+template <int DIM>
+__device__ inline void pass_isect(const double* is, float& I, float& di, float& dj, float& dk) {
+   // (Realf) conversion then the i<->k (dim 0) or j<->k (dim 1) swap, cpu_acc_map.cpp:128-176
+   I = (float)is[4 * DIM + 0];
+   const float a = (float)is[4 * DIM + 1], b = (float)is[4 * DIM + 2], c = (float)is[4 * DIM + 3];
+   if (DIM == 0) { di = c; dj = b; dk = a; }
+   else if (DIM == 1) { di = a; dj = c; dk = b; }
+   else { di = a; dj = b; dk = c; }
+}
+
+__device__ inline bool pass_is_copy(float dv, float di, float dj, float dk) {
+   // cells whose Lagrangian spacing is not positive, or whose per-cell deposit span would not
+   // fit the target ring, are passed through unchanged and flagged VACC_STATUS_DK
+   if (!(dk > 0.0f)) return true;
+   const float span = (dv + 3.0f * fabsf(di) + 3.0f * fabsf(dj)) / dk;
+   return !(span <= (float)MAX_SPAN_CELLS);
+}
+
+__device__ inline uint32_t wave_incl_scan(uint32_t v) {
+   const int lane = threadIdx.x & 63;
+#pragma unroll
+   for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t u = __shfl_up(v, o, 64);
+      if (lane >= o) v += u;
+   }
+   return v;
+}
+
+// Exclusive scan of NV values per thread over the workgroup (blockDim.x threads).
+template <int NV>
+__device__ inline void block_scan(const uint32_t (&v)[NV], uint32_t (&ex)[NV], uint32_t (&tot)[NV], uint32_t* sh) {
+   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+   uint32_t inc[NV];
+#pragma unroll
+   for (int q = 0; q < NV; ++q) inc[q] = wave_incl_scan(v[q]);
+   if (lane == 63) {
+#pragma unroll
+      for (int q = 0; q < NV; ++q) sh[w * NV + q] = inc[q];
+   }
+   __syncthreads();
+#pragma unroll
+   for (int q = 0; q < NV; ++q) {
+      uint32_t base = 0, all = 0;
+      for (int i = 0; i < nw; ++i) {
+         const uint32_t x = sh[i * NV + q];
+         if (i < w) base += x;
+         all += x;
+      }
+      ex[q] = base + inc[q] - v[q];
+      tot[q] = all;
+   }
+   __syncthreads();
+}
+
+__device__ inline int next_one(const uint32_t* w, int W, int p) {
+   int q = p >> 5;
+   if (q >= W) return -1;
+   uint32_t x = w[q] & (0xFFFFFFFFu << (p & 31));
+   for (;;) {
+      if (x) return q * 32 + __ffs(x) - 1;
+      if (++q >= W) return -1;
+      x = w[q];
+   }
+}
+__device__ inline int next_zero(const uint32_t* w, int W, int p) {
+   int q = p >> 5;
+   if (q >= W) return W * 32;
+   uint32_t x = ~w[q] & (0xFFFFFFFFu << (p & 31));
+   for (;;) {
+      if (x) return q * 32 + __ffs(x) - 1;
+      if (++q >= W) return W * 32;
+      x = ~w[q];
+   }
+}
+__device__ inline uint32_t rank_in_row(const uint32_t* w, int kk) {
+   uint32_t r = 0;
+   const int q = kk >> 5;
+   for (int i = 0; i < q; ++i) r += __popc(w[i]);
+   const int b = kk & 31;
+   if (b) r += __popc(w[q] & ((1u << b) - 1u));
+   return r;
+}
+
+// Target extent of a column set: max/min of the Lagrangian base over the 4 corner cells,
+// cpu_acc_map.cpp:258-288 (std::max(a,b) = a<b?b:a, std::min(a,b) = b<a?b:a).
+struct SetGeom {
+   float maxI, minI;
+};
+__device__ inline SetGeom set_geom(int b0, int b1, float I, float di, float dj) {
+   const float x0 = (float)(uint32_t)(b0 * WID), x3 = (float)(uint32_t)(b0 * WID + WID - 1);
+   const float y0 = (float)(uint32_t)(b1 * WID), y3 = (float)(uint32_t)(b1 * WID + WID - 1);
+   const float c00 = (I + x0 * di) + y0 * dj;
+   const float c03 = (I + x0 * di) + y3 * dj;
+   const float c30 = (I + x3 * di) + y0 * dj;
+   const float c33 = (I + x3 * di) + y3 * dj;
+   SetGeom g;
+   float m = c00;
+   m = (m < c03) ? c03 : m;
+   m = (m < c30) ? c30 : m;
+   m = (m < c33) ? c33 : m;
+   g.maxI = m;
+   m = c00;
+   m = (c03 < m) ? c03 : m;
+   m = (c30 < m) ? c30 : m;
+   m = (c33 < m) ? c33 : m;
+   g.minI = m;
+   return g;
+}
+
+// Target block range of one column, cpu_acc_map.cpp:307-338.
+__device__ inline void column_extent(int ks, int ke, const SetGeom& g, float dv, float vmin, float dk,
+                                     int maxlen, int wall, int& fk, int& lk, bool& wallhit) {
+   const float fbv = (float)(uint32_t)(WID * ks) * dv + vmin;
+   const float lbv = (float)(uint32_t)(WID * (ke + 1)) * dv + vmin;
+   const int fgk = (int)((fbv - g.maxI) / dk);
+   const int lgk = (int)((lbv - g.minI) / dk);
+   fk = fgk / WID;
+   lk = lgk / WID;
+   fk = (fk >= 0) ? fk : 0;
+   fk = (fk < maxlen) ? fk : maxlen - 1;
+   lk = (lk >= 0) ? lk : 0;
+   lk = (lk < maxlen) ? lk : maxlen - 1;
+   wallhit = (fk < wall) || (fk >= maxlen - wall) || (lk < wall) || (lk >= maxlen - wall);
+}
+
+// ---------------------------------------------------------------------------
+// k_columns
+// ---------------------------------------------------------------------------
+
+template <int DIM>
+__global__ __launch_bounds__(WG) void k_columns(ColumnArgs a) {
+   __shared__ uint32_t bm[BM_WORDS];
+   __shared__ uint32_t rowv[4][ROW_MAX];
+   __shared__ int sbox[6];
+   __shared__ uint32_t sh[4 * 4];
+   __shared__ uint32_t carry[4];
+   __shared__ uint32_t sstat;
+   const int t = threadIdx.x;
+   const uint32_t c = blockIdx.x;
+   const uint32_t off = a.cell_off[c];
+   const uint32_t n = a.cell_off[c + 1] - off;
+   if (n == 0) {
+      if (t == 0) {
+         a.n_sets[c] = 0;
+         a.n_targets[c] = 0;
+      }
+      return;
+   }
+   if (t < 3) {
+      sbox[t] = INT_MAX;
+      sbox[t + 3] = INT_MIN;
+   }
+   if (t < 4) carry[t] = 0;
+   if (t == 0) sstat = 0;
+   __syncthreads();
+   {
+      int lo0 = INT_MAX, lo1 = INT_MAX, lok = INT_MAX, hi0 = INT_MIN, hi1 = INT_MIN, hik = INT_MIN;
+      for (uint32_t i = t; i < n; i += WG) {
+         int b0, b1, k;
+         pass_coords<DIM>(a.gid[off + i], a.len, b0, b1, k);
+         lo0 = min(lo0, b0); hi0 = max(hi0, b0);
+         lo1 = min(lo1, b1); hi1 = max(hi1, b1);
+         lok = min(lok, k); hik = max(hik, k);
+      }
+      atomicMin(&sbox[0], lo0); atomicMin(&sbox[1], lo1); atomicMin(&sbox[2], lok);
+      atomicMax(&sbox[3], hi0); atomicMax(&sbox[4], hi1); atomicMax(&sbox[5], hik);
+   }
+   __syncthreads();
+   const int mn0 = sbox[0], mn1 = sbox[1], mnk = sbox[2];
+   const int R1 = sbox[4] - mn1 + 1;
+   const int R0 = sbox[3] - mn0 + 1;
+   const int K = sbox[5] - mnk + 1;
+   const int W = (K + 31) >> 5;
+   const int rows = R0 * R1;
+   const int rpc = min(ROW_MAX, BM_WORDS / W);
+   float I, di, dj, dk;
+   pass_isect<DIM>(a.isect + 12 * (size_t)c, I, di, dj, dk);
+   const bool copy = pass_is_copy(a.dv, di, dj, dk);
+   const int maxlen = (int)a.len[DIM];
+   uint32_t mystat = copy ? VACC_STATUS_DK : 0u;
+
+   for (int r0 = 0; r0 < rows; r0 += rpc) {
+      const int nr = min(rpc, rows - r0);
+      for (int i = t; i < nr * W; i += WG) bm[i] = 0;
+      __syncthreads();
+      for (uint32_t i = t; i < n; i += WG) {
+         int b0, b1, k;
+         pass_coords<DIM>(a.gid[off + i], a.len, b0, b1, k);
+         const int row = (b0 - mn0) * R1 + (b1 - mn1) - r0;
+         if (row >= 0 && row < nr) {
+            const int kk = k - mnk;
+            atomicOr(&bm[row * W + (kk >> 5)], 1u << (kk & 31));
+         }
+      }
+      __syncthreads();
+      // per-row counts: blocks, columns, set (0/1), target blocks
+      for (int rr = t; rr < nr; rr += WG) {
+         const uint32_t* w = bm + rr * W;
+         uint32_t nb = 0;
+         for (int q = 0; q < W; ++q) nb += __popc(w[q]);
+         uint32_t ncol = 0, ntgt = 0;
+         if (nb) {
+            const int grow = r0 + rr;
+            const int b0 = mn0 + grow / R1, b1 = mn1 + grow % R1;
+            const SetGeom g = set_geom(b0, b1, I, di, dj);
+            int end = -1;
+            int p = 0;
+            for (;;) {
+               const int s = next_one(w, W, p);
+               if (s < 0) break;
+               const int e = next_zero(w, W, s) - 1;
+               int fk, lk;
+               if (copy) {
+                  fk = mnk + s;
+                  lk = mnk + e;
+               } else {
+                  bool wh;
+                  column_extent(mnk + s, mnk + e, g, a.dv, a.vmin, dk, maxlen, a.wall_margin, fk, lk, wh);
+                  if (wh) mystat |= VACC_STATUS_WALL;
+               }
+               if (lk > end) {
+                  ntgt += (uint32_t)(lk - max(fk, end + 1) + 1);
+                  end = lk;
+               }
+               ++ncol;
+               p = e + 1;
+            }
+         }
+         rowv[0][rr] = nb;
+         rowv[1][rr] = ncol;
+         rowv[2][rr] = nb ? 1u : 0u;
+         rowv[3][rr] = ntgt;
+      }
+      __syncthreads();
+      // exclusive scan of the four row quantities (contiguous chunk of rows per thread)
+      const int per = (nr + WG - 1) / WG;
+      const int rb = t * per, re = min(nr, rb + per);
+      uint32_t s4[4] = {0, 0, 0, 0}, ex[4], tot[4];
+      for (int i = rb; i < re; ++i)
+#pragma unroll
+         for (int q = 0; q < 4; ++q) s4[q] += rowv[q][i];
+      block_scan<4>(s4, ex, tot, sh);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+         uint32_t run = ex[q] + carry[q];
+         for (int i = rb; i < re; ++i) {
+            const uint32_t x = rowv[q][i];
+            rowv[q][i] = run;
+            run += x;
+         }
+      }
+      __syncthreads();
+      // emit set and column records
+      for (int rr = t; rr < nr; rr += WG) {
+         const uint32_t* w = bm + rr * W;
+         bool any = false;
+         for (int q = 0; q < W; ++q) any |= (w[q] != 0);
+         if (!any) continue;
+         const int grow = r0 + rr;
+         const int b0 = mn0 + grow / R1, b1 = mn1 + grow % R1;
+         const SetGeom g = set_geom(b0, b1, I, di, dj);
+         const uint32_t si = rowv[2][rr], cb = rowv[1][rr], spos0 = rowv[0][rr], slot0 = rowv[3][rr];
+         int end = -1;
+         uint32_t cnt = 0, ncol = 0, bpos = 0;
+         int p = 0;
+         for (;;) {
+            const int s = next_one(w, W, p);
+            if (s < 0) break;
+            const int e = next_zero(w, W, s) - 1;
+            int fk, lk;
+            if (copy) {
+               fk = mnk + s;
+               lk = mnk + e;
+            } else {
+               bool wh;
+               column_extent(mnk + s, mnk + e, g, a.dv, a.vmin, dk, maxlen, a.wall_margin, fk, lk, wh);
+            }
+            // slot of block fk in the union of target ranges (ranges ascend in start and end)
+            const uint32_t first = (fk <= end) ? (cnt - 1u - (uint32_t)(end - fk)) : cnt;
+            if (lk > end) {
+               cnt += (uint32_t)(lk - max(fk, end + 1) + 1);
+               end = lk;
+            }
+            const uint32_t len = (uint32_t)(e - s + 1);
+            a.col_rec[off + cb + ncol] = make_uint4((uint32_t)(mnk + s) | (len << 16), spos0 + bpos,
+                                                    (uint32_t)fk | ((uint32_t)lk << 16), slot0 + first);
+            bpos += len;
+            ++ncol;
+            p = e + 1;
+         }
+         a.set_rec[off + si] = make_uint4((uint32_t)b0 | ((uint32_t)b1 << 16), cb, ncol | (copy ? COPY_FLAG : 0u), 0u);
+      }
+      // source index of every block in sorted (set, k) order
+      for (uint32_t i = t; i < n; i += WG) {
+         int b0, b1, k;
+         pass_coords<DIM>(a.gid[off + i], a.len, b0, b1, k);
+         const int row = (b0 - mn0) * R1 + (b1 - mn1) - r0;
+         if (row >= 0 && row < nr) {
+            const uint32_t pos = rowv[0][row] + rank_in_row(bm + row * W, k - mnk);
+            a.sorted_src[off + pos] = a.src[off + i];
+         }
+      }
+      __syncthreads();
+      if (t < 4) carry[t] += tot[t];
+      __syncthreads();
+   }
+   if (mystat) atomicOr(&sstat, mystat);
+   __syncthreads();
+   if (t == 0) {
+      a.n_sets[c] = carry[2];
+      a.n_targets[c] = carry[3];
+      if (sstat) atomicOr(&a.status[c], sstat);
+   }
+}
+
+// ---------------------------------------------------------------------------
+// k_scan2: exclusive scans of one or two per-cell count arrays
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(1024) void k_scan2(ScanArgs a) {
+   __shared__ uint32_t sh[16 * 2];
+   __shared__ uint32_t carry[2];
+   if (threadIdx.x < 2) carry[threadIdx.x] = 0;
+   __syncthreads();
+   for (uint32_t base = 0; base < a.n; base += 1024) {
+      const uint32_t i = base + threadIdx.x;
+      uint32_t v[2] = {i < a.n ? a.a[i] : 0u, (a.b && i < a.n) ? a.b[i] : 0u};
+      uint32_t ex[2], tot[2];
+      block_scan<2>(v, ex, tot, sh);
+      if (i < a.n) {
+         a.a_off[i] = carry[0] + ex[0];
+         if (a.b) a.b_off[i] = carry[1] + ex[1];
+      }
+      __syncthreads();
+      if (threadIdx.x < 2) carry[threadIdx.x] += tot[threadIdx.x];
+      __syncthreads();
+   }
+   if (threadIdx.x == 0) {
+      a.a_off[a.n] = carry[0];
+      if (a.b) a.b_off[a.n] = carry[1];
+      if (a.cap_a && carry[0] > a.cap_a && a.flags) atomicOr(a.flags, VACC_STATUS_CAPACITY);
+   }
+}
+
+// ---------------------------------------------------------------------------
+// Reconstructions (one lane): v points at the source cell, v[-4..4] valid.
+// cpu_face_estimates.hpp, cpu_slope_limiters.hpp, cpu_1d_plm/ppm/pqm.hpp
+// ---------------------------------------------------------------------------
+
+__device__ inline float face_h4(const float* v) {
+   // left face of cell v[0], cpu_face_estimates.hpp:160-167
+   float s = dmul(-1.0, v[-2]);
+   s = s + dmul(7.0, v[-1]);
+   s = s + dmul(7.0, v[0]);
+   s = s - dmul(1.0, v[1]);
+   return (float)((1.0 / 12.0) * (double)s);
+}
+__device__ inline float face_h8(const float* v) {
+   // cpu_face_estimates.hpp:45-56
+   float s = dmul(-3.0, v[-4]);
+   s = s + dmul(29.0, v[-3]);
+   s = s - dmul(139.0, v[-2]);
+   s = s + dmul(533.0, v[-1]);
+   s = s + dmul(533.0, v[0]);
+   s = s - dmul(139.0, v[1]);
+   s = s + dmul(29.0, v[2]);
+   s = s - dmul(3.0, v[3]);
+   return (float)((1.0 / 840.0) * (double)s);
+}
+__device__ inline float deriv_h7(const float* v) {
+   // cpu_face_estimates.hpp:68-78
+   float s = dmul(9.0, v[-4]);
+   s = s - dmul(119.0, v[-3]);
+   s = s + dmul(889.0, v[-2]);
+   s = s - dmul(7175.0, v[-1]);
+   s = s + dmul(7175.0, v[0]);
+   s = s - dmul(889.0, v[1]);
+   s = s + dmul(119.0, v[2]);
+   s = s - dmul(9.0, v[3]);
+   return (float)((1.0 / 5040.0) * (double)s);
+}
+__device__ inline float mod_min(float s1, float s2) {
+   const float slope = (fabsf(s1) < fabsf(s2)) ? s1 : s2;
+   const float p = s1 * s2;
+   return (p <= 0.0f) ? 0.0f : slope;
+}
+__device__ inline float mod_max(float s1, float s2) {
+   const float slope = (fabsf(s1) > fabsf(s2)) ? s1 : s2;
+   const float p = s1 * s2;
+   return (p <= 0.0f) ? 0.0f : slope;
+}
+__device__ inline float superbee(float l, float m, float r) {
+   const float a = r - m;
+   const float b = m - l;
+   return mod_max(mod_min(a, 2.0f * b), mod_min(2.0f * a, b));
+}
+
+template <int METHOD>
+__device__ inline void reconstruct(const float* v, float thr, float scale, float* c);
+
+template <>
+__device__ inline void reconstruct<VACC_PLM>(const float* v, float thr, float scale, float* c) {
+   const float d = superbee(v[-1] * scale, v[0] * scale, v[1] * scale) * thr;
+   c[0] = v[0] - dmul(0.5, d);
+   c[1] = dmul(0.5, d);
+}
+
+template <>
+__device__ inline void reconstruct<VACC_PPM>(const float* v, float thr, float scale, float* c) {
+   float m = face_h4(v), p = face_h4(v + 1);
+   const float sl = superbee(v[-1] * scale, v[0] * scale, v[1] * scale);
+   const float sabs = fabsf(sl) * thr;
+   const float ssgn = (sl > 0.0f) ? 1.0f : -1.0f;
+   if (sabs == 0.0f) {
+      p = v[0];
+      m = v[0];
+   }
+   if ((v[-1] - m) * (m - v[0]) < 0.0f) m = v[0] - dmul(0.5, ssgn) * sabs;
+   if ((v[1] - p) * (p - v[0]) < 0.0f) p = v[0] + dmul(0.5, ssgn) * sabs;
+   const float one_sixth = (float)(1.0 / 6.0);
+   {
+      const float d = p - m;
+      const float mid = v[0] - dmul(0.5, m + p);
+      if (d * mid > (d * d) * one_sixth) m = 3.0f * v[0] - 2.0f * p;
+   }
+   {
+      const float d = p - m;
+      const float mid = v[0] - dmul(0.5, m + p);
+      if (((-d) * d) * one_sixth > d * mid) p = 3.0f * v[0] - 2.0f * m;
+   }
+   c[0] = m;
+   c[1] = (dmul(3.0, v[0]) - dmul(2.0, m)) - p;
+   c[2] = (m + p) - dmul(2.0, v[0]);
+}
+
+template <>
+__device__ inline void reconstruct<VACC_PQM>(const float* v, float thr, float scale, float* c) {
+   float fvl = face_h8(v), fvr = face_h8(v + 1);
+   float fdl = deriv_h7(v), fdr = deriv_h7(v + 1);
+   const float sl = superbee(v[-1] * scale, v[0] * scale, v[1] * scale);
+   const float sabs = fabsf(sl) * thr;
+   const float ssgn = (sl > 0.0f) ? 1.0f : -1.0f;
+   const float vk = v[0];
+   if (sabs == 0.0f) {
+      fvr = vk;
+      fvl = vk;
+      fdl = 0.0f;
+      fdr = 0.0f;
+   }
+   if ((v[-1] - fvl) * (fvl - vk) < 0.0f || ssgn * fdl < 0.0f) {
+      fvl = vk - dmul(0.5, ssgn) * sabs;
+      fdl = ssgn * sabs;
+   }
+   if ((v[1] - fvr) * (fvr - vk) < 0.0f || ssgn * fdr < 0.0f) {
+      fvr = vk + dmul(0.5, ssgn) * sabs;
+      fdr = ssgn * sabs;
+   }
+   // monotonicity filter, cpu_1d_pqm.hpp:31-146
+   const float b0 = ((dmul(60.0, vk) - dmul(24.0, fvr)) - dmul(36.0, fvl)) + dmul(3.0, fdr - dmul(3.0, fdl));
+   const float b1 = (((dmul(-360.0, vk) + dmul(36.0, fdl)) - dmul(24.0, fdr)) + dmul(168.0, fvr)) + dmul(192.0, fvl);
+   const float b2 = (dmul(360.0, vk) + dmul(30.0, fdr - fdl)) - dmul(180.0, fvl + fvr);
+   const float disc = (b1 * b1) - (4.0f * b0) * b2;
+   const float sq = (disc < 0.0f) ? (b1 + dmul(200.0, b2)) : sqrtf(disc);
+   const float tb2 = 2.0f * b2;
+   const float root1 = ((-b1) + sq) / tb2;
+   const float root2 = ((-b1) - sq) / tb2;
+   const float psl = dmul(2.0, vk - v[-1]);
+   const float psr = dmul(2.0, v[1] - vk);
+   const float ssg = psl + psr;
+   const float c1 = b0;
+   const float c2 = (float)((double)b1 / 2.0);
+   const float c3 = (float)((double)b2 / 3.0);
+   const float r1s = (root1 >= 0.0f && root1 <= 1.0f) ? (fdl + root1 * (c1 + root1 * (c2 + root1 * c3))) : ssg;
+   const float r2s = (root2 >= 0.0f && root2 <= 1.0f) ? (fdl + root2 * (c1 + root2 * (c2 + root2 * c3))) : ssg;
+   if (r1s * ssg < 0.0f || r2s * ssg < 0.0f) {
+      if (fabsf(psl) <= fabsf(psr)) {
+         fdl = (float)((1.0 / 3.0) * (((double)(10.0f * vk) - 2.0 * (double)fvr) - 8.0 * (double)fvl));
+         fdr = (float)((-10.0 * (double)vk + 6.0 * (double)fvr) + 4.0 * (double)fvl);
+         if (ssg * fdl < 0.0f) {
+            fdl = 0.0f;
+            fvr = 5.0f * vk - 4.0f * fvl;
+            fdr = 20.0f * (vk - fvl);
+         } else if (ssg * fdr < 0.0f) {
+            fdr = 0.0f;
+            fvl = (float)(0.5 * (double)(5.0f * vk - 3.0f * fvr));
+            fdl = (float)((10.0 / 3.0) * (double)(-vk + fvr));
+         }
+      } else {
+         fdl = (float)((10.0 * (double)vk - 6.0 * (double)fvl) - 4.0 * (double)fvr);
+         fdr = (float)((1.0 / 3.0) * ((-10.0 * (double)vk + (double)(2.0f * fvl)) + (double)(8.0f * fvr)));
+         if (ssg * fdl < 0.0f) {
+            fdl = 0.0f;
+            fvr = (float)(0.5 * (double)(5.0f * vk - 3.0f * fvl));
+            fdr = (float)((10.0 / 3.0) * (double)(vk - fvl));
+         } else if (ssg * fdr < 0.0f) {
+            fdr = 0.0f;
+            fvl = 5.0f * vk - 4.0f * fvr;
+            fdl = (float)(20.0 * (double)(-vk + fvr));
+         }
+      }
+   }
+   c[0] = fvl;
+   c[1] = (float)((double)fdl / 2.0);
+   c[2] = ((dmul(10.0, vk) - dmul(4.0, fvr)) - dmul(6.0, fvl)) + dmul(0.5, fdr - 3.0f * fdl);
+   c[3] = (((dmul(-15.0, vk) + dmul(1.5, fdl)) - fdr) + dmul(7.0, fvr)) + 8.0f * fvl;
+   c[4] = (dmul(6.0, vk) + dmul(0.5, fdr - fdl)) - dmul(3.0, fvl + fvr);
+}
+
+template <int METHOD>
+__device__ inline float integral(float x, const float* c) {
+   // cpu_acc_map.cpp:590-602
+   if (METHOD == VACC_PLM) return x * (c[0] + x * c[1]);
+   if (METHOD == VACC_PPM) return x * (c[0] + x * (c[1] + x * c[2]));
+   return x * (c[0] + x * (c[1] + x * (c[2] + x * (c[3] + x * c[4]))));
+}
+
+// ---------------------------------------------------------------------------
+// k_remap
+// ---------------------------------------------------------------------------
+
+template <int DIM>
+struct Pencil {
+   int o;   // offset of the pencil's first cell in a block
+   static constexpr int st = (DIM == 0) ? 1 : (DIM == 1) ? 4 : 16;   // stride along the pass axis
+};
+
+template <int DIM>
+__device__ inline void load_pencil(const float* __restrict__ f, uint32_t sidx, int o, float (&w)[4]) {
+   if (sidx == INVALID) {
+      w[0] = w[1] = w[2] = w[3] = 0.0f;
+      return;
+   }
+   const float* p = f + (size_t)sidx * WID3 + o;
+   if (DIM == 0) {
+      const float4 q = *reinterpret_cast<const float4*>(p);
+      w[0] = q.x; w[1] = q.y; w[2] = q.z; w[3] = q.w;
+   } else {
+      constexpr int st = Pencil<DIM>::st;
+      w[0] = p[0]; w[1] = p[st]; w[2] = p[2 * st]; w[3] = p[3 * st];
+   }
+}
+
+template <int DIM>
+__device__ inline void store_pencil(float* __restrict__ f, size_t slot, int o, const float (&w)[4]) {
+   float* p = f + slot * WID3 + o;
+   if (DIM == 0) {
+      *reinterpret_cast<float4*>(p) = make_float4(w[0], w[1], w[2], w[3]);
+   } else {
+      constexpr int st = Pencil<DIM>::st;
+      p[0] = w[0]; p[st] = w[1]; p[2 * st] = w[2]; p[3 * st] = w[3];
+   }
+}
+
+template <int DIM>
+__device__ inline void read_pencil(const float* f, size_t slot, int o, float (&w)[4]) {
+   const float* p = f + slot * WID3 + o;
+   if (DIM == 0) {
+      const float4 q = *reinterpret_cast<const float4*>(p);
+      w[0] = q.x; w[1] = q.y; w[2] = q.z; w[3] = q.w;
+   } else {
+      constexpr int st = Pencil<DIM>::st;
+      w[0] = p[0]; w[1] = p[st]; w[2] = p[2 * st]; w[3] = p[3 * st];
+   }
+}
+
+__device__ inline void set_content(uint8_t* content, size_t slot, const float (&w)[4], double mv, int setlane0, int l16) {
+   const bool has = ((double)w[0] >= mv) || ((double)w[1] >= mv) || ((double)w[2] >= mv) || ((double)w[3] >= mv);
+   const unsigned long long bal = __ballot(has);
+   if (l16 == 0) content[slot] = (uint8_t)(((bal >> setlane0) & 0xFFFFull) != 0ull);
+}
+
+template <int DIM, int METHOD>
+__global__ __launch_bounds__(WG) void k_remap(RemapArgs a, double i_dv) {
+   __shared__ float ring[RING][WG];
+   if (*a.flags & VACC_STATUS_CAPACITY) return;
+   const int tid = threadIdx.x;
+   const int l16 = tid & 15;
+   const int ip = l16 & 3, jp = l16 >> 2;
+   const int setlane0 = (tid & 63) & ~15;
+   const int o = (DIM == 0) ? (ip * WID2 + jp * WID) : (DIM == 1) ? (jp * WID2 + ip) : (jp * WID + ip);
+   const float thr = a.threshold;
+   const float scale = (float)(1.0 / (double)thr);
+   const double mv = a.min_value;
+   const uint32_t total = a.set_base[a.ncells];
+   const uint32_t gstride = gridDim.x * (WG / 16);
+   for (uint32_t g = blockIdx.x * (WG / 16) + (tid >> 4); g < total; g += gstride) {
+      uint32_t lo = 0, hi = a.ncells;
+      while (hi - lo > 1) {
+         const uint32_t mid = (lo + hi) >> 1;
+         if (a.set_base[mid] <= g) lo = mid;
+         else hi = mid;
+      }
+      const uint32_t c = lo;
+      const uint32_t cbase = a.cell_off[c];
+      const uint4 sr = a.set_rec[cbase + (g - a.set_base[c])];
+      const int b0 = (int)(sr.x & 0xFFFFu), b1 = (int)(sr.x >> 16);
+      const uint32_t col0 = sr.y;
+      const uint32_t ncol = sr.z & ~COPY_FLAG;
+      const bool copy = (sr.z & COPY_FLAG) != 0u;
+      const uint32_t obase = a.out_off[c];
+      float I, di, dj, dk;
+      pass_isect<DIM>(a.isect + 12 * (size_t)c, I, di, dj, dk);
+      // Lagrangian base of this lane's pencil, cpu_acc_map.cpp:493-496
+      const float Imin = (I + (float)(uint32_t)(b0 * WID + ip) * di) + (float)(uint32_t)(b1 * WID + jp) * dj;
+      int prev_end = -1;
+      for (uint32_t ci = 0; ci < ncol; ++ci) {
+         const uint4 cr = a.col_rec[cbase + col0 + ci];
+         const int ks = (int)(cr.x & 0xFFFFu), len = (int)(cr.x >> 16);
+         const int fk = (int)(cr.z & 0xFFFFu), lk = (int)(cr.z >> 16);
+         const uint32_t* srcs = a.sorted_src + cbase + cr.y;
+         const size_t slotb = (size_t)obase + cr.w;   // output slot of block fk
+         for (int tb = max(fk, prev_end + 1) + l16; tb <= lk; tb += 16) {
+            const size_t slot = slotb + (size_t)(tb - fk);
+            a.gid_out[slot] = pass_gid<DIM>(b0, b1, tb, a.len);
+            a.src_out[slot] = (uint32_t)slot;
+         }
+         if (copy) {
+            for (int b = 0; b < len; ++b) {
+               float w[4];
+               load_pencil<DIM>(a.f_in, srcs[b], o, w);
+               store_pencil<DIM>(a.f_out, slotb + b, o, w);
+               set_content(a.content_out, slotb + b, w, mv, setlane0, l16);
+            }
+            prev_end = max(prev_end, lk);
+            continue;
+         }
+         float v_r = (float)(uint32_t)(WID * ks) * a.dv + a.vmin;
+         const float lvr = (v_r - Imin) / dk;
+         int gkr = (int)lvr;
+         // lanes of min/max Lagrangian index within the 8-lane vector (cpu_acc_map.cpp:515-529)
+         int minL = 0, maxL = 0;
+         {
+            double maxV = DBL_MIN, minV = DBL_MAX;
+#pragma unroll
+            for (int L = 0; L < 8; ++L) {
+               const float x = __shfl(lvr, L, 8);
+               if ((double)x > maxV) { maxV = (double)x; maxL = L; }
+               if ((double)x < minV) { minV = (double)x; minL = L; }
+            }
+         }
+         float w0[4] = {0.0f, 0.0f, 0.0f, 0.0f}, w1[4], w2[4];
+         load_pencil<DIM>(a.f_in, srcs[0], o, w1);
+         if (len > 1) load_pencil<DIM>(a.f_in, srcs[1], o, w2);
+         else w2[0] = w2[1] = w2[2] = w2[3] = 0.0f;
+         int next_open = fk, next_flush = fk;
+         const int lo_gk = WID * fk, hi_gk = WID * lk + WID - 1;
+         for (int b = 0; b < len; ++b) {
+            float w3[4];
+            if (b + 2 < len) load_pencil<DIM>(a.f_in, srcs[b + 2], o, w3);
+            else w3[0] = w3[1] = w3[2] = w3[3] = 0.0f;
+            const float v[12] = {w0[0], w0[1], w0[2], w0[3], w1[0], w1[1], w1[2], w1[3], w2[0], w2[1], w2[2], w2[3]};
+#pragma unroll
+            for (int kk = 0; kk < WID; ++kk) {
+               float cf[5];
+               reconstruct<METHOD>(v + 4 + kk, thr, scale, cf);
+               const float v_l = v_r;
+               v_r = v_r + a.dv;
+               const int gkl = gkr;
+               gkr = (int)((v_r - Imin) / dk);
+               const int mnG = max(__shfl(gkl, minL, 8), lo_gk);
+               const int mxG = min(__shfl(gkr, maxL, 8), hi_gk);
+               const int flushLim = min(mnG, __shfl_xor(mnG, 8, 16));
+               const int openLim = max(max(mxG, __shfl_xor(mxG, 8, 16)), flushLim);
+               while (next_open <= lk && WID * next_open <= openLim) {
+                  if (next_open - next_flush >= RING / WID) {   // ring full: cannot happen for checked spans
+                     atomicOr(&a.status[c], VACC_STATUS_DK);
+                     float q[4];
+#pragma unroll
+                     for (int e = 0; e < 4; ++e) q[e] = ring[(WID * next_flush + e) & (RING - 1)][tid];
+                     store_pencil<DIM>(a.f_out, slotb + (next_flush - fk), o, q);
+                     ++next_flush;
+                  }
+                  float q[4];
+                  if (next_open <= prev_end) read_pencil<DIM>(a.f_out, slotb + (next_open - fk), o, q);
+                  else q[0] = q[1] = q[2] = q[3] = 0.0f;
+#pragma unroll
+                  for (int e = 0; e < 4; ++e) ring[(WID * next_open + e) & (RING - 1)][tid] = q[e];
+                  ++next_open;
+               }
+               while (next_flush < next_open && WID * next_flush + WID - 1 < flushLim) {
+                  float q[4];
+#pragma unroll
+                  for (int e = 0; e < 4; ++e) q[e] = ring[(WID * next_flush + e) & (RING - 1)][tid];
+                  const size_t slot = slotb + (next_flush - fk);
+                  store_pencil<DIM>(a.f_out, slot, o, q);
+                  set_content(a.content_out, slot, q, mv, setlane0, l16);
+                  ++next_flush;
+               }
+               float tdr = 0.0f;
+               for (int gk = mnG; gk <= mxG; ++gk) {
+                  const float y = (float)(gk + 1) * dk + Imin;
+                  const float mx = (y > v_l) ? y : v_l;
+                  const float mn = (mx < v_r) ? mx : v_r;
+                  const float vn = (float)((double)(mn - v_l) * i_dv);
+                  const float tdl = tdr;
+                  tdr = integral<METHOD>(vn, cf);
+                  float& r = ring[gk & (RING - 1)][tid];
+                  r = r + (tdr - tdl);
+               }
+            }
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+               w0[e] = w1[e];
+               w1[e] = w2[e];
+               w2[e] = w3[e];
+            }
+         }
+         while (next_flush < next_open) {
+            float q[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) q[e] = ring[(WID * next_flush + e) & (RING - 1)][tid];
+            const size_t slot = slotb + (next_flush - fk);
+            store_pencil<DIM>(a.f_out, slot, o, q);
+            set_content(a.content_out, slot, q, mv, setlane0, l16);
+            ++next_flush;
+         }
+         for (int tb = next_open; tb <= lk; ++tb) {
+            if (tb <= prev_end) continue;
+            const float q[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+            const size_t slot = slotb + (tb - fk);
+            store_pencil<DIM>(a.f_out, slot, o, q);
+            set_content(a.content_out, slot, q, mv, setlane0, l16);
+         }
+         prev_end = max(prev_end, lk);
+      }
+   }
+}
+
+// ---------------------------------------------------------------------------
+// k_adjust: keep = content blocks and their (2w+1)^3 velocity neighbours
+// (spatial_cell_cpu.cpp:190-210), or, for w < 0, the existing blocks.  The
+// count phase writes the kept count and the mass of removed blocks
+// (:227-267); the write phase writes the kept list in ascending GID order.
+// ---------------------------------------------------------------------------
+
+template <bool WRITE>
+__global__ __launch_bounds__(WG) void k_adjust(AdjustArgs a) {
+   __shared__ uint32_t bm[BM_WORDS];
+   __shared__ uint32_t wpre[WRITE ? BM_WORDS : 1];
+   __shared__ int sbox[6];
+   __shared__ uint32_t sh[4];
+   __shared__ uint32_t carry;
+   __shared__ double sloss;
+   if (WRITE && (*a.flags & VACC_STATUS_CAPACITY)) return;
+   const int t = threadIdx.x;
+   const uint32_t c = blockIdx.x;
+   const uint32_t off = a.cell_off[c];
+   const uint32_t n = a.cell_off[c + 1] - off;
+   const int w = a.add_width;
+   const int ww = (w > 0) ? w : 0;
+   if (n == 0) {
+      if (!WRITE && t == 0) {
+         a.n_keep[c] = 0;
+         if (a.rho_loss) a.rho_loss[c] = 0.0;
+      }
+      return;
+   }
+   if (t < 3) {
+      sbox[t] = INT_MAX;
+      sbox[t + 3] = INT_MIN;
+   }
+   if (t == 0) {
+      carry = 0;
+      sloss = 0.0;
+   }
+   __syncthreads();
+   const uint32_t nx = a.len[0], ny = a.len[1], nz = a.len[2];
+   {
+      int lx = INT_MAX, ly = INT_MAX, lz = INT_MAX, hx = INT_MIN, hy = INT_MIN, hz = INT_MIN;
+      for (uint32_t i = t; i < n; i += WG) {
+         const uint32_t g = a.gid[off + i];
+         const int x = (int)(g % nx), y = (int)((g / nx) % ny), z = (int)(g / (nx * ny));
+         lx = min(lx, x); hx = max(hx, x);
+         ly = min(ly, y); hy = max(hy, y);
+         lz = min(lz, z); hz = max(hz, z);
+      }
+      atomicMin(&sbox[0], lx); atomicMin(&sbox[1], ly); atomicMin(&sbox[2], lz);
+      atomicMax(&sbox[3], hx); atomicMax(&sbox[4], hy); atomicMax(&sbox[5], hz);
+   }
+   __syncthreads();
+   const int mnx = max(sbox[0] - ww, 0), mny = max(sbox[1] - ww, 0), mnz = max(sbox[2] - ww, 0);
+   const int mxx = min(sbox[3] + ww, (int)nx - 1), mxy = min(sbox[4] + ww, (int)ny - 1), mxz = min(sbox[5] + ww, (int)nz - 1);
+   const int Rx = mxx - mnx + 1, Ry = mxy - mny + 1, Rz = mxz - mnz + 1;
+   const int W = (Rx + 31) >> 5;
+   const int rows = Ry * Rz;
+   const int rpc = BM_WORDS / W;
+   double myloss = 0.0;
+   for (int r0 = 0; r0 < rows; r0 += rpc) {
+      const int nr = min(rpc, rows - r0);
+      for (int i = t; i < nr * W; i += WG) bm[i] = 0;
+      __syncthreads();
+      for (uint32_t i = t; i < n; i += WG) {
+         const uint32_t g = a.gid[off + i];
+         const int x = (int)(g % nx), y = (int)((g / nx) % ny), z = (int)(g / (nx * ny));
+         if (w < 0) {
+            const int row = (z - mnz) * Ry + (y - mny) - r0;
+            if (row >= 0 && row < nr) {
+               const int kx = x - mnx;
+               atomicOr(&bm[row * W + (kx >> 5)], 1u << (kx & 31));
+            }
+         } else if (a.content[off + i]) {
+            const int x0 = max(x - w, mnx), x1 = min(x + w, mxx);
+            for (int dz = -w; dz <= w; ++dz) {
+               const int zz = z + dz;
+               if (zz < mnz || zz > mxz) continue;
+               for (int dy = -w; dy <= w; ++dy) {
+                  const int yy = y + dy;
+                  if (yy < mny || yy > mxy) continue;
+                  const int row = (zz - mnz) * Ry + (yy - mny) - r0;
+                  if (row < 0 || row >= nr) continue;
+                  for (int xx = x0; xx <= x1; ++xx) {
+                     const int kx = xx - mnx;
+                     atomicOr(&bm[row * W + (kx >> 5)], 1u << (kx & 31));
+                  }
+               }
+            }
+         }
+      }
+      __syncthreads();
+      // word prefix counts over the chunk (contiguous words per thread)
+      const int nwords = nr * W;
+      const int per = (nwords + WG - 1) / WG;
+      const int wb = t * per, we = min(nwords, wb + per);
+      uint32_t s1[1] = {0}, ex1[1], tot1[1];
+      for (int q = wb; q < we; ++q) s1[0] += __popc(bm[q]);
+      block_scan<1>(s1, ex1, tot1, sh);
+      if (WRITE) {
+         uint32_t run = ex1[0] + carry;
+         for (int q = wb; q < we; ++q) {
+            wpre[q] = run;
+            run += __popc(bm[q]);
+         }
+         __syncthreads();
+         const uint32_t obase = a.out_off[c];
+         for (int q = t; q < nwords; q += WG) {
+            uint32_t x = bm[q];
+            uint32_t pos = wpre[q];
+            const int row = q / W + r0;
+            const int zz = mnz + row / Ry, yy = mny + row % Ry;
+            const int xb = mnx + (q % W) * 32;
+            while (x) {
+               const int b = __ffs(x) - 1;
+               x &= x - 1;
+               a.gid_out[obase + pos] = (uint32_t)(xb + b) + (uint32_t)yy * nx + (uint32_t)zz * nx * ny;
+               a.src_out[obase + pos] = INVALID;
+               ++pos;
+            }
+         }
+         __syncthreads();
+         for (uint32_t i = t; i < n; i += WG) {
+            const uint32_t g = a.gid[off + i];
+            const int x = (int)(g % nx), y = (int)((g / nx) % ny), z = (int)(g / (nx * ny));
+            const int row = (z - mnz) * Ry + (y - mny) - r0;
+            if (row < 0 || row >= nr) continue;
+            const int kx = x - mnx;
+            const int q = row * W + (kx >> 5);
+            const uint32_t bit = 1u << (kx & 31);
+            if (bm[q] & bit) {
+               const uint32_t pos = wpre[q] + __popc(bm[q] & (bit - 1u));
+               a.src_out[obase + pos] = a.src[off + i];
+            }
+         }
+      } else {
+         for (uint32_t i = t; i < n; i += WG) {
+            if (w < 0 || a.content[off + i]) continue;
+            const uint32_t g = a.gid[off + i];
+            const int x = (int)(g % nx), y = (int)((g / nx) % ny), z = (int)(g / (nx * ny));
+            const int row = (z - mnz) * Ry + (y - mny) - r0;
+            if (row < 0 || row >= nr) continue;
+            const int kx = x - mnx;
+            if (bm[row * W + (kx >> 5)] & (1u << (kx & 31))) continue;
+            // removed block: RHOLOSSADJUST += DV3 * sum(f)
+            const uint32_t s = a.src[off + i];
+            double sum = 0.0;
+            if (s != INVALID) {
+               const float* fb = a.f + (size_t)s * WID3;
+               for (int e = 0; e < WID3; ++e) sum += fb[e];
+            }
+            myloss += a.dv3 * sum;
+         }
+      }
+      __syncthreads();
+      if (t == 0) carry += tot1[0];
+      __syncthreads();
+   }
+   if (!WRITE) {
+      if (myloss != 0.0) atomicAdd(&sloss, myloss);
+      __syncthreads();
+      if (t == 0) {
+         a.n_keep[c] = carry;
+         if (a.rho_loss) a.rho_loss[c] = sloss;
+      }
+   }
+}
+
+// content flag of each block of a list, from f (spatial_cell_cpu.cpp:290-316)
+__global__ __launch_bounds__(WG) void k_content(const uint32_t* src, const float* f, uint8_t* content,
+                                                uint32_t n, double mv) {
+   const uint32_t wave = (blockIdx.x * WG + threadIdx.x) >> 6;
+   const int lane = threadIdx.x & 63;
+   if (wave >= n) return;
+   const uint32_t s = src[wave];
+   const float x = (s == INVALID) ? 0.0f : f[(size_t)s * WID3 + lane];
+   const unsigned long long b = __ballot((double)x >= mv);
+   if (lane == 0) content[wave] = (uint8_t)(b != 0ull);
+}
+
+// ---------------------------------------------------------------------------
+// moments, transform, gather
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(WG) void k_moments(MomentArgs a) {
+   __shared__ double red[4][WG / 64];
+   const int t = threadIdx.x;
+   const uint32_t c = blockIdx.x;
+   const uint32_t off = a.cell_off[c];
+   const uint32_t n = a.cell_off[c + 1] - off;
+   const double DV3 = a.cell[0] * a.cell[1] * a.cell[2];
+   const int e = t & 63;
+   const int i = e & 3, j = (e >> 2) & 3, k = e >> 4;
+   double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+   for (uint32_t b = t >> 6; b < n; b += WG / 64) {
+      const uint32_t s = a.src[off + b];
+      if (s == INVALID) continue;
+      const uint32_t g = a.gid[off + b];
+      const uint32_t x = g % a.len[0], y = (g / a.len[0]) % a.len[1], z = g / (a.len[0] * a.len[1]);
+      const double VX = (a.vmin[0] + x * (a.cell[0] * WID)) + (i + 0.5) * a.cell[0];
+      const double VY = (a.vmin[1] + y * (a.cell[1] * WID)) + (j + 0.5) * a.cell[1];
+      const double VZ = (a.vmin[2] + z * (a.cell[2] * WID)) + (k + 0.5) * a.cell[2];
+      const double v = (double)a.f[(size_t)s * WID3 + e];
+      s0 += v * DV3;
+      s1 += v * VX * DV3;
+      s2 += v * VY * DV3;
+      s3 += v * VZ * DV3;
+   }
+   for (int o = 32; o > 0; o >>= 1) {
+      s0 += __shfl_down(s0, o, 64);
+      s1 += __shfl_down(s1, o, 64);
+      s2 += __shfl_down(s2, o, 64);
+      s3 += __shfl_down(s3, o, 64);
+   }
+   if ((t & 63) == 0) {
+      red[0][t >> 6] = s0;
+      red[1][t >> 6] = s1;
+      red[2][t >> 6] = s2;
+      red[3][t >> 6] = s3;
+   }
+   __syncthreads();
+   if (t < 4) {
+      double s = 0.0;
+      for (int q = 0; q < WG / 64; ++q) s += red[t][q];
+      a.out[4 * (size_t)c + t] = s;
+   }
+}
+
+__global__ void k_transform(MeshInfo m, const TransformIn* fields, const double* moments, int use_moments,
+                            double mass, double charge, double hall_min_rhoq, int ohm_gradpe, int map_order,
+                            double* isect_out, uint32_t ncells) {
+   const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+   if (c >= ncells) return;
+   TransformIn in = fields[c];
+   if (use_moments) {
+      const double* mo = moments + 4 * (size_t)c;
+      if (mo[0] > 0.0) {
+         in.V[0] = mo[1] / mo[0];
+         in.V[1] = mo[2] / mo[0];
+         in.V[2] = mo[3] / mo[0];
+      }
+      in.rhoq = charge * mo[0];
+   }
+   const Affine fwd = acceleration_transform(in, mass, charge, hall_min_rhoq, ohm_gradpe);
+   cell_intersections(m, fwd, map_order, isect_out + 12 * (size_t)c);
+}
+
+__global__ void k_gather(const uint32_t* src, const float4* f_in, float4* f_out, uint32_t n) {
+   const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+   if (q >= (size_t)n * 16) return;
+   const uint32_t b = (uint32_t)(q >> 4);
+   const uint32_t s = src[b];
+   f_out[q] = (s == INVALID) ? make_float4(0.f, 0.f, 0.f, 0.f) : f_in[(size_t)s * 16 + (q & 15)];
+}
+
+__global__ void k_iota(uint32_t* dst, uint32_t n) {
+   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+   if (i < n) dst[i] = i;
+}
+
+__global__ void k_fill_u32(uint32_t* dst, uint32_t v, uint32_t n) {
+   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+   if (i < n) dst[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// launch wrappers
+// ---------------------------------------------------------------------------
+
+void launch_columns(int dim, const ColumnArgs& a, hipStream_t s) {
+   if (a.ncells == 0) return;
+   if (dim == 0) hipLaunchKernelGGL(k_columns<0>, dim3(a.ncells), dim3(WG), 0, s, a);
+   else if (dim == 1) hipLaunchKernelGGL(k_columns<1>, dim3(a.ncells), dim3(WG), 0, s, a);
+   else hipLaunchKernelGGL(k_columns<2>, dim3(a.ncells), dim3(WG), 0, s, a);
+}
+
+void launch_scan(const ScanArgs& a, hipStream_t s) {
+   hipLaunchKernelGGL(k_scan2, dim3(1), dim3(1024), 0, s, a);
+}
+
+static int g_remap_grid = 0;
+int remap_grid_size() {
+   if (g_remap_grid == 0) {
+      int dev = 0, cus = 0;
+      (void)hipGetDevice(&dev);
+      (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+      if (cus <= 0) cus = 256;
+      g_remap_grid = cus * 4;
+   }
+   return g_remap_grid;
+}
+
+template <int DIM, int METHOD>
+static void remap_dm(const RemapArgs& a, uint32_t grid, double i_dv, hipStream_t s) {
+   hipLaunchKernelGGL((k_remap<DIM, METHOD>), dim3(grid), dim3(WG), 0, s, a, i_dv);
+}
+
+template <int DIM>
+static void remap_d(int method, const RemapArgs& a, uint32_t grid, double i_dv, hipStream_t s) {
+   if (method == VACC_PLM) remap_dm<DIM, VACC_PLM>(a, grid, i_dv, s);
+   else if (method == VACC_PPM) remap_dm<DIM, VACC_PPM>(a, grid, i_dv, s);
+   else remap_dm<DIM, VACC_PQM>(a, grid, i_dv, s);
+}
+
+void launch_remap(int dim, int method, const RemapArgs& a, uint32_t max_sets, hipStream_t s) {
+   if (a.ncells == 0 || max_sets == 0) return;
+   uint32_t grid = (uint32_t)remap_grid_size();
+   const uint64_t need = ((uint64_t)max_sets + (WG / 16) - 1) / (WG / 16);
+   if (need < grid) grid = (uint32_t)need;
+   const double i_dv = 1.0 / (double)a.dv;   // cpu_acc_map.cpp:201, Real i_dv = 1.0/dv
+   if (dim == 0) remap_d<0>(method, a, grid, i_dv, s);
+   else if (dim == 1) remap_d<1>(method, a, grid, i_dv, s);
+   else remap_d<2>(method, a, grid, i_dv, s);
+}
+
+void launch_adjust(const AdjustArgs& a, bool write_phase, hipStream_t s) {
+   if (a.ncells == 0) return;
+   if (write_phase) hipLaunchKernelGGL(k_adjust<true>, dim3(a.ncells), dim3(WG), 0, s, a);
+   else hipLaunchKernelGGL(k_adjust<false>, dim3(a.ncells), dim3(WG), 0, s, a);
+}
+
+void launch_content(const uint32_t* src, const float* f, uint8_t* content, uint32_t n, double mv, hipStream_t s) {
+   if (n == 0) return;
+   const uint32_t blocks = (uint32_t)(((size_t)n * 64 + WG - 1) / WG);
+   hipLaunchKernelGGL(k_content, dim3(blocks), dim3(WG), 0, s, src, f, content, n, mv);
+}
+
+void launch_moments(const MomentArgs& a, hipStream_t s) {
+   if (a.ncells == 0) return;
+   hipLaunchKernelGGL(k_moments, dim3(a.ncells), dim3(WG), 0, s, a);
+}
+
+void launch_transform(const MeshInfo& m, const TransformIn* fields, const double* moments, int use_moments,
+                      double mass, double charge, double hall_min_rhoq, int ohm_gradpe, int map_order,
+                      double* isect_out, uint32_t ncells, hipStream_t s) {
+   if (ncells == 0) return;
+   hipLaunchKernelGGL(k_transform, dim3((ncells + 127) / 128), dim3(128), 0, s, m, fields, moments, use_moments,
+                      mass, charge, hall_min_rhoq, ohm_gradpe, map_order, isect_out, ncells);
+}
+
+void launch_gather(const uint32_t* src, const float* f_in, float* f_out, uint32_t n, hipStream_t s) {
+   if (n == 0) return;
+   const size_t q = (size_t)n * 16;
+   hipLaunchKernelGGL(k_gather, dim3((uint32_t)((q + 255) / 256)), dim3(256), 0, s, src,
+                      reinterpret_cast<const float4*>(f_in), reinterpret_cast<float4*>(f_out), n);
+}
+
+void launch_iota(uint32_t* dst, uint32_t n, hipStream_t s) {
+   if (n == 0) return;
+   hipLaunchKernelGGL(k_iota, dim3((n + 255) / 256), dim3(256), 0, s, dst, n);
+}
+
+void launch_fill_u32(uint32_t* dst, uint32_t v, uint32_t n, hipStream_t s) {
+   if (n == 0) return;
+   hipLaunchKernelGGL(k_fill_u32, dim3((n + 255) / 256), dim3(256), 0, s, dst, v, n);
+}
+
+}  // namespace vacc
