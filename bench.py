@@ -1,0 +1,205 @@
+#!/usr/bin/env python3
+"""Benchmark of the velocity-space acceleration step on one MI355X.
+
+A step is one full acceleration of every spatial cell of the workload: FP64 moments
+(bulk-velocity pivot) + transform + SLICE-3D intersections on the device, the three 1-D
+remaps in the step's map order, and the sparsity block adjust.  Inputs are resident in
+HBM before the timed region.  The metric is velocity-cell updates per second:
+sum over cells of 64 * (blocks at step start), per step, over the timed steps.
+
+The JSON line also carries the roofline of the dominant kernel (the remap, k_remap),
+measured with HIP events on the library's stream over the timed region, and a CPU
+baseline: the C restatement of the reference CPU path (oracle/, OpenMP
+schedule(dynamic,1) over cells) timed on a bounded sample of the same cells.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 2] [--cells N]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from vlasiator_amd import vacc as V  # noqa: E402
+from vlasiator_amd import workloads as W  # noqa: E402
+
+METRIC = "velocity-cell updates/s per 3-D accel step; % HBM peak; FP32 vs BF16-store"
+HBM_PEAK_GBPS = 8000.0
+BYTES_PER_BLOCK_F32 = 64 * 4
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_context(w: W.Workload):
+    mesh = V.make_mesh(w.grid_length, w.vmin, w.vmax)
+    sp = V.make_species(w.mass, w.charge, w.min_value, w.add_width)
+    op = V.make_options(w.method)
+    return V.Context(mesh, sp, op)
+
+
+def run_step(ctx, fields, step, ncells, adjust):
+    mo = step % 3
+    ctx.compute_intersections(fields, mo, use_moments=True)
+    ctx.accelerate(mo)
+    if adjust:
+        ctx.adjust(ncells)
+
+
+def cpu_baseline(w, sample_cells, isect, target_s, nthreads):
+    """Oracle (C restatement of the reference CPU path) on a bounded sample of cells."""
+    import oracle as O
+    O.build(ref=False) if not os.path.exists(O.LIB_PATH) else None
+    m = O.make_mesh(w.grid_length, w.vmin, w.vmax)
+
+    def run(ncell):
+        cells = sample_cells[:ncell]
+        off = np.zeros(ncell + 1, np.uint32)
+        off[1:] = np.cumsum([len(g) for g, _ in cells])
+        g = np.concatenate([c[0] for c in cells])
+        f = np.concatenate([c[1] for c in cells])
+        t0 = time.perf_counter()
+        O.accelerate_cells(m, off, g, f, isect[:ncell], 0, w.method, w.min_value, do_adjust=w.adjust_every_step,
+                           add_width=w.add_width, nthreads=nthreads, want_output=False)
+        dt = time.perf_counter() - t0
+        return 64.0 * float(off[-1]), dt
+
+    n = min(8, len(sample_cells))
+    upd, dt = run(n)
+    rate = upd / dt
+    per_cell = dt / n
+    ncell = int(min(len(sample_cells), max(n, target_s / max(per_cell, 1e-9))))
+    if ncell > n:
+        upd, dt = run(ncell)
+        rate = upd / dt
+    return {"value": rate, "unit": "cell-updates/s", "cores": nthreads, "kind": "port",
+            "sample": f"first {ncell} cells of the workload, one step (3 remaps"
+                      f"{' + adjust' if w.adjust_every_step else ''}), {dt:.1f} s of CPU work"}
+
+
+def plain_form(w, cells, isect, steps):
+    """One cell per launch: every cell in its own single-cell context (cpu_accelerate_cell form)."""
+    ctxs = []
+    for (g, f), x in zip(cells, isect):
+        c = make_context(w)
+        c.upload_cells([(g, f)])
+        c.set_intersections(x[None, :])
+        ctxs.append(c)
+    for c in ctxs:   # warm-up
+        c.accelerate(0)
+    n0 = sum(c.total_blocks() for c in ctxs)
+    t0 = time.perf_counter()
+    upd = 0
+    for s in range(steps):
+        for c in ctxs:
+            upd += 64 * c.total_blocks()
+            c.accelerate(s % 3)
+            if w.adjust_every_step:
+                c.adjust(1)
+    dt = time.perf_counter() - t0
+    for c in ctxs:
+        c.close()
+    return upd / dt, n0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--cells", type=int, default=0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--plain-cells", type=int, default=128)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    w = W.config(args.config, ncells=args.cells or None)
+    ncells = len(w.cells)
+    log(f"workload {w.name}: {ncells} cells, mesh {w.grid_length}, method {w.method}")
+    ctx = make_context(w)
+    t0 = time.perf_counter()
+    ctx.init_peaks([W.to_vacc_peaks(c) for c in w.cells], W.boxes(w))
+    n_init = ctx.total_blocks()
+    log(f"initial state: {n_init} blocks ({n_init * 256 / 1e9:.2f} GB f32) in {time.perf_counter() - t0:.1f} s")
+    fields = [V.make_fields(c.B, c.V, c.dt) for c in w.cells]
+
+    # bounded samples for the CPU baseline and the plain form, taken from the initial state
+    sample_n = min(ncells, 2048)
+    sample_cells = None
+    isect0 = None
+    if not args.no_cpu or args.plain_cells:
+        # the same initial state for the first sample_n cells, built in a context of its own
+        sw = W.config(args.config, ncells=sample_n)
+        sctx = make_context(sw)
+        sctx.init_peaks([W.to_vacc_peaks(c) for c in sw.cells], W.boxes(sw))
+        sample_cells = sctx.download_cells()
+        sctx.compute_intersections(fields[:sample_n], 0, use_moments=True)
+        isect0 = sctx.get_intersections(sample_n)
+        sctx.close()
+
+    for s in range(args.warmup):
+        run_step(ctx, fields, s, ncells, w.adjust_every_step)
+    ctx.sync()
+    ctx.profile_begin()
+    updates = 0
+    t0 = time.perf_counter()
+    for s in range(args.warmup, args.warmup + args.steps):
+        updates += 64 * ctx.total_blocks()
+        run_step(ctx, fields, s, ncells, w.adjust_every_step)
+    status = ctx.sync()
+    elapsed = time.perf_counter() - t0
+    prof = ctx.profile_read()
+    value = updates / elapsed
+    ms_per_step = 1e3 * elapsed / args.steps
+    log(f"timed {args.steps} steps: {ms_per_step:.3f} ms/step, {value:.4g} cell-updates/s, status {status}")
+
+    remap_bytes = (prof["remap_src_blocks"] + prof["remap_tgt_blocks"]) * BYTES_PER_BLOCK_F32
+    remap_s = prof["remap_ms"] / 1e3
+    achieved = remap_bytes / remap_s / 1e9 if remap_s > 0 else 0.0
+    launches = max(prof["remap_launches"], 1)
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                "kernel": "k_remap", "launches": prof["remap_launches"],
+                "avg_launch_ms": round(prof["remap_ms"] / launches, 4),
+                "algorithmic_bytes_per_launch": int(remap_bytes / launches)}
+    breakdown = {k: round(prof[k] / args.steps, 4) for k in ("remap_ms", "build_ms", "adjust_ms", "transform_ms")}
+    breakdown["host_other_ms"] = round(ms_per_step - sum(breakdown.values()), 4)
+
+    out = {"metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": w.name, "cells": ncells, "grid_length": list(w.grid_length),
+                      "method": ["PLM", "PPM", "PQM"][w.method], "storage": "f32",
+                      "adjust_every_step": w.adjust_every_step, "blocks_at_start": n_init,
+                      "rotation_deg_per_step": w.rotation_deg},
+           "roofline": roofline, "breakdown_ms_per_step": breakdown}
+
+    if args.plain_cells and sample_cells is not None:
+        npl = min(args.plain_cells, len(sample_cells))
+        log(f"plain form: {npl} single-cell contexts")
+        rate_plain, _ = plain_form(w, sample_cells[:npl], isect0[:npl], 3)
+        out["plain_form"] = {"value": rate_plain, "unit": "cell-updates/s", "cells": npl,
+                             "batched_over_plain": value / rate_plain}
+    if not args.no_cpu:
+        nthreads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        log(f"cpu baseline: {nthreads} threads, ~{args.cpu_seconds} s")
+        out["cpu_baseline"] = cpu_baseline(w, sample_cells, isect0, args.cpu_seconds, nthreads)
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

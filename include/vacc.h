@@ -109,6 +109,8 @@ const char* vacc_last_error(void);
  * Blocks of a cell must have distinct valid GIDs. */
 int vacc_upload_cells(vacc_ctx* ctx, uint32_t ncells, const uint64_t* offsets,
                       const uint32_t* gids, const float* f);
+/* Total blocks held as of the last call that changed the block lists (no device sync). */
+int vacc_total_blocks(vacc_ctx* ctx, uint64_t* total);
 /* Number of cells and blocks per cell (counts[ncells]) currently held. */
 int vacc_get_counts(vacc_ctx* ctx, uint32_t* ncells, uint32_t* counts);
 /* Download: offsets[ncells+1] written; gids/f sized by the caller from vacc_get_counts.
@@ -160,6 +162,24 @@ typedef struct vacc_peak {
 } vacc_peak;
 int vacc_init_peaks(vacc_ctx* ctx, uint32_t ncells, uint32_t max_peaks, const vacc_peak* peaks,
                     const int32_t* npeaks, const int32_t* bbox);
+/* Kernel timing with HIP events recorded on the context's stream (no counterpart in the
+ * reference, whose phiprof timers measure host wall time).  Between vacc_profile_begin and
+ * vacc_profile_read every remap launch (k_remap), column build (k_columns + scan) and block
+ * adjust is bracketed by events; the block counts each remap read and wrote are logged. */
+typedef struct vacc_profile {
+   uint64_t remap_launches;
+   double remap_ms;           /* summed k_remap durations */
+   uint64_t remap_src_blocks; /* summed source blocks over the remap launches */
+   uint64_t remap_tgt_blocks; /* summed target blocks */
+   uint64_t build_launches;
+   double build_ms;           /* k_columns + scan */
+   uint64_t adjust_launches;
+   double adjust_ms;          /* adjust count + scan + write */
+   uint64_t transform_launches;
+   double transform_ms;       /* moments + transform/intersections */
+} vacc_profile;
+int vacc_profile_begin(vacc_ctx* ctx);
+int vacc_profile_read(vacc_ctx* ctx, vacc_profile* out);
 /* Wait for the context's stream; returns the OR of per-cell status bits since the last call. */
 int vacc_sync(vacc_ctx* ctx);
 /* Per-cell status words since the last vacc_sync (ncells entries). */

@@ -65,6 +65,49 @@ struct vacc_ctx {
    TransformIn* fields = nullptr;
    double* moments = nullptr;
    double* rho_loss = nullptr;
+   // event timing
+   struct Rec {
+      hipEvent_t a, b;
+      int kind;   // 0 remap, 1 build, 2 adjust, 3 transform
+   };
+   bool prof = false;
+   std::vector<Rec> recs;
+   std::vector<hipEvent_t> ev_free;
+   uint32_t* prof_counts = nullptr;   // [2*PROF_LOG] input/output totals of each remap
+   uint32_t nlog = 0;
+};
+
+static constexpr uint32_t PROF_LOG = 16384;
+
+static hipEvent_t take_event(vacc_ctx* c) {
+   if (!c->ev_free.empty()) {
+      hipEvent_t e = c->ev_free.back();
+      c->ev_free.pop_back();
+      return e;
+   }
+   hipEvent_t e = nullptr;
+   (void)hipEventCreate(&e);
+   return e;
+}
+
+// Bracket a region of launches on the context stream with events when profiling.
+struct ProfScope {
+   vacc_ctx* c;
+   int kind;
+   hipEvent_t a = nullptr;
+   ProfScope(vacc_ctx* c_, int kind_) : c(c_), kind(kind_) {
+      if (c->prof) {
+         a = take_event(c);
+         (void)hipEventRecord(a, c->stream);
+      }
+   }
+   ~ProfScope() {
+      if (c->prof && a) {
+         hipEvent_t b = take_event(c);
+         (void)hipEventRecord(b, c->stream);
+         c->recs.push_back({a, b, kind});
+      }
+   }
 };
 
 static void free_blocks(vacc_ctx* c) {
@@ -203,16 +246,19 @@ static int do_pass(vacc_ctx* c, int dim) {
    ca.dv = cell_f(c, dim);                 // Realf dv = cellSize (cpu_acc_map.cpp:147)
    ca.vmin = (float)c->mi.vmin[dim];       // Realf v_min (cpu_acc_map.cpp:148)
    ca.wall_margin = c->opt.wall_margin;
-   launch_columns(dim, ca, c->stream);
-   ScanArgs sa{};
-   sa.a = c->n_tgt;
-   sa.b = c->n_sets;
-   sa.a_off = c->off[lo];
-   sa.b_off = c->set_base;
-   sa.n = c->ncells;
-   sa.cap_a = (uint32_t)std::min<size_t>(c->cap, 0xFFFFFFFFull);
-   sa.flags = c->flags;
-   launch_scan(sa, c->stream);
+   {
+      ProfScope ps(c, 1);
+      launch_columns(dim, ca, c->stream);
+      ScanArgs sa{};
+      sa.a = c->n_tgt;
+      sa.b = c->n_sets;
+      sa.a_off = c->off[lo];
+      sa.b_off = c->set_base;
+      sa.n = c->ncells;
+      sa.cap_a = (uint32_t)std::min<size_t>(c->cap, 0xFFFFFFFFull);
+      sa.flags = c->flags;
+      launch_scan(sa, c->stream);
+   }
    RemapArgs ra{};
    ra.cell_off = c->off[li];
    ra.set_base = c->set_base;
@@ -234,8 +280,18 @@ static int do_pass(vacc_ctx* c, int dim) {
    ra.vmin = ca.vmin;
    ra.threshold = (float)c->sp.sparse_min_value;   // Realf threshold argument of compute_*_coeff
    ra.min_value = c->sp.sparse_min_value;
-   launch_remap(dim, c->opt.method, ra, 0xFFFFFFFFu, c->stream);
+   {
+      ProfScope ps(c, 0);
+      launch_remap(dim, c->opt.method, ra, 0xFFFFFFFFu, c->stream);
+   }
    HIPCHK(hipGetLastError());
+   if (c->prof && c->nlog < PROF_LOG) {
+      HIPCHK(hipMemcpyAsync(c->prof_counts + 2 * c->nlog, c->off[li] + c->ncells, sizeof(uint32_t),
+                            hipMemcpyDeviceToDevice, c->stream));
+      HIPCHK(hipMemcpyAsync(c->prof_counts + 2 * c->nlog + 1, c->off[lo] + c->ncells, sizeof(uint32_t),
+                            hipMemcpyDeviceToDevice, c->stream));
+      c->nlog++;
+   }
    c->lcur = lo;
    c->fcur = fo;
    c->content_valid = true;
@@ -244,6 +300,7 @@ static int do_pass(vacc_ctx* c, int dim) {
 
 // Rebuild the list: add_width >= 0 is the sparsity adjust, add_width < 0 a re-sort by GID.
 static int do_relist(vacc_ctx* c, int add_width, bool want_loss) {
+   ProfScope ps(c, add_width >= 0 ? 2 : 4);
    const int li = c->lcur, lo = 1 - c->lcur;
    if (add_width >= 0 && !c->content_valid) {
       launch_content(c->src[li], c->f[c->fcur], c->content, (uint32_t)c->nblocks, c->sp.sparse_min_value, c->stream);
@@ -351,6 +408,12 @@ int vacc_destroy(vacc_ctx* c) {
    if (c->stream) (void)hipStreamSynchronize(c->stream);
    free_blocks(c);
    free_cells(c);
+   for (auto& r : c->recs) {
+      (void)hipEventDestroy(r.a);
+      (void)hipEventDestroy(r.b);
+   }
+   for (auto e : c->ev_free) (void)hipEventDestroy(e);
+   if (c->prof_counts) (void)hipFree(c->prof_counts);
    if (c->flags) (void)hipFree(c->flags);
    if (c->stream) (void)hipStreamDestroy(c->stream);
    delete c;
@@ -392,6 +455,12 @@ int vacc_upload_cells(vacc_ctx* c, uint32_t ncells, const uint64_t* offsets, con
    HIPCHK(hipStreamSynchronize(c->stream));
    c->nblocks = total;
    c->content_valid = false;
+   return 0;
+}
+
+int vacc_total_blocks(vacc_ctx* c, uint64_t* total) {
+   if (!c || !total) return set_err(VACC_E_ARG, "null argument");
+   *total = c->nblocks;
    return 0;
 }
 
@@ -459,6 +528,7 @@ int vacc_compute_cell_intersections(vacc_ctx* c, const vacc_cell_fields* fields,
    std::vector<TransformIn> tin(c->ncells);
    for (uint32_t i = 0; i < c->ncells; ++i) to_tin(fields[i], tin[i]);
    HIPCHK(hipMemcpyAsync(c->fields, tin.data(), tin.size() * sizeof(TransformIn), hipMemcpyHostToDevice, c->stream));
+   ProfScope ps(c, 3);
    if (use_moments) {
       MomentArgs ma{};
       ma.cell_off = c->off[c->lcur];
@@ -652,6 +722,43 @@ int vacc_init_peaks(vacc_ctx* c, uint32_t ncells, uint32_t max_peaks, const vacc
    (void)hipFree(d_bb);
    (void)hipFree(d_bmo);
    (void)hipFree(d_bm);
+   return 0;
+}
+
+int vacc_profile_begin(vacc_ctx* c) {
+   if (!c) return set_err(VACC_E_ARG, "null context");
+   HIPCHK(hipStreamSynchronize(c->stream));
+   if (!c->prof_counts) HIPCHK(hipMalloc(&c->prof_counts, 2 * PROF_LOG * sizeof(uint32_t)));
+   for (auto& r : c->recs) {
+      c->ev_free.push_back(r.a);
+      c->ev_free.push_back(r.b);
+   }
+   c->recs.clear();
+   c->nlog = 0;
+   c->prof = true;
+   return 0;
+}
+
+int vacc_profile_read(vacc_ctx* c, vacc_profile* out) {
+   if (!c || !out) return set_err(VACC_E_ARG, "null argument");
+   HIPCHK(hipStreamSynchronize(c->stream));
+   std::memset(out, 0, sizeof(*out));
+   for (auto& r : c->recs) {
+      float ms = 0.0f;
+      HIPCHK(hipEventElapsedTime(&ms, r.a, r.b));
+      if (r.kind == 0) { out->remap_launches++; out->remap_ms += ms; }
+      else if (r.kind == 1) { out->build_launches++; out->build_ms += ms; }
+      else if (r.kind == 2) { out->adjust_launches++; out->adjust_ms += ms; }
+      else if (r.kind == 3) { out->transform_launches++; out->transform_ms += ms; }
+   }
+   if (c->nlog) {
+      std::vector<uint32_t> cnt(2 * c->nlog);
+      HIPCHK(hipMemcpy(cnt.data(), c->prof_counts, cnt.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+      for (uint32_t i = 0; i < c->nlog; ++i) {
+         out->remap_src_blocks += cnt[2 * i];
+         out->remap_tgt_blocks += cnt[2 * i + 1];
+      }
+   }
    return 0;
 }
 

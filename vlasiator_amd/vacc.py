@@ -21,11 +21,11 @@ WID3 = 64
 
 # symbols declared in include/vacc.h
 EXPORTS = [
-    "vacc_create", "vacc_destroy", "vacc_last_error", "vacc_upload_cells", "vacc_get_counts",
+    "vacc_create", "vacc_destroy", "vacc_last_error", "vacc_upload_cells", "vacc_total_blocks", "vacc_get_counts",
     "vacc_download_cells", "vacc_set_intersections", "vacc_get_intersections",
     "vacc_compute_cell_intersections", "vacc_cell_intersections_host", "vacc_max_v_dt", "vacc_map_1d",
     "vacc_accelerate_cells", "vacc_adjust_blocks", "vacc_moments", "vacc_sync", "vacc_get_status",
-    "vacc_init_peaks",
+    "vacc_init_peaks", "vacc_profile_begin", "vacc_profile_read",
 ]
 
 
@@ -53,6 +53,16 @@ class CellFields(C.Structure):
                 ("EgradPe", C.c_double * 3), ("dt", C.c_double)]
 
 
+class Profile(C.Structure):
+    _fields_ = [("remap_launches", C.c_uint64), ("remap_ms", C.c_double), ("remap_src_blocks", C.c_uint64),
+                ("remap_tgt_blocks", C.c_uint64), ("build_launches", C.c_uint64), ("build_ms", C.c_double),
+                ("adjust_launches", C.c_uint64), ("adjust_ms", C.c_double), ("transform_launches", C.c_uint64),
+                ("transform_ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class Peak(C.Structure):
     _fields_ = [("kind", C.c_int32), ("reserved", C.c_int32), ("n", C.c_double), ("V", C.c_double * 3),
                 ("T", C.c_double), ("axis", C.c_double * 3), ("vperp", C.c_double), ("vth", C.c_double)]
@@ -73,6 +83,7 @@ def lib():
         L.vacc_destroy.argtypes = [vp]
         L.vacc_upload_cells.argtypes = [vp, C.c_uint32, vp, vp, vp]
         L.vacc_get_counts.argtypes = [vp, C.POINTER(C.c_uint32), vp]
+        L.vacc_total_blocks.argtypes = [vp, C.POINTER(C.c_uint64)]
         L.vacc_download_cells.argtypes = [vp, vp, vp, vp]
         L.vacc_set_intersections.argtypes = [vp, vp]
         L.vacc_get_intersections.argtypes = [vp, vp]
@@ -88,6 +99,8 @@ def lib():
         L.vacc_sync.argtypes = [vp]
         L.vacc_get_status.argtypes = [vp, vp]
         L.vacc_init_peaks.argtypes = [vp, C.c_uint32, C.c_uint32, vp, vp, vp]
+        L.vacc_profile_begin.argtypes = [vp]
+        L.vacc_profile_read.argtypes = [vp, C.POINTER(Profile)]
         _lib = L
     return _lib
 
@@ -181,6 +194,11 @@ class Context:
             np.zeros((0, WID3), np.float32)
         self.upload(off, g, f)
 
+    def total_blocks(self) -> int:
+        n = C.c_uint64()
+        _check(lib().vacc_total_blocks(self.h, C.byref(n)))
+        return n.value
+
     def counts(self):
         n = C.c_uint32()
         _check(lib().vacc_get_counts(self.h, C.byref(n), None))
@@ -234,6 +252,14 @@ class Context:
         out = np.zeros(max(ncells, 1) * 4, np.float64)
         _check(lib().vacc_moments(self.h, _ptr(out)))
         return out[:ncells * 4].reshape(ncells, 4)
+
+    def profile_begin(self):
+        _check(lib().vacc_profile_begin(self.h))
+
+    def profile_read(self) -> dict:
+        p = Profile()
+        _check(lib().vacc_profile_read(self.h, C.byref(p)))
+        return p.as_dict()
 
     def sync(self) -> int:
         return _check(lib().vacc_sync(self.h))
