@@ -1,0 +1,66 @@
+"""CPU check of the device reconstruction source: vlasiator_amd/csrc/vacc_recon.h is compiled
+for the host (same flags as the library: no FP contraction) and streamed over long columns;
+every coefficient must equal the oracle's bit for bit (the oracle is itself pinned to the
+reference's reconstruction headers by the golden vectors)."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SRC = os.path.join(HERE, "recon_host_check.hip")
+OUT = os.path.join(HERE, "_build", "librecon_host_check.so")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not os.path.exists(HIPCC):
+        pytest.skip("hipcc not available")
+    os.makedirs(os.path.dirname(OUT), exist_ok=True)
+    deps = [SRC, os.path.join(HERE, "..", "vlasiator_amd", "csrc", "vacc_recon.h"),
+            os.path.join(HERE, "..", "vlasiator_amd", "csrc", "vacc_common.h")]
+    if not os.path.exists(OUT) or os.path.getmtime(OUT) < max(os.path.getmtime(d) for d in deps):
+        subprocess.run([HIPCC, "--offload-arch=gfx950", "-O2", "-std=c++17", "-fPIC", "-shared",
+                        "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+                        SRC, "-o", OUT], check=True, capture_output=True)
+    L = C.CDLL(OUT)
+    L.recon_stream.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_float, C.c_void_p]
+    return L
+
+
+def columns(seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(40):
+        n = int(rng.integers(4, 120))
+        kind = rng.integers(0, 4)
+        x = np.linspace(-3, 3, n)
+        if kind == 0:
+            v = 10 ** rng.uniform(-17, -9, n)
+        elif kind == 1:
+            v = 1e-10 * np.exp(-((x - rng.uniform(-1, 1)) / rng.uniform(0.3, 2)) ** 2)
+        elif kind == 2:
+            v = np.where(rng.random(n) < 0.4, 0.0, 10 ** rng.uniform(-16, -10, n))
+        else:
+            v = 1e-15 * rng.uniform(0.3, 3, n)
+        col = np.zeros(n + 8, np.float32)
+        col[4:n + 4] = v
+        out.append(col)
+    return out
+
+
+@pytest.mark.parametrize("method", [O.PLM, O.PPM, O.PQM])
+def test_streamed_device_reconstruction_matches_oracle(lib, method):
+    nc = O.ncoef(method)
+    for col in columns(31 + method):
+        n = col.size - 8
+        got = np.zeros(n * nc, np.float32)
+        lib.recon_stream(method, col.ctypes.data, n, np.float32(1e-15), got.ctypes.data)
+        st = np.stack([col[k:k + 9] for k in range(n)])
+        want = O.coeff_batch(method, st, 1e-15).reshape(-1)
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32))

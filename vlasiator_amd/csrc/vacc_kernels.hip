@@ -21,6 +21,7 @@
 // float, a float combined with a double literal is evaluated in double and
 // rounded.  The file is compiled with -ffp-contract=off.
 #include "vacc_kernels.h"
+#include "vacc_recon.h"
 
 #include <float.h>
 #include <limits.h>
@@ -34,8 +35,6 @@ constexpr int ROW_MAX = 2048;    // rows per chunk (k_columns)
 // ---------------------------------------------------------------------------
 // small helpers
 // ---------------------------------------------------------------------------
-
-__device__ inline float dmul(double c, float x) { return (float)(c * (double)x); }
 
 template <int DIM>
 __device__ inline void pass_coords(uint32_t gid, const uint32_t* len, int& b0, int& b1, int& k) {
@@ -402,178 +401,6 @@ __global__ __launch_bounds__(1024) void k_scan2(ScanArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Reconstructions (one lane): v points at the source cell, v[-4..4] valid.
-// cpu_face_estimates.hpp, cpu_slope_limiters.hpp, cpu_1d_plm/ppm/pqm.hpp
-// ---------------------------------------------------------------------------
-
-__device__ inline float face_h4(const float* v) {
-   // left face of cell v[0], cpu_face_estimates.hpp:160-167
-   float s = dmul(-1.0, v[-2]);
-   s = s + dmul(7.0, v[-1]);
-   s = s + dmul(7.0, v[0]);
-   s = s - dmul(1.0, v[1]);
-   return (float)((1.0 / 12.0) * (double)s);
-}
-__device__ inline float face_h8(const float* v) {
-   // cpu_face_estimates.hpp:45-56
-   float s = dmul(-3.0, v[-4]);
-   s = s + dmul(29.0, v[-3]);
-   s = s - dmul(139.0, v[-2]);
-   s = s + dmul(533.0, v[-1]);
-   s = s + dmul(533.0, v[0]);
-   s = s - dmul(139.0, v[1]);
-   s = s + dmul(29.0, v[2]);
-   s = s - dmul(3.0, v[3]);
-   return (float)((1.0 / 840.0) * (double)s);
-}
-__device__ inline float deriv_h7(const float* v) {
-   // cpu_face_estimates.hpp:68-78
-   float s = dmul(9.0, v[-4]);
-   s = s - dmul(119.0, v[-3]);
-   s = s + dmul(889.0, v[-2]);
-   s = s - dmul(7175.0, v[-1]);
-   s = s + dmul(7175.0, v[0]);
-   s = s - dmul(889.0, v[1]);
-   s = s + dmul(119.0, v[2]);
-   s = s - dmul(9.0, v[3]);
-   return (float)((1.0 / 5040.0) * (double)s);
-}
-__device__ inline float mod_min(float s1, float s2) {
-   const float slope = (fabsf(s1) < fabsf(s2)) ? s1 : s2;
-   const float p = s1 * s2;
-   return (p <= 0.0f) ? 0.0f : slope;
-}
-__device__ inline float mod_max(float s1, float s2) {
-   const float slope = (fabsf(s1) > fabsf(s2)) ? s1 : s2;
-   const float p = s1 * s2;
-   return (p <= 0.0f) ? 0.0f : slope;
-}
-__device__ inline float superbee(float l, float m, float r) {
-   const float a = r - m;
-   const float b = m - l;
-   return mod_max(mod_min(a, 2.0f * b), mod_min(2.0f * a, b));
-}
-
-template <int METHOD>
-__device__ inline void reconstruct(const float* v, float thr, float scale, float* c);
-
-template <>
-__device__ inline void reconstruct<VACC_PLM>(const float* v, float thr, float scale, float* c) {
-   const float d = superbee(v[-1] * scale, v[0] * scale, v[1] * scale) * thr;
-   c[0] = v[0] - dmul(0.5, d);
-   c[1] = dmul(0.5, d);
-}
-
-template <>
-__device__ inline void reconstruct<VACC_PPM>(const float* v, float thr, float scale, float* c) {
-   float m = face_h4(v), p = face_h4(v + 1);
-   const float sl = superbee(v[-1] * scale, v[0] * scale, v[1] * scale);
-   const float sabs = fabsf(sl) * thr;
-   const float ssgn = (sl > 0.0f) ? 1.0f : -1.0f;
-   if (sabs == 0.0f) {
-      p = v[0];
-      m = v[0];
-   }
-   if ((v[-1] - m) * (m - v[0]) < 0.0f) m = v[0] - dmul(0.5, ssgn) * sabs;
-   if ((v[1] - p) * (p - v[0]) < 0.0f) p = v[0] + dmul(0.5, ssgn) * sabs;
-   const float one_sixth = (float)(1.0 / 6.0);
-   {
-      const float d = p - m;
-      const float mid = v[0] - dmul(0.5, m + p);
-      if (d * mid > (d * d) * one_sixth) m = 3.0f * v[0] - 2.0f * p;
-   }
-   {
-      const float d = p - m;
-      const float mid = v[0] - dmul(0.5, m + p);
-      if (((-d) * d) * one_sixth > d * mid) p = 3.0f * v[0] - 2.0f * m;
-   }
-   c[0] = m;
-   c[1] = (dmul(3.0, v[0]) - dmul(2.0, m)) - p;
-   c[2] = (m + p) - dmul(2.0, v[0]);
-}
-
-template <>
-__device__ inline void reconstruct<VACC_PQM>(const float* v, float thr, float scale, float* c) {
-   float fvl = face_h8(v), fvr = face_h8(v + 1);
-   float fdl = deriv_h7(v), fdr = deriv_h7(v + 1);
-   const float sl = superbee(v[-1] * scale, v[0] * scale, v[1] * scale);
-   const float sabs = fabsf(sl) * thr;
-   const float ssgn = (sl > 0.0f) ? 1.0f : -1.0f;
-   const float vk = v[0];
-   if (sabs == 0.0f) {
-      fvr = vk;
-      fvl = vk;
-      fdl = 0.0f;
-      fdr = 0.0f;
-   }
-   if ((v[-1] - fvl) * (fvl - vk) < 0.0f || ssgn * fdl < 0.0f) {
-      fvl = vk - dmul(0.5, ssgn) * sabs;
-      fdl = ssgn * sabs;
-   }
-   if ((v[1] - fvr) * (fvr - vk) < 0.0f || ssgn * fdr < 0.0f) {
-      fvr = vk + dmul(0.5, ssgn) * sabs;
-      fdr = ssgn * sabs;
-   }
-   // monotonicity filter, cpu_1d_pqm.hpp:31-146
-   const float b0 = ((dmul(60.0, vk) - dmul(24.0, fvr)) - dmul(36.0, fvl)) + dmul(3.0, fdr - dmul(3.0, fdl));
-   const float b1 = (((dmul(-360.0, vk) + dmul(36.0, fdl)) - dmul(24.0, fdr)) + dmul(168.0, fvr)) + dmul(192.0, fvl);
-   const float b2 = (dmul(360.0, vk) + dmul(30.0, fdr - fdl)) - dmul(180.0, fvl + fvr);
-   const float disc = (b1 * b1) - (4.0f * b0) * b2;
-   const float sq = (disc < 0.0f) ? (b1 + dmul(200.0, b2)) : sqrtf(disc);
-   const float tb2 = 2.0f * b2;
-   const float root1 = ((-b1) + sq) / tb2;
-   const float root2 = ((-b1) - sq) / tb2;
-   const float psl = dmul(2.0, vk - v[-1]);
-   const float psr = dmul(2.0, v[1] - vk);
-   const float ssg = psl + psr;
-   const float c1 = b0;
-   const float c2 = (float)((double)b1 / 2.0);
-   const float c3 = (float)((double)b2 / 3.0);
-   const float r1s = (root1 >= 0.0f && root1 <= 1.0f) ? (fdl + root1 * (c1 + root1 * (c2 + root1 * c3))) : ssg;
-   const float r2s = (root2 >= 0.0f && root2 <= 1.0f) ? (fdl + root2 * (c1 + root2 * (c2 + root2 * c3))) : ssg;
-   if (r1s * ssg < 0.0f || r2s * ssg < 0.0f) {
-      if (fabsf(psl) <= fabsf(psr)) {
-         fdl = (float)((1.0 / 3.0) * (((double)(10.0f * vk) - 2.0 * (double)fvr) - 8.0 * (double)fvl));
-         fdr = (float)((-10.0 * (double)vk + 6.0 * (double)fvr) + 4.0 * (double)fvl);
-         if (ssg * fdl < 0.0f) {
-            fdl = 0.0f;
-            fvr = 5.0f * vk - 4.0f * fvl;
-            fdr = 20.0f * (vk - fvl);
-         } else if (ssg * fdr < 0.0f) {
-            fdr = 0.0f;
-            fvl = (float)(0.5 * (double)(5.0f * vk - 3.0f * fvr));
-            fdl = (float)((10.0 / 3.0) * (double)(-vk + fvr));
-         }
-      } else {
-         fdl = (float)((10.0 * (double)vk - 6.0 * (double)fvl) - 4.0 * (double)fvr);
-         fdr = (float)((1.0 / 3.0) * ((-10.0 * (double)vk + (double)(2.0f * fvl)) + (double)(8.0f * fvr)));
-         if (ssg * fdl < 0.0f) {
-            fdl = 0.0f;
-            fvr = (float)(0.5 * (double)(5.0f * vk - 3.0f * fvl));
-            fdr = (float)((10.0 / 3.0) * (double)(vk - fvl));
-         } else if (ssg * fdr < 0.0f) {
-            fdr = 0.0f;
-            fvl = 5.0f * vk - 4.0f * fvr;
-            fdl = (float)(20.0 * (double)(-vk + fvr));
-         }
-      }
-   }
-   c[0] = fvl;
-   c[1] = (float)((double)fdl / 2.0);
-   c[2] = ((dmul(10.0, vk) - dmul(4.0, fvr)) - dmul(6.0, fvl)) + dmul(0.5, fdr - 3.0f * fdl);
-   c[3] = (((dmul(-15.0, vk) + dmul(1.5, fdl)) - fdr) + dmul(7.0, fvr)) + 8.0f * fvl;
-   c[4] = (dmul(6.0, vk) + dmul(0.5, fdr - fdl)) - dmul(3.0, fvl + fvr);
-}
-
-template <int METHOD>
-__device__ inline float integral(float x, const float* c) {
-   // cpu_acc_map.cpp:590-602
-   if (METHOD == VACC_PLM) return x * (c[0] + x * c[1]);
-   if (METHOD == VACC_PPM) return x * (c[0] + x * (c[1] + x * c[2]));
-   return x * (c[0] + x * (c[1] + x * (c[2] + x * (c[3] + x * c[4]))));
-}
-
-// ---------------------------------------------------------------------------
 // k_remap
 // ---------------------------------------------------------------------------
 
@@ -703,6 +530,11 @@ __global__ __launch_bounds__(WG) void k_remap(RemapArgs a, double i_dv) {
          else w2[0] = w2[1] = w2[2] = w2[3] = 0.0f;
          int next_open = fk, next_flush = fk;
          const int lo_gk = WID * fk, hi_gk = WID * lk + WID - 1;
+         Recon<METHOD> rc;
+         {
+            const float v0[12] = {w0[0], w0[1], w0[2], w0[3], w1[0], w1[1], w1[2], w1[3], w2[0], w2[1], w2[2], w2[3]};
+            rc.start(v0 + 4, scale);
+         }
          for (int b = 0; b < len; ++b) {
             float w3[4];
             if (b + 2 < len) load_pencil<DIM>(a.f_in, srcs[b + 2], o, w3);
@@ -711,7 +543,7 @@ __global__ __launch_bounds__(WG) void k_remap(RemapArgs a, double i_dv) {
 #pragma unroll
             for (int kk = 0; kk < WID; ++kk) {
                float cf[5];
-               reconstruct<METHOD>(v + 4 + kk, thr, scale, cf);
+               rc.coeffs(v + 4 + kk, thr, scale, cf);
                const float v_l = v_r;
                v_r = v_r + a.dv;
                const int gkl = gkr;
