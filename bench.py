@@ -37,10 +37,13 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+FAST = False
+
+
 def make_context(w: W.Workload):
     mesh = V.make_mesh(w.grid_length, w.vmin, w.vmax)
     sp = V.make_species(w.mass, w.charge, w.min_value, w.add_width)
-    op = V.make_options(w.method)
+    op = V.make_options(w.method, fast_arith=FAST)
     return V.Context(mesh, sp, op)
 
 
@@ -119,7 +122,10 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-cells", type=int, default=128)
+    ap.add_argument("--fast", action="store_true", help="fused-multiply-add remap arithmetic (fast_arith=1)")
     args = ap.parse_args()
+    global FAST
+    FAST = args.fast
 
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
@@ -183,6 +189,7 @@ def main():
            "vs_baseline": None, "dtype": "f32", "data": "synthetic",
            "config": {"workload": w.name, "cells": ncells, "grid_length": list(w.grid_length),
                       "method": ["PLM", "PPM", "PQM"][w.method], "storage": "f32",
+                      "arith": "fast" if FAST else "exact",
                       "adjust_every_step": w.adjust_every_step, "blocks_at_start": n_init,
                       "rotation_deg_per_step": w.rotation_deg},
            "roofline": roofline, "breakdown_ms_per_step": breakdown}

@@ -85,6 +85,11 @@ typedef struct vacc_options {
    int32_t ohm_gradpe_term;   /* fieldsolver.ohmGradPeTerm */
    double hall_minimum_rhoq;  /* P::hallMinimumRhoq */
    double max_rotation_deg;   /* vlasovsolver.maxSlAccelerationRotation */
+   int32_t fast_arith;        /* 0: remap arithmetic rounds exactly as the reference CPU path
+                                 (bit-identical f); 1: fused multiply-adds and hardware
+                                 reciprocal/sqrt in the reconstruction and integration
+                                 (f within a few ulp; block lists unchanged) */
+   int32_t reserved;
 } vacc_options;
 
 /* Per-cell field input of the transform (cpu_acc_transform.cpp:71-115). */

@@ -225,3 +225,37 @@ def test_init_peaks_matches_oracle_initial_condition():
         # device exp/pow vs host libm: a few ulp on the sampled values
         assert np.allclose(f1, f2s, rtol=1e-6, atol=0), f"cell {i}"
     ctx.close()
+
+
+@pytest.mark.parametrize("cfg,method", [(0, O.PPM), (2, O.PQM), (1, O.PLM)])
+def test_fast_arith_within_tolerance(cfg, method):
+    """fast_arith=1: block lists equal to the oracle's, f within the stated tolerance:
+    per cell L-inf error <= 2e-5 of the cell's max f, L1 error <= 2e-6 of its L1 norm, and
+    density (sum f) within 1e-6 relative."""
+    w = W.config(cfg, ncells=8)
+    cells = cases.build_cells(w)
+    m = cases.oracle_mesh(w)
+    mesh = V.make_mesh(w.grid_length, w.vmin, w.vmax)
+    sp = V.make_species(w.mass, w.charge, w.min_value, w.add_width)
+    ctx = V.Context(mesh, sp, V.make_options(method, fast_arith=True))
+    ctx.upload_cells(cells)
+    worst = [0.0, 0.0, 0.0]
+    ref = cells
+    for step in range(3):
+        isx = cases.oracle_intersections(w, step % 3)
+        _say(f"fast arith cfg {cfg} step {step}")
+        ctx.set_intersections(isx)
+        ctx.accelerate(step % 3)
+        got = ctx.download_cells()
+        ref = [O.accelerate_cell(m, g, f, isx[i], step % 3, method, w.min_value)[:2] for i, (g, f) in enumerate(ref)]
+        for (g1, f1), (g2, f2) in zip(got, ref):
+            g2, f2 = cases.sort_cell(g2, f2)
+            assert np.array_equal(g1, g2)
+            d = np.abs(f1.astype(np.float64) - f2)
+            worst[0] = max(worst[0], d.max() / np.abs(f2).max())
+            worst[1] = max(worst[1], d.sum() / np.abs(f2).sum())
+            worst[2] = max(worst[2], abs(f1.astype(np.float64).sum() / f2.astype(np.float64).sum() - 1))
+        ctx.upload_cells(ref)   # next step from identical inputs
+    print("fast arith worst (Linf, L1, density):", worst, flush=True)
+    assert worst[0] <= 2e-5 and worst[1] <= 2e-6 and worst[2] <= 1e-6
+    ctx.close()

@@ -282,7 +282,7 @@ static int do_pass(vacc_ctx* c, int dim) {
    ra.min_value = c->sp.sparse_min_value;
    {
       ProfScope ps(c, 0);
-      launch_remap(dim, c->opt.method, ra, 0xFFFFFFFFu, c->stream);
+      launch_remap(dim, c->opt.method, c->opt.fast_arith != 0, ra, 0xFFFFFFFFu, c->stream);
    }
    HIPCHK(hipGetLastError());
    if (c->prof && c->nlog < PROF_LOG) {

@@ -126,7 +126,7 @@ void launch_content(const uint32_t* src, const float* f, uint8_t* content, uint3
 
 void launch_columns(int dim, const ColumnArgs& a, hipStream_t s);
 void launch_scan(const ScanArgs& a, hipStream_t s);
-void launch_remap(int dim, int method, const RemapArgs& a, uint32_t max_sets, hipStream_t s);
+void launch_remap(int dim, int method, bool fast, const RemapArgs& a, uint32_t max_sets, hipStream_t s);
 void launch_adjust(const AdjustArgs& a, bool write_phase, hipStream_t s);
 void launch_moments(const MomentArgs& a, hipStream_t s);
 void launch_transform(const MeshInfo& m, const TransformIn* fields, const double* moments, int use_moments,

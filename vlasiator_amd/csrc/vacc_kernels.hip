@@ -24,6 +24,7 @@
 #include "vacc_recon.h"
 
 #include <float.h>
+#include <type_traits>
 #include <limits.h>
 
 namespace vacc {
@@ -455,7 +456,7 @@ __device__ inline void set_content(uint8_t* content, size_t slot, const float (&
    if (l16 == 0) content[slot] = (uint8_t)(((bal >> setlane0) & 0xFFFFull) != 0ull);
 }
 
-template <int DIM, int METHOD>
+template <int DIM, int METHOD, bool FAST>
 __global__ __launch_bounds__(WG) void k_remap(RemapArgs a, double i_dv) {
    __shared__ float ring[RING][WG];
    if (*a.flags & VACC_STATUS_CAPACITY) return;
@@ -466,6 +467,7 @@ __global__ __launch_bounds__(WG) void k_remap(RemapArgs a, double i_dv) {
    const int o = (DIM == 0) ? (ip * WID2 + jp * WID) : (DIM == 1) ? (jp * WID2 + ip) : (jp * WID + ip);
    const float thr = a.threshold;
    const float scale = (float)(1.0 / (double)thr);
+   const float i_dv_f = (float)i_dv;
    const double mv = a.min_value;
    const uint32_t total = a.set_base[a.ncells];
    const uint32_t gstride = gridDim.x * (WG / 16);
@@ -530,7 +532,7 @@ __global__ __launch_bounds__(WG) void k_remap(RemapArgs a, double i_dv) {
          else w2[0] = w2[1] = w2[2] = w2[3] = 0.0f;
          int next_open = fk, next_flush = fk;
          const int lo_gk = WID * fk, hi_gk = WID * lk + WID - 1;
-         Recon<METHOD> rc;
+         typename std::conditional<FAST, ReconFast<METHOD>, Recon<METHOD>>::type rc;
          {
             const float v0[12] = {w0[0], w0[1], w0[2], w0[3], w1[0], w1[1], w1[2], w1[3], w2[0], w2[1], w2[2], w2[3]};
             rc.start(v0 + 4, scale);
@@ -582,9 +584,13 @@ __global__ __launch_bounds__(WG) void k_remap(RemapArgs a, double i_dv) {
                   const float y = (float)(gk + 1) * dk + Imin;
                   const float mx = (y > v_l) ? y : v_l;
                   const float mn = (mx < v_r) ? mx : v_r;
-                  const float vn = (float)((double)(mn - v_l) * i_dv);
                   const float tdl = tdr;
-                  tdr = integral<METHOD>(vn, cf);
+                  if (FAST) {
+                     tdr = integral_fast<METHOD>((mn - v_l) * i_dv_f, cf);
+                  } else {
+                     const float vn = (float)((double)(mn - v_l) * i_dv);
+                     tdr = integral<METHOD>(vn, cf);
+                  }
                   float& r = ring[gk & (RING - 1)][tid];
                   r = r + (tdr - tdl);
                }
@@ -801,6 +807,8 @@ __global__ __launch_bounds__(WG) void k_content(const uint32_t* src, const float
 // ---------------------------------------------------------------------------
 
 __global__ __launch_bounds__(WG) void k_moments(MomentArgs a) {
+   // one workgroup per cell, one wave per block and one lane per velocity cell; the block
+   // coordinates are decoded once per wave in scalar registers
    __shared__ double red[4][WG / 64];
    const int t = threadIdx.x;
    const uint32_t c = blockIdx.x;
@@ -809,15 +817,19 @@ __global__ __launch_bounds__(WG) void k_moments(MomentArgs a) {
    const double DV3 = a.cell[0] * a.cell[1] * a.cell[2];
    const int e = t & 63;
    const int i = e & 3, j = (e >> 2) & 3, k = e >> 4;
+   const double ci = (i + 0.5) * a.cell[0], cj = (j + 0.5) * a.cell[1], ck = (k + 0.5) * a.cell[2];
+   const double bx = a.cell[0] * WID, by = a.cell[1] * WID, bz = a.cell[2] * WID;
+   const uint32_t nx = a.len[0], nxy = a.len[0] * a.len[1];
    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-   for (uint32_t b = t >> 6; b < n; b += WG / 64) {
-      const uint32_t s = a.src[off + b];
+   const uint32_t wv = __builtin_amdgcn_readfirstlane(t >> 6);
+   for (uint32_t b = wv; b < n; b += WG / 64) {
+      const uint32_t s = __builtin_amdgcn_readfirstlane(a.src[off + b]);
       if (s == INVALID) continue;
-      const uint32_t g = a.gid[off + b];
-      const uint32_t x = g % a.len[0], y = (g / a.len[0]) % a.len[1], z = g / (a.len[0] * a.len[1]);
-      const double VX = (a.vmin[0] + x * (a.cell[0] * WID)) + (i + 0.5) * a.cell[0];
-      const double VY = (a.vmin[1] + y * (a.cell[1] * WID)) + (j + 0.5) * a.cell[1];
-      const double VZ = (a.vmin[2] + z * (a.cell[2] * WID)) + (k + 0.5) * a.cell[2];
+      const uint32_t g = __builtin_amdgcn_readfirstlane(a.gid[off + b]);
+      const uint32_t z = g / nxy, r = g - z * nxy, y = r / nx, x = r - y * nx;
+      const double VX = (a.vmin[0] + x * bx) + ci;
+      const double VY = (a.vmin[1] + y * by) + cj;
+      const double VZ = (a.vmin[2] + z * bz) + ck;
       const double v = (double)a.f[(size_t)s * WID3 + e];
       s0 += v * DV3;
       s1 += v * VX * DV3;
@@ -909,26 +921,27 @@ int remap_grid_size() {
 }
 
 template <int DIM, int METHOD>
-static void remap_dm(const RemapArgs& a, uint32_t grid, double i_dv, hipStream_t s) {
-   hipLaunchKernelGGL((k_remap<DIM, METHOD>), dim3(grid), dim3(WG), 0, s, a, i_dv);
+static void remap_dm(const RemapArgs& a, uint32_t grid, double i_dv, bool fast, hipStream_t s) {
+   if (fast) hipLaunchKernelGGL((k_remap<DIM, METHOD, true>), dim3(grid), dim3(WG), 0, s, a, i_dv);
+   else hipLaunchKernelGGL((k_remap<DIM, METHOD, false>), dim3(grid), dim3(WG), 0, s, a, i_dv);
 }
 
 template <int DIM>
-static void remap_d(int method, const RemapArgs& a, uint32_t grid, double i_dv, hipStream_t s) {
-   if (method == VACC_PLM) remap_dm<DIM, VACC_PLM>(a, grid, i_dv, s);
-   else if (method == VACC_PPM) remap_dm<DIM, VACC_PPM>(a, grid, i_dv, s);
-   else remap_dm<DIM, VACC_PQM>(a, grid, i_dv, s);
+static void remap_d(int method, const RemapArgs& a, uint32_t grid, double i_dv, bool fast, hipStream_t s) {
+   if (method == VACC_PLM) remap_dm<DIM, VACC_PLM>(a, grid, i_dv, fast, s);
+   else if (method == VACC_PPM) remap_dm<DIM, VACC_PPM>(a, grid, i_dv, fast, s);
+   else remap_dm<DIM, VACC_PQM>(a, grid, i_dv, fast, s);
 }
 
-void launch_remap(int dim, int method, const RemapArgs& a, uint32_t max_sets, hipStream_t s) {
+void launch_remap(int dim, int method, bool fast, const RemapArgs& a, uint32_t max_sets, hipStream_t s) {
    if (a.ncells == 0 || max_sets == 0) return;
    uint32_t grid = (uint32_t)remap_grid_size();
    const uint64_t need = ((uint64_t)max_sets + (WG / 16) - 1) / (WG / 16);
    if (need < grid) grid = (uint32_t)need;
    const double i_dv = 1.0 / (double)a.dv;   // cpu_acc_map.cpp:201, Real i_dv = 1.0/dv
-   if (dim == 0) remap_d<0>(method, a, grid, i_dv, s);
-   else if (dim == 1) remap_d<1>(method, a, grid, i_dv, s);
-   else remap_d<2>(method, a, grid, i_dv, s);
+   if (dim == 0) remap_d<0>(method, a, grid, i_dv, fast, s);
+   else if (dim == 1) remap_d<1>(method, a, grid, i_dv, fast, s);
+   else remap_d<2>(method, a, grid, i_dv, fast, s);
 }
 
 void launch_adjust(const AdjustArgs& a, bool write_phase, hipStream_t s) {
