@@ -259,3 +259,54 @@ def test_fast_arith_within_tolerance(cfg, method):
     print("fast arith worst (Linf, L1, density):", worst, flush=True)
     assert worst[0] <= 2e-5 and worst[1] <= 2e-6 and worst[2] <= 1e-6
     ctx.close()
+
+
+@pytest.mark.parametrize("per_cell", [0, 1])
+def test_cpp_host_adapter_driver(tmp_path, per_cell):
+    """The C++ host adapter (vlasiator_amd/host/vacc_host.hpp) through its example driver:
+    three steps of intersections + remaps (batched, or one cell per launch) + adjust, checked
+    bit for bit against the oracle run with the intersections the driver computed."""
+    import os
+    import struct
+    import subprocess
+    drv = os.path.join(os.path.dirname(V.LIB_PATH), "vacc_acc_driver")
+    if not os.path.exists(drv):
+        pytest.fail("vacc_acc_driver not built")
+    w = W.config(0, ncells=4)
+    cells = cases.build_cells(w)
+    steps = 3
+    inp = tmp_path / "in.bin"
+    with open(inp, "wb") as fp:
+        fp.write(b"VACC" + struct.pack("<IIiiii", 1, len(cells), O.PPM, steps, per_cell, 1))
+        fp.write(struct.pack("<3I", *w.grid_length) + struct.pack("<3d", *w.vmin) + struct.pack("<3d", *w.vmax))
+        fp.write(struct.pack("<3di", w.mass, w.charge, w.min_value, w.add_width))
+        for c, (g, f) in zip(w.cells, cells):
+            fp.write(struct.pack("<7d", *c.B, *c.V, c.dt) + struct.pack("<I", len(g)))
+            fp.write(np.asarray(g, np.uint32).tobytes() + np.asarray(f, np.float32).tobytes())
+    out = tmp_path / "out.bin"
+    _say(f"driver per_cell={per_cell}")
+    subprocess.run([drv, str(inp), str(out)], check=True, timeout=120)
+    buf = open(out, "rb").read()
+    pos = 0
+    isx = []
+    for s in range(steps):
+        isx.append(np.frombuffer(buf, np.float64, 12 * len(cells), pos).reshape(len(cells), 12))
+        pos += 8 * 12 * len(cells)
+    got = []
+    for _ in cells:
+        (n,) = struct.unpack_from("<I", buf, pos)
+        pos += 4
+        g = np.frombuffer(buf, np.uint32, n, pos)
+        pos += 4 * n
+        f = np.frombuffer(buf, np.float32, 64 * n, pos).reshape(n, 64)
+        pos += 256 * n + 8
+        got.append((g, f))
+    m = cases.oracle_mesh(w)
+    ref = cells
+    for s in range(steps):
+        ref = [O.adjust(m, *O.accelerate_cell(m, g, f, isx[s][i], s % 3, O.PPM, w.min_value)[:2], w.min_value,
+                        w.add_width)[:2] for i, (g, f) in enumerate(ref)]
+    cases.assert_cells_equal(got, ref, "C++ driver")
+    # the host intersections agree with the oracle transform
+    assert np.allclose(isx[0], cases.oracle_intersections(w, 0), rtol=0,
+                       atol=1e-12 * 4e6)

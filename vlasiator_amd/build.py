@@ -39,6 +39,15 @@ def build(verbose: bool = False) -> str:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    # example C++ host program over the adapter (vlasiator_amd/host)
+    drv = os.path.join(LIB_DIR, "vacc_acc_driver")
+    cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall", "-I", os.path.join(HERE, "..", "include"),
+           os.path.join(HERE, "host", "acc_driver.cpp"), "-o", drv + ".tmp", "-L", LIB_DIR, "-lvacc",
+           "-Wl,-rpath,$ORIGIN"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(drv + ".tmp", drv)
     return LIB
 
 

@@ -1,0 +1,166 @@
+// vacc_host.hpp -- C++ host adapter over the C ABI of libvacc (include/vacc.h).
+//
+// Mirrors the part of the reference's host interface that the acceleration path uses, so
+// that a caller written against it can switch to the GPU library:
+//   Population          <- spatial_cells/spatial_cell_cpu.hpp:66-92 (velocity mesh GIDs, block
+//                          data, intersection_*, subcycleDt, velocityBlockMinValue, RHOLOSSADJUST)
+//   SpatialCell         <- the fields the transform reads (BGB*VOL + PERB*VOL, dPERB*VOL/d*,
+//                          V_V, RHOQ_V, E*GRADPE; cpu_acc_transform.cpp:71-115)
+//   compute_cell_intersections(SpatialCell*, popID, map_order, dt)
+//                       <- vlasovsolver/cpu_acc_intersections.cpp:44
+//   accelerate_cell(SpatialCell*, popID, map_order)
+//                       <- cpu_accelerate_cell, vlasovsolver/cpu_acc_semilag.cpp:92 (one cell
+//                          per launch)
+//   accelerate_cells(cells, popID, map_order)
+//                       <- cpu_accelerate_cells, vlasovsolver/cpu_acc_semilag.cpp:45 (all cells
+//                          in one batch: every kernel covers every cell)
+//   adjust_blocks(cells, popID)
+//                       <- SpatialCell::adjustSingleCellVelocityBlocks(popID, true),
+//                          spatial_cells/spatial_cell_cpu.cpp:275
+// Errors are reported as in the C ABI: the functions return the library's status code
+// (0 ok, > 0 per-cell status bits, < 0 error) and never abort.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/vacc.h"
+
+namespace vacc_host {
+
+struct Population {
+   std::vector<uint32_t> gids;          // block GIDs in local (LID) order
+   std::vector<float> f;                // 64 values per block, in-block index i + 4j + 16k
+   double intersection[12] = {0};       // x {I,di,dj,dk}, y {...}, z {...}
+   double subcycleDt = 0.0;
+   double RHOLOSSADJUST = 0.0;
+   uint32_t status = 0;                 // VACC_STATUS_* bits of the last call
+};
+
+struct SpatialCell {
+   double B[3] = {0, 0, 0};             // BGB*VOL + PERB*VOL
+   double dPERB[6] = {0, 0, 0, 0, 0, 0};// dPERBXVOLdy dPERBXVOLdz dPERBYVOLdx dPERBYVOLdz dPERBZVOLdx dPERBZVOLdy
+   double V[3] = {0, 0, 0};             // V_V
+   double RHOQ = 0.0;                   // RHOQ_V
+   double EGRADPE[3] = {0, 0, 0};
+   std::vector<Population> populations;
+};
+
+class AccelerationEngine {
+public:
+   // one velocity mesh and species per population, shared options
+   AccelerationEngine(const std::vector<vacc_mesh>& meshes, const std::vector<vacc_species>& species,
+                      const vacc_options& opts)
+       : meshes_(meshes), species_(species), opts_(opts), ctx_(meshes.size(), nullptr) {}
+   ~AccelerationEngine() {
+      for (auto c : ctx_)
+         if (c) vacc_destroy(c);
+   }
+   AccelerationEngine(const AccelerationEngine&) = delete;
+   AccelerationEngine& operator=(const AccelerationEngine&) = delete;
+
+   static std::string last_error() { return vacc_last_error(); }
+
+   // FP64 transform + SLICE-3D intersections of one cell (host side, like the reference).
+   int compute_cell_intersections(SpatialCell* sc, uint32_t popID, uint32_t map_order, double dt) {
+      vacc_cell_fields fl = fields(*sc, dt);
+      Population& pop = sc->populations.at(popID);
+      return vacc_cell_intersections_host(&meshes_.at(popID), &species_.at(popID), &opts_, &fl, map_order,
+                                          pop.intersection);
+   }
+
+   // Three 1-D remaps of one cell (one cell per launch).
+   int accelerate_cell(SpatialCell* sc, uint32_t popID, uint32_t map_order) {
+      std::vector<SpatialCell*> one{sc};
+      return run(one, popID, map_order, false);
+   }
+
+   // Three 1-D remaps of every cell in one batch.
+   int accelerate_cells(std::vector<SpatialCell*>& cells, uint32_t popID, uint32_t map_order) {
+      return run(cells, popID, map_order, false);
+   }
+
+   // Single-cell sparsity adjust of every cell (doDeleteEmptyBlocks = true).
+   int adjust_blocks(std::vector<SpatialCell*>& cells, uint32_t popID) {
+      return run(cells, popID, 0, true);
+   }
+
+private:
+   static vacc_cell_fields fields(const SpatialCell& sc, double dt) {
+      vacc_cell_fields fl{};
+      for (int i = 0; i < 3; ++i) {
+         fl.B[i] = sc.B[i];
+         fl.V[i] = sc.V[i];
+         fl.EgradPe[i] = sc.EGRADPE[i];
+      }
+      for (int i = 0; i < 6; ++i) fl.dB[i] = sc.dPERB[i];
+      fl.rhoq = sc.RHOQ;
+      fl.dt = dt;
+      return fl;
+   }
+
+   int context(uint32_t popID, vacc_ctx** out) {
+      if (popID >= ctx_.size()) return VACC_E_ARG;
+      if (!ctx_[popID]) {
+         const int rc = vacc_create(&meshes_[popID], &species_[popID], &opts_, &ctx_[popID]);
+         if (rc) return rc;
+      }
+      *out = ctx_[popID];
+      return 0;
+   }
+
+   // upload the cells' populations, remap or adjust, and write the new state back
+   int run(std::vector<SpatialCell*>& cells, uint32_t popID, uint32_t map_order, bool adjust) {
+      vacc_ctx* c = nullptr;
+      int rc = context(popID, &c);
+      if (rc) return rc;
+      const size_t n = cells.size();
+      std::vector<uint64_t> off(n + 1, 0);
+      for (size_t i = 0; i < n; ++i) off[i + 1] = off[i] + cells[i]->populations.at(popID).gids.size();
+      std::vector<uint32_t> gids(off[n]);
+      std::vector<float> f(off[n] * VACC_WID3);
+      std::vector<double> isect(12 * n);
+      for (size_t i = 0; i < n; ++i) {
+         const Population& p = cells[i]->populations[popID];
+         std::copy(p.gids.begin(), p.gids.end(), gids.begin() + off[i]);
+         std::copy(p.f.begin(), p.f.end(), f.begin() + off[i] * VACC_WID3);
+         std::copy(p.intersection, p.intersection + 12, isect.begin() + 12 * i);
+      }
+      if ((rc = vacc_upload_cells(c, (uint32_t)n, off.data(), gids.data(), f.data())) < 0) return rc;
+      int st = 0;
+      std::vector<double> loss(n, 0.0);
+      if (adjust) {
+         st = vacc_adjust_blocks(c, loss.data());
+      } else {
+         if ((rc = vacc_set_intersections(c, isect.data())) < 0) return rc;
+         st = vacc_accelerate_cells(c, map_order);
+      }
+      if (st < 0) return st;
+      std::vector<uint32_t> status(n, 0);
+      if ((rc = vacc_get_status(c, status.data())) < 0) return rc;
+      std::vector<uint32_t> counts(n);
+      uint32_t nc = 0;
+      if ((rc = vacc_get_counts(c, &nc, counts.data())) < 0) return rc;
+      uint64_t tot = 0;
+      for (uint32_t v : counts) tot += v;
+      gids.resize(tot);
+      f.resize(tot * VACC_WID3);
+      if ((rc = vacc_download_cells(c, off.data(), gids.data(), f.data())) < 0) return rc;
+      for (size_t i = 0; i < n; ++i) {
+         Population& p = cells[i]->populations[popID];
+         p.gids.assign(gids.begin() + off[i], gids.begin() + off[i + 1]);
+         p.f.assign(f.begin() + off[i] * VACC_WID3, f.begin() + off[i + 1] * VACC_WID3);
+         p.status = status[i];
+         if (adjust) p.RHOLOSSADJUST += loss[i];
+      }
+      return st;
+   }
+
+   std::vector<vacc_mesh> meshes_;
+   std::vector<vacc_species> species_;
+   vacc_options opts_;
+   std::vector<vacc_ctx*> ctx_;
+};
+
+}  // namespace vacc_host
