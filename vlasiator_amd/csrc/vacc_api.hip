@@ -54,11 +54,13 @@ struct vacc_ctx {
    uint8_t* content = nullptr;
    // pass scratch
    uint32_t* sorted_src = nullptr;
-   uint4* set_rec = nullptr;
+   uint4* unit_rec = nullptr;
    uint4* col_rec = nullptr;
-   uint32_t* n_sets = nullptr;
+   uint32_t* perm = nullptr;
+   uint32_t* n_units = nullptr;
    uint32_t* n_tgt = nullptr;
-   uint32_t* set_base = nullptr;
+   uint32_t* unit_base = nullptr;
+   uint32_t* len_hist = nullptr;     // [2*LEN_BINS]: histogram, cursors
    uint32_t* status = nullptr;
    uint32_t* flags = nullptr;
    double* isect = nullptr;
@@ -120,11 +122,13 @@ static void free_blocks(vacc_ctx* c) {
    }
    if (c->content) (void)hipFree(c->content);
    if (c->sorted_src) (void)hipFree(c->sorted_src);
-   if (c->set_rec) (void)hipFree(c->set_rec);
+   if (c->unit_rec) (void)hipFree(c->unit_rec);
    if (c->col_rec) (void)hipFree(c->col_rec);
+   if (c->perm) (void)hipFree(c->perm);
    c->content = nullptr;
    c->sorted_src = nullptr;
-   c->set_rec = c->col_rec = nullptr;
+   c->unit_rec = c->col_rec = nullptr;
+   c->perm = nullptr;
    c->cap = 0;
 }
 
@@ -133,7 +137,7 @@ static void free_cells(vacc_ctx* c) {
       if (c->off[i]) (void)hipFree(c->off[i]);
       c->off[i] = nullptr;
    }
-   uint32_t** u32s[] = {&c->n_sets, &c->n_tgt, &c->set_base, &c->status};
+   uint32_t** u32s[] = {&c->n_units, &c->n_tgt, &c->unit_base, &c->status, &c->len_hist};
    for (auto p : u32s) {
       if (*p) (void)hipFree(*p);
       *p = nullptr;
@@ -154,9 +158,10 @@ static int alloc_cells(vacc_ctx* c, uint32_t n) {
    free_cells(c);
    const size_t m = (size_t)n + 1;
    for (int i = 0; i < 2; ++i) HIPCHK(hipMalloc(&c->off[i], m * sizeof(uint32_t)));
-   HIPCHK(hipMalloc(&c->n_sets, m * sizeof(uint32_t)));
+   HIPCHK(hipMalloc(&c->n_units, m * sizeof(uint32_t)));
    HIPCHK(hipMalloc(&c->n_tgt, m * sizeof(uint32_t)));
-   HIPCHK(hipMalloc(&c->set_base, m * sizeof(uint32_t)));
+   HIPCHK(hipMalloc(&c->unit_base, m * sizeof(uint32_t)));
+   HIPCHK(hipMalloc(&c->len_hist, 2 * LEN_BINS * sizeof(uint32_t)));
    HIPCHK(hipMalloc(&c->status, m * sizeof(uint32_t)));
    HIPCHK(hipMemset(c->status, 0, m * sizeof(uint32_t)));
    HIPCHK(hipMalloc(&c->isect, m * 12 * sizeof(double)));
@@ -176,7 +181,7 @@ static int ensure_blocks(vacc_ctx* c, size_t want) {
    uint32_t *g[2], *s[2];
    float* f[2];
    uint8_t* ct;
-   uint32_t* ss;
+   uint32_t *ss, *pm;
    uint4 *sr, *cr;
    for (int i = 0; i < 2; ++i) {
       HIPCHK(hipMalloc(&g[i], ncap * sizeof(uint32_t)));
@@ -187,6 +192,7 @@ static int ensure_blocks(vacc_ctx* c, size_t want) {
    HIPCHK(hipMalloc(&ss, ncap * sizeof(uint32_t)));
    HIPCHK(hipMalloc(&sr, ncap * sizeof(uint4)));
    HIPCHK(hipMalloc(&cr, ncap * sizeof(uint4)));
+   HIPCHK(hipMalloc(&pm, ncap * sizeof(uint32_t)));
    if (c->cap && c->nblocks) {
       // carry the current state: list entries and the whole current pool extent
       const size_t n = c->nblocks;
@@ -204,8 +210,9 @@ static int ensure_blocks(vacc_ctx* c, size_t want) {
    }
    c->content = ct;
    c->sorted_src = ss;
-   c->set_rec = sr;
+   c->unit_rec = sr;
    c->col_rec = cr;
+   c->perm = pm;
    c->cap = ncap;
    return 0;
 }
@@ -236,10 +243,11 @@ static int do_pass(vacc_ctx* c, int dim) {
    ca.src = c->src[li];
    ca.isect = c->isect;
    ca.sorted_src = c->sorted_src;
-   ca.set_rec = c->set_rec;
+   ca.unit_rec = c->unit_rec;
    ca.col_rec = c->col_rec;
-   ca.n_sets = c->n_sets;
+   ca.n_units = c->n_units;
    ca.n_targets = c->n_tgt;
+   ca.len_hist = c->len_hist;
    ca.status = c->status;
    ca.ncells = c->ncells;
    for (int d = 0; d < 3; ++d) ca.len[d] = c->mi.len[d];
@@ -248,22 +256,35 @@ static int do_pass(vacc_ctx* c, int dim) {
    ca.wall_margin = c->opt.wall_margin;
    {
       ProfScope ps(c, 1);
+      launch_fill_u32(c->len_hist, 0u, LEN_BINS, c->stream);
       launch_columns(dim, ca, c->stream);
       ScanArgs sa{};
       sa.a = c->n_tgt;
-      sa.b = c->n_sets;
+      sa.b = c->n_units;
       sa.a_off = c->off[lo];
-      sa.b_off = c->set_base;
+      sa.b_off = c->unit_base;
       sa.n = c->ncells;
       sa.cap_a = (uint32_t)std::min<size_t>(c->cap, 0xFFFFFFFFull);
       sa.flags = c->flags;
+      sa.hist = c->len_hist;
+      sa.hist_cursor = c->len_hist + LEN_BINS;
       launch_scan(sa, c->stream);
+      OrderArgs oa{};
+      oa.cell_off = c->off[li];
+      oa.unit_base = c->unit_base;
+      oa.unit_rec = c->unit_rec;
+      oa.hist_cursor = c->len_hist + LEN_BINS;
+      oa.perm = c->perm;
+      oa.flags = c->flags;
+      oa.ncells = c->ncells;
+      launch_order(oa, c->stream);
    }
    RemapArgs ra{};
    ra.cell_off = c->off[li];
-   ra.set_base = c->set_base;
+   ra.unit_base = c->unit_base;
    ra.out_off = c->off[lo];
-   ra.set_rec = c->set_rec;
+   ra.perm = c->perm;
+   ra.unit_rec = c->unit_rec;
    ra.col_rec = c->col_rec;
    ra.sorted_src = c->sorted_src;
    ra.isect = c->isect;

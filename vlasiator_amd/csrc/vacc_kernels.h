@@ -5,17 +5,19 @@
 
 namespace vacc {
 
-// Column-set record (one per transverse position with blocks), 16 bytes.
+// Work unit of the remap (one column, or a whole column set whose columns' target ranges
+// share a block), 16 bytes.
 //   x: b0 | b1 << 16   transverse block indices in the pass frame
 //   y: first column (cell-local)
 //   z: number of columns | COPY_FLAG
-//   w: unused
+//   w: source blocks of the unit (its work)
 // Column record, 16 bytes.
 //   x: first block index along the pass axis | length << 16
 //   y: position of the first block in the sorted source list (cell-local)
 //   z: first target block | last target block << 16
 //   w: output slot of the first target block (cell-local)
 constexpr uint32_t COPY_FLAG = 0x80000000u;
+constexpr int LEN_BINS = 256;   // length bins of the work-unit ordering
 
 struct ColumnArgs {
    const uint32_t* cell_off;   // [ncells+1] input list offsets
@@ -23,9 +25,10 @@ struct ColumnArgs {
    const uint32_t* src;        // input list -> f pool index (INVALID = zero block)
    const double* isect;        // [ncells*12]
    uint32_t* sorted_src;       // [N] at cell_off
-   uint4* set_rec;             // [N] at cell_off
+   uint4* unit_rec;            // [N] at cell_off
    uint4* col_rec;             // [N] at cell_off
-   uint32_t* n_sets;           // [ncells]
+   uint32_t* n_units;          // [ncells]
+   uint32_t* len_hist;         // [LEN_BINS] work units per length bin (zeroed before the launch)
    uint32_t* n_targets;        // [ncells]
    uint32_t* status;           // [ncells] OR-ed status bits
    uint32_t ncells;
@@ -43,13 +46,26 @@ struct ScanArgs {
    uint32_t n;
    uint32_t cap_a;         // capacity check on the a total (0 = none)
    uint32_t* flags;        // device flags word: bit VACC_STATUS_CAPACITY set on overflow
+   const uint32_t* hist;   // [LEN_BINS] optional histogram to scan into hist_cursor
+   uint32_t* hist_cursor;
+};
+
+struct OrderArgs {
+   const uint32_t* cell_off;
+   const uint32_t* unit_base;   // [ncells+1]
+   const uint4* unit_rec;
+   uint32_t* hist_cursor;       // [LEN_BINS] insertion cursors
+   uint32_t* perm;              // [total units] global unit indices, longest first
+   const uint32_t* flags;
+   uint32_t ncells;
 };
 
 struct RemapArgs {
    const uint32_t* cell_off;    // input list offsets (record base of each cell)
-   const uint32_t* set_base;    // [ncells+1] global set index base
+   const uint32_t* unit_base;   // [ncells+1] global work-unit index base
    const uint32_t* out_off;     // [ncells+1] output block offsets
-   const uint4* set_rec;
+   const uint32_t* perm;        // [total units] processing order
+   const uint4* unit_rec;
    const uint4* col_rec;
    const uint32_t* sorted_src;
    const double* isect;
@@ -126,6 +142,7 @@ void launch_content(const uint32_t* src, const float* f, uint8_t* content, uint3
 
 void launch_columns(int dim, const ColumnArgs& a, hipStream_t s);
 void launch_scan(const ScanArgs& a, hipStream_t s);
+void launch_order(const OrderArgs& a, hipStream_t s);
 void launch_remap(int dim, int method, bool fast, const RemapArgs& a, uint32_t max_sets, hipStream_t s);
 void launch_adjust(const AdjustArgs& a, bool write_phase, hipStream_t s);
 void launch_moments(const MomentArgs& a, hipStream_t s);

@@ -67,6 +67,11 @@ __device__ inline void pass_isect(const double* is, float& I, float& di, float& 
    else { di = a; dj = b; dk = c; }
 }
 
+// work units are ordered longest first: bin 0 holds units of >= LEN_BINS-1 source blocks
+__device__ inline uint32_t len_bin(uint32_t nblocks) {
+   return (uint32_t)(LEN_BINS - 1) - min(nblocks, (uint32_t)(LEN_BINS - 1));
+}
+
 __device__ inline bool pass_is_copy(float dv, float di, float dj, float dk) {
    // cells whose Lagrangian spacing is not positive, or whose per-cell deposit span would not
    // fit the target ring, are passed through unchanged and flagged VACC_STATUS_DK
@@ -194,17 +199,19 @@ __global__ __launch_bounds__(WG) void k_columns(ColumnArgs a) {
    __shared__ uint32_t sh[4 * 4];
    __shared__ uint32_t carry[4];
    __shared__ uint32_t sstat;
+   __shared__ uint32_t hist[LEN_BINS];
    const int t = threadIdx.x;
    const uint32_t c = blockIdx.x;
    const uint32_t off = a.cell_off[c];
    const uint32_t n = a.cell_off[c + 1] - off;
    if (n == 0) {
       if (t == 0) {
-         a.n_sets[c] = 0;
+         a.n_units[c] = 0;
          a.n_targets[c] = 0;
       }
       return;
    }
+   for (int i = t; i < LEN_BINS; i += WG) hist[i] = 0;
    if (t < 3) {
       sbox[t] = INT_MAX;
       sbox[t + 3] = INT_MIN;
@@ -252,12 +259,15 @@ __global__ __launch_bounds__(WG) void k_columns(ColumnArgs a) {
          }
       }
       __syncthreads();
-      // per-row counts: blocks, columns, set (0/1), target blocks
+      // per-row counts: blocks, columns, work units, target blocks.  The columns of a set are
+      // separate work units unless the target ranges of two of them share a block, in which
+      // case the set is one unit (its columns must add into the shared block in order).
       for (int rr = t; rr < nr; rr += WG) {
          const uint32_t* w = bm + rr * W;
          uint32_t nb = 0;
          for (int q = 0; q < W; ++q) nb += __popc(w[q]);
          uint32_t ncol = 0, ntgt = 0;
+         bool overlap = false;
          if (nb) {
             const int grow = r0 + rr;
             const int b0 = mn0 + grow / R1, b1 = mn1 + grow % R1;
@@ -277,6 +287,7 @@ __global__ __launch_bounds__(WG) void k_columns(ColumnArgs a) {
                   column_extent(mnk + s, mnk + e, g, a.dv, a.vmin, dk, maxlen, a.wall_margin, fk, lk, wh);
                   if (wh) mystat |= VACC_STATUS_WALL;
                }
+               if (fk <= end) overlap = true;
                if (lk > end) {
                   ntgt += (uint32_t)(lk - max(fk, end + 1) + 1);
                   end = lk;
@@ -287,7 +298,7 @@ __global__ __launch_bounds__(WG) void k_columns(ColumnArgs a) {
          }
          rowv[0][rr] = nb;
          rowv[1][rr] = ncol;
-         rowv[2][rr] = nb ? 1u : 0u;
+         rowv[2][rr] = overlap ? 1u : ncol;
          rowv[3][rr] = ntgt;
       }
       __syncthreads();
@@ -318,9 +329,10 @@ __global__ __launch_bounds__(WG) void k_columns(ColumnArgs a) {
          const int grow = r0 + rr;
          const int b0 = mn0 + grow / R1, b1 = mn1 + grow % R1;
          const SetGeom g = set_geom(b0, b1, I, di, dj);
-         const uint32_t si = rowv[2][rr], cb = rowv[1][rr], spos0 = rowv[0][rr], slot0 = rowv[3][rr];
+         const uint32_t ui = rowv[2][rr], cb = rowv[1][rr], spos0 = rowv[0][rr], slot0 = rowv[3][rr];
          int end = -1;
          uint32_t cnt = 0, ncol = 0, bpos = 0;
+         bool overlap = false;
          int p = 0;
          for (;;) {
             const int s = next_one(w, W, p);
@@ -336,6 +348,7 @@ __global__ __launch_bounds__(WG) void k_columns(ColumnArgs a) {
             }
             // slot of block fk in the union of target ranges (ranges ascend in start and end)
             const uint32_t first = (fk <= end) ? (cnt - 1u - (uint32_t)(end - fk)) : cnt;
+            if (fk <= end) overlap = true;
             if (lk > end) {
                cnt += (uint32_t)(lk - max(fk, end + 1) + 1);
                end = lk;
@@ -347,7 +360,18 @@ __global__ __launch_bounds__(WG) void k_columns(ColumnArgs a) {
             ++ncol;
             p = e + 1;
          }
-         a.set_rec[off + si] = make_uint4((uint32_t)b0 | ((uint32_t)b1 << 16), cb, ncol | (copy ? COPY_FLAG : 0u), 0u);
+         // work units: {b0 | b1 << 16, first column, columns | COPY_FLAG, source blocks}
+         const uint32_t bb = (uint32_t)b0 | ((uint32_t)b1 << 16), cf = copy ? COPY_FLAG : 0u;
+         if (overlap) {
+            a.unit_rec[off + ui] = make_uint4(bb, cb, ncol | cf, bpos);
+            atomicAdd(&hist[len_bin(bpos)], 1u);
+         } else {
+            for (uint32_t j = 0; j < ncol; ++j) {
+               const uint32_t len = a.col_rec[off + cb + j].x >> 16;
+               a.unit_rec[off + ui + j] = make_uint4(bb, cb + j, 1u | cf, len);
+               atomicAdd(&hist[len_bin(len)], 1u);
+            }
+         }
       }
       // source index of every block in sorted (set, k) order
       for (uint32_t i = t; i < n; i += WG) {
@@ -366,10 +390,12 @@ __global__ __launch_bounds__(WG) void k_columns(ColumnArgs a) {
    if (mystat) atomicOr(&sstat, mystat);
    __syncthreads();
    if (t == 0) {
-      a.n_sets[c] = carry[2];
+      a.n_units[c] = carry[2];
       a.n_targets[c] = carry[3];
       if (sstat) atomicOr(&a.status[c], sstat);
    }
+   for (int i = t; i < LEN_BINS; i += WG)
+      if (hist[i]) atomicAdd(&a.len_hist[i], hist[i]);
 }
 
 // ---------------------------------------------------------------------------
@@ -398,6 +424,41 @@ __global__ __launch_bounds__(1024) void k_scan2(ScanArgs a) {
       a.a_off[a.n] = carry[0];
       if (a.b) a.b_off[a.n] = carry[1];
       if (a.cap_a && carry[0] > a.cap_a && a.flags) atomicOr(a.flags, VACC_STATUS_CAPACITY);
+   }
+   if (a.hist) {
+      // exclusive scan of the work-unit length histogram -> insertion cursors of k_order
+      uint32_t v[2] = {threadIdx.x < LEN_BINS ? a.hist[threadIdx.x] : 0u, 0u};
+      uint32_t ex[2], tot[2];
+      block_scan<2>(v, ex, tot, sh);
+      if (threadIdx.x < LEN_BINS) a.hist_cursor[threadIdx.x] = ex[0];
+   }
+}
+
+// Global work list of the remap: every unit's global index, ordered by source length
+// (longest first; order within a length is arbitrary, units are independent).
+__global__ __launch_bounds__(WG) void k_order(OrderArgs a) {
+   // one workgroup per cell: count the cell's units per length bin in LDS, reserve each bin's
+   // range with one global atomic, then place the units
+   __shared__ uint32_t cnt[LEN_BINS];
+   __shared__ uint32_t base_of[LEN_BINS];
+   if (*a.flags & VACC_STATUS_CAPACITY) return;
+   const uint32_t c = blockIdx.x;
+   const uint32_t base = a.unit_base[c];
+   const uint32_t nu = a.unit_base[c + 1] - base;
+   if (nu == 0) return;
+   const uint32_t off = a.cell_off[c];
+   for (int i = threadIdx.x; i < LEN_BINS; i += WG) cnt[i] = 0;
+   __syncthreads();
+   for (uint32_t u = threadIdx.x; u < nu; u += WG) atomicAdd(&cnt[len_bin(a.unit_rec[off + u].w)], 1u);
+   __syncthreads();
+   for (int i = threadIdx.x; i < LEN_BINS; i += WG) {
+      base_of[i] = cnt[i] ? atomicAdd(&a.hist_cursor[i], cnt[i]) : 0u;
+      cnt[i] = 0;
+   }
+   __syncthreads();
+   for (uint32_t u = threadIdx.x; u < nu; u += WG) {
+      const uint32_t b = len_bin(a.unit_rec[off + u].w);
+      a.perm[base_of[b] + atomicAdd(&cnt[b], 1u)] = base + u;
    }
 }
 
@@ -469,18 +530,19 @@ __global__ __launch_bounds__(WG) void k_remap(RemapArgs a, double i_dv) {
    const float scale = (float)(1.0 / (double)thr);
    const float i_dv_f = (float)i_dv;
    const double mv = a.min_value;
-   const uint32_t total = a.set_base[a.ncells];
+   const uint32_t total = a.unit_base[a.ncells];
    const uint32_t gstride = gridDim.x * (WG / 16);
-   for (uint32_t g = blockIdx.x * (WG / 16) + (tid >> 4); g < total; g += gstride) {
+   for (uint32_t gi = blockIdx.x * (WG / 16) + (tid >> 4); gi < total; gi += gstride) {
+      const uint32_t g = a.perm[gi];
       uint32_t lo = 0, hi = a.ncells;
       while (hi - lo > 1) {
          const uint32_t mid = (lo + hi) >> 1;
-         if (a.set_base[mid] <= g) lo = mid;
+         if (a.unit_base[mid] <= g) lo = mid;
          else hi = mid;
       }
       const uint32_t c = lo;
       const uint32_t cbase = a.cell_off[c];
-      const uint4 sr = a.set_rec[cbase + (g - a.set_base[c])];
+      const uint4 sr = a.unit_rec[cbase + (g - a.unit_base[c])];
       const int b0 = (int)(sr.x & 0xFFFFu), b1 = (int)(sr.x >> 16);
       const uint32_t col0 = sr.y;
       const uint32_t ncol = sr.z & ~COPY_FLAG;
@@ -902,6 +964,11 @@ void launch_columns(int dim, const ColumnArgs& a, hipStream_t s) {
    if (dim == 0) hipLaunchKernelGGL(k_columns<0>, dim3(a.ncells), dim3(WG), 0, s, a);
    else if (dim == 1) hipLaunchKernelGGL(k_columns<1>, dim3(a.ncells), dim3(WG), 0, s, a);
    else hipLaunchKernelGGL(k_columns<2>, dim3(a.ncells), dim3(WG), 0, s, a);
+}
+
+void launch_order(const OrderArgs& a, hipStream_t s) {
+   if (a.ncells == 0) return;
+   hipLaunchKernelGGL(k_order, dim3(a.ncells), dim3(WG), 0, s, a);
 }
 
 void launch_scan(const ScanArgs& a, hipStream_t s) {
