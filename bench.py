@@ -47,12 +47,9 @@ def make_context(w: W.Workload):
     return V.Context(mesh, sp, op)
 
 
-def run_step(ctx, fields, step, ncells, adjust):
-    mo = step % 3
-    ctx.compute_intersections(fields, mo, use_moments=True)
-    ctx.accelerate(mo)
-    if adjust:
-        ctx.adjust(ncells)
+def run_step(ctx, step, adjust):
+    """One subcycle of every cell: moments pivot, transform, three remaps, adjust."""
+    return ctx.step(step % 3, use_moments=True, adjust=adjust)
 
 
 def cpu_baseline(w, sample_cells, isect, target_s, nthreads):
@@ -141,6 +138,7 @@ def main():
     n_init = ctx.total_blocks()
     log(f"initial state: {n_init} blocks ({n_init * 256 / 1e9:.2f} GB f32) in {time.perf_counter() - t0:.1f} s")
     fields = [V.make_fields(c.B, c.V, c.dt) for c in w.cells]
+    ctx.set_fields(fields)
 
     # bounded samples for the CPU baseline and the plain form, taken from the initial state
     sample_n = min(ncells, 2048)
@@ -157,14 +155,14 @@ def main():
         sctx.close()
 
     for s in range(args.warmup):
-        run_step(ctx, fields, s, ncells, w.adjust_every_step)
+        run_step(ctx, s, w.adjust_every_step)
     ctx.sync()
     ctx.profile_begin()
     updates = 0
     t0 = time.perf_counter()
     for s in range(args.warmup, args.warmup + args.steps):
         updates += 64 * ctx.total_blocks()
-        run_step(ctx, fields, s, ncells, w.adjust_every_step)
+        run_step(ctx, s, w.adjust_every_step)
     status = ctx.sync()
     elapsed = time.perf_counter() - t0
     prof = ctx.profile_read()

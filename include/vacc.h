@@ -128,7 +128,8 @@ int vacc_set_intersections(vacc_ctx* ctx, const double* isect);
 int vacc_get_intersections(vacc_ctx* ctx, double* isect);
 /* Transform + intersections for every cell on the device (FP64).  If use_moments != 0 the
  * pivot bulk velocity is the first moment of the current f (calculateMoments_V), otherwise
- * fields[c].V.  fields: ncells entries. */
+ * fields[c].V.  fields: ncells entries, kept resident for later calls; NULL reuses the
+ * fields of the last vacc_set_fields / vacc_compute_cell_intersections call. */
 int vacc_compute_cell_intersections(vacc_ctx* ctx, const vacc_cell_fields* fields, uint32_t map_order,
                                     int32_t use_moments);
 /* Host FP64 transform + intersections of a single cell (no device work). */
@@ -146,8 +147,21 @@ int vacc_accelerate_cells(vacc_ctx* ctx, uint32_t map_order);
 /* Single-cell sparsity adjust of every cell (doDeleteEmptyBlocks = true).  rho_loss may be
  * NULL; otherwise ncells FP64 values of the removed mass (RHOLOSSADJUST increments). */
 int vacc_adjust_blocks(vacc_ctx* ctx, double* rho_loss);
-/* FP64 moments per cell: out[4*c+0] = n, [1..3] = n*V (arch_moments.h:83-108). */
+/* FP64 moments per cell: out[4*c+0] = n, [1..3] = n*V (arch_moments.h:83-108).  They are
+ * kept resident until f changes, so a following use_moments transform does not recompute
+ * them. */
 int vacc_moments(vacc_ctx* ctx, double* out);
+
+/* Make the per-cell transform inputs resident (ncells entries; the dt of each entry is the
+ * cell's subcycle dt).  Replaced by the next call; cleared by an upload. */
+int vacc_set_fields(vacc_ctx* ctx, const vacc_cell_fields* fields);
+/* One acceleration subcycle of every cell with a single host synchronisation:
+ * calculateAcceleration(popID, ..., acceleratedCells, dt), vlasovsolver/vlasovmover.cpp:410 --
+ * moments when use_moments != 0 (calculateMoments_V, :420), transform + intersections from
+ * the resident fields, the three remaps in map_order (cpu_accelerate_cells, :462) and, when
+ * adjust != 0, the sparsity adjust (adjustVelocityBlocks, :473).  rho_loss as in
+ * vacc_adjust_blocks (may be NULL).  Returns the OR of the per-cell status bits. */
+int vacc_step(vacc_ctx* ctx, uint32_t map_order, int32_t use_moments, int32_t adjust, double* rho_loss);
 /* Synthetic initial state, the project helper used by tests and benchmarks: mirrors
  * Project::setVelocitySpace + MultiPeak::fillPhaseSpace (projects/project.cpp:239-263,
  * projects/MultiPeak/MultiPeak.cpp:127-222) followed by the initial single-cell adjust

@@ -212,6 +212,101 @@ def test_moments_match_oracle(cfg0):
     ctx.close()
 
 
+def _moment_tol(w, want):
+    """FP64 sums in another order: n to 1e-12 relative, n*V to 1e-12 of n * max|v| of the mesh."""
+    vmax = float(np.max(np.abs(np.concatenate([np.asarray(w.vmin), np.asarray(w.vmax)]))))
+    return 1e-12 * np.abs(want[:, :1]) * np.array([[1.0, vmax, vmax, vmax]]) + 1e-300
+
+
+def test_moments_follow_the_state(cfg0):
+    """Resident moments are recomputed whenever f changes (remaps, adjust, a lone 1-D remap):
+    after each, vacc_moments against the oracle's moments of the downloaded state."""
+    w, cells = cfg0
+    m = cases.oracle_mesh(w)
+    ctx = _ctx(w, O.PQM)
+    ctx.upload_cells(cells)
+    for step in range(3):
+        mo = (step + 2) % 3
+        ctx.set_intersections(cases.oracle_intersections(w, mo))
+        ctx.accelerate(mo)
+        for what in ("accelerate", "adjust"):
+            if what == "adjust":
+                ctx.adjust(len(cells))
+            _say(f"moments step {step} after {what}")
+            got = ctx.moments(len(cells))
+            want = np.array([O.moments(m, g, f) for g, f in ctx.download_cells()])
+            assert np.all(np.abs(got - want) <= _moment_tol(w, want)), f"step {step} {what}"
+    ctx.map_1d(1)
+    got = ctx.moments(len(cells))
+    want = np.array([O.moments(m, g, f) for g, f in ctx.download_cells()])
+    assert np.all(np.abs(got - want) <= _moment_tol(w, want))
+    ctx.close()
+
+
+def test_step_bit_exact_against_oracle(cfg0):
+    """vacc_step (transform from the resident fields, three remaps, adjust, one sync) against
+    the oracle's remaps and adjust driven by the intersections the step computed; those
+    intersections against the oracle transform."""
+    w, cells = cfg0
+    m = cases.oracle_mesh(w)
+    ctx = _ctx(w, O.PPM)
+    ctx.upload_cells(cells)
+    ctx.set_fields([V.make_fields(c.B, c.V, c.dt) for c in w.cells])
+    ref = cells
+    for step in range(4):
+        mo = step % 3
+        _say(f"vacc_step {step} order {mo}")
+        loss = np.zeros(len(cells))
+        st = ctx.step(mo, use_moments=False, adjust=True, rho_loss=loss)
+        assert st == 0
+        isx = ctx.get_intersections(len(cells))
+        want_isx = cases.oracle_intersections(w, mo)
+        scale = np.abs(want_isx).max(axis=0) + 1e-300
+        assert np.all(np.abs(isx - want_isx) <= 1e-12 * scale)
+        new, want_loss = [], []
+        for i, (g, f) in enumerate(ref):
+            g2, f2, _ = O.accelerate_cell(m, g, f, isx[i], mo, O.PPM, w.min_value)
+            g3, f3, l3 = O.adjust(m, g2, f2, w.min_value, w.add_width)
+            new.append((g3, f3))
+            want_loss.append(l3)
+        ref = new
+        cases.assert_cells_equal(ctx.download_cells(), ref, f"vacc_step {step}")
+        assert np.allclose(loss, want_loss, rtol=1e-12, atol=0)
+    ctx.close()
+
+
+def test_step_with_moment_pivot_matches_separate_calls():
+    """vacc_step(use_moments=1) equals vacc_compute_cell_intersections(use_moments=1) +
+    vacc_accelerate_cells + vacc_adjust_blocks bit for bit, and its pivot is the bulk
+    velocity of the oracle's moments of the state the step starts from."""
+    w = W.config(1, ncells=16)
+    cells = cases.build_cells(w)
+    m = cases.oracle_mesh(w)
+    fields = [V.make_fields(c.B, c.V, c.dt) for c in w.cells]
+    a, b = _ctx(w), _ctx(w)
+    a.upload_cells(cells)
+    b.upload_cells(cells)
+    a.set_fields(fields)
+    state = cells
+    for step in range(3):
+        mo = (step + 1) % 3
+        _say(f"moment pivot step {step}")
+        a.step(mo, use_moments=True, adjust=True)
+        b.compute_intersections(fields, mo, use_moments=True)
+        b.accelerate(mo)
+        b.adjust(len(cells))
+        ia, ib = a.get_intersections(len(cells)), b.get_intersections(len(cells))
+        assert np.array_equal(ia, ib)
+        mom = np.array([O.moments(m, g, f) for g, f in state])
+        want_isx = cases.oracle_intersections(w, mo, pivots=mom[:, 1:] / mom[:, :1])
+        scale = np.abs(want_isx).max(axis=0) + 1e-300
+        assert np.all(np.abs(ia - want_isx) <= 1e-10 * scale)
+        state = a.download_cells()
+        cases.assert_cells_equal(state, b.download_cells(), f"step {step}")
+    a.close()
+    b.close()
+
+
 def test_init_peaks_matches_oracle_initial_condition():
     w = W.config(2, ncells=8)
     cells = cases.build_cells(w)

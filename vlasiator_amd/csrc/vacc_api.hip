@@ -46,6 +46,8 @@ struct vacc_ctx {
    int lcur = 0, fcur = 0;
    uint64_t nblocks = 0;       // total blocks as of the last synchronisation point
    bool content_valid = false;
+   bool moments_valid = false; // `moments` holds the moments of the current list
+   bool fields_valid = false;  // `fields` holds the caller's transform inputs
    // lists and pools
    uint32_t* off[2] = {nullptr, nullptr};
    uint32_t* gid[2] = {nullptr, nullptr};
@@ -67,6 +69,7 @@ struct vacc_ctx {
    TransformIn* fields = nullptr;
    double* moments = nullptr;
    double* rho_loss = nullptr;
+   uint32_t* h_sum = nullptr;        // pinned host words: capacity flag, block total, status OR
    // event timing
    struct Rec {
       hipEvent_t a, b;
@@ -235,6 +238,7 @@ static int reserve_for_step(vacc_ctx* c) {
 
 static float cell_f(const vacc_ctx* c, int d) { return (float)c->mi.cell[d]; }
 
+// One 1-D remap of every cell.
 static int do_pass(vacc_ctx* c, int dim) {
    const int li = c->lcur, lo = 1 - c->lcur, fi = c->fcur, fo = 1 - c->fcur;
    ColumnArgs ca{};
@@ -316,6 +320,7 @@ static int do_pass(vacc_ctx* c, int dim) {
    c->lcur = lo;
    c->fcur = fo;
    c->content_valid = true;
+   c->moments_valid = false;
    return 0;
 }
 
@@ -358,6 +363,7 @@ static int do_relist(vacc_ctx* c, int add_width, bool want_loss) {
    HIPCHK(hipGetLastError());
    c->lcur = lo;
    c->content_valid = false;
+   if (add_width >= 0) c->moments_valid = false;   // blocks may have been removed
    return 0;
 }
 
@@ -370,23 +376,26 @@ static int do_compact(vacc_ctx* c) {
    return 0;
 }
 
+// The one synchronisation of a work call: reads the capacity flag, the block total of the
+// current list and the OR of the cell status words (kept in status[ncells]) with three small
+// copies into pinned memory.  Returns the status OR or an error.
+static int finish(vacc_ctx* c) {
+   HIPCHK(hipMemcpyAsync(c->h_sum + 0, c->flags, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+   HIPCHK(hipMemcpyAsync(c->h_sum + 1, c->off[c->lcur] + c->ncells, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+   HIPCHK(hipMemcpyAsync(c->h_sum + 2, c->status + c->ncells, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+   HIPCHK(hipStreamSynchronize(c->stream));
+   if (c->h_sum[0] & VACC_STATUS_CAPACITY)
+      return set_err(VACC_E_STATE, "block pool capacity exceeded; state is undefined");
+   c->nblocks = c->h_sum[1];
+   return (int)(c->h_sum[2] & 0x7FFFFFFFu);
+}
+
 // OR of the per-cell status words (synchronises the stream).
 static int status_or(vacc_ctx* c) {
    if (!c->ncells) return 0;
-   std::vector<uint32_t> st(c->ncells);
-   HIPCHK(hipMemcpyAsync(st.data(), c->status, c->ncells * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+   HIPCHK(hipMemcpyAsync(c->h_sum + 2, c->status + c->ncells, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
    HIPCHK(hipStreamSynchronize(c->stream));
-   uint32_t any = 0;
-   for (uint32_t v : st) any |= v;
-   return (int)(any & 0x7FFFFFFFu);
-}
-
-static int check_flags(vacc_ctx* c) {
-   uint32_t fl = 0;
-   HIPCHK(hipMemcpyAsync(&fl, c->flags, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
-   HIPCHK(hipStreamSynchronize(c->stream));
-   if (fl & VACC_STATUS_CAPACITY) return set_err(VACC_E_STATE, "block pool capacity exceeded; state is undefined");
-   return 0;
+   return (int)(c->h_sum[2] & 0x7FFFFFFFu);
 }
 
 extern "C" {
@@ -416,7 +425,8 @@ int vacc_create(const vacc_mesh* mesh, const vacc_species* species, const vacc_o
       c->mi.cell[d] = cell_size(mesh->min_limits[d], mesh->max_limits[d], mesh->grid_length[d]);
    }
    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-       hipMalloc(&c->flags, sizeof(uint32_t)) != hipSuccess || hipMemset(c->flags, 0, sizeof(uint32_t)) != hipSuccess) {
+       hipMalloc(&c->flags, sizeof(uint32_t)) != hipSuccess || hipMemset(c->flags, 0, sizeof(uint32_t)) != hipSuccess ||
+       hipHostMalloc(&c->h_sum, 4 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess) {
       delete c;
       return set_err(VACC_E_HIP, "stream/flag allocation failed");
    }
@@ -436,6 +446,7 @@ int vacc_destroy(vacc_ctx* c) {
    for (auto e : c->ev_free) (void)hipEventDestroy(e);
    if (c->prof_counts) (void)hipFree(c->prof_counts);
    if (c->flags) (void)hipFree(c->flags);
+   if (c->h_sum) (void)hipHostFree(c->h_sum);
    if (c->stream) (void)hipStreamDestroy(c->stream);
    delete c;
    return 0;
@@ -476,6 +487,8 @@ int vacc_upload_cells(vacc_ctx* c, uint32_t ncells, const uint64_t* offsets, con
    HIPCHK(hipStreamSynchronize(c->stream));
    c->nblocks = total;
    c->content_valid = false;
+   c->moments_valid = false;
+   c->fields_valid = false;
    return 0;
 }
 
@@ -543,14 +556,10 @@ static void to_tin(const vacc_cell_fields& in, TransformIn& t) {
    t.dt = in.dt;
 }
 
-int vacc_compute_cell_intersections(vacc_ctx* c, const vacc_cell_fields* fields, uint32_t map_order, int32_t use_moments) {
-   if (!c || (!fields && c->ncells) || map_order > 2) return set_err(VACC_E_ARG, "bad argument");
-   if (!c->ncells) return 0;
-   std::vector<TransformIn> tin(c->ncells);
-   for (uint32_t i = 0; i < c->ncells; ++i) to_tin(fields[i], tin[i]);
-   HIPCHK(hipMemcpyAsync(c->fields, tin.data(), tin.size() * sizeof(TransformIn), hipMemcpyHostToDevice, c->stream));
+// Enqueue the moments (when the pivot needs them and none are resident) and the transform.
+static void enqueue_transform(vacc_ctx* c, uint32_t map_order, bool use_moments) {
    ProfScope ps(c, 3);
-   if (use_moments) {
+   if (use_moments && !c->moments_valid) {
       MomentArgs ma{};
       ma.cell_off = c->off[c->lcur];
       ma.gid = c->gid[c->lcur];
@@ -564,11 +573,39 @@ int vacc_compute_cell_intersections(vacc_ctx* c, const vacc_cell_fields* fields,
          ma.cell[d] = c->mi.cell[d];
       }
       launch_moments(ma, c->stream);
+      c->moments_valid = true;
    }
    launch_transform(c->mi, c->fields, c->moments, use_moments ? 1 : 0, c->sp.mass, c->sp.charge,
                     c->opt.hall_minimum_rhoq, c->opt.ohm_gradpe_term, (int)map_order, c->isect, c->ncells, c->stream);
+}
+
+static int upload_fields(vacc_ctx* c, const vacc_cell_fields* fields) {
+   std::vector<TransformIn> tin(c->ncells);
+   for (uint32_t i = 0; i < c->ncells; ++i) to_tin(fields[i], tin[i]);
+   HIPCHK(hipMemcpyAsync(c->fields, tin.data(), tin.size() * sizeof(TransformIn), hipMemcpyHostToDevice, c->stream));
+   HIPCHK(hipStreamSynchronize(c->stream));   // tin is released on return
+   c->fields_valid = true;
+   return 0;
+}
+
+int vacc_set_fields(vacc_ctx* c, const vacc_cell_fields* fields) {
+   if (!c || (!fields && c->ncells)) return set_err(VACC_E_ARG, "null argument");
+   if (!c->ncells) return 0;
+   return upload_fields(c, fields);
+}
+
+int vacc_compute_cell_intersections(vacc_ctx* c, const vacc_cell_fields* fields, uint32_t map_order, int32_t use_moments) {
+   if (!c || map_order > 2) return set_err(VACC_E_ARG, "bad argument");
+   if (!c->ncells) return 0;
+   if (fields) {
+      const int rc = upload_fields(c, fields);
+      if (rc) return rc;
+   } else if (!c->fields_valid) {
+      return set_err(VACC_E_STATE, "no cell fields: pass fields or call vacc_set_fields first");
+   }
+   enqueue_transform(c, map_order, use_moments != 0);
    HIPCHK(hipGetLastError());
-   HIPCHK(hipStreamSynchronize(c->stream));   // fields buffer is host-owned until here
+   HIPCHK(hipStreamSynchronize(c->stream));
    return 0;
 }
 
@@ -601,11 +638,7 @@ int vacc_map_1d(vacc_ctx* c, uint32_t dim) {
    if (rc) return rc;
    rc = do_pass(c, (int)dim);
    if (rc) return rc;
-   rc = check_flags(c);
-   if (rc) return rc;
-   rc = refresh_count(c);
-   if (rc) return rc;
-   return status_or(c);
+   return finish(c);
 }
 
 int vacc_accelerate_cells(vacc_ctx* c, uint32_t map_order) {
@@ -617,11 +650,7 @@ int vacc_accelerate_cells(vacc_ctx* c, uint32_t map_order) {
       rc = do_pass(c, pass_dim((int)map_order, p));
       if (rc) return rc;
    }
-   rc = check_flags(c);
-   if (rc) return rc;
-   rc = refresh_count(c);
-   if (rc) return rc;
-   return status_or(c);
+   return finish(c);
 }
 
 int vacc_adjust_blocks(vacc_ctx* c, double* rho_loss) {
@@ -631,29 +660,53 @@ int vacc_adjust_blocks(vacc_ctx* c, double* rho_loss) {
    if (rc) return rc;
    rc = do_relist(c, c->sp.sparse_add_width_v, true);
    if (rc) return rc;
-   rc = check_flags(c);
-   if (rc) return rc;
+   rc = finish(c);
+   if (rc < 0) return rc;
    if (rho_loss) HIPCHK(hipMemcpy(rho_loss, c->rho_loss, c->ncells * sizeof(double), hipMemcpyDeviceToHost));
-   return refresh_count(c);
+   return rc;
+}
+
+int vacc_step(vacc_ctx* c, uint32_t map_order, int32_t use_moments, int32_t adjust, double* rho_loss) {
+   if (!c || map_order > 2) return set_err(VACC_E_ARG, "bad argument");
+   if (!c->ncells) return 0;
+   if (!c->fields_valid) return set_err(VACC_E_STATE, "no cell fields: call vacc_set_fields first");
+   int rc = reserve_for_step(c);
+   if (rc) return rc;
+   enqueue_transform(c, map_order, use_moments != 0);
+   for (int p = 0; p < 3; ++p) {
+      rc = do_pass(c, pass_dim((int)map_order, p));
+      if (rc) return rc;
+   }
+   if (adjust) {
+      rc = do_relist(c, c->sp.sparse_add_width_v, true);
+      if (rc) return rc;
+   }
+   rc = finish(c);
+   if (rc < 0) return rc;
+   if (adjust && rho_loss) HIPCHK(hipMemcpy(rho_loss, c->rho_loss, c->ncells * sizeof(double), hipMemcpyDeviceToHost));
+   return rc;
 }
 
 int vacc_moments(vacc_ctx* c, double* out) {
    if (!c || (!out && c->ncells)) return set_err(VACC_E_ARG, "null argument");
    if (!c->ncells) return 0;
-   MomentArgs ma{};
-   ma.cell_off = c->off[c->lcur];
-   ma.gid = c->gid[c->lcur];
-   ma.src = c->src[c->lcur];
-   ma.f = c->f[c->fcur];
-   ma.out = c->moments;
-   ma.ncells = c->ncells;
-   for (int d = 0; d < 3; ++d) {
-      ma.len[d] = c->mi.len[d];
-      ma.vmin[d] = c->mi.vmin[d];
-      ma.cell[d] = c->mi.cell[d];
+   if (!c->moments_valid) {
+      MomentArgs ma{};
+      ma.cell_off = c->off[c->lcur];
+      ma.gid = c->gid[c->lcur];
+      ma.src = c->src[c->lcur];
+      ma.f = c->f[c->fcur];
+      ma.out = c->moments;
+      ma.ncells = c->ncells;
+      for (int d = 0; d < 3; ++d) {
+         ma.len[d] = c->mi.len[d];
+         ma.vmin[d] = c->mi.vmin[d];
+         ma.cell[d] = c->mi.cell[d];
+      }
+      launch_moments(ma, c->stream);
+      HIPCHK(hipGetLastError());
+      c->moments_valid = true;
    }
-   launch_moments(ma, c->stream);
-   HIPCHK(hipGetLastError());
    HIPCHK(hipMemcpyAsync(out, c->moments, (size_t)c->ncells * 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
    HIPCHK(hipStreamSynchronize(c->stream));
    return 0;
@@ -685,6 +738,8 @@ int vacc_init_peaks(vacc_ctx* c, uint32_t ncells, uint32_t max_peaks, const vacc
    c->fcur = 0;
    c->nblocks = 0;
    c->content_valid = false;
+   c->moments_valid = false;
+   c->fields_valid = false;
    if (!ncells) return 0;
    vacc_peak* d_pk = nullptr;
    int32_t *d_np = nullptr, *d_bb = nullptr;
@@ -794,7 +849,7 @@ int vacc_get_status(vacc_ctx* c, uint32_t* status) {
    HIPCHK(hipStreamSynchronize(c->stream));
    if (c->ncells) {
       HIPCHK(hipMemcpy(status, c->status, c->ncells * sizeof(uint32_t), hipMemcpyDeviceToHost));
-      HIPCHK(hipMemset(c->status, 0, c->ncells * sizeof(uint32_t)));
+      HIPCHK(hipMemset(c->status, 0, (c->ncells + 1) * sizeof(uint32_t)));
    }
    return 0;
 }

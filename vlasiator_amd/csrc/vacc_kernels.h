@@ -30,7 +30,7 @@ struct ColumnArgs {
    uint32_t* n_units;          // [ncells]
    uint32_t* len_hist;         // [LEN_BINS] work units per length bin (zeroed before the launch)
    uint32_t* n_targets;        // [ncells]
-   uint32_t* status;           // [ncells] OR-ed status bits
+   uint32_t* status;           // [ncells + 1] OR-ed status bits; entry ncells ORs all cells
    uint32_t ncells;
    uint32_t len[3];
    float dv;                   // (float) cell size along the pass axis
@@ -74,7 +74,7 @@ struct RemapArgs {
    uint32_t* gid_out;
    uint32_t* src_out;
    uint8_t* content_out;
-   uint32_t* status;
+   uint32_t* status;            // [ncells + 1], entry ncells ORs all cells
    const uint32_t* flags;       // global flags (capacity overflow -> skip)
    uint32_t ncells;
    uint32_t len[3];

@@ -392,7 +392,10 @@ __global__ __launch_bounds__(WG) void k_columns(ColumnArgs a) {
    if (t == 0) {
       a.n_units[c] = carry[2];
       a.n_targets[c] = carry[3];
-      if (sstat) atomicOr(&a.status[c], sstat);
+      if (sstat) {
+         atomicOr(&a.status[c], sstat);
+         atomicOr(&a.status[a.ncells], sstat);   // OR over all cells
+      }
    }
    for (int i = t; i < LEN_BINS; i += WG)
       if (hist[i]) atomicAdd(&a.len_hist[i], hist[i]);
@@ -619,6 +622,7 @@ __global__ __launch_bounds__(WG) void k_remap(RemapArgs a, double i_dv) {
                while (next_open <= lk && WID * next_open <= openLim) {
                   if (next_open - next_flush >= RING / WID) {   // ring full: cannot happen for checked spans
                      atomicOr(&a.status[c], VACC_STATUS_DK);
+                     atomicOr(&a.status[a.ncells], VACC_STATUS_DK);
                      float q[4];
 #pragma unroll
                      for (int e = 0; e < 4; ++e) q[e] = ring[(WID * next_flush + e) & (RING - 1)][tid];
@@ -869,34 +873,50 @@ __global__ __launch_bounds__(WG) void k_content(const uint32_t* src, const float
 // ---------------------------------------------------------------------------
 
 __global__ __launch_bounds__(WG) void k_moments(MomentArgs a) {
-   // one workgroup per cell, one wave per block and one lane per velocity cell; the block
-   // coordinates are decoded once per wave in scalar registers
+   // One workgroup per cell; 16 lanes per block, each lane one float4 (i = 0..3 at fixed j, k),
+   // UNR blocks per 16-lane group in flight.  Per value the reference adds f*DV3, f*VX*DV3, ...
+   // (arch_moments.h:98-108); here DV3 is applied once to the cell's FP64 sums.
    __shared__ double red[4][WG / 64];
    const int t = threadIdx.x;
    const uint32_t c = blockIdx.x;
    const uint32_t off = a.cell_off[c];
    const uint32_t n = a.cell_off[c + 1] - off;
-   const double DV3 = a.cell[0] * a.cell[1] * a.cell[2];
-   const int e = t & 63;
-   const int i = e & 3, j = (e >> 2) & 3, k = e >> 4;
-   const double ci = (i + 0.5) * a.cell[0], cj = (j + 0.5) * a.cell[1], ck = (k + 0.5) * a.cell[2];
+   const int q = t & 15;
+   const int j = q & 3, k = q >> 2;
+   const double cx0 = 0.5 * a.cell[0], cx1 = 1.5 * a.cell[0], cx2 = 2.5 * a.cell[0], cx3 = 3.5 * a.cell[0];
+   const double cj = (j + 0.5) * a.cell[1], ck = (k + 0.5) * a.cell[2];
    const double bx = a.cell[0] * WID, by = a.cell[1] * WID, bz = a.cell[2] * WID;
    const uint32_t nx = a.len[0], nxy = a.len[0] * a.len[1];
+   const float4* f4 = reinterpret_cast<const float4*>(a.f);
    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-   const uint32_t wv = __builtin_amdgcn_readfirstlane(t >> 6);
-   for (uint32_t b = wv; b < n; b += WG / 64) {
-      const uint32_t s = __builtin_amdgcn_readfirstlane(a.src[off + b]);
-      if (s == INVALID) continue;
-      const uint32_t g = __builtin_amdgcn_readfirstlane(a.gid[off + b]);
-      const uint32_t z = g / nxy, r = g - z * nxy, y = r / nx, x = r - y * nx;
-      const double VX = (a.vmin[0] + x * bx) + ci;
-      const double VY = (a.vmin[1] + y * by) + cj;
-      const double VZ = (a.vmin[2] + z * bz) + ck;
-      const double v = (double)a.f[(size_t)s * WID3 + e];
-      s0 += v * DV3;
-      s1 += v * VX * DV3;
-      s2 += v * VY * DV3;
-      s3 += v * VZ * DV3;
+   constexpr uint32_t GROUPS = WG / 16;
+   constexpr int UNR = 4;
+   for (uint32_t b = t >> 4; b < n; b += UNR * GROUPS) {
+      uint32_t sv[UNR], gv[UNR];
+      float4 v[UNR];
+#pragma unroll
+      for (int h = 0; h < UNR; ++h) {
+         const uint32_t bb = b + h * GROUPS;
+         sv[h] = (bb < n) ? a.src[off + bb] : INVALID;
+         gv[h] = (bb < n) ? a.gid[off + bb] : 0u;
+      }
+#pragma unroll
+      for (int h = 0; h < UNR; ++h)
+         v[h] = (sv[h] != INVALID) ? f4[(size_t)sv[h] * (WID3 / 4) + q] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+      for (int h = 0; h < UNR; ++h) {
+         const uint32_t g = gv[h];
+         const uint32_t z = g / nxy, r = g - z * nxy, y = r / nx, x = r - y * nx;
+         const double X0 = a.vmin[0] + x * bx;
+         const double VY = (a.vmin[1] + y * by) + cj;
+         const double VZ = (a.vmin[2] + z * bz) + ck;
+         const double v0 = v[h].x, v1 = v[h].y, v2 = v[h].z, v3 = v[h].w;
+         const double sum = (v0 + v1) + (v2 + v3);
+         s0 += sum;
+         s1 += ((v0 * (X0 + cx0) + v1 * (X0 + cx1)) + (v2 * (X0 + cx2) + v3 * (X0 + cx3)));
+         s2 += sum * VY;
+         s3 += sum * VZ;
+      }
    }
    for (int o = 32; o > 0; o >>= 1) {
       s0 += __shfl_down(s0, o, 64);
@@ -912,9 +932,10 @@ __global__ __launch_bounds__(WG) void k_moments(MomentArgs a) {
    }
    __syncthreads();
    if (t < 4) {
-      double s = 0.0;
-      for (int q = 0; q < WG / 64; ++q) s += red[t][q];
-      a.out[4 * (size_t)c + t] = s;
+      const double DV3 = a.cell[0] * a.cell[1] * a.cell[2];
+      double sum = 0.0;
+      for (int w = 0; w < WG / 64; ++w) sum += red[t][w];
+      a.out[4 * (size_t)c + t] = sum * DV3;
    }
 }
 

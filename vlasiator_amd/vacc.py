@@ -25,7 +25,7 @@ EXPORTS = [
     "vacc_download_cells", "vacc_set_intersections", "vacc_get_intersections",
     "vacc_compute_cell_intersections", "vacc_cell_intersections_host", "vacc_max_v_dt", "vacc_map_1d",
     "vacc_accelerate_cells", "vacc_adjust_blocks", "vacc_moments", "vacc_sync", "vacc_get_status",
-    "vacc_init_peaks", "vacc_profile_begin", "vacc_profile_read",
+    "vacc_init_peaks", "vacc_profile_begin", "vacc_profile_read", "vacc_set_fields", "vacc_step",
 ]
 
 
@@ -101,6 +101,8 @@ def lib():
         L.vacc_init_peaks.argtypes = [vp, C.c_uint32, C.c_uint32, vp, vp, vp]
         L.vacc_profile_begin.argtypes = [vp]
         L.vacc_profile_read.argtypes = [vp, C.POINTER(Profile)]
+        L.vacc_set_fields.argtypes = [vp, vp]
+        L.vacc_step.argtypes = [vp, C.c_uint32, C.c_int32, C.c_int32, vp]
         _lib = L
     return _lib
 
@@ -231,9 +233,19 @@ class Context:
         return a.reshape(ncells, 12)
 
     def compute_intersections(self, fields, map_order: int, use_moments: bool = False):
+        """fields=None reuses the resident fields."""
+        arr = None if fields is None else C.cast((CellFields * len(fields))(*fields), C.c_void_p)
+        _check(lib().vacc_compute_cell_intersections(self.h, arr, int(map_order), 1 if use_moments else 0))
+
+    def set_fields(self, fields):
         arr = (CellFields * len(fields))(*fields)
-        _check(lib().vacc_compute_cell_intersections(self.h, C.cast(arr, C.c_void_p), int(map_order),
-                                                     1 if use_moments else 0))
+        _check(lib().vacc_set_fields(self.h, C.cast(arr, C.c_void_p)))
+
+    def step(self, map_order: int, use_moments: bool = True, adjust: bool = True, rho_loss=None) -> int:
+        """One subcycle (moments, transform, three remaps, adjust) with one synchronisation.
+        rho_loss: optional float64 array of ncells entries that receives the removed mass."""
+        ptr = None if rho_loss is None else _ptr(rho_loss)
+        return _check(lib().vacc_step(self.h, int(map_order), 1 if use_moments else 0, 1 if adjust else 0, ptr))
 
     # -- work --------------------------------------------------------------------
     def map_1d(self, dim: int) -> int:
