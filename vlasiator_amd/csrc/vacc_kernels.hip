@@ -502,18 +502,6 @@ __device__ inline void store_pencil(float* __restrict__ f, size_t slot, int o, c
    }
 }
 
-template <int DIM>
-__device__ inline void read_pencil(const float* f, size_t slot, int o, float (&w)[4]) {
-   const float* p = f + slot * WID3 + o;
-   if (DIM == 0) {
-      const float4 q = *reinterpret_cast<const float4*>(p);
-      w[0] = q.x; w[1] = q.y; w[2] = q.z; w[3] = q.w;
-   } else {
-      constexpr int st = Pencil<DIM>::st;
-      w[0] = p[0]; w[1] = p[st]; w[2] = p[2 * st]; w[3] = p[3 * st];
-   }
-}
-
 __device__ inline void set_content(uint8_t* content, size_t slot, const float (&w)[4], double mv, int setlane0, int l16) {
    const bool has = ((double)w[0] >= mv) || ((double)w[1] >= mv) || ((double)w[2] >= mv) || ((double)w[3] >= mv);
    const unsigned long long bal = __ballot(has);
@@ -555,11 +543,17 @@ __global__ __launch_bounds__(WG) void k_remap(RemapArgs a, double i_dv) {
       pass_isect<DIM>(a.isect + 12 * (size_t)c, I, di, dj, dk);
       // Lagrangian base of this lane's pencil, cpu_acc_map.cpp:493-496
       const float Imin = (I + (float)(uint32_t)(b0 * WID + ip) * di) + (float)(uint32_t)(b1 * WID + jp) * dj;
+      // Columns of a unit have non-decreasing target ranges that overlap only where one ends
+      // and the next begins.  Blocks from the next column's first target block on stay open
+      // in the ring across the column boundary (carry_open), so every target block is
+      // accumulated in FP32 registers/LDS and stored exactly once.
       int prev_end = -1;
+      int carry_open = -1;
       for (uint32_t ci = 0; ci < ncol; ++ci) {
          const uint4 cr = a.col_rec[cbase + col0 + ci];
          const int ks = (int)(cr.x & 0xFFFFu), len = (int)(cr.x >> 16);
          const int fk = (int)(cr.z & 0xFFFFu), lk = (int)(cr.z >> 16);
+         const int nfk = (ci + 1 < ncol) ? (int)(a.col_rec[cbase + col0 + ci + 1].z & 0xFFFFu) : INT_MAX;
          const uint32_t* srcs = a.sorted_src + cbase + cr.y;
          const size_t slotb = (size_t)obase + cr.w;   // output slot of block fk
          for (int tb = max(fk, prev_end + 1) + l16; tb <= lk; tb += 16) {
@@ -595,7 +589,8 @@ __global__ __launch_bounds__(WG) void k_remap(RemapArgs a, double i_dv) {
          load_pencil<DIM>(a.f_in, srcs[0], o, w1);
          if (len > 1) load_pencil<DIM>(a.f_in, srcs[1], o, w2);
          else w2[0] = w2[1] = w2[2] = w2[3] = 0.0f;
-         int next_open = fk, next_flush = fk;
+         int next_flush = fk;
+         int next_open = (carry_open >= 0) ? carry_open : fk;
          const int lo_gk = WID * fk, hi_gk = WID * lk + WID - 1;
          typename std::conditional<FAST, ReconFast<METHOD>, Recon<METHOD>>::type rc;
          {
@@ -629,14 +624,11 @@ __global__ __launch_bounds__(WG) void k_remap(RemapArgs a, double i_dv) {
                      store_pencil<DIM>(a.f_out, slotb + (next_flush - fk), o, q);
                      ++next_flush;
                   }
-                  float q[4];
-                  if (next_open <= prev_end) read_pencil<DIM>(a.f_out, slotb + (next_open - fk), o, q);
-                  else q[0] = q[1] = q[2] = q[3] = 0.0f;
 #pragma unroll
-                  for (int e = 0; e < 4; ++e) ring[(WID * next_open + e) & (RING - 1)][tid] = q[e];
+                  for (int e = 0; e < 4; ++e) ring[(WID * next_open + e) & (RING - 1)][tid] = 0.0f;
                   ++next_open;
                }
-               while (next_flush < next_open && WID * next_flush + WID - 1 < flushLim) {
+               while (next_flush < next_open && next_flush < nfk && WID * next_flush + WID - 1 < flushLim) {
                   float q[4];
 #pragma unroll
                   for (int e = 0; e < 4; ++e) q[e] = ring[(WID * next_flush + e) & (RING - 1)][tid];
@@ -668,7 +660,7 @@ __global__ __launch_bounds__(WG) void k_remap(RemapArgs a, double i_dv) {
                w2[e] = w3[e];
             }
          }
-         while (next_flush < next_open) {
+         while (next_flush < next_open && next_flush < nfk) {
             float q[4];
 #pragma unroll
             for (int e = 0; e < 4; ++e) q[e] = ring[(WID * next_flush + e) & (RING - 1)][tid];
@@ -677,13 +669,13 @@ __global__ __launch_bounds__(WG) void k_remap(RemapArgs a, double i_dv) {
             set_content(a.content_out, slot, q, mv, setlane0, l16);
             ++next_flush;
          }
-         for (int tb = next_open; tb <= lk; ++tb) {
-            if (tb <= prev_end) continue;
+         for (int tb = next_open; tb <= lk && tb < nfk; ++tb) {
             const float q[4] = {0.0f, 0.0f, 0.0f, 0.0f};
             const size_t slot = slotb + (tb - fk);
             store_pencil<DIM>(a.f_out, slot, o, q);
             set_content(a.content_out, slot, q, mv, setlane0, l16);
          }
+         carry_open = (next_open > nfk) ? next_open : -1;
          prev_end = max(prev_end, lk);
       }
    }
