@@ -48,8 +48,13 @@ extern "C" {
 /* reconstruction (Makefile ACC_SEMILAG_PLM / _PPM / _PQM) */
 enum { VACC_PLM = 0, VACC_PPM = 1, VACC_PQM = 2 };
 
-/* storage type of the distribution function in HBM */
-enum { VACC_STORE_F32 = 0 };
+/* Storage of the distribution function in HBM.  F32: 64 floats per block.  BF16 / F16: 64
+ * 16-bit codes plus one float scale per block, f = scale * code; the remap reads and writes
+ * this form and computes in FP32.  A block is encoded as: m = max|f| = mant * 2^p,
+ * code = RNE(f * 2^-p), scale = S_f / S_code (the block's FP32 sums, so the block mass is
+ * kept) when that ratio is within [2^(p-1), 2^(p+1)], else 2^p.  Upload and download convert
+ * from and to FP32. */
+enum { VACC_STORE_F32 = 0, VACC_STORE_BF16 = 1, VACC_STORE_F16 = 2 };
 
 /* per-cell status bits (OR-ed over cells in return values) */
 #define VACC_STATUS_WALL 1u        /* target blocks within wall margin (cpu_acc_map.cpp:325-338) */
@@ -80,7 +85,7 @@ typedef struct vacc_species {
 
 typedef struct vacc_options {
    int32_t method;            /* VACC_PLM / VACC_PPM / VACC_PQM */
-   int32_t storage;           /* VACC_STORE_F32 */
+   int32_t storage;           /* VACC_STORE_* */
    int32_t wall_margin;       /* bailout.velocity_space_wall_block_margin */
    int32_t ohm_gradpe_term;   /* fieldsolver.ohmGradPeTerm */
    double hall_minimum_rhoq;  /* P::hallMinimumRhoq */

@@ -17,10 +17,12 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libvacc_oracle.so")
+LIB64_PATH = os.path.join(HERE, "libvacc_oracle64.so")
 REF_PATH = os.path.join(HERE, "_ref", "libref_coeffs.so")
 
 PLM, PPM, PQM = 0, 1, 2
 WID3 = 64
+ST_F32, ST_BF16, ST_F16 = 0, 1, 2
 
 
 class Mesh(C.Structure):
@@ -47,6 +49,7 @@ def build(ref: bool | None = None) -> None:
 
 
 _lib = None
+_lib64 = None
 _ref = None
 
 _u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
@@ -54,6 +57,7 @@ _f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
 _i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
 _u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+_u16p = np.ctypeslib.ndpointer(np.uint16, flags="C_CONTIGUOUS")
 
 
 def lib():
@@ -88,8 +92,35 @@ def lib():
         L.vo_accelerate_cells.argtypes = [C.POINTER(Mesh), C.c_int, _u32p, _u32p, _f32p, _f64p, C.c_int,
                                           C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, C.c_int,
                                           C.c_uint64, _u32p, _u32p, _f32p, _i32p]
+        L.vo_convert16.argtypes = [C.c_int, _f32p, C.c_uint32, _u16p]
+        L.vo_encode_block.argtypes = [_f32p, C.c_int, _u16p, C.POINTER(C.c_float)]
+        L.vo_decode_block.argtypes = [_u16p, C.c_float, C.c_int, _f32p]
+        L.vo_block_csum.restype = C.c_float
+        L.vo_block_csum.argtypes = [_f32p]
+        L.vo_requantize.argtypes = [C.c_uint32, _f32p, C.c_int]
         _lib = L
     return _lib
+
+
+def lib64():
+    """The oracle built with Realf = double (FP64 f): the FP64 reference run."""
+    global _lib64
+    if _lib64 is None:
+        if not os.path.exists(LIB64_PATH):
+            build(ref=False)
+        L = C.CDLL(LIB64_PATH)
+        L.vo_map_1d.argtypes = [C.POINTER(Mesh), C.c_uint32, _u32p, _f64p, C.c_double, C.c_double,
+                                C.c_double, C.c_double, C.c_int, C.c_int, C.c_double, C.c_int,
+                                C.c_uint32, C.POINTER(C.c_uint32), _u32p, _f64p, C.POINTER(C.c_int)]
+        L.vo_adjust.argtypes = [C.POINTER(Mesh), C.c_uint32, _u32p, _f64p, C.c_double, C.c_int,
+                                C.c_uint32, C.POINTER(C.c_uint32), _u32p, _f64p, C.POINTER(C.c_double)]
+        L.vo_moments.argtypes = [C.POINTER(Mesh), C.c_uint32, _u32p, _f64p, _f64p]
+        _lib64 = L
+    return _lib64
+
+
+def _flib(fp64: bool):
+    return (lib64(), np.float64) if fp64 else (lib(), np.float32)
 
 
 def ref_lib():
@@ -152,16 +183,18 @@ def sort_columns(mesh: Mesh, gids: np.ndarray, dim: int):
     return (sorted_[:n], col_off[:nc.value], col_len[:nc.value], set_off[:ns.value], set_nc[:ns.value])
 
 
-def map_1d(mesh: Mesh, gids, f, isect4, dim: int, method: int, min_value: float, wall_margin: int = 0):
+def map_1d(mesh: Mesh, gids, f, isect4, dim: int, method: int, min_value: float, wall_margin: int = 0,
+           fp64: bool = False):
+    L, dt = _flib(fp64)
     g = np.ascontiguousarray(gids, dtype=np.uint32)
-    ff = np.ascontiguousarray(f, dtype=np.float32).reshape(-1)
+    ff = np.ascontiguousarray(f, dtype=dt).reshape(-1)
     cap = g.size + g.size // 2 + 64
     while True:
         go = np.zeros(cap, np.uint32)
-        fo = np.zeros(cap * WID3, np.float32)
+        fo = np.zeros(cap * WID3, dt)
         nout = C.c_uint32()
         wf = C.c_int(0)
-        rc = lib().vo_map_1d(C.byref(mesh), g.size, g, ff, *[float(x) for x in isect4], dim, method,
+        rc = L.vo_map_1d(C.byref(mesh), g.size, g, ff, *[float(x) for x in isect4], dim, method,
                              float(min_value), int(wall_margin), cap, C.byref(nout), go, fo, C.byref(wf))
         if rc == 0:
             n = nout.value
@@ -170,26 +203,33 @@ def map_1d(mesh: Mesh, gids, f, isect4, dim: int, method: int, min_value: float,
 
 
 def accelerate_cell(mesh: Mesh, gids, f, intersections12, map_order: int, method: int,
-                    min_value: float, wall_margin: int = 0):
-    g, ff = np.asarray(gids, np.uint32), np.asarray(f, np.float32).reshape(-1, WID3)
+                    min_value: float, wall_margin: int = 0, fp64: bool = False, storage: int = ST_F32):
+    """Three map_1d passes in map_order.  storage: the blocks written by each pass are stored
+    in that form (BF16/FP16 codes with a per-block scale), i.e. re-quantized after each pass."""
+    g = np.asarray(gids, np.uint32)
+    ff = np.asarray(f, np.float64 if fp64 else np.float32).reshape(-1, WID3)
     order = [(0, 1, 2), (1, 2, 0), (2, 0, 1)][map_order]
     wall = 0
     for dim in order:
-        g, ff, wf = map_1d(mesh, g, ff, intersections12[4 * dim:4 * dim + 4], dim, method, min_value, wall_margin)
+        g, ff, wf = map_1d(mesh, g, ff, intersections12[4 * dim:4 * dim + 4], dim, method, min_value, wall_margin,
+                           fp64=fp64)
+        if storage != ST_F32:
+            ff = requantize(ff, storage)
         wall |= wf
     return g, ff, wall
 
 
-def adjust(mesh: Mesh, gids, f, min_value: float, add_width: int = 1):
+def adjust(mesh: Mesh, gids, f, min_value: float, add_width: int = 1, fp64: bool = False):
+    L, dt = _flib(fp64)
     g = np.ascontiguousarray(gids, dtype=np.uint32)
-    ff = np.ascontiguousarray(f, dtype=np.float32).reshape(-1)
+    ff = np.ascontiguousarray(f, dtype=dt).reshape(-1)
     cap = 2 * g.size + 4096
     while True:
         go = np.zeros(cap, np.uint32)
-        fo = np.zeros(cap * WID3, np.float32)
+        fo = np.zeros(cap * WID3, dt)
         nout = C.c_uint32()
         loss = C.c_double()
-        rc = lib().vo_adjust(C.byref(mesh), g.size, g, ff, float(min_value), int(add_width), cap,
+        rc = L.vo_adjust(C.byref(mesh), g.size, g, ff, float(min_value), int(add_width), cap,
                              C.byref(nout), go, fo, C.byref(loss))
         if rc == 0:
             n = nout.value
@@ -205,12 +245,49 @@ def content_flags(f, min_value: float) -> np.ndarray:
     return out[:n]
 
 
-def moments(mesh: Mesh, gids, f) -> np.ndarray:
+def moments(mesh: Mesh, gids, f, fp64: bool = False) -> np.ndarray:
+    L, dt = _flib(fp64)
     g = np.ascontiguousarray(gids, dtype=np.uint32)
-    ff = np.ascontiguousarray(f, dtype=np.float32).reshape(-1)
+    ff = np.ascontiguousarray(f, dtype=dt).reshape(-1)
     out = np.zeros(4, np.float64)
-    lib().vo_moments(C.byref(mesh), g.size, g, ff, out)
+    L.vo_moments(C.byref(mesh), g.size, g, ff, out)
     return out
+
+
+# ---- 16-bit block storage -------------------------------------------------------------
+
+def convert16(x, storage: int) -> np.ndarray:
+    """Element-wise round-to-nearest-even FP32 -> BF16 / FP16 bit patterns."""
+    a = np.ascontiguousarray(x, dtype=np.float32).reshape(-1)
+    out = np.zeros(max(a.size, 1), np.uint16)
+    lib().vo_convert16(storage, a, a.size, out)
+    return out[:a.size]
+
+
+def encode_block(x, storage: int):
+    a = np.ascontiguousarray(x, dtype=np.float32).reshape(WID3)
+    code = np.zeros(WID3, np.uint16)
+    sc = C.c_float()
+    lib().vo_encode_block(a, storage, code, C.byref(sc))
+    return code, np.float32(sc.value)
+
+
+def decode_block(code, scale, storage: int) -> np.ndarray:
+    out = np.zeros(WID3, np.float32)
+    lib().vo_decode_block(np.ascontiguousarray(code, np.uint16), float(scale), storage, out)
+    return out
+
+
+def block_csum(x) -> np.float32:
+    return np.float32(lib().vo_block_csum(np.ascontiguousarray(x, dtype=np.float32).reshape(WID3)))
+
+
+def requantize(f, storage: int) -> np.ndarray:
+    """The values a 16-bit store holds for these blocks (encode, then decode), as FP32."""
+    a = np.array(f, dtype=np.float32, copy=True).reshape(-1, WID3)
+    if a.shape[0]:
+        lib().vo_requantize(a.shape[0], a.reshape(-1), storage)
+    return a
 
 
 def make_fields(B, V, rhoq=0.0, dB=(0, 0, 0, 0, 0, 0), EgradPe=(0, 0, 0)) -> Fields:

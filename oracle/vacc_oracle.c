@@ -9,15 +9,26 @@
  * (vectorclass_fallback.h:98-226 and insert() :663), a Vec combined with an
  * int or float scalar is evaluated in float.  Build with -ffp-contract=off.
  */
-#include "vacc_oracle.h"
-
 #include <float.h>
 #include <math.h>
+#include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
+
+#ifdef VO_REALF_DOUBLE
+/* FP64 build (libvacc_oracle64.so): Realf = double, as the reference built with DPF
+ * (definitions.h: typedef double Realf).  Every FP32 lane value and rounding of the
+ * restatement becomes FP64; the API takes double f arrays.  Used as the FP64 run that the
+ * storage forms are measured against. */
+#define float double
+#define sqrtf sqrt
+#define fabsf fabs
+#endif
+
+#include "vacc_oracle.h"
 
 /* Vec<float> (op) double-scalar: computed in double, rounded into the float lane. */
 static inline float dm(double c, float x) { return (float)(c * (double)x); }
@@ -1116,3 +1127,112 @@ int64_t vo_accelerate_cells(const vo_mesh* m, int ncells, const uint32_t* offset
    free(cells);
    return ret;
 }
+
+#ifndef VO_REALF_DOUBLE
+/* ------------------------------------------------------------------------ */
+/* 16-bit block storage: codes plus a per-block FP32 scale (the library's     */
+/* BF16 / FP16 storage forms).  No reference counterpart: the reference       */
+/* stores Realf.  Restated from the format definition:                        */
+/*   m = max|x| = mant*2^e (mant in [0.5,1)); code = RNE(x * 2^-e);           */
+/*   S_x, S_d = canonical FP32 sums of x and of the decoded codes;             */
+/*   scale = S_x/S_d if S_x > 0, S_d > 0 and 2^(e-1) <= S_x/S_d <= 2^(e+1),    */
+/*   else 2^e (0 for an all-zero block); f = scale * code.                     */
+/* ------------------------------------------------------------------------ */
+
+static inline uint32_t f2u(float x) { uint32_t u; memcpy(&u, &x, 4); return u; }
+static inline float u2f(uint32_t u) { float x; memcpy(&x, &u, 4); return x; }
+
+uint16_t vo_f32_to_bf16(float x) {
+   const uint32_t u = f2u(x);
+   return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);   /* round to nearest even, finite x */
+}
+
+float vo_bf16_to_f32(uint16_t h) { return u2f((uint32_t)h << 16); }
+
+/* IEEE binary32 -> binary16, round to nearest even, with subnormal results (finite x). */
+uint16_t vo_f32_to_f16(float x) {
+   const uint32_t u = f2u(x);
+   const uint32_t sign = (u >> 16) & 0x8000u;
+   const uint32_t a = u & 0x7FFFFFFFu;
+   if (a >= 0x477FF000u) return (uint16_t)(sign | 0x7C00u);     /* >= 65520: overflow to inf */
+   if (a >= 0x38800000u) {                                          /* normal binary16 */
+      uint32_t h = (a - 0x38000000u) >> 13;
+      const uint32_t rem = a & 0x1FFFu;
+      if (rem > 0x1000u || (rem == 0x1000u && (h & 1u))) ++h;
+      return (uint16_t)(sign | h);
+   }
+   if (a < 0x33000000u) return (uint16_t)sign;                     /* < 2^-25: rounds to zero */
+   /* subnormal binary16: value / 2^-24 rounded to an integer */
+   const uint32_t e = a >> 23;
+   const uint32_t mant = (a & 0x7FFFFFu) | 0x800000u;               /* value = mant * 2^(e-150) */
+   const uint32_t shift = 126u - e;                                 /* 14..24 */
+   uint32_t q = mant >> shift;
+   const uint32_t rem = mant & ((1u << shift) - 1u), half = 1u << (shift - 1u);
+   if (rem > half || (rem == half && (q & 1u))) ++q;
+   return (uint16_t)(sign | q);
+}
+
+float vo_f16_to_f32(uint16_t h) {
+   const uint32_t sign = ((uint32_t)h & 0x8000u) << 16;
+   const uint32_t e = ((uint32_t)h >> 10) & 0x1Fu, mant = (uint32_t)h & 0x3FFu;
+   if (e == 0) {
+      const float v = (float)mant * u2f(0x33800000u);               /* mant * 2^-24, exact */
+      return sign ? -v : v;
+   }
+   if (e == 31) return u2f(sign | 0x7F800000u | (mant << 13));
+   return u2f(sign | ((e + 112u) << 23) | (mant << 13));
+}
+
+static uint16_t enc16(int st, float x) { return st == VO_ST_BF16 ? vo_f32_to_bf16(x) : vo_f32_to_f16(x); }
+static float dec16(int st, uint16_t h) { return st == VO_ST_BF16 ? vo_bf16_to_f32(h) : vo_f16_to_f32(h); }
+
+/* canonical sum of a block: rows L of (i = 0..3) then the 16 row sums pairwise s_L + s_{L^m},
+ * m = 8, 4, 2, 1 */
+float vo_block_csum(const float x[64]) {
+   float s[16], t[16];
+   for (int L = 0; L < 16; ++L) s[L] = (x[4 * L] + x[4 * L + 1]) + (x[4 * L + 2] + x[4 * L + 3]);
+   for (int m = 8; m > 0; m >>= 1) {
+      for (int L = 0; L < 16; ++L) t[L] = s[L] + s[L ^ m];
+      memcpy(s, t, sizeof(s));
+   }
+   return s[0];
+}
+
+void vo_encode_block(const float x[64], int storage, uint16_t code[64], float* scale) {
+   float m = 0.0f;
+   for (int e = 0; e < 64; ++e) m = fmaxf(m, fabsf(x[e]));
+   if (!(m > 0.0f)) {
+      memset(code, 0, 64 * sizeof(uint16_t));
+      *scale = 0.0f;
+      return;
+   }
+   int ex;
+   (void)frexpf(m, &ex);                     /* m = mant * 2^ex, mant in [0.5, 1) */
+   const float p = ldexpf(1.0f, ex);
+   float d[64];
+   for (int e = 0; e < 64; ++e) {
+      code[e] = enc16(storage, ldexpf(x[e], -ex));   /* exact power-of-two normalisation */
+      d[e] = dec16(storage, code[e]);
+   }
+   const float sf = vo_block_csum(x), sq = vo_block_csum(d);
+   const float r = (sq > 0.0f) ? sf / sq : 0.0f;
+   *scale = (sf > 0.0f && r >= 0.5f * p && r <= 2.0f * p) ? r : p;
+}
+
+void vo_decode_block(const uint16_t code[64], float scale, int storage, float out[64]) {
+   for (int e = 0; e < 64; ++e) out[e] = scale * dec16(storage, code[e]);
+}
+
+void vo_convert16(int storage, const float* x, uint32_t n, uint16_t* out) {
+   for (uint32_t i = 0; i < n; ++i) out[i] = enc16(storage, x[i]);
+}
+
+void vo_requantize(uint32_t nblocks, float* f, int storage) {
+   uint16_t code[64];
+   float sc;
+   for (uint32_t b = 0; b < nblocks; ++b) {
+      vo_encode_block(f + (size_t)b * 64, storage, code, &sc);
+      vo_decode_block(code, sc, storage, f + (size_t)b * 64);
+   }
+}
+#endif /* !VO_REALF_DOUBLE */

@@ -132,6 +132,22 @@ int64_t vo_accelerate_cells(const vo_mesh* m, int ncells, const uint32_t* offset
                             uint64_t cap, uint32_t* out_offsets, uint32_t* gids_out, float* f_out,
                             int* wall_flags);
 
+#ifndef VO_REALF_DOUBLE
+/* ---------------- 16-bit block storage (the library's BF16 / FP16 forms) ---------------- */
+enum { VO_ST_F32 = 0, VO_ST_BF16 = 1, VO_ST_F16 = 2 };
+uint16_t vo_f32_to_bf16(float x);
+float vo_bf16_to_f32(uint16_t h);
+uint16_t vo_f32_to_f16(float x);
+float vo_f16_to_f32(uint16_t h);
+float vo_block_csum(const float x[64]);
+void vo_encode_block(const float x[64], int storage, uint16_t code[64], float* scale);
+void vo_decode_block(const uint16_t code[64], float scale, int storage, float out[64]);
+/* element-wise RNE conversion to the 16-bit code */
+void vo_convert16(int storage, const float* x, uint32_t n, uint16_t* out);
+/* encode then decode every block in place (the stored values of the 16-bit forms) */
+void vo_requantize(uint32_t nblocks, float* f, int storage);
+#endif
+
 #ifdef __cplusplus
 }
 #endif

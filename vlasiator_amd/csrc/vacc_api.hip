@@ -52,7 +52,8 @@ struct vacc_ctx {
    uint32_t* off[2] = {nullptr, nullptr};
    uint32_t* gid[2] = {nullptr, nullptr};
    uint32_t* src[2] = {nullptr, nullptr};
-   float* f[2] = {nullptr, nullptr};
+   void* f[2] = {nullptr, nullptr};      // block data: FP32, or 16-bit codes
+   float* scl[2] = {nullptr, nullptr};   // per-block scales of the 16-bit forms
    uint8_t* content = nullptr;
    // pass scratch
    uint32_t* sorted_src = nullptr;
@@ -120,6 +121,8 @@ static void free_blocks(vacc_ctx* c) {
       if (c->gid[i]) (void)hipFree(c->gid[i]);
       if (c->src[i]) (void)hipFree(c->src[i]);
       if (c->f[i]) (void)hipFree(c->f[i]);
+      if (c->scl[i]) (void)hipFree(c->scl[i]);
+      c->scl[i] = nullptr;
       c->gid[i] = c->src[i] = nullptr;
       c->f[i] = nullptr;
    }
@@ -176,20 +179,26 @@ static int alloc_cells(vacc_ctx* c, uint32_t n) {
    return 0;
 }
 
+static size_t elem_bytes(const vacc_ctx* c) { return c->opt.storage == VACC_STORE_F32 ? sizeof(float) : sizeof(uint16_t); }
+
+static Pool pool(const vacc_ctx* c, int i) { return Pool{c->f[i], c->scl[i], c->opt.storage}; }
+
 // (Re)allocate block storage for `want` blocks, preserving the current list and pool.
 static int ensure_blocks(vacc_ctx* c, size_t want) {
    if (want <= c->cap && c->gid[0]) return 0;
    if (want > 0xF0000000ull) return set_err(VACC_E_NOMEM, "block count exceeds 32-bit list indices");
    const size_t ncap = std::max<size_t>(want, 1024);
    uint32_t *g[2], *s[2];
-   float* f[2];
+   void* f[2];
+   float* sc[2] = {nullptr, nullptr};
    uint8_t* ct;
    uint32_t *ss, *pm;
    uint4 *sr, *cr;
    for (int i = 0; i < 2; ++i) {
       HIPCHK(hipMalloc(&g[i], ncap * sizeof(uint32_t)));
       HIPCHK(hipMalloc(&s[i], ncap * sizeof(uint32_t)));
-      HIPCHK(hipMalloc(&f[i], ncap * WID3 * sizeof(float)));
+      HIPCHK(hipMalloc(&f[i], ncap * WID3 * elem_bytes(c)));
+      if (c->opt.storage != VACC_STORE_F32) HIPCHK(hipMalloc(&sc[i], ncap * sizeof(float)));
    }
    HIPCHK(hipMalloc(&ct, ncap));
    HIPCHK(hipMalloc(&ss, ncap * sizeof(uint32_t)));
@@ -201,7 +210,9 @@ static int ensure_blocks(vacc_ctx* c, size_t want) {
       const size_t n = c->nblocks;
       HIPCHK(hipMemcpyAsync(g[c->lcur], c->gid[c->lcur], n * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
       HIPCHK(hipMemcpyAsync(s[c->lcur], c->src[c->lcur], n * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
-      HIPCHK(hipMemcpyAsync(f[c->fcur], c->f[c->fcur], c->cap * WID3 * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+      HIPCHK(hipMemcpyAsync(f[c->fcur], c->f[c->fcur], c->cap * WID3 * elem_bytes(c), hipMemcpyDeviceToDevice, c->stream));
+      if (sc[c->fcur])
+         HIPCHK(hipMemcpyAsync(sc[c->fcur], c->scl[c->fcur], c->cap * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
       HIPCHK(hipMemcpyAsync(ct, c->content, n, hipMemcpyDeviceToDevice, c->stream));
       HIPCHK(hipStreamSynchronize(c->stream));
    }
@@ -210,6 +221,7 @@ static int ensure_blocks(vacc_ctx* c, size_t want) {
       c->gid[i] = g[i];
       c->src[i] = s[i];
       c->f[i] = f[i];
+      c->scl[i] = sc[i];
    }
    c->content = ct;
    c->sorted_src = ss;
@@ -292,8 +304,8 @@ static int do_pass(vacc_ctx* c, int dim) {
    ra.col_rec = c->col_rec;
    ra.sorted_src = c->sorted_src;
    ra.isect = c->isect;
-   ra.f_in = c->f[fi];
-   ra.f_out = c->f[fo];
+   ra.in = pool(c, fi);
+   ra.out = pool(c, fo);
    ra.gid_out = c->gid[lo];
    ra.src_out = c->src[lo];
    ra.content_out = c->content;
@@ -329,7 +341,7 @@ static int do_relist(vacc_ctx* c, int add_width, bool want_loss) {
    ProfScope ps(c, add_width >= 0 ? 2 : 4);
    const int li = c->lcur, lo = 1 - c->lcur;
    if (add_width >= 0 && !c->content_valid) {
-      launch_content(c->src[li], c->f[c->fcur], c->content, (uint32_t)c->nblocks, c->sp.sparse_min_value, c->stream);
+      launch_content(c->src[li], pool(c, c->fcur), c->content, (uint32_t)c->nblocks, c->sp.sparse_min_value, c->stream);
       c->content_valid = true;
    }
    AdjustArgs aa{};
@@ -337,7 +349,7 @@ static int do_relist(vacc_ctx* c, int add_width, bool want_loss) {
    aa.gid = c->gid[li];
    aa.src = c->src[li];
    aa.content = c->content;
-   aa.f = c->f[c->fcur];
+   aa.pool = pool(c, c->fcur);
    aa.n_keep = c->n_tgt;
    aa.rho_loss = want_loss ? c->rho_loss : nullptr;
    aa.ncells = c->ncells;
@@ -369,7 +381,7 @@ static int do_relist(vacc_ctx* c, int add_width, bool want_loss) {
 
 // Compact the pool so that src is the identity (block i of the list at pool index i).
 static int do_compact(vacc_ctx* c) {
-   launch_gather(c->src[c->lcur], c->f[c->fcur], c->f[1 - c->fcur], (uint32_t)c->nblocks, c->stream);
+   launch_gather(c->src[c->lcur], pool(c, c->fcur), pool(c, 1 - c->fcur), (uint32_t)c->nblocks, c->stream);
    launch_iota(c->src[c->lcur], (uint32_t)c->nblocks, c->stream);
    HIPCHK(hipGetLastError());
    c->fcur = 1 - c->fcur;
@@ -410,7 +422,7 @@ int vacc_create(const vacc_mesh* mesh, const vacc_species* species, const vacc_o
       if (!(mesh->max_limits[d] > mesh->min_limits[d])) return set_err(VACC_E_ARG, "empty mesh extent");
    }
    if (opts->method < VACC_PLM || opts->method > VACC_PQM) return set_err(VACC_E_ARG, "unknown method");
-   if (opts->storage != VACC_STORE_F32) return set_err(VACC_E_ARG, "unsupported storage");
+   if (opts->storage < VACC_STORE_F32 || opts->storage > VACC_STORE_F16) return set_err(VACC_E_ARG, "unknown storage");
    if (species->sparse_add_width_v < 0 || species->sparse_add_width_v > 8)
       return set_err(VACC_E_ARG, "sparse_add_width_v must be in [0, 8]");
    int ndev = 0;
@@ -479,7 +491,16 @@ int vacc_upload_cells(vacc_ctx* c, uint32_t ncells, const uint64_t* offsets, con
    HIPCHK(hipMemcpyAsync(c->off[0], off32.data(), (ncells + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
    if (total) {
       HIPCHK(hipMemcpyAsync(c->gid[0], gids, total * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
-      HIPCHK(hipMemcpyAsync(c->f[0], f, total * WID3 * sizeof(float), hipMemcpyHostToDevice, c->stream));
+      if (c->opt.storage == VACC_STORE_F32) {
+         HIPCHK(hipMemcpyAsync(c->f[0], f, total * WID3 * sizeof(float), hipMemcpyHostToDevice, c->stream));
+      } else {
+         float* stage = nullptr;
+         HIPCHK(hipMalloc(&stage, total * WID3 * sizeof(float)));
+         HIPCHK(hipMemcpyAsync(stage, f, total * WID3 * sizeof(float), hipMemcpyHostToDevice, c->stream));
+         launch_encode(stage, pool(c, 0), (uint32_t)total, c->stream);
+         HIPCHK(hipStreamSynchronize(c->stream));
+         HIPCHK(hipFree(stage));
+      }
    }
    launch_iota(c->src[0], (uint32_t)total, c->stream);
    HIPCHK(hipMemsetAsync(c->status, 0, (ncells + 1) * sizeof(uint32_t), c->stream));
@@ -525,7 +546,18 @@ int vacc_download_cells(vacc_ctx* c, uint64_t* offsets, uint32_t* gids, float* f
    for (uint32_t i = 0; i <= c->ncells; ++i) offsets[i] = off[i];
    const size_t n = off[c->ncells];
    if (n && gids) HIPCHK(hipMemcpy(gids, c->gid[c->lcur], n * sizeof(uint32_t), hipMemcpyDeviceToHost));
-   if (n && f) HIPCHK(hipMemcpy(f, c->f[c->fcur], n * WID3 * sizeof(float), hipMemcpyDeviceToHost));
+   if (n && f) {
+      if (c->opt.storage == VACC_STORE_F32) {
+         HIPCHK(hipMemcpy(f, c->f[c->fcur], n * WID3 * sizeof(float), hipMemcpyDeviceToHost));
+      } else {
+         float* stage = nullptr;
+         HIPCHK(hipMalloc(&stage, n * WID3 * sizeof(float)));
+         launch_decode(pool(c, c->fcur), stage, (uint32_t)n, c->stream);
+         HIPCHK(hipMemcpyAsync(f, stage, n * WID3 * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+         HIPCHK(hipStreamSynchronize(c->stream));
+         HIPCHK(hipFree(stage));
+      }
+   }
    return 0;
 }
 
@@ -564,7 +596,7 @@ static void enqueue_transform(vacc_ctx* c, uint32_t map_order, bool use_moments)
       ma.cell_off = c->off[c->lcur];
       ma.gid = c->gid[c->lcur];
       ma.src = c->src[c->lcur];
-      ma.f = c->f[c->fcur];
+      ma.pool = pool(c, c->fcur);
       ma.out = c->moments;
       ma.ncells = c->ncells;
       for (int d = 0; d < 3; ++d) {
@@ -695,7 +727,7 @@ int vacc_moments(vacc_ctx* c, double* out) {
       ma.cell_off = c->off[c->lcur];
       ma.gid = c->gid[c->lcur];
       ma.src = c->src[c->lcur];
-      ma.f = c->f[c->fcur];
+      ma.pool = pool(c, c->fcur);
       ma.out = c->moments;
       ma.ncells = c->ncells;
       for (int d = 0; d < 3; ++d) {
@@ -786,13 +818,21 @@ int vacc_init_peaks(vacc_ctx* c, uint32_t ncells, uint32_t max_peaks, const vacc
    ia.out_off = c->off[0];
    ia.gid_out = c->gid[0];
    ia.src_out = c->src[0];
-   ia.f_out = c->f[0];
+   float* stage = nullptr;
+   if (c->opt.storage == VACC_STORE_F32) {
+      ia.f_out = static_cast<float*>(c->f[0]);
+   } else {
+      HIPCHK(hipMalloc(&stage, std::max<size_t>(total, 1) * WID3 * sizeof(float)));
+      ia.f_out = stage;
+   }
    launch_init(ia, true, c->stream);
+   if (stage) launch_encode(stage, pool(c, 0), (uint32_t)total, c->stream);
    HIPCHK(hipGetLastError());
    HIPCHK(hipMemsetAsync(c->status, 0, (ncells + 1) * sizeof(uint32_t), c->stream));
    HIPCHK(hipMemsetAsync(c->flags, 0, sizeof(uint32_t), c->stream));
    HIPCHK(hipStreamSynchronize(c->stream));
    c->nblocks = total;
+   if (stage) (void)hipFree(stage);
    (void)hipFree(d_pk);
    (void)hipFree(d_np);
    (void)hipFree(d_bb);

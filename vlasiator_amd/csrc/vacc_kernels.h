@@ -5,6 +5,14 @@
 
 namespace vacc {
 
+// A pool of block data: FP32 (64 floats per block) or 16-bit codes (64 per block) plus one
+// FP32 scale per block (vacc_codec.h).  storage: 0 FP32, 1 BF16, 2 FP16.
+struct Pool {
+   void* f;
+   float* scale;
+   int storage;
+};
+
 // Work unit of the remap (one column, or a whole column set whose columns' target ranges
 // share a block), 16 bytes.
 //   x: b0 | b1 << 16   transverse block indices in the pass frame
@@ -69,8 +77,7 @@ struct RemapArgs {
    const uint4* col_rec;
    const uint32_t* sorted_src;
    const double* isect;
-   const float* f_in;
-   float* f_out;
+   Pool in, out;                // source and target pools
    uint32_t* gid_out;
    uint32_t* src_out;
    uint8_t* content_out;
@@ -88,7 +95,7 @@ struct AdjustArgs {
    const uint32_t* gid;
    const uint32_t* src;
    const uint8_t* content;     // per input block, or null -> computed from f
-   const float* f;             // pool the src indices point into
+   Pool pool;                  // pool the src indices point into
    const uint32_t* out_off;    // [ncells+1] (write phase)
    uint32_t* n_keep;           // [ncells] (count phase)
    uint32_t* gid_out;          // (write phase)
@@ -106,7 +113,7 @@ struct MomentArgs {
    const uint32_t* cell_off;
    const uint32_t* gid;
    const uint32_t* src;
-   const float* f;
+   Pool pool;
    double* out;                // [ncells*4]
    uint32_t ncells;
    uint32_t len[3];
@@ -138,7 +145,7 @@ struct InitArgs {
 
 void launch_init(const InitArgs& a, bool write_phase, hipStream_t s);
 int init_box_words_max();
-void launch_content(const uint32_t* src, const float* f, uint8_t* content, uint32_t n, double mv, hipStream_t s);
+void launch_content(const uint32_t* src, const Pool& pool, uint8_t* content, uint32_t n, double mv, hipStream_t s);
 
 void launch_columns(int dim, const ColumnArgs& a, hipStream_t s);
 void launch_scan(const ScanArgs& a, hipStream_t s);
@@ -149,7 +156,10 @@ void launch_moments(const MomentArgs& a, hipStream_t s);
 void launch_transform(const MeshInfo& m, const TransformIn* fields, const double* moments, int use_moments,
                       double mass, double charge, double hall_min_rhoq, int ohm_gradpe, int map_order,
                       double* isect_out, uint32_t ncells, hipStream_t s);
-void launch_gather(const uint32_t* src, const float* f_in, float* f_out, uint32_t n, hipStream_t s);
+void launch_gather(const uint32_t* src, const Pool& in, const Pool& out, uint32_t n, hipStream_t s);
+// FP32 blocks <-> 16-bit pool (one wave per block)
+void launch_encode(const float* f, const Pool& out, uint32_t n, hipStream_t s);
+void launch_decode(const Pool& in, float* f, uint32_t n, hipStream_t s);
 void launch_iota(uint32_t* dst, uint32_t n, hipStream_t s);
 void launch_fill_u32(uint32_t* dst, uint32_t v, uint32_t n, hipStream_t s);
 

@@ -22,6 +22,7 @@
 // rounded.  The file is compiled with -ffp-contract=off.
 #include "vacc_kernels.h"
 #include "vacc_recon.h"
+#include "vacc_codec.h"
 
 #include <float.h>
 #include <type_traits>
@@ -469,25 +470,46 @@ __global__ __launch_bounds__(WG) void k_order(OrderArgs a) {
 // k_remap
 // ---------------------------------------------------------------------------
 
+// Value e of block b of a pool (any storage), decoded to FP32.
+__device__ inline float pool_value(const Pool& pl, uint32_t b, int e) {
+   if (pl.storage == ST_F32) return static_cast<const float*>(pl.f)[(size_t)b * WID3 + e];
+   return pl.scale[b] * dec16(pl.storage, static_cast<const uint16_t*>(pl.f)[(size_t)b * WID3 + e]);
+}
+
 template <int DIM>
 struct Pencil {
    int o;   // offset of the pencil's first cell in a block
    static constexpr int st = (DIM == 0) ? 1 : (DIM == 1) ? 4 : 16;   // stride along the pass axis
 };
 
-template <int DIM>
-__device__ inline void load_pencil(const float* __restrict__ f, uint32_t sidx, int o, float (&w)[4]) {
+// Load the lane's pencil (4 cells along the pass axis) of source block sidx, decoded to FP32.
+template <int DIM, int ST>
+__device__ inline void load_pencil(const Pool& pl, uint32_t sidx, int o, float (&w)[4]) {
    if (sidx == INVALID) {
       w[0] = w[1] = w[2] = w[3] = 0.0f;
       return;
    }
-   const float* p = f + (size_t)sidx * WID3 + o;
-   if (DIM == 0) {
-      const float4 q = *reinterpret_cast<const float4*>(p);
-      w[0] = q.x; w[1] = q.y; w[2] = q.z; w[3] = q.w;
+   constexpr int st = Pencil<DIM>::st;
+   if constexpr (ST == ST_F32) {
+      const float* p = static_cast<const float*>(pl.f) + (size_t)sidx * WID3 + o;
+      if (DIM == 0) {
+         const float4 q = *reinterpret_cast<const float4*>(p);
+         w[0] = q.x; w[1] = q.y; w[2] = q.z; w[3] = q.w;
+      } else {
+         w[0] = p[0]; w[1] = p[st]; w[2] = p[2 * st]; w[3] = p[3 * st];
+      }
    } else {
-      constexpr int st = Pencil<DIM>::st;
-      w[0] = p[0]; w[1] = p[st]; w[2] = p[2 * st]; w[3] = p[3 * st];
+      const uint16_t* p = static_cast<const uint16_t*>(pl.f) + (size_t)sidx * WID3 + o;
+      const float sc = pl.scale[sidx];
+      uint32_t u[4];
+      if (DIM == 0) {
+         const uint2 q = *reinterpret_cast<const uint2*>(p);
+         u[0] = q.x & 0xFFFFu; u[1] = q.x >> 16; u[2] = q.y & 0xFFFFu; u[3] = q.y >> 16;
+      } else {
+         u[0] = p[0]; u[1] = p[st]; u[2] = p[2 * st]; u[3] = p[3 * st];
+      }
+#pragma unroll
+      for (int e = 0; e < 4; ++e) w[e] = sc * Codec<ST>::dec(u[e]);
    }
 }
 
@@ -508,7 +530,83 @@ __device__ inline void set_content(uint8_t* content, size_t slot, const float (&
    if (l16 == 0) content[slot] = (uint8_t)(((bal >> setlane0) & 0xFFFFull) != 0ull);
 }
 
-template <int DIM, int METHOD, bool FAST>
+// Store one finished target block from the ring (all 16 lanes of the unit active).  FP32: each
+// lane stores its own pencil.  16-bit forms: lane L reads the canonical row (j,k) = (L&3, L>>2)
+// of the block from the ring columns of the unit's lanes, encodes the block and stores its 4
+// codes (8 contiguous bytes) and, lane 0, the scale.  The content flag is taken on the stored
+// (decoded) values.
+template <int DIM, int ST>
+__device__ inline void flush_block(const RemapArgs& a, float (*ring)[WG], int tid, int l16, int setlane0, int o,
+                                   int blk, size_t slot, double mv) {
+   if constexpr (ST == ST_F32) {
+      float q[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) q[e] = ring[(WID * blk + e) & (RING - 1)][tid];
+      store_pencil<DIM>(static_cast<float*>(a.out.f), slot, o, q);
+      set_content(a.content_out, slot, q, mv, setlane0, l16);
+   } else {
+      const int L = l16, base = tid - l16;
+      float x[4];
+      __builtin_amdgcn_wave_barrier();
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+         // lane (ip, jp) of the unit holds f(i=kk, j=jp, k=ip) (dim 0), f(ip, kk, jp) (dim 1),
+         // f(ip, jp, kk) (dim 2) in ring row kk
+         const int lane = (DIM == 0) ? ((L >> 2) + 4 * (L & 3)) : (DIM == 1) ? (i + 4 * (L >> 2)) : (i + 4 * (L & 3));
+         const int row = (DIM == 0) ? i : (DIM == 1) ? (L & 3) : (L >> 2);
+         x[i] = ring[(WID * blk + row) & (RING - 1)][base + lane];
+      }
+      uint32_t code[4];
+      const float sc = encode_rows16<ST>(x, code);
+      uint16_t* p = static_cast<uint16_t*>(a.out.f) + slot * WID3 + 4 * L;
+      *reinterpret_cast<uint2*>(p) = make_uint2(code[0] | (code[1] << 16), code[2] | (code[3] << 16));
+      if (L == 0) a.out.scale[slot] = sc;
+      float d[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) d[i] = sc * Codec<ST>::dec(code[i]);
+      set_content(a.content_out, slot, d, mv, setlane0, l16);
+   }
+}
+
+// Store an all-zero target block (a target block that received no contribution).
+template <int DIM, int ST>
+__device__ inline void zero_block(const RemapArgs& a, int l16, int setlane0, int o, size_t slot, double mv) {
+   const float q[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+   if constexpr (ST == ST_F32) {
+      store_pencil<DIM>(static_cast<float*>(a.out.f), slot, o, q);
+   } else {
+      uint16_t* p = static_cast<uint16_t*>(a.out.f) + slot * WID3 + 4 * l16;
+      *reinterpret_cast<uint2*>(p) = make_uint2(0u, 0u);
+      if (l16 == 0) a.out.scale[slot] = 0.0f;
+   }
+   set_content(a.content_out, slot, q, mv, setlane0, l16);
+}
+
+// Pass-through copy of a source block (cells in copy mode keep their stored representation).
+template <int DIM, int ST>
+__device__ inline void copy_block(const RemapArgs& a, int l16, int setlane0, int o, uint32_t sidx, size_t slot,
+                                  double mv) {
+   if constexpr (ST == ST_F32) {
+      float w[4];
+      load_pencil<DIM, ST>(a.in, sidx, o, w);
+      store_pencil<DIM>(static_cast<float*>(a.out.f), slot, o, w);
+      set_content(a.content_out, slot, w, mv, setlane0, l16);
+   } else {
+      uint2 raw = make_uint2(0u, 0u);
+      float sc = 0.0f;
+      if (sidx != INVALID) {
+         raw = *reinterpret_cast<const uint2*>(static_cast<const uint16_t*>(a.in.f) + (size_t)sidx * WID3 + 4 * l16);
+         sc = a.in.scale[sidx];
+      }
+      *reinterpret_cast<uint2*>(static_cast<uint16_t*>(a.out.f) + slot * WID3 + 4 * l16) = raw;
+      if (l16 == 0) a.out.scale[slot] = sc;
+      const float d[4] = {sc * Codec<ST>::dec(raw.x & 0xFFFFu), sc * Codec<ST>::dec(raw.x >> 16),
+                          sc * Codec<ST>::dec(raw.y & 0xFFFFu), sc * Codec<ST>::dec(raw.y >> 16)};
+      set_content(a.content_out, slot, d, mv, setlane0, l16);
+   }
+}
+
+template <int DIM, int METHOD, bool FAST, int ST>
 __global__ __launch_bounds__(WG) void k_remap(RemapArgs a, double i_dv) {
    __shared__ float ring[RING][WG];
    if (*a.flags & VACC_STATUS_CAPACITY) return;
@@ -562,12 +660,7 @@ __global__ __launch_bounds__(WG) void k_remap(RemapArgs a, double i_dv) {
             a.src_out[slot] = (uint32_t)slot;
          }
          if (copy) {
-            for (int b = 0; b < len; ++b) {
-               float w[4];
-               load_pencil<DIM>(a.f_in, srcs[b], o, w);
-               store_pencil<DIM>(a.f_out, slotb + b, o, w);
-               set_content(a.content_out, slotb + b, w, mv, setlane0, l16);
-            }
+            for (int b = 0; b < len; ++b) copy_block<DIM, ST>(a, l16, setlane0, o, srcs[b], slotb + b, mv);
             prev_end = max(prev_end, lk);
             continue;
          }
@@ -586,8 +679,8 @@ __global__ __launch_bounds__(WG) void k_remap(RemapArgs a, double i_dv) {
             }
          }
          float w0[4] = {0.0f, 0.0f, 0.0f, 0.0f}, w1[4], w2[4];
-         load_pencil<DIM>(a.f_in, srcs[0], o, w1);
-         if (len > 1) load_pencil<DIM>(a.f_in, srcs[1], o, w2);
+         load_pencil<DIM, ST>(a.in, srcs[0], o, w1);
+         if (len > 1) load_pencil<DIM, ST>(a.in, srcs[1], o, w2);
          else w2[0] = w2[1] = w2[2] = w2[3] = 0.0f;
          int next_flush = fk;
          int next_open = (carry_open >= 0) ? carry_open : fk;
@@ -599,7 +692,7 @@ __global__ __launch_bounds__(WG) void k_remap(RemapArgs a, double i_dv) {
          }
          for (int b = 0; b < len; ++b) {
             float w3[4];
-            if (b + 2 < len) load_pencil<DIM>(a.f_in, srcs[b + 2], o, w3);
+            if (b + 2 < len) load_pencil<DIM, ST>(a.in, srcs[b + 2], o, w3);
             else w3[0] = w3[1] = w3[2] = w3[3] = 0.0f;
             const float v[12] = {w0[0], w0[1], w0[2], w0[3], w1[0], w1[1], w1[2], w1[3], w2[0], w2[1], w2[2], w2[3]};
 #pragma unroll
@@ -618,10 +711,7 @@ __global__ __launch_bounds__(WG) void k_remap(RemapArgs a, double i_dv) {
                   if (next_open - next_flush >= RING / WID) {   // ring full: cannot happen for checked spans
                      atomicOr(&a.status[c], VACC_STATUS_DK);
                      atomicOr(&a.status[a.ncells], VACC_STATUS_DK);
-                     float q[4];
-#pragma unroll
-                     for (int e = 0; e < 4; ++e) q[e] = ring[(WID * next_flush + e) & (RING - 1)][tid];
-                     store_pencil<DIM>(a.f_out, slotb + (next_flush - fk), o, q);
+                     flush_block<DIM, ST>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), mv);
                      ++next_flush;
                   }
 #pragma unroll
@@ -629,12 +719,7 @@ __global__ __launch_bounds__(WG) void k_remap(RemapArgs a, double i_dv) {
                   ++next_open;
                }
                while (next_flush < next_open && next_flush < nfk && WID * next_flush + WID - 1 < flushLim) {
-                  float q[4];
-#pragma unroll
-                  for (int e = 0; e < 4; ++e) q[e] = ring[(WID * next_flush + e) & (RING - 1)][tid];
-                  const size_t slot = slotb + (next_flush - fk);
-                  store_pencil<DIM>(a.f_out, slot, o, q);
-                  set_content(a.content_out, slot, q, mv, setlane0, l16);
+                  flush_block<DIM, ST>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), mv);
                   ++next_flush;
                }
                float tdr = 0.0f;
@@ -661,20 +746,10 @@ __global__ __launch_bounds__(WG) void k_remap(RemapArgs a, double i_dv) {
             }
          }
          while (next_flush < next_open && next_flush < nfk) {
-            float q[4];
-#pragma unroll
-            for (int e = 0; e < 4; ++e) q[e] = ring[(WID * next_flush + e) & (RING - 1)][tid];
-            const size_t slot = slotb + (next_flush - fk);
-            store_pencil<DIM>(a.f_out, slot, o, q);
-            set_content(a.content_out, slot, q, mv, setlane0, l16);
+            flush_block<DIM, ST>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), mv);
             ++next_flush;
          }
-         for (int tb = next_open; tb <= lk && tb < nfk; ++tb) {
-            const float q[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-            const size_t slot = slotb + (tb - fk);
-            store_pencil<DIM>(a.f_out, slot, o, q);
-            set_content(a.content_out, slot, q, mv, setlane0, l16);
-         }
+         for (int tb = next_open; tb <= lk && tb < nfk; ++tb) zero_block<DIM, ST>(a, l16, setlane0, o, slotb + (tb - fk), mv);
          carry_open = (next_open > nfk) ? next_open : -1;
          prev_end = max(prev_end, lk);
       }
@@ -827,10 +902,8 @@ __global__ __launch_bounds__(WG) void k_adjust(AdjustArgs a) {
             // removed block: RHOLOSSADJUST += DV3 * sum(f)
             const uint32_t s = a.src[off + i];
             double sum = 0.0;
-            if (s != INVALID) {
-               const float* fb = a.f + (size_t)s * WID3;
-               for (int e = 0; e < WID3; ++e) sum += fb[e];
-            }
+            if (s != INVALID)
+               for (int e = 0; e < WID3; ++e) sum += pool_value(a.pool, s, e);
             myloss += a.dv3 * sum;
          }
       }
@@ -849,13 +922,13 @@ __global__ __launch_bounds__(WG) void k_adjust(AdjustArgs a) {
 }
 
 // content flag of each block of a list, from f (spatial_cell_cpu.cpp:290-316)
-__global__ __launch_bounds__(WG) void k_content(const uint32_t* src, const float* f, uint8_t* content,
+__global__ __launch_bounds__(WG) void k_content(const uint32_t* src, Pool pl, uint8_t* content,
                                                 uint32_t n, double mv) {
    const uint32_t wave = (blockIdx.x * WG + threadIdx.x) >> 6;
    const int lane = threadIdx.x & 63;
    if (wave >= n) return;
    const uint32_t s = src[wave];
-   const float x = (s == INVALID) ? 0.0f : f[(size_t)s * WID3 + lane];
+   const float x = (s == INVALID) ? 0.0f : pool_value(pl, s, lane);
    const unsigned long long b = __ballot((double)x >= mv);
    if (lane == 0) content[wave] = (uint8_t)(b != 0ull);
 }
@@ -863,6 +936,16 @@ __global__ __launch_bounds__(WG) void k_content(const uint32_t* src, const float
 // ---------------------------------------------------------------------------
 // moments, transform, gather
 // ---------------------------------------------------------------------------
+
+// Row q (cells 4q..4q+3, i = 0..3 at j = q & 3, k = q >> 2) of block b, decoded to FP32.
+__device__ inline float4 pool_row(const Pool& pl, uint32_t b, int q) {
+   if (b == INVALID) return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+   if (pl.storage == ST_F32) return reinterpret_cast<const float4*>(pl.f)[(size_t)b * (WID3 / 4) + q];
+   const uint2 r = reinterpret_cast<const uint2*>(pl.f)[(size_t)b * (WID3 / 4) + q];
+   const float sc = pl.scale[b];
+   return make_float4(sc * dec16(pl.storage, r.x & 0xFFFFu), sc * dec16(pl.storage, r.x >> 16),
+                      sc * dec16(pl.storage, r.y & 0xFFFFu), sc * dec16(pl.storage, r.y >> 16));
+}
 
 __global__ __launch_bounds__(WG) void k_moments(MomentArgs a) {
    // One workgroup per cell; 16 lanes per block, each lane one float4 (i = 0..3 at fixed j, k),
@@ -879,7 +962,6 @@ __global__ __launch_bounds__(WG) void k_moments(MomentArgs a) {
    const double cj = (j + 0.5) * a.cell[1], ck = (k + 0.5) * a.cell[2];
    const double bx = a.cell[0] * WID, by = a.cell[1] * WID, bz = a.cell[2] * WID;
    const uint32_t nx = a.len[0], nxy = a.len[0] * a.len[1];
-   const float4* f4 = reinterpret_cast<const float4*>(a.f);
    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
    constexpr uint32_t GROUPS = WG / 16;
    constexpr int UNR = 4;
@@ -893,8 +975,7 @@ __global__ __launch_bounds__(WG) void k_moments(MomentArgs a) {
          gv[h] = (bb < n) ? a.gid[off + bb] : 0u;
       }
 #pragma unroll
-      for (int h = 0; h < UNR; ++h)
-         v[h] = (sv[h] != INVALID) ? f4[(size_t)sv[h] * (WID3 / 4) + q] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      for (int h = 0; h < UNR; ++h) v[h] = pool_row(a.pool, sv[h], q);
 #pragma unroll
       for (int h = 0; h < UNR; ++h) {
          const uint32_t g = gv[h];
@@ -958,6 +1039,36 @@ __global__ void k_gather(const uint32_t* src, const float4* f_in, float4* f_out,
    f_out[q] = (s == INVALID) ? make_float4(0.f, 0.f, 0.f, 0.f) : f_in[(size_t)s * 16 + (q & 15)];
 }
 
+// 16-bit pools: 16 lanes per block, 8 bytes of codes each, lane 0 the scale.
+__global__ void k_gather16(const uint32_t* src, const uint2* q_in, const float* s_in, uint2* q_out, float* s_out,
+                           uint32_t n) {
+   const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+   if (q >= (size_t)n * 16) return;
+   const uint32_t b = (uint32_t)(q >> 4);
+   const uint32_t s = src[b];
+   q_out[q] = (s == INVALID) ? make_uint2(0u, 0u) : q_in[(size_t)s * 16 + (q & 15)];
+   if ((q & 15) == 0) s_out[b] = (s == INVALID) ? 0.0f : s_in[s];
+}
+
+// FP32 blocks -> 16-bit pool, one wave per block.
+template <int ST>
+__global__ __launch_bounds__(WG) void k_encode(const float* f, uint16_t* q, float* scale, uint32_t n) {
+   const uint32_t b = (blockIdx.x * WG + threadIdx.x) >> 6;
+   const int e = threadIdx.x & 63;
+   if (b >= n) return;   // whole waves exit together
+   uint32_t code;
+   const float sc = encode_wave64<ST>(f[(size_t)b * WID3 + e], code);
+   q[(size_t)b * WID3 + e] = (uint16_t)code;
+   if (e == 0) scale[b] = sc;
+}
+
+// 16-bit pool -> FP32 blocks.
+__global__ __launch_bounds__(WG) void k_decode(Pool pl, float* f, uint32_t n) {
+   const size_t i = (size_t)blockIdx.x * WG + threadIdx.x;
+   if (i >= (size_t)n * WID3) return;
+   f[i] = pool_value(pl, (uint32_t)(i / WID3), (int)(i % WID3));
+}
+
 __global__ void k_iota(uint32_t* dst, uint32_t n) {
    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
    if (i < n) dst[i] = i;
@@ -1002,8 +1113,18 @@ int remap_grid_size() {
 
 template <int DIM, int METHOD>
 static void remap_dm(const RemapArgs& a, uint32_t grid, double i_dv, bool fast, hipStream_t s) {
-   if (fast) hipLaunchKernelGGL((k_remap<DIM, METHOD, true>), dim3(grid), dim3(WG), 0, s, a, i_dv);
-   else hipLaunchKernelGGL((k_remap<DIM, METHOD, false>), dim3(grid), dim3(WG), 0, s, a, i_dv);
+   // the source and target pools share one storage form
+   const int st = a.in.storage;
+   if (st == ST_F32) {
+      if (fast) hipLaunchKernelGGL((k_remap<DIM, METHOD, true, ST_F32>), dim3(grid), dim3(WG), 0, s, a, i_dv);
+      else hipLaunchKernelGGL((k_remap<DIM, METHOD, false, ST_F32>), dim3(grid), dim3(WG), 0, s, a, i_dv);
+   } else if (st == ST_BF16) {
+      if (fast) hipLaunchKernelGGL((k_remap<DIM, METHOD, true, ST_BF16>), dim3(grid), dim3(WG), 0, s, a, i_dv);
+      else hipLaunchKernelGGL((k_remap<DIM, METHOD, false, ST_BF16>), dim3(grid), dim3(WG), 0, s, a, i_dv);
+   } else {
+      if (fast) hipLaunchKernelGGL((k_remap<DIM, METHOD, true, ST_F16>), dim3(grid), dim3(WG), 0, s, a, i_dv);
+      else hipLaunchKernelGGL((k_remap<DIM, METHOD, false, ST_F16>), dim3(grid), dim3(WG), 0, s, a, i_dv);
+   }
 }
 
 template <int DIM>
@@ -1030,10 +1151,10 @@ void launch_adjust(const AdjustArgs& a, bool write_phase, hipStream_t s) {
    else hipLaunchKernelGGL(k_adjust<false>, dim3(a.ncells), dim3(WG), 0, s, a);
 }
 
-void launch_content(const uint32_t* src, const float* f, uint8_t* content, uint32_t n, double mv, hipStream_t s) {
+void launch_content(const uint32_t* src, const Pool& pool, uint8_t* content, uint32_t n, double mv, hipStream_t s) {
    if (n == 0) return;
    const uint32_t blocks = (uint32_t)(((size_t)n * 64 + WG - 1) / WG);
-   hipLaunchKernelGGL(k_content, dim3(blocks), dim3(WG), 0, s, src, f, content, n, mv);
+   hipLaunchKernelGGL(k_content, dim3(blocks), dim3(WG), 0, s, src, pool, content, n, mv);
 }
 
 void launch_moments(const MomentArgs& a, hipStream_t s) {
@@ -1049,11 +1170,30 @@ void launch_transform(const MeshInfo& m, const TransformIn* fields, const double
                       mass, charge, hall_min_rhoq, ohm_gradpe, map_order, isect_out, ncells);
 }
 
-void launch_gather(const uint32_t* src, const float* f_in, float* f_out, uint32_t n, hipStream_t s) {
+void launch_gather(const uint32_t* src, const Pool& in, const Pool& out, uint32_t n, hipStream_t s) {
    if (n == 0) return;
    const size_t q = (size_t)n * 16;
-   hipLaunchKernelGGL(k_gather, dim3((uint32_t)((q + 255) / 256)), dim3(256), 0, s, src,
-                      reinterpret_cast<const float4*>(f_in), reinterpret_cast<float4*>(f_out), n);
+   if (in.storage == ST_F32)
+      hipLaunchKernelGGL(k_gather, dim3((uint32_t)((q + 255) / 256)), dim3(256), 0, s, src,
+                         static_cast<const float4*>(in.f), static_cast<float4*>(out.f), n);
+   else
+      hipLaunchKernelGGL(k_gather16, dim3((uint32_t)((q + 255) / 256)), dim3(256), 0, s, src,
+                         static_cast<const uint2*>(in.f), in.scale, static_cast<uint2*>(out.f), out.scale, n);
+}
+
+void launch_encode(const float* f, const Pool& out, uint32_t n, hipStream_t s) {
+   if (n == 0) return;
+   const uint32_t blocks = (uint32_t)(((size_t)n * 64 + WG - 1) / WG);
+   if (out.storage == ST_BF16)
+      hipLaunchKernelGGL(k_encode<ST_BF16>, dim3(blocks), dim3(WG), 0, s, f, static_cast<uint16_t*>(out.f), out.scale, n);
+   else
+      hipLaunchKernelGGL(k_encode<ST_F16>, dim3(blocks), dim3(WG), 0, s, f, static_cast<uint16_t*>(out.f), out.scale, n);
+}
+
+void launch_decode(const Pool& in, float* f, uint32_t n, hipStream_t s) {
+   if (n == 0) return;
+   const uint32_t blocks = (uint32_t)(((size_t)n * 64 + WG - 1) / WG);
+   hipLaunchKernelGGL(k_decode, dim3(blocks), dim3(WG), 0, s, in, f, n);
 }
 
 void launch_iota(uint32_t* dst, uint32_t n, hipStream_t s) {

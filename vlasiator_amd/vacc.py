@@ -15,7 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libvacc.so")
 
 PLM, PPM, PQM = 0, 1, 2
-STORE_F32 = 0
+STORE_F32, STORE_BF16, STORE_F16 = 0, 1, 2
 STATUS_WALL, STATUS_DK, STATUS_CAPACITY = 1, 2, 4
 WID3 = 64
 
