@@ -167,6 +167,16 @@ int vacc_set_fields(vacc_ctx* ctx, const vacc_cell_fields* fields);
  * adjust != 0, the sparsity adjust (adjustVelocityBlocks, :473).  rho_loss as in
  * vacc_adjust_blocks (may be NULL).  Returns the OR of the per-cell status bits. */
 int vacc_step(vacc_ctx* ctx, uint32_t map_order, int32_t use_moments, int32_t adjust, double* rho_loss);
+/* One acceleration step of several populations of the same cells (one context each, npop <=
+ * 8, e.g. protons and He2+): calculateAcceleration(mpiGrid, dt), vlasovmover.cpp:481-565, with
+ * one subcycle per population.  Populations are advanced in order; with use_moments != 0 each
+ * one's transform pivots on the plasma moments of all populations as they stand
+ * (calculateMoments_V, arch_moments.cpp:342-440: V = sum m nV / sum m n, RHOQ = sum q n).
+ * Then its three remaps and, when adjust != 0, its adjust.  rho_loss may be NULL, or hold npop
+ * pointers (each NULL or ncells doubles).  One host synchronisation; returns the OR of the
+ * per-cell status bits of all populations. */
+int vacc_accelerate_populations(vacc_ctx* const* ctxs, uint32_t npop, uint32_t map_order, int32_t use_moments,
+                                int32_t adjust, double* const* rho_loss);
 /* Synthetic initial state, the project helper used by tests and benchmarks: mirrors
  * Project::setVelocitySpace + MultiPeak::fillPhaseSpace (projects/project.cpp:239-263,
  * projects/MultiPeak/MultiPeak.cpp:127-222) followed by the initial single-cell adjust

@@ -405,3 +405,43 @@ def test_cpp_host_adapter_driver(tmp_path, per_cell):
     # the host intersections agree with the oracle transform
     assert np.allclose(isx[0], cases.oracle_intersections(w, 0), rtol=0,
                        atol=1e-12 * 4e6)
+
+
+def test_two_population_step_matches_sequential_calls():
+    """vacc_accelerate_populations (protons + He2+) against the same step written with
+    single-population calls: for each population in order, the plasma pivot
+    V = sum(m nV) / sum(m n), RHOQ = sum(q n) from both populations' current moments
+    (arch_moments.cpp:406-440), then its transform, remaps and adjust.  Bit for bit."""
+    ws = W.populations(4, ncells=4)
+    cells = [cases.build_cells(w) for w in ws]
+    fields = [[V.make_fields(c.B, c.V, c.dt) for c in w.cells] for w in ws]
+    group = [_ctx(w) for w in ws]
+    seq = [_ctx(w) for w in ws]
+    for p in range(2):
+        group[p].upload_cells(cells[p])
+        group[p].set_fields(fields[p])
+        seq[p].upload_cells(cells[p])
+    n = len(ws[0].cells)
+    for step in range(2):
+        mo = step % 3
+        _say(f"two populations step {step}")
+        V.accelerate_populations(group, mo, use_moments=True, adjust=True)
+        for p in range(2):
+            mom = [c.moments(n) for c in seq]
+            rhom = np.zeros(n)
+            nv = np.zeros((n, 3))
+            rhoq = np.zeros(n)
+            for q in range(2):
+                rhom = rhom + mom[q][:, 0] * ws[q].mass
+                nv = nv + mom[q][:, 1:] * ws[q].mass
+                rhoq = rhoq + mom[q][:, 0] * ws[q].charge
+            piv = nv / rhom[:, None]
+            fl = [V.make_fields(c.B, tuple(piv[i]), c.dt, rhoq=rhoq[i]) for i, c in enumerate(ws[p].cells)]
+            seq[p].compute_intersections(fl, mo, use_moments=False)
+            assert np.array_equal(seq[p].get_intersections(n), group[p].get_intersections(n)), f"pop {p}"
+            seq[p].accelerate(mo)
+            seq[p].adjust(n)
+        for p in range(2):
+            cases.assert_cells_equal(group[p].download_cells(), seq[p].download_cells(), f"pop {p} step {step}")
+    for c in group + seq:
+        c.close()

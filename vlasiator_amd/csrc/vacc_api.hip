@@ -69,7 +69,9 @@ struct vacc_ctx {
    double* isect = nullptr;
    TransformIn* fields = nullptr;
    double* moments = nullptr;
+   double* pivot = nullptr;          // [4*ncells] plasma pivot of a multi-population step
    double* rho_loss = nullptr;
+   hipEvent_t mom_ev = nullptr;      // moments of this population are ready (multi-population step)
    uint32_t* h_sum = nullptr;        // pinned host words: capacity flag, block total, status OR
    // event timing
    struct Rec {
@@ -151,10 +153,12 @@ static void free_cells(vacc_ctx* c) {
    if (c->isect) (void)hipFree(c->isect);
    if (c->fields) (void)hipFree(c->fields);
    if (c->moments) (void)hipFree(c->moments);
+   if (c->pivot) (void)hipFree(c->pivot);
    if (c->rho_loss) (void)hipFree(c->rho_loss);
    c->isect = nullptr;
    c->fields = nullptr;
    c->moments = nullptr;
+   c->pivot = nullptr;
    c->rho_loss = nullptr;
    c->cell_cap = 0;
 }
@@ -174,6 +178,7 @@ static int alloc_cells(vacc_ctx* c, uint32_t n) {
    HIPCHK(hipMemset(c->isect, 0, m * 12 * sizeof(double)));
    HIPCHK(hipMalloc(&c->fields, m * sizeof(TransformIn)));
    HIPCHK(hipMalloc(&c->moments, m * 4 * sizeof(double)));
+   HIPCHK(hipMalloc(&c->pivot, m * 4 * sizeof(double)));
    HIPCHK(hipMalloc(&c->rho_loss, m * sizeof(double)));
    c->cell_cap = n;
    return 0;
@@ -438,7 +443,8 @@ int vacc_create(const vacc_mesh* mesh, const vacc_species* species, const vacc_o
    }
    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
        hipMalloc(&c->flags, sizeof(uint32_t)) != hipSuccess || hipMemset(c->flags, 0, sizeof(uint32_t)) != hipSuccess ||
-       hipHostMalloc(&c->h_sum, 4 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess) {
+       hipHostMalloc(&c->h_sum, 4 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess ||
+       hipEventCreateWithFlags(&c->mom_ev, hipEventDisableTiming) != hipSuccess) {
       delete c;
       return set_err(VACC_E_HIP, "stream/flag allocation failed");
    }
@@ -459,6 +465,7 @@ int vacc_destroy(vacc_ctx* c) {
    if (c->prof_counts) (void)hipFree(c->prof_counts);
    if (c->flags) (void)hipFree(c->flags);
    if (c->h_sum) (void)hipHostFree(c->h_sum);
+   if (c->mom_ev) (void)hipEventDestroy(c->mom_ev);
    if (c->stream) (void)hipStreamDestroy(c->stream);
    delete c;
    return 0;
@@ -588,27 +595,34 @@ static void to_tin(const vacc_cell_fields& in, TransformIn& t) {
    t.dt = in.dt;
 }
 
-// Enqueue the moments (when the pivot needs them and none are resident) and the transform.
-static void enqueue_transform(vacc_ctx* c, uint32_t map_order, bool use_moments) {
+// Enqueue the moments of the current list unless they are resident.
+static void enqueue_moments(vacc_ctx* c) {
+   if (c->moments_valid) return;
    ProfScope ps(c, 3);
-   if (use_moments && !c->moments_valid) {
-      MomentArgs ma{};
-      ma.cell_off = c->off[c->lcur];
-      ma.gid = c->gid[c->lcur];
-      ma.src = c->src[c->lcur];
-      ma.pool = pool(c, c->fcur);
-      ma.out = c->moments;
-      ma.ncells = c->ncells;
-      for (int d = 0; d < 3; ++d) {
-         ma.len[d] = c->mi.len[d];
-         ma.vmin[d] = c->mi.vmin[d];
-         ma.cell[d] = c->mi.cell[d];
-      }
-      launch_moments(ma, c->stream);
-      c->moments_valid = true;
+   MomentArgs ma{};
+   ma.cell_off = c->off[c->lcur];
+   ma.gid = c->gid[c->lcur];
+   ma.src = c->src[c->lcur];
+   ma.pool = pool(c, c->fcur);
+   ma.out = c->moments;
+   ma.ncells = c->ncells;
+   for (int d = 0; d < 3; ++d) {
+      ma.len[d] = c->mi.len[d];
+      ma.vmin[d] = c->mi.vmin[d];
+      ma.cell[d] = c->mi.cell[d];
    }
-   launch_transform(c->mi, c->fields, c->moments, use_moments ? 1 : 0, c->sp.mass, c->sp.charge,
-                    c->opt.hall_minimum_rhoq, c->opt.ohm_gradpe_term, (int)map_order, c->isect, c->ncells, c->stream);
+   launch_moments(ma, c->stream);
+   c->moments_valid = true;
+}
+
+// Enqueue the transform + intersections; mode 0: pivot from the fields, 1: from this
+// population's moments (computed here when not resident), 2: from c->pivot.
+static void enqueue_transform(vacc_ctx* c, uint32_t map_order, int mode) {
+   if (mode == 1) enqueue_moments(c);
+   ProfScope ps(c, 3);
+   const double* mo = (mode == 1) ? c->moments : (mode == 2) ? c->pivot : nullptr;
+   launch_transform(c->mi, c->fields, mo, mode, c->sp.mass, c->sp.charge, c->opt.hall_minimum_rhoq,
+                    c->opt.ohm_gradpe_term, (int)map_order, c->isect, c->ncells, c->stream);
 }
 
 static int upload_fields(vacc_ctx* c, const vacc_cell_fields* fields) {
@@ -635,7 +649,7 @@ int vacc_compute_cell_intersections(vacc_ctx* c, const vacc_cell_fields* fields,
    } else if (!c->fields_valid) {
       return set_err(VACC_E_STATE, "no cell fields: pass fields or call vacc_set_fields first");
    }
-   enqueue_transform(c, map_order, use_moments != 0);
+   enqueue_transform(c, map_order, use_moments ? 1 : 0);
    HIPCHK(hipGetLastError());
    HIPCHK(hipStreamSynchronize(c->stream));
    return 0;
@@ -704,7 +718,7 @@ int vacc_step(vacc_ctx* c, uint32_t map_order, int32_t use_moments, int32_t adju
    if (!c->fields_valid) return set_err(VACC_E_STATE, "no cell fields: call vacc_set_fields first");
    int rc = reserve_for_step(c);
    if (rc) return rc;
-   enqueue_transform(c, map_order, use_moments != 0);
+   enqueue_transform(c, map_order, use_moments ? 1 : 0);
    for (int p = 0; p < 3; ++p) {
       rc = do_pass(c, pass_dim((int)map_order, p));
       if (rc) return rc;
@@ -719,26 +733,78 @@ int vacc_step(vacc_ctx* c, uint32_t map_order, int32_t use_moments, int32_t adju
    return rc;
 }
 
+int vacc_accelerate_populations(vacc_ctx* const* ctxs, uint32_t npop, uint32_t map_order, int32_t use_moments,
+                                int32_t adjust, double* const* rho_loss) {
+   if (!ctxs || npop == 0 || npop > (uint32_t)MAX_POP || map_order > 2) return set_err(VACC_E_ARG, "bad argument");
+   for (uint32_t p = 0; p < npop; ++p) {
+      if (!ctxs[p]) return set_err(VACC_E_ARG, "null context");
+      if (ctxs[p]->ncells != ctxs[0]->ncells) return set_err(VACC_E_ARG, "populations hold different cell counts");
+      if (!ctxs[p]->fields_valid && ctxs[p]->ncells)
+         return set_err(VACC_E_STATE, "no cell fields: call vacc_set_fields on every population");
+      for (uint32_t q = 0; q < p; ++q)
+         if (ctxs[q] == ctxs[p]) return set_err(VACC_E_ARG, "a context appears twice");
+   }
+   if (!ctxs[0]->ncells) return 0;
+   for (uint32_t p = 0; p < npop; ++p) {
+      const int rc = reserve_for_step(ctxs[p]);
+      if (rc) return rc;
+   }
+   // populations one after another, as calculateAcceleration's loop over popID
+   // (vlasovmover.cpp:503-565): each one's pivot sees the others' current state
+   for (uint32_t p = 0; p < npop; ++p) {
+      vacc_ctx* c = ctxs[p];
+      if (use_moments) {
+         PlasmaArgs pa{};
+         pa.npop = (int)npop;
+         pa.ncells = c->ncells;
+         pa.out = c->pivot;
+         for (uint32_t q = 0; q < npop; ++q) {
+            vacc_ctx* o = ctxs[q];
+            enqueue_moments(o);
+            if (q != p) {
+               HIPCHK(hipEventRecord(o->mom_ev, o->stream));
+               HIPCHK(hipStreamWaitEvent(c->stream, o->mom_ev, 0));
+            }
+            pa.mom[q] = o->moments;
+            pa.mass[q] = o->sp.mass;
+            pa.charge[q] = o->sp.charge;
+         }
+         {
+            ProfScope ps(c, 3);
+            launch_plasma(pa, c->stream);
+         }
+         enqueue_transform(c, map_order, 2);
+      } else {
+         enqueue_transform(c, map_order, 0);
+      }
+      for (int d = 0; d < 3; ++d) {
+         const int rc = do_pass(c, pass_dim((int)map_order, d));
+         if (rc) return rc;
+      }
+      if (adjust) {
+         const int rc = do_relist(c, c->sp.sparse_add_width_v, true);
+         if (rc) return rc;
+      }
+      HIPCHK(hipGetLastError());
+   }
+   int st = 0;
+   for (uint32_t p = 0; p < npop; ++p) {
+      const int rc = finish(ctxs[p]);
+      if (rc < 0) return rc;
+      st |= rc;
+   }
+   if (adjust && rho_loss)
+      for (uint32_t p = 0; p < npop; ++p)
+         if (rho_loss[p])
+            HIPCHK(hipMemcpy(rho_loss[p], ctxs[p]->rho_loss, ctxs[p]->ncells * sizeof(double), hipMemcpyDeviceToHost));
+   return st;
+}
+
 int vacc_moments(vacc_ctx* c, double* out) {
    if (!c || (!out && c->ncells)) return set_err(VACC_E_ARG, "null argument");
    if (!c->ncells) return 0;
-   if (!c->moments_valid) {
-      MomentArgs ma{};
-      ma.cell_off = c->off[c->lcur];
-      ma.gid = c->gid[c->lcur];
-      ma.src = c->src[c->lcur];
-      ma.pool = pool(c, c->fcur);
-      ma.out = c->moments;
-      ma.ncells = c->ncells;
-      for (int d = 0; d < 3; ++d) {
-         ma.len[d] = c->mi.len[d];
-         ma.vmin[d] = c->mi.vmin[d];
-         ma.cell[d] = c->mi.cell[d];
-      }
-      launch_moments(ma, c->stream);
-      HIPCHK(hipGetLastError());
-      c->moments_valid = true;
-   }
+   enqueue_moments(c);
+   HIPCHK(hipGetLastError());
    HIPCHK(hipMemcpyAsync(out, c->moments, (size_t)c->ncells * 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
    HIPCHK(hipStreamSynchronize(c->stream));
    return 0;

@@ -153,6 +153,18 @@ void launch_order(const OrderArgs& a, hipStream_t s);
 void launch_remap(int dim, int method, bool fast, const RemapArgs& a, uint32_t max_sets, hipStream_t s);
 void launch_adjust(const AdjustArgs& a, bool write_phase, hipStream_t s);
 void launch_moments(const MomentArgs& a, hipStream_t s);
+// plasma pivot from the moments of up to MAX_POP populations holding the same cells
+constexpr int MAX_POP = 8;
+struct PlasmaArgs {
+   const double* mom[MAX_POP];   // [4*ncells] per population: n, nVx, nVy, nVz
+   double mass[MAX_POP];
+   double charge[MAX_POP];
+   int npop;
+   uint32_t ncells;
+   double* out;                  // [4*ncells]: VX_V, VY_V, VZ_V, RHOQ_V
+};
+void launch_plasma(const PlasmaArgs& a, hipStream_t s);
+// use_moments: 0 fields V and rhoq; 1 this population's moments; 2 plasma pivot buffer
 void launch_transform(const MeshInfo& m, const TransformIn* fields, const double* moments, int use_moments,
                       double mass, double charge, double hall_min_rhoq, int ohm_gradpe, int map_order,
                       double* isect_out, uint32_t ncells, hipStream_t s);

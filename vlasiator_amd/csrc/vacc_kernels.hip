@@ -1018,17 +1018,52 @@ __global__ void k_transform(MeshInfo m, const TransformIn* fields, const double*
    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
    if (c >= ncells) return;
    TransformIn in = fields[c];
-   if (use_moments) {
+   if (use_moments == 1) {
+      // one population: VX_V = (nVx * m) / (n * m), RHOQ_V = n * q (arch_moments.cpp:419-440)
       const double* mo = moments + 4 * (size_t)c;
-      if (mo[0] > 0.0) {
-         in.V[0] = mo[1] / mo[0];
-         in.V[1] = mo[2] / mo[0];
-         in.V[2] = mo[3] / mo[0];
+      const double rhom = mo[0] * mass;
+      if (rhom != 0.0) {
+         in.V[0] = (mo[1] * mass) / rhom;
+         in.V[1] = (mo[2] * mass) / rhom;
+         in.V[2] = (mo[3] * mass) / rhom;
       }
-      in.rhoq = charge * mo[0];
+      in.rhoq = mo[0] * charge;
+   } else if (use_moments == 2) {
+      // plasma pivot of several populations (k_plasma): {VX_V, VY_V, VZ_V, RHOQ_V}
+      const double* pv = moments + 4 * (size_t)c;
+      in.V[0] = pv[0];
+      in.V[1] = pv[1];
+      in.V[2] = pv[2];
+      in.rhoq = pv[3];
    }
    const Affine fwd = acceleration_transform(in, mass, charge, hall_min_rhoq, ohm_gradpe);
    cell_intersections(m, fwd, map_order, isect_out + 12 * (size_t)c);
+}
+
+// Plasma moments of several populations (arch_moments.cpp:406-440): RHOM_V = sum n*m,
+// V_V = sum (nV*m) / RHOM_V (0 where RHOM_V = 0), RHOQ_V = sum n*q, summed in population order.
+__global__ void k_plasma(PlasmaArgs a) {
+   const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+   if (c >= a.ncells) return;
+   double rhom = 0.0, vx = 0.0, vy = 0.0, vz = 0.0, rhoq = 0.0;
+   for (int p = 0; p < a.npop; ++p) {
+      const double* mo = a.mom[p] + 4 * (size_t)c;
+      rhom += mo[0] * a.mass[p];
+      vx += mo[1] * a.mass[p];
+      vy += mo[2] * a.mass[p];
+      vz += mo[3] * a.mass[p];
+      rhoq += mo[0] * a.charge[p];
+   }
+   double* o = a.out + 4 * (size_t)c;
+   o[0] = (rhom != 0.0) ? vx / rhom : 0.0;
+   o[1] = (rhom != 0.0) ? vy / rhom : 0.0;
+   o[2] = (rhom != 0.0) ? vz / rhom : 0.0;
+   o[3] = rhoq;
+}
+
+void launch_plasma(const PlasmaArgs& a, hipStream_t s) {
+   if (a.ncells == 0) return;
+   hipLaunchKernelGGL(k_plasma, dim3((a.ncells + 255) / 256), dim3(256), 0, s, a);
 }
 
 __global__ void k_gather(const uint32_t* src, const float4* f_in, float4* f_out, uint32_t n) {

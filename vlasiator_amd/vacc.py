@@ -26,6 +26,7 @@ EXPORTS = [
     "vacc_compute_cell_intersections", "vacc_cell_intersections_host", "vacc_max_v_dt", "vacc_map_1d",
     "vacc_accelerate_cells", "vacc_adjust_blocks", "vacc_moments", "vacc_sync", "vacc_get_status",
     "vacc_init_peaks", "vacc_profile_begin", "vacc_profile_read", "vacc_set_fields", "vacc_step",
+    "vacc_accelerate_populations",
 ]
 
 
@@ -103,6 +104,7 @@ def lib():
         L.vacc_profile_read.argtypes = [vp, C.POINTER(Profile)]
         L.vacc_set_fields.argtypes = [vp, vp]
         L.vacc_step.argtypes = [vp, C.c_uint32, C.c_int32, C.c_int32, vp]
+        L.vacc_accelerate_populations.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_int32, C.c_int32, vp]
         _lib = L
     return _lib
 
@@ -295,3 +297,17 @@ class Context:
                 arr[c * max_peaks + q] = p
         bb = np.ascontiguousarray(bboxes, np.int32).reshape(nc, 6)
         _check(lib().vacc_init_peaks(self.h, nc, max_peaks, C.cast(arr, C.c_void_p), _ptr(npk), _ptr(bb)))
+
+
+def accelerate_populations(ctxs, map_order: int, use_moments: bool = True, adjust: bool = True,
+                           rho_loss=None) -> int:
+    """One step of several populations of the same cells (vacc_accelerate_populations).
+    rho_loss: optional list of float64 arrays (ncells each, or None per population)."""
+    n = len(ctxs)
+    hs = (C.c_void_p * n)(*[c.h for c in ctxs])
+    rl = None
+    if rho_loss is not None:
+        rl = (C.c_void_p * n)(*[None if a is None else a.ctypes.data for a in rho_loss])
+    return _check(lib().vacc_accelerate_populations(C.cast(hs, C.c_void_p), n, int(map_order),
+                                                    1 if use_moments else 0, 1 if adjust else 0,
+                                                    None if rl is None else C.cast(rl, C.c_void_p)))

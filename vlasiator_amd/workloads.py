@@ -179,6 +179,28 @@ def config(index: int, ncells: int | None = None, seed: int = 1000) -> Workload:
     raise ValueError(index)
 
 
+HE_MASS = 4.0 * MASS_PROTON   # He2+ as a population with 4x the proton mass-to-charge ratio
+HE_CHARGE = CHARGE             # (4 m_p, 1 e); the density fraction is HE_FRACTION
+HE_FRACTION = 0.04
+
+
+def populations(index: int, ncells: int | None = None, seed: int = 1000) -> list:
+    """The populations of a configuration, one Workload each over the same cells.  Config 4
+    has protons plus a He2+ population at 4 % density with 4x the proton mass-to-charge
+    ratio; the He2+ peak has the protons' temperature and bulk velocity, and both
+    populations share each cell's field and time step (5 degrees of proton gyration)."""
+    w = config(index, ncells, seed)
+    if index != 4:
+        return [w]
+    he = Workload(w.name.replace("two-species", "he2"), w.grid_length, w.vmin, w.vmax, w.method, w.min_value,
+                  w.add_width, w.steps, w.adjust_every_step, mass=HE_MASS, charge=HE_CHARGE,
+                  rotation_deg=w.rotation_deg * MASS_PROTON / HE_MASS)
+    for c in w.cells:
+        p = c.peaks[0]
+        he.cells.append(CellSpec([PeakSpec(0, p.n * HE_FRACTION, p.V, p.T)], c.B, c.V, c.dt))
+    return [w, he]
+
+
 def to_vacc_peaks(cell: CellSpec):
     from . import vacc as V
     out = []
