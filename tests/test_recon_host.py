@@ -87,3 +87,33 @@ def test_fast_reconstruction_close_to_oracle(lib, method):
         total += n
         flipped += int((~close).sum())
     assert flipped <= 0.02 * total
+
+
+def test_div3_equals_ieee_division_for_every_float(lib):
+    """vacc::div3 (FMA-corrected product) against x / 3 over all 2^32 bit patterns."""
+    import concurrent.futures as cf
+    lib.div3_mismatches.restype = C.c_longlong
+    lib.div3_mismatches.argtypes = [C.c_ulonglong, C.c_ulonglong]
+    parts = 16
+    step = (1 << 32) // parts
+    with cf.ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
+        bad = sum(ex.map(lambda k: lib.div3_mismatches(k * step, (k + 1) * step), range(parts)))
+    assert bad == 0
+
+
+def test_lagrangian_index_equals_truncated_quotient(lib):
+    """vacc::lagr_index against (int)(x / dk): random velocities and spacings, plus quotients
+    placed within a few ulp of integers (where the product and the quotient can straddle)."""
+    lib.lagr_mismatches.restype = C.c_longlong
+    lib.lagr_mismatches.argtypes = [C.c_void_p, C.c_void_p, C.c_longlong]
+    rng = np.random.default_rng(17)
+    n = 2_000_000
+    dk = (2e4 * rng.uniform(0.5, 2.0, n)).astype(np.float32)
+    x = rng.uniform(-8e6, 8e6, n).astype(np.float32)
+    # adversarial: x = (k + tiny) * dk for integers k, nudged by a few ulp either way
+    k = rng.integers(-400, 400, n).astype(np.float32)
+    xa = (k * dk).astype(np.float32)
+    xa = (xa.view(np.int32) + rng.integers(-4, 5, n).astype(np.int32)).view(np.float32)
+    for xs in (x, xa):
+        xs = np.ascontiguousarray(xs)
+        assert lib.lagr_mismatches(xs.ctypes.data, dk.ctypes.data, n) == 0

@@ -667,6 +667,7 @@ __global__ __launch_bounds__(WG) void k_remap(RemapArgs a, double i_dv) {
          float v_r = (float)(uint32_t)(WID * ks) * a.dv + a.vmin;
          const float lvr = (v_r - Imin) / dk;
          int gkr = (int)lvr;
+         const float rdk = 1.0f / dk;
          // lanes of min/max Lagrangian index within the 8-lane vector (cpu_acc_map.cpp:515-529)
          int minL = 0, maxL = 0;
          {
@@ -702,7 +703,7 @@ __global__ __launch_bounds__(WG) void k_remap(RemapArgs a, double i_dv) {
                const float v_l = v_r;
                v_r = v_r + a.dv;
                const int gkl = gkr;
-               gkr = (int)((v_r - Imin) / dk);
+               gkr = lagr_index(v_r - Imin, dk, rdk);
                const int mnG = max(__shfl(gkl, minL, 8), lo_gk);
                const int mxG = min(__shfl(gkr, maxL, 8), hi_gk);
                const int flushLim = min(mnG, __shfl_xor(mnG, 8, 16));
@@ -725,8 +726,10 @@ __global__ __launch_bounds__(WG) void k_remap(RemapArgs a, double i_dv) {
                float tdr = 0.0f;
                for (int gk = mnG; gk <= mxG; ++gk) {
                   const float y = (float)(gk + 1) * dk + Imin;
-                  const float mx = (y > v_l) ? y : v_l;
-                  const float mn = (mx < v_r) ? mx : v_r;
+                  // (y > v_l ? y : v_l), then (mx < v_r ? mx : v_r) (cpu_acc_map.cpp:580-584); no
+                  // operand is NaN or a negative zero here, so max/min give the same value
+                  const float mx = fmaxf(y, v_l);
+                  const float mn = fminf(mx, v_r);
                   const float tdl = tdr;
                   if (FAST) {
                      tdr = integral_fast<METHOD>((mn - v_l) * i_dv_f, cf);
@@ -763,6 +766,52 @@ __global__ __launch_bounds__(WG) void k_remap(RemapArgs a, double i_dv) {
 // (:227-267); the write phase writes the kept list in ascending GID order.
 // ---------------------------------------------------------------------------
 
+// Dilate an LDS bitmap (rows of W words over an Rx x Ry x Rz box, row = z*Ry + y) by one
+// block along x, y and z in place: the union of the 3x3x3 neighbourhoods of the set bits,
+// clipped to the box.  Applied w times this is the (2w+1)^3 neighbourhood.
+__device__ inline void dilate_box1(uint32_t* bm, int W, int Rx, int Ry, int Rz) {
+   const int t = threadIdx.x;
+   const int rows = Ry * Rz;
+   const uint32_t last = ((Rx & 31) == 0) ? 0xFFFFFFFFu : ((1u << (Rx & 31)) - 1u);
+   for (int r = t; r < rows; r += WG) {            // x: one thread per row, words in order
+      uint32_t* row = bm + r * W;
+      uint32_t prev = 0u, cur = row[0];
+      for (int q = 0; q < W; ++q) {
+         const uint32_t next = (q + 1 < W) ? row[q + 1] : 0u;
+         uint32_t d = cur | (cur << 1) | (cur >> 1) | (prev >> 31) | (next << 31);
+         if (q == W - 1) d &= last;
+         row[q] = d;
+         prev = cur;
+         cur = next;
+      }
+   }
+   __syncthreads();
+   for (int i = t; i < Rz * W; i += WG) {           // y: one thread per (z, word) column
+      const int z = i / W, q = i - z * W;
+      uint32_t* col = bm + (size_t)z * Ry * W + q;
+      uint32_t prev = 0u, cur = col[0];
+      for (int y = 0; y < Ry; ++y) {
+         const uint32_t next = (y + 1 < Ry) ? col[(y + 1) * W] : 0u;
+         col[y * W] = prev | cur | next;
+         prev = cur;
+         cur = next;
+      }
+   }
+   __syncthreads();
+   for (int i = t; i < Ry * W; i += WG) {           // z: one thread per (y, word) column
+      uint32_t* col = bm + i;
+      const int stride = Ry * W;
+      uint32_t prev = 0u, cur = col[0];
+      for (int z = 0; z < Rz; ++z) {
+         const uint32_t next = (z + 1 < Rz) ? col[(z + 1) * stride] : 0u;
+         col[z * stride] = prev | cur | next;
+         prev = cur;
+         cur = next;
+      }
+   }
+   __syncthreads();
+}
+
 template <bool WRITE>
 __global__ __launch_bounds__(WG) void k_adjust(AdjustArgs a) {
    __shared__ uint32_t bm[BM_WORDS];
@@ -771,6 +820,8 @@ __global__ __launch_bounds__(WG) void k_adjust(AdjustArgs a) {
    __shared__ uint32_t sh[4];
    __shared__ uint32_t carry;
    __shared__ double sloss;
+   __shared__ uint32_t nrem;
+   __shared__ uint32_t rem[WRITE ? 1 : WG];   // removed blocks among WG list entries (positions)
    if (WRITE && (*a.flags & VACC_STATUS_CAPACITY)) return;
    const int t = threadIdx.x;
    const uint32_t c = blockIdx.x;
@@ -792,6 +843,7 @@ __global__ __launch_bounds__(WG) void k_adjust(AdjustArgs a) {
    if (t == 0) {
       carry = 0;
       sloss = 0.0;
+      nrem = 0;
    }
    __syncthreads();
    const uint32_t nx = a.len[0], ny = a.len[1], nz = a.len[2];
@@ -814,11 +866,24 @@ __global__ __launch_bounds__(WG) void k_adjust(AdjustArgs a) {
    const int W = (Rx + 31) >> 5;
    const int rows = Ry * Rz;
    const int rpc = BM_WORDS / W;
+   // one chunk holds the whole box: mark content blocks, then dilate separably in LDS
+   const bool separable = (w >= 0) && (rows <= rpc);
    double myloss = 0.0;
    for (int r0 = 0; r0 < rows; r0 += rpc) {
       const int nr = min(rpc, rows - r0);
       for (int i = t; i < nr * W; i += WG) bm[i] = 0;
       __syncthreads();
+      if (separable) {
+         for (uint32_t i = t; i < n; i += WG) {
+            if (!a.content[off + i]) continue;
+            const uint32_t g = a.gid[off + i];
+            const int x = (int)(g % nx), y = (int)((g / nx) % ny), z = (int)(g / (nx * ny));
+            const int kx = x - mnx;
+            atomicOr(&bm[((z - mnz) * Ry + (y - mny)) * W + (kx >> 5)], 1u << (kx & 31));
+         }
+         __syncthreads();
+         for (int k = 0; k < w; ++k) dilate_box1(bm, W, Rx, Ry, Rz);
+      } else
       for (uint32_t i = t; i < n; i += WG) {
          const uint32_t g = a.gid[off + i];
          const int x = (int)(g % nx), y = (int)((g / nx) % ny), z = (int)(g / (nx * ny));
@@ -891,20 +956,33 @@ __global__ __launch_bounds__(WG) void k_adjust(AdjustArgs a) {
             }
          }
       } else {
-         for (uint32_t i = t; i < n; i += WG) {
-            if (w < 0 || a.content[off + i]) continue;
-            const uint32_t g = a.gid[off + i];
-            const int x = (int)(g % nx), y = (int)((g / nx) % ny), z = (int)(g / (nx * ny));
-            const int row = (z - mnz) * Ry + (y - mny) - r0;
-            if (row < 0 || row >= nr) continue;
-            const int kx = x - mnx;
-            if (bm[row * W + (kx >> 5)] & (1u << (kx & 31))) continue;
-            // removed block: RHOLOSSADJUST += DV3 * sum(f)
-            const uint32_t s = a.src[off + i];
-            double sum = 0.0;
-            if (s != INVALID)
-               for (int e = 0; e < WID3; ++e) sum += pool_value(a.pool, s, e);
-            myloss += a.dv3 * sum;
+         // removed blocks: no content and not kept; their mass goes to RHOLOSSADJUST
+         // (spatial_cell_cpu.cpp:227-267), summed one wave per block
+         for (uint32_t i0 = 0; i0 < n; i0 += WG) {
+            const uint32_t i = i0 + t;
+            bool removed = false;
+            if (i < n && w >= 0 && !a.content[off + i]) {
+               const uint32_t g = a.gid[off + i];
+               const int x = (int)(g % nx), y = (int)((g / nx) % ny), z = (int)(g / (nx * ny));
+               const int row = (z - mnz) * Ry + (y - mny) - r0;
+               if (row >= 0 && row < nr) {
+                  const int kx = x - mnx;
+                  removed = !(bm[row * W + (kx >> 5)] & (1u << (kx & 31)));
+               }
+            }
+            if (removed) rem[atomicAdd(&nrem, 1u)] = i;   // at most WG entries
+            __syncthreads();
+            const uint32_t nk = nrem;
+            const int lane = t & 63;
+            for (uint32_t k = t >> 6; k < nk; k += WG / 64) {   // whole waves: uniform trip count
+               const uint32_t sidx = a.src[off + rem[k]];
+               double v = (sidx != INVALID) ? (double)pool_value(a.pool, sidx, lane) : 0.0;
+               for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+               if (lane == 0) myloss += a.dv3 * v;
+            }
+            __syncthreads();
+            if (t == 0) nrem = 0;
+            __syncthreads();
          }
       }
       __syncthreads();

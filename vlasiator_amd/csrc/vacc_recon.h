@@ -71,6 +71,28 @@ VACC_HD float superbee_ab(float a, float b) {
    return mod_max(mod_min(a, 2.0f * b), mod_min(2.0f * a, b));
 }
 
+// x / 3 with IEEE rounding from a multiply by RN(1/3) and one FMA residual correction
+// (checked against the division for every finite FP32 x; a zero residual returns the
+// product, which also keeps the sign of a zero x).
+VACC_HD float div3(float x) {
+   const float r3 = 1.0f / 3.0f;
+   const float q0 = x * r3;
+   const float e = fmaf(-q0, 3.0f, x);
+   return (e == 0.0f) ? q0 : fmaf(e, r3, q0);
+}
+
+// (int)(x / dk), the truncated IEEE quotient (Lagrangian cell index, cpu_acc_map.cpp:505-508,
+// :564), from q = x * rdk with rdk = RN(1/dk).  q is within 2^-22.4 |q| of the IEEE quotient,
+// so when no integer lies within 2^-21 |q| of q both truncate alike; otherwise divide.
+VACC_HD int lagr_index(float x, float dk, float rdk) {
+   const float q = x * rdk;
+   const float t = truncf(q);
+   const float fr = fabsf(q - t);
+   const float d = fminf(fr, 1.0f - fr);
+   if (d > fabsf(q) * 0x1p-21f) return (int)t;
+   return (int)(x / dk);
+}
+
 template <int METHOD>
 struct Recon;
 
@@ -194,7 +216,7 @@ struct Recon<VACC_PQM> {
       const float ssg = psl + psr;
       const float c1 = b0;
       const float c2 = 0.5f * b1;
-      const float c3 = b2 / 3.0f;
+      const float c3 = div3(b2);
       const float r1s = (root1 >= 0.0f && root1 <= 1.0f) ? (fdl + root1 * (c1 + root1 * (c2 + root1 * c3))) : ssg;
       const float r2s = (root2 >= 0.0f && root2 <= 1.0f) ? (fdl + root2 * (c1 + root2 * (c2 + root2 * c3))) : ssg;
       if (r1s * ssg < 0.0f || r2s * ssg < 0.0f) {
