@@ -116,3 +116,25 @@ def test_moments_and_init_16bit(store, st):
     want = np.array([O.moments(m, g, f) for g, f in got])
     assert np.allclose(mom[:, 0], want[:, 0], rtol=1e-12, atol=0)
     ctx.close()
+
+
+@pytest.mark.parametrize("store", [V.STORE_F32, V.STORE_BF16])
+def test_state_beyond_2_to_32_thread_indices(store):
+    """About 9e7 blocks: kernels with one wave per block then index past 2^32 threads
+    (64 x 67 108 864).  The initial state's density must equal the sum of the sampled
+    peaks (FP32 store as the reference for BF16), and an adjust right after the initial
+    condition (content flags computed from f for every block) must keep every block."""
+    w = W.populations(4, ncells=24000)[0]
+    _say(f"storage {store}: large initial state")
+    ctx = _ctx(w, O.PQM, store)
+    ctx.init_peaks([W.to_vacc_peaks(c) for c in w.cells], W.boxes(w))
+    n0 = ctx.total_blocks()
+    assert n0 > (1 << 32) // 64
+    dens = ctx.moments(len(w.cells))[:, 0]
+    want = np.array([sum(p.n for p in c.peaks) for c in w.cells])
+    # sampled Maxwellians on the mesh carry their density to well within 1e-3
+    assert np.all(np.abs(dens - want) <= 1e-3 * want)
+    ctx.adjust(len(w.cells))
+    assert ctx.total_blocks() == n0
+    assert np.allclose(ctx.moments(len(w.cells))[:, 0], dens, rtol=1e-12, atol=0)
+    ctx.close()

@@ -1000,15 +1000,18 @@ __global__ __launch_bounds__(WG) void k_adjust(AdjustArgs a) {
 }
 
 // content flag of each block of a list, from f (spatial_cell_cpu.cpp:290-316)
+// One wave per block, grid-stride: a dispatch holds fewer than 2^32 work-items, so the grid is
+// capped (launch_waves) and each wave walks blocks wave, wave + nwaves, ...
 __global__ __launch_bounds__(WG) void k_content(const uint32_t* src, Pool pl, uint8_t* content,
                                                 uint32_t n, double mv) {
-   const uint32_t wave = (blockIdx.x * WG + threadIdx.x) >> 6;
    const int lane = threadIdx.x & 63;
-   if (wave >= n) return;
-   const uint32_t s = src[wave];
-   const float x = (s == INVALID) ? 0.0f : pool_value(pl, s, lane);
-   const unsigned long long b = __ballot((double)x >= mv);
-   if (lane == 0) content[wave] = (uint8_t)(b != 0ull);
+   const uint32_t nw = gridDim.x * (WG / 64);
+   for (uint32_t wave = blockIdx.x * (WG / 64) + (threadIdx.x >> 6); wave < n; wave += nw) {
+      const uint32_t s = src[wave];
+      const float x = (s == INVALID) ? 0.0f : pool_value(pl, s, lane);
+      const unsigned long long b = __ballot((double)x >= mv);
+      if (lane == 0) content[wave] = (uint8_t)(b != 0ull);
+   }
 }
 
 // ---------------------------------------------------------------------------
@@ -1144,42 +1147,48 @@ void launch_plasma(const PlasmaArgs& a, hipStream_t s) {
    hipLaunchKernelGGL(k_plasma, dim3((a.ncells + 255) / 256), dim3(256), 0, s, a);
 }
 
+// 16 lanes per block, grid-stride (capped grid, see launch_gather)
 __global__ void k_gather(const uint32_t* src, const float4* f_in, float4* f_out, uint32_t n) {
-   const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-   if (q >= (size_t)n * 16) return;
-   const uint32_t b = (uint32_t)(q >> 4);
-   const uint32_t s = src[b];
-   f_out[q] = (s == INVALID) ? make_float4(0.f, 0.f, 0.f, 0.f) : f_in[(size_t)s * 16 + (q & 15)];
+   const size_t stride = (size_t)gridDim.x * blockDim.x;
+   for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < (size_t)n * 16; q += stride) {
+      const uint32_t b = (uint32_t)(q >> 4);
+      const uint32_t s = src[b];
+      f_out[q] = (s == INVALID) ? make_float4(0.f, 0.f, 0.f, 0.f) : f_in[(size_t)s * 16 + (q & 15)];
+   }
 }
 
 // 16-bit pools: 16 lanes per block, 8 bytes of codes each, lane 0 the scale.
 __global__ void k_gather16(const uint32_t* src, const uint2* q_in, const float* s_in, uint2* q_out, float* s_out,
                            uint32_t n) {
-   const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-   if (q >= (size_t)n * 16) return;
-   const uint32_t b = (uint32_t)(q >> 4);
-   const uint32_t s = src[b];
-   q_out[q] = (s == INVALID) ? make_uint2(0u, 0u) : q_in[(size_t)s * 16 + (q & 15)];
-   if ((q & 15) == 0) s_out[b] = (s == INVALID) ? 0.0f : s_in[s];
+   const size_t stride = (size_t)gridDim.x * blockDim.x;
+   for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < (size_t)n * 16; q += stride) {
+      const uint32_t b = (uint32_t)(q >> 4);
+      const uint32_t s = src[b];
+      q_out[q] = (s == INVALID) ? make_uint2(0u, 0u) : q_in[(size_t)s * 16 + (q & 15)];
+      if ((q & 15) == 0) s_out[b] = (s == INVALID) ? 0.0f : s_in[s];
+   }
 }
 
 // FP32 blocks -> 16-bit pool, one wave per block.
 template <int ST>
 __global__ __launch_bounds__(WG) void k_encode(const float* f, uint16_t* q, float* scale, uint32_t n) {
-   const uint32_t b = (blockIdx.x * WG + threadIdx.x) >> 6;
    const int e = threadIdx.x & 63;
-   if (b >= n) return;   // whole waves exit together
-   uint32_t code;
-   const float sc = encode_wave64<ST>(f[(size_t)b * WID3 + e], code);
-   q[(size_t)b * WID3 + e] = (uint16_t)code;
-   if (e == 0) scale[b] = sc;
+   const uint32_t nw = gridDim.x * (WG / 64);
+   // whole waves take the same blocks, so the wave-wide shuffles see 64 active lanes
+   for (uint32_t b = blockIdx.x * (WG / 64) + (threadIdx.x >> 6); b < n; b += nw) {
+      uint32_t code;
+      const float sc = encode_wave64<ST>(f[(size_t)b * WID3 + e], code);
+      q[(size_t)b * WID3 + e] = (uint16_t)code;
+      if (e == 0) scale[b] = sc;
+   }
 }
 
-// 16-bit pool -> FP32 blocks.
+// 16-bit pool -> FP32 blocks, one wave per block, grid-stride.
 __global__ __launch_bounds__(WG) void k_decode(Pool pl, float* f, uint32_t n) {
-   const size_t i = (size_t)blockIdx.x * WG + threadIdx.x;
-   if (i >= (size_t)n * WID3) return;
-   f[i] = pool_value(pl, (uint32_t)(i / WID3), (int)(i % WID3));
+   const int e = threadIdx.x & 63;
+   const uint32_t nw = gridDim.x * (WG / 64);
+   for (uint32_t b = blockIdx.x * (WG / 64) + (threadIdx.x >> 6); b < n; b += nw)
+      f[(size_t)b * WID3 + e] = pool_value(pl, b, e);
 }
 
 __global__ void k_iota(uint32_t* dst, uint32_t n) {
@@ -1264,9 +1273,15 @@ void launch_adjust(const AdjustArgs& a, bool write_phase, hipStream_t s) {
    else hipLaunchKernelGGL(k_adjust<false>, dim3(a.ncells), dim3(WG), 0, s, a);
 }
 
+// Workgroups for one wave per block, capped so that a dispatch stays far below 2^32 work-items.
+static uint32_t launch_waves(uint32_t n) {
+   const uint64_t want = ((uint64_t)n + (WG / 64) - 1) / (WG / 64);
+   return (uint32_t)std::min<uint64_t>(want, 1u << 20);
+}
+
 void launch_content(const uint32_t* src, const Pool& pool, uint8_t* content, uint32_t n, double mv, hipStream_t s) {
    if (n == 0) return;
-   const uint32_t blocks = (uint32_t)(((size_t)n * 64 + WG - 1) / WG);
+   const uint32_t blocks = launch_waves(n);
    hipLaunchKernelGGL(k_content, dim3(blocks), dim3(WG), 0, s, src, pool, content, n, mv);
 }
 
@@ -1285,18 +1300,18 @@ void launch_transform(const MeshInfo& m, const TransformIn* fields, const double
 
 void launch_gather(const uint32_t* src, const Pool& in, const Pool& out, uint32_t n, hipStream_t s) {
    if (n == 0) return;
-   const size_t q = (size_t)n * 16;
+   const uint32_t grid = (uint32_t)std::min<uint64_t>(((uint64_t)n * 16 + 255) / 256, 1u << 20);
    if (in.storage == ST_F32)
-      hipLaunchKernelGGL(k_gather, dim3((uint32_t)((q + 255) / 256)), dim3(256), 0, s, src,
+      hipLaunchKernelGGL(k_gather, dim3(grid), dim3(256), 0, s, src,
                          static_cast<const float4*>(in.f), static_cast<float4*>(out.f), n);
    else
-      hipLaunchKernelGGL(k_gather16, dim3((uint32_t)((q + 255) / 256)), dim3(256), 0, s, src,
+      hipLaunchKernelGGL(k_gather16, dim3(grid), dim3(256), 0, s, src,
                          static_cast<const uint2*>(in.f), in.scale, static_cast<uint2*>(out.f), out.scale, n);
 }
 
 void launch_encode(const float* f, const Pool& out, uint32_t n, hipStream_t s) {
    if (n == 0) return;
-   const uint32_t blocks = (uint32_t)(((size_t)n * 64 + WG - 1) / WG);
+   const uint32_t blocks = launch_waves(n);
    if (out.storage == ST_BF16)
       hipLaunchKernelGGL(k_encode<ST_BF16>, dim3(blocks), dim3(WG), 0, s, f, static_cast<uint16_t*>(out.f), out.scale, n);
    else
@@ -1305,7 +1320,7 @@ void launch_encode(const float* f, const Pool& out, uint32_t n, hipStream_t s) {
 
 void launch_decode(const Pool& in, float* f, uint32_t n, hipStream_t s) {
    if (n == 0) return;
-   const uint32_t blocks = (uint32_t)(((size_t)n * 64 + WG - 1) / WG);
+   const uint32_t blocks = launch_waves(n);
    hipLaunchKernelGGL(k_decode, dim3(blocks), dim3(WG), 0, s, in, f, n);
 }
 
