@@ -152,6 +152,17 @@ int vacc_accelerate_cells(vacc_ctx* ctx, uint32_t map_order);
 /* Single-cell sparsity adjust of every cell (doDeleteEmptyBlocks = true).  rho_loss may be
  * NULL; otherwise ncells FP64 values of the removed mass (RHOLOSSADJUST increments). */
 int vacc_adjust_blocks(vacc_ctx* ctx, double* rho_loss);
+/* Sparsity adjust with spatial neighbours: SpatialCell::adjust_velocity_blocks(spatial_neighbors,
+ * popID, true), spatial_cells/spatial_cell_cpu.cpp:179-273, as driven by adjustVelocityBlocks
+ * (grid.cpp:775-814).  Besides the cell's content blocks and their velocity neighbours, every
+ * block with content in a spatial neighbour is kept (or added, zero) as it is, without velocity
+ * dilation (:212-222).  Neighbours are given as cells of this context, nbr_off[ncells+1] into
+ * nbr[] (their content as of the current f), and/or as caller-supplied content GIDs,
+ * extra_off[ncells+1] into extra_gids[] (e.g. remote neighbours).  Either pair may be NULL.
+ * rho_loss as in vacc_adjust_blocks.  The block storage grows when the neighbours add more
+ * blocks than the reserve holds. */
+int vacc_adjust_blocks_spatial(vacc_ctx* ctx, const uint32_t* nbr_off, const uint32_t* nbr, const uint64_t* extra_off,
+                               const uint32_t* extra_gids, double* rho_loss);
 /* FP64 moments per cell: out[4*c+0] = n, [1..3] = n*V (arch_moments.h:83-108).  They are
  * kept resident until f changes, so a following use_moments transform does not recompute
  * them. */

@@ -76,6 +76,9 @@ def lib():
                                 C.c_uint32, C.POINTER(C.c_uint32), _u32p, _f32p, C.POINTER(C.c_int)]
         L.vo_adjust.argtypes = [C.POINTER(Mesh), C.c_uint32, _u32p, _f32p, C.c_double, C.c_int,
                                 C.c_uint32, C.POINTER(C.c_uint32), _u32p, _f32p, C.POINTER(C.c_double)]
+        L.vo_adjust_spatial.argtypes = [C.POINTER(Mesh), C.c_uint32, _u32p, _f32p, C.c_double, C.c_int,
+                                        C.c_uint32, _u32p, C.c_uint32, C.POINTER(C.c_uint32), _u32p, _f32p,
+                                        C.POINTER(C.c_double)]
         L.vo_content_flags.argtypes = [C.c_uint32, _f32p, C.c_double, _u8p]
         L.vo_moments.argtypes = [C.POINTER(Mesh), C.c_uint32, _u32p, _f32p, _f64p]
         L.vo_transform.argtypes = [C.POINTER(Fields), C.c_double, C.c_double, C.c_double, C.c_double,
@@ -231,6 +234,26 @@ def adjust(mesh: Mesh, gids, f, min_value: float, add_width: int = 1, fp64: bool
         loss = C.c_double()
         rc = L.vo_adjust(C.byref(mesh), g.size, g, ff, float(min_value), int(add_width), cap,
                              C.byref(nout), go, fo, C.byref(loss))
+        if rc == 0:
+            n = nout.value
+            return go[:n].copy(), fo[:n * WID3].reshape(n, WID3).copy(), loss.value
+        cap = nout.value + 16
+
+
+def adjust_spatial(mesh: Mesh, gids, f, min_value: float, add_width: int, extra_gids):
+    """Adjust with the spatial neighbours' content GIDs kept as they are."""
+    g = np.ascontiguousarray(gids, dtype=np.uint32)
+    ff = np.ascontiguousarray(f, dtype=np.float32).reshape(-1)
+    ex = np.ascontiguousarray(extra_gids, dtype=np.uint32)
+    exn = ex if ex.size else np.zeros(1, np.uint32)
+    cap = 2 * g.size + ex.size + 4096
+    while True:
+        go = np.zeros(cap, np.uint32)
+        fo = np.zeros(cap * WID3, np.float32)
+        nout = C.c_uint32()
+        loss = C.c_double()
+        rc = lib().vo_adjust_spatial(C.byref(mesh), g.size, g, ff, float(min_value), int(add_width), ex.size, exn,
+                                     cap, C.byref(nout), go, fo, C.byref(loss))
         if rc == 0:
             n = nout.value
             return go[:n].copy(), fo[:n * WID3].reshape(n, WID3).copy(), loss.value

@@ -668,9 +668,9 @@ static int cmp_u32(const void* a, const void* b) {
    return (x < y) ? -1 : (x > y) ? 1 : 0;
 }
 
-int vo_adjust(const vo_mesh* m, uint32_t n_in, const uint32_t* gids, const float* f,
-              double min_value, int add_width,
-              uint32_t cap, uint32_t* n_out, uint32_t* gids_out, float* f_out, double* rho_loss) {
+int vo_adjust_spatial(const vo_mesh* m, uint32_t n_in, const uint32_t* gids, const float* f,
+                      double min_value, int add_width, uint32_t n_extra, const uint32_t* extra_gids,
+                      uint32_t cap, uint32_t* n_out, uint32_t* gids_out, float* f_out, double* rho_loss) {
    const uint32_t nx = m->len[0], ny = m->len[1], nz = m->len[2];
    const size_t ntot = (size_t)nx * ny * nz;
    /* dense maps over the mesh: keep flag and LID+1 of existing blocks */
@@ -694,6 +694,9 @@ int vo_adjust(const vo_mesh* m, uint32_t n_in, const uint32_t* gids, const float
                keep[x + y * nx + (size_t)z * nx * ny] = 1;
             }
    }
+   /* :212-222 content blocks of the spatial neighbours, kept as they are (no dilation) */
+   for (uint32_t i = 0; i < n_extra; ++i)
+      if ((size_t)extra_gids[i] < ntot) keep[extra_gids[i]] = 1;
    /* :227-267 remove no-content blocks not kept, iterating the no-content list in reverse */
    const double DV3 = vo_cell_size(m, 0) * vo_cell_size(m, 1) * vo_cell_size(m, 2);
    double loss = 0.0;
@@ -724,6 +727,12 @@ int vo_adjust(const vo_mesh* m, uint32_t n_in, const uint32_t* gids, const float
    free(keep); free(lid1); free(content);
    (void)cmp_u32;
    return 0;
+}
+
+int vo_adjust(const vo_mesh* m, uint32_t n_in, const uint32_t* gids, const float* f,
+              double min_value, int add_width,
+              uint32_t cap, uint32_t* n_out, uint32_t* gids_out, float* f_out, double* rho_loss) {
+   return vo_adjust_spatial(m, n_in, gids, f, min_value, add_width, 0, NULL, cap, n_out, gids_out, f_out, rho_loss);
 }
 
 /* ------------------------------------------------------------------------ */

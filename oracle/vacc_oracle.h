@@ -79,6 +79,12 @@ int  vo_accelerate_cell(const vo_mesh* m, vo_cell* c, const double intersections
 int vo_adjust(const vo_mesh* m, uint32_t n_in, const uint32_t* gids, const float* f,
               double min_value, int add_width,
               uint32_t cap, uint32_t* n_out, uint32_t* gids_out, float* f_out, double* rho_loss);
+/* The same with the spatial neighbours' content blocks: extra_gids (n_extra, any order, may
+ * repeat) are kept and added as they are, without velocity dilation (:212-222); this is
+ * SpatialCell::adjust_velocity_blocks(spatial_neighbors, popID, true), :179-273. */
+int vo_adjust_spatial(const vo_mesh* m, uint32_t n_in, const uint32_t* gids, const float* f,
+                      double min_value, int add_width, uint32_t n_extra, const uint32_t* extra_gids,
+                      uint32_t cap, uint32_t* n_out, uint32_t* gids_out, float* f_out, double* rho_loss);
 /* content flag per block: any f >= min_value (spatial_cell_cpu.cpp:290-316) */
 void vo_content_flags(uint32_t n, const float* f, double min_value, uint8_t* flags);
 

@@ -445,3 +445,45 @@ def test_two_population_step_matches_sequential_calls():
             cases.assert_cells_equal(group[p].download_cells(), seq[p].download_cells(), f"pop {p} step {step}")
     for c in group + seq:
         c.close()
+
+
+@pytest.mark.parametrize("corners", [False, True])
+def test_spatial_neighbour_adjust_bit_exact(cfg0, corners):
+    """vacc_adjust_blocks_spatial after a remap: ring neighbours (i-1, i+1) of the context plus
+    caller GIDs, against the oracle's adjust with the union of the neighbours' content GIDs
+    (spatial_cell_cpu.cpp:212-222).  One cell starts empty and only receives neighbour blocks.
+    corners=True adds GIDs at opposite mesh corners (and one GID outside the mesh, ignored),
+    so the cell's box spans the mesh and the LDS bitmap is processed in chunks."""
+    w, cells = cfg0
+    cells = list(cells)
+    cells[3] = (np.zeros(0, np.uint32), np.zeros((0, 64), np.float32))
+    n = len(cells)
+    m = cases.oracle_mesh(w)
+    ntot = int(np.prod(w.grid_length))
+    isx = cases.oracle_intersections(w, 1)
+    ctx = _ctx(w, O.PPM)
+    ctx.upload_cells(cells)
+    ctx.set_intersections(isx)
+    ctx.accelerate(1)
+    state = [O.accelerate_cell(m, g, f, isx[i], 1, O.PPM, w.min_value)[:2] if g.size else (g, f)
+             for i, (g, f) in enumerate(cells)]
+    nbr_off = np.arange(0, 2 * n + 1, 2, dtype=np.uint32)
+    nbr = np.array([[(i - 1) % n, (i + 1) % n] for i in range(n)], np.uint32).reshape(-1)
+    rng = np.random.default_rng(5)
+    extra = [rng.choice(ntot, 5).astype(np.uint32) for _ in range(n)]
+    if corners:
+        extra = [np.concatenate([e, np.array([0, ntot - 1, ntot], np.uint32)]) for e in extra]
+    extra_off = np.zeros(n + 1, np.uint64)
+    extra_off[1:] = np.cumsum([e.size for e in extra])
+    _say(f"spatial adjust corners={corners}")
+    loss = ctx.adjust_spatial(n, nbr_off, nbr, extra_off, np.concatenate(extra))
+    want, want_loss = [], []
+    for i, (g, f) in enumerate(state):
+        nb = [state[(i - 1) % n], state[(i + 1) % n]]
+        ex = [gg[O.content_flags(ff, w.min_value).astype(bool)] for gg, ff in nb] + [extra[i]]
+        g2, f2, l2 = O.adjust_spatial(m, g, f, w.min_value, w.add_width, np.concatenate(ex))
+        want.append((g2, f2))
+        want_loss.append(l2)
+    cases.assert_cells_equal(ctx.download_cells(), want, f"spatial adjust corners={corners}")
+    assert np.allclose(loss, want_loss, rtol=1e-12, atol=0)
+    ctx.close()

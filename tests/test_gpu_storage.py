@@ -123,7 +123,8 @@ def test_state_beyond_2_to_32_thread_indices(store):
     """About 9e7 blocks: kernels with one wave per block then index past 2^32 threads
     (64 x 67 108 864).  The initial state's density must equal the sum of the sampled
     peaks (FP32 store as the reference for BF16), and an adjust right after the initial
-    condition (content flags computed from f for every block) must keep every block."""
+    condition (content flags computed from f for every block) must keep every block (FP32),
+    or all but the blocks whose content sat within a code step of the threshold (BF16)."""
     w = W.populations(4, ncells=24000)[0]
     _say(f"storage {store}: large initial state")
     ctx = _ctx(w, O.PQM, store)
@@ -134,7 +135,15 @@ def test_state_beyond_2_to_32_thread_indices(store):
     want = np.array([sum(p.n for p in c.peaks) for c in w.cells])
     # sampled Maxwellians on the mesh carry their density to well within 1e-3
     assert np.all(np.abs(dens - want) <= 1e-3 * want)
-    ctx.adjust(len(w.cells))
-    assert ctx.total_blocks() == n0
-    assert np.allclose(ctx.moments(len(w.cells))[:, 0], dens, rtol=1e-12, atol=0)
+    loss = ctx.adjust(len(w.cells))
+    after = ctx.moments(len(w.cells))[:, 0]
+    if store == V.STORE_F32:
+        assert ctx.total_blocks() == n0
+        assert np.allclose(after, dens, rtol=1e-12, atol=0)
+    else:
+        # the initial keep set follows the FP32 samples; the adjust sees the stored codes, so
+        # blocks whose only content was within a code step of the threshold may go (with
+        # their halo), a few in 1e4; their mass is the reported loss
+        assert n0 >= ctx.total_blocks() >= n0 - n0 // 1000
+        assert np.allclose(after + loss, dens, rtol=1e-9, atol=0)
     ctx.close()

@@ -225,6 +225,45 @@ def test_adjust_brute_force():
     assert abs(loss - dv3 * float(np.float32(1e-20)) * len(removed)) <= 1e-12 * abs(loss) + 1e-300
 
 
+def test_adjust_spatial_neighbours_brute_force():
+    """Spatial-neighbour content GIDs are kept and added without velocity dilation; own
+    content is dilated by add_width (spatial_cell_cpu.cpp:190-222), here w = 2."""
+    rng = np.random.default_rng(12)
+    m = O.make_mesh((12, 12, 12), (-1, -1, -1), (1, 1, 1))
+    n = 200
+    gids = rng.choice(1728, n, replace=False).astype(np.uint32)
+    f = np.zeros((n, 64), np.float32)
+    has = rng.random(n) < 0.1
+    f[has, rng.integers(0, 64, has.sum())] = 1.0
+    f[~has, 5] = 2e-20
+    extra = rng.choice(1728, 60, replace=True).astype(np.uint32)
+    g2, f2, loss = O.adjust_spatial(m, gids, f, 1e-15, 2, extra)
+    keep = set(int(x) for x in extra)
+    for g, h in zip(gids.tolist(), has):
+        if not h:
+            continue
+        x, y, z = g % 12, (g // 12) % 12, g // 144
+        for dx in range(-2, 3):
+            for dy in range(-2, 3):
+                for dz in range(-2, 3):
+                    a, b, c = x + dx, y + dy, z + dz
+                    if 0 <= a < 12 and 0 <= b < 12 and 0 <= c < 12:
+                        keep.add(int(a + 12 * b + 144 * c))
+    assert sorted(keep) == g2.tolist()
+    # kept blocks carry their values; added ones are zero
+    src = {int(g): i for i, g in enumerate(gids)}
+    for j, g in enumerate(g2.tolist()):
+        want = f[src[g]] if g in src else np.zeros(64, np.float32)
+        assert np.array_equal(f2[j], want)
+    removed = [i for i, g in enumerate(gids) if not has[i] and int(g) not in keep]
+    dv3 = (2.0 / 12 / 4) ** 3
+    assert abs(loss - dv3 * float(np.float32(2e-20)) * len(removed)) <= 1e-12 * abs(loss) + 1e-300
+    # no extra GIDs: the single-cell adjust
+    g3, _, _ = O.adjust_spatial(m, gids, f, 1e-15, 1, np.zeros(0, np.uint32))
+    g4, _, _ = O.adjust(m, gids, f, 1e-15, 1)
+    assert np.array_equal(g3, g4)
+
+
 def test_moments_of_maxwellian():
     V = (1e5, -2e5, 5e4)
     g, f = _maxwellian_cell(T=4e5, V=V)

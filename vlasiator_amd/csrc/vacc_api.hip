@@ -72,6 +72,12 @@ struct vacc_ctx {
    double* pivot = nullptr;          // [4*ncells] plasma pivot of a multi-population step
    double* rho_loss = nullptr;
    hipEvent_t mom_ev = nullptr;      // moments of this population are ready (multi-population step)
+   // spatial-neighbour adjust inputs (device copies, grown on demand)
+   uint32_t* d_nbr_off = nullptr;
+   uint32_t* d_nbr = nullptr;
+   uint64_t* d_ext_off = nullptr;
+   uint32_t* d_ext = nullptr;
+   size_t nbr_cap = 0, ext_cap = 0, off_cap = 0, ext_off_cap = 0;
    uint32_t* h_sum = nullptr;        // pinned host words: capacity flag, block total, status OR
    // event timing
    struct Rec {
@@ -341,8 +347,17 @@ static int do_pass(vacc_ctx* c, int dim) {
    return 0;
 }
 
+struct SpatialIn {
+   const uint32_t* nbr_off = nullptr;   // device pointers or null
+   const uint32_t* nbr = nullptr;
+   const uint64_t* ext_off = nullptr;
+   const uint32_t* ext = nullptr;
+};
+
 // Rebuild the list: add_width >= 0 is the sparsity adjust, add_width < 0 a re-sort by GID.
-static int do_relist(vacc_ctx* c, int add_width, bool want_loss) {
+// With spatial-neighbour inputs the kept count can exceed the reserve: the total is read back
+// after the count phase and the block storage grown before the write phase.
+static int do_relist(vacc_ctx* c, int add_width, bool want_loss, const SpatialIn* sp = nullptr) {
    ProfScope ps(c, add_width >= 0 ? 2 : 4);
    const int li = c->lcur, lo = 1 - c->lcur;
    if (add_width >= 0 && !c->content_valid) {
@@ -363,6 +378,12 @@ static int do_relist(vacc_ctx* c, int add_width, bool want_loss) {
    aa.dv3 = c->mi.cell[0] * c->mi.cell[1] * c->mi.cell[2];
    aa.add_width = add_width;
    aa.flags = c->flags;
+   if (sp) {
+      aa.nbr_off = sp->nbr_off;
+      aa.nbr = sp->nbr;
+      aa.ext_off = sp->ext_off;
+      aa.ext_gid = sp->ext;
+   }
    launch_adjust(aa, false, c->stream);
    ScanArgs sa{};
    sa.a = c->n_tgt;
@@ -370,9 +391,22 @@ static int do_relist(vacc_ctx* c, int add_width, bool want_loss) {
    sa.a_off = c->off[lo];
    sa.b_off = nullptr;
    sa.n = c->ncells;
-   sa.cap_a = (uint32_t)std::min<size_t>(c->cap, 0xFFFFFFFFull);
+   sa.cap_a = sp ? 0xFFFFFFFFu : (uint32_t)std::min<size_t>(c->cap, 0xFFFFFFFFull);
    sa.flags = c->flags;
    launch_scan(sa, c->stream);
+   if (sp) {
+      uint32_t total = 0;
+      HIPCHK(hipMemcpyAsync(&total, c->off[lo] + c->ncells, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+      if (total > c->cap) {
+         const int rc = ensure_blocks(c, (size_t)total + total / 4 + 4096);
+         if (rc) return rc;
+         aa.gid = c->gid[li];
+         aa.src = c->src[li];
+         aa.content = c->content;
+         aa.pool = pool(c, c->fcur);
+      }
+   }
    aa.out_off = c->off[lo];
    aa.gid_out = c->gid[lo];
    aa.src_out = c->src[lo];
@@ -466,6 +500,9 @@ int vacc_destroy(vacc_ctx* c) {
    if (c->flags) (void)hipFree(c->flags);
    if (c->h_sum) (void)hipHostFree(c->h_sum);
    if (c->mom_ev) (void)hipEventDestroy(c->mom_ev);
+   void* sp[] = {c->d_nbr_off, c->d_nbr, c->d_ext_off, c->d_ext};
+   for (void* p : sp)
+      if (p) (void)hipFree(p);
    if (c->stream) (void)hipStreamDestroy(c->stream);
    delete c;
    return 0;
@@ -705,6 +742,62 @@ int vacc_adjust_blocks(vacc_ctx* c, double* rho_loss) {
    int rc = reserve_for_step(c);
    if (rc) return rc;
    rc = do_relist(c, c->sp.sparse_add_width_v, true);
+   if (rc) return rc;
+   rc = finish(c);
+   if (rc < 0) return rc;
+   if (rho_loss) HIPCHK(hipMemcpy(rho_loss, c->rho_loss, c->ncells * sizeof(double), hipMemcpyDeviceToHost));
+   return rc;
+}
+
+// grow a device buffer to hold n elements of size es
+static int grow(void** p, size_t* cap, size_t n, size_t es) {
+   if (n <= *cap && *p) return 0;
+   if (*p) (void)hipFree(*p);
+   *p = nullptr;
+   const size_t nc = std::max<size_t>(n + n / 2, 64);
+   HIPCHK(hipMalloc(p, nc * es));
+   *cap = nc;
+   return 0;
+}
+
+int vacc_adjust_blocks_spatial(vacc_ctx* c, const uint32_t* nbr_off, const uint32_t* nbr, const uint64_t* extra_off,
+                               const uint32_t* extra_gids, double* rho_loss) {
+   if (!c || (nbr_off && !nbr && nbr_off[c->ncells]) || (extra_off && !extra_gids && extra_off[c->ncells]))
+      return set_err(VACC_E_ARG, "null argument");
+   if (!c->ncells) return 0;
+   SpatialIn sp;
+   const size_t m = (size_t)c->ncells + 1;
+   if (nbr_off) {
+      for (uint32_t i = 0; i < c->ncells; ++i)
+         if (nbr_off[i + 1] < nbr_off[i]) return set_err(VACC_E_ARG, "neighbour offsets not ascending");
+      const size_t nn = nbr_off[c->ncells];
+      for (size_t k = 0; k < nn; ++k)
+         if (nbr[k] >= c->ncells) return set_err(VACC_E_ARG, "neighbour index outside the context");
+      int rc = grow((void**)&c->d_nbr_off, &c->off_cap, m, sizeof(uint32_t));
+      if (rc) return rc;
+      rc = grow((void**)&c->d_nbr, &c->nbr_cap, std::max<size_t>(nn, 1), sizeof(uint32_t));
+      if (rc) return rc;
+      HIPCHK(hipMemcpyAsync(c->d_nbr_off, nbr_off, m * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+      if (nn) HIPCHK(hipMemcpyAsync(c->d_nbr, nbr, nn * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+      sp.nbr_off = c->d_nbr_off;
+      sp.nbr = c->d_nbr;
+   }
+   if (extra_off) {
+      for (uint32_t i = 0; i < c->ncells; ++i)
+         if (extra_off[i + 1] < extra_off[i]) return set_err(VACC_E_ARG, "extra GID offsets not ascending");
+      const size_t ne = extra_off[c->ncells];
+      int rc = grow((void**)&c->d_ext_off, &c->ext_off_cap, m, sizeof(uint64_t));
+      if (rc) return rc;
+      rc = grow((void**)&c->d_ext, &c->ext_cap, std::max<size_t>(ne, 1), sizeof(uint32_t));
+      if (rc) return rc;
+      HIPCHK(hipMemcpyAsync(c->d_ext_off, extra_off, m * sizeof(uint64_t), hipMemcpyHostToDevice, c->stream));
+      if (ne) HIPCHK(hipMemcpyAsync(c->d_ext, extra_gids, ne * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+      sp.ext_off = c->d_ext_off;
+      sp.ext = c->d_ext;
+   }
+   int rc = reserve_for_step(c);
+   if (rc) return rc;
+   rc = do_relist(c, c->sp.sparse_add_width_v, true, &sp);
    if (rc) return rc;
    rc = finish(c);
    if (rc < 0) return rc;

@@ -107,6 +107,12 @@ struct AdjustArgs {
    double min_value;
    double dv3;
    int add_width;              // < 0: keep exactly the existing blocks (re-sort only)
+   // spatial neighbours (null: none): cells of this list whose content blocks are kept
+   // (nbr_off [ncells+1] into nbr), and caller GIDs to keep (ext_off [ncells+1] into ext_gid)
+   const uint32_t* nbr_off;
+   const uint32_t* nbr;
+   const uint64_t* ext_off;
+   const uint32_t* ext_gid;
 };
 
 struct MomentArgs {

@@ -829,7 +829,33 @@ __global__ __launch_bounds__(WG) void k_adjust(AdjustArgs a) {
    const uint32_t n = a.cell_off[c + 1] - off;
    const int w = a.add_width;
    const int ww = (w > 0) ? w : 0;
-   if (n == 0) {
+   const uint32_t nx = a.len[0], ny = a.len[1], nz = a.len[2];
+   const uint32_t ntot = nx * ny * nz;
+   // content GIDs of the spatial neighbours (spatial_cell_cpu.cpp:212-222): content blocks of
+   // neighbour cells of this context, and caller-supplied GIDs; kept as they are
+   const bool spatial = (w >= 0) && (a.nbr_off || a.ext_off);
+   auto for_spatial = [&](auto&& fn) {
+      if (!spatial) return;
+      if (a.nbr_off)
+         for (uint32_t k = a.nbr_off[c]; k < a.nbr_off[c + 1]; ++k) {
+            const uint32_t o = a.cell_off[a.nbr[k]], m = a.cell_off[a.nbr[k] + 1] - o;
+            for (uint32_t i = t; i < m; i += WG)
+               if (a.content[o + i]) fn(a.gid[o + i]);
+         }
+      if (a.ext_off)
+         for (uint64_t i = a.ext_off[c] + t; i < a.ext_off[c + 1]; i += WG) {
+            const uint32_t g = a.ext_gid[i];
+            if (g < ntot) fn(g);
+         }
+   };
+   bool any_spatial = false;
+   if (spatial) {
+      if (a.ext_off && a.ext_off[c + 1] > a.ext_off[c]) any_spatial = true;
+      if (a.nbr_off)
+         for (uint32_t k = a.nbr_off[c]; k < a.nbr_off[c + 1]; ++k)
+            if (a.cell_off[a.nbr[k] + 1] > a.cell_off[a.nbr[k]]) any_spatial = true;
+   }
+   if (n == 0 && !any_spatial) {
       if (!WRITE && t == 0) {
          a.n_keep[c] = 0;
          if (a.rho_loss) a.rho_loss[c] = 0.0;
@@ -846,20 +872,27 @@ __global__ __launch_bounds__(WG) void k_adjust(AdjustArgs a) {
       nrem = 0;
    }
    __syncthreads();
-   const uint32_t nx = a.len[0], ny = a.len[1], nz = a.len[2];
    {
       int lx = INT_MAX, ly = INT_MAX, lz = INT_MAX, hx = INT_MIN, hy = INT_MIN, hz = INT_MIN;
-      for (uint32_t i = t; i < n; i += WG) {
-         const uint32_t g = a.gid[off + i];
+      auto grow = [&](uint32_t g) {
          const int x = (int)(g % nx), y = (int)((g / nx) % ny), z = (int)(g / (nx * ny));
          lx = min(lx, x); hx = max(hx, x);
          ly = min(ly, y); hy = max(hy, y);
          lz = min(lz, z); hz = max(hz, z);
-      }
+      };
+      for (uint32_t i = t; i < n; i += WG) grow(a.gid[off + i]);
+      for_spatial(grow);
       atomicMin(&sbox[0], lx); atomicMin(&sbox[1], ly); atomicMin(&sbox[2], lz);
       atomicMax(&sbox[3], hx); atomicMax(&sbox[4], hy); atomicMax(&sbox[5], hz);
    }
    __syncthreads();
+   if (sbox[0] == INT_MAX) {   // no block of its own and no neighbour content: nothing to keep
+      if (!WRITE && t == 0) {
+         a.n_keep[c] = 0;
+         if (a.rho_loss) a.rho_loss[c] = 0.0;
+      }
+      return;
+   }
    const int mnx = max(sbox[0] - ww, 0), mny = max(sbox[1] - ww, 0), mnz = max(sbox[2] - ww, 0);
    const int mxx = min(sbox[3] + ww, (int)nx - 1), mxy = min(sbox[4] + ww, (int)ny - 1), mxz = min(sbox[5] + ww, (int)nz - 1);
    const int Rx = mxx - mnx + 1, Ry = mxy - mny + 1, Rz = mxz - mnz + 1;
@@ -883,7 +916,21 @@ __global__ __launch_bounds__(WG) void k_adjust(AdjustArgs a) {
          }
          __syncthreads();
          for (int k = 0; k < w; ++k) dilate_box1(bm, W, Rx, Ry, Rz);
-      } else
+         for_spatial([&](uint32_t g) {
+            const int x = (int)(g % nx), y = (int)((g / nx) % ny), z = (int)(g / (nx * ny));
+            const int kx = x - mnx;
+            atomicOr(&bm[((z - mnz) * Ry + (y - mny)) * W + (kx >> 5)], 1u << (kx & 31));
+         });
+         __syncthreads();
+      } else {
+      for_spatial([&](uint32_t g) {
+         const int x = (int)(g % nx), y = (int)((g / nx) % ny), z = (int)(g / (nx * ny));
+         const int row = (z - mnz) * Ry + (y - mny) - r0;
+         if (row >= 0 && row < nr) {
+            const int kx = x - mnx;
+            atomicOr(&bm[row * W + (kx >> 5)], 1u << (kx & 31));
+         }
+      });
       for (uint32_t i = t; i < n; i += WG) {
          const uint32_t g = a.gid[off + i];
          const int x = (int)(g % nx), y = (int)((g / nx) % ny), z = (int)(g / (nx * ny));
@@ -910,6 +957,7 @@ __global__ __launch_bounds__(WG) void k_adjust(AdjustArgs a) {
                }
             }
          }
+      }
       }
       __syncthreads();
       // word prefix counts over the chunk (contiguous words per thread)

@@ -26,7 +26,7 @@ EXPORTS = [
     "vacc_compute_cell_intersections", "vacc_cell_intersections_host", "vacc_max_v_dt", "vacc_map_1d",
     "vacc_accelerate_cells", "vacc_adjust_blocks", "vacc_moments", "vacc_sync", "vacc_get_status",
     "vacc_init_peaks", "vacc_profile_begin", "vacc_profile_read", "vacc_set_fields", "vacc_step",
-    "vacc_accelerate_populations",
+    "vacc_accelerate_populations", "vacc_adjust_blocks_spatial",
 ]
 
 
@@ -105,6 +105,7 @@ def lib():
         L.vacc_set_fields.argtypes = [vp, vp]
         L.vacc_step.argtypes = [vp, C.c_uint32, C.c_int32, C.c_int32, vp]
         L.vacc_accelerate_populations.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_int32, C.c_int32, vp]
+        L.vacc_adjust_blocks_spatial.argtypes = [vp, vp, vp, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -261,6 +262,22 @@ class Context:
             ncells = self.counts().size
         loss = np.zeros(max(ncells, 1), np.float64)
         _check(lib().vacc_adjust_blocks(self.h, _ptr(loss)))
+        return loss[:ncells]
+
+    def adjust_spatial(self, ncells, nbr_off=None, nbr=None, extra_off=None, extra_gids=None):
+        """Adjust with spatial neighbours: nbr_off/nbr (cells of this context, CSR) and/or
+        extra_off/extra_gids (content GIDs per cell, CSR).  Returns the removed mass per cell."""
+        def arr(a, dt):
+            return None if a is None else np.ascontiguousarray(a, dt)
+        no, nb = arr(nbr_off, np.uint32), arr(nbr, np.uint32)
+        eo, eg = arr(extra_off, np.uint64), arr(extra_gids, np.uint32)
+        if nb is not None and nb.size == 0:
+            nb = np.zeros(1, np.uint32)
+        if eg is not None and eg.size == 0:
+            eg = np.zeros(1, np.uint32)
+        loss = np.zeros(max(ncells, 1), np.float64)
+        p = lambda a: None if a is None else _ptr(a)
+        _check(lib().vacc_adjust_blocks_spatial(self.h, p(no), p(nb), p(eo), p(eg), _ptr(loss)))
         return loss[:ncells]
 
     def moments(self, ncells):
