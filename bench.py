@@ -114,6 +114,19 @@ def plain_form(w, cells, isect, steps):
     return upd / dt, n0
 
 
+def profiled_traffic(workload: str, storage: str):
+    """HBM bytes per k_remap launch relative to the algorithmic bytes, as measured with
+    rocprofv3 FETCH_SIZE / WRITE_SIZE (copy-calibrated) for this workload and storage form
+    and recorded in a profiles/ summary ("traffic-ratio <workload> <storage> <ratio>")."""
+    import glob
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        for line in open(path):
+            p = line.split()
+            if len(p) == 4 and p[0] == "traffic-ratio" and p[1] == workload and p[2] == storage:
+                return float(p[3]), os.path.relpath(path, ROOT)
+    return None, None
+
+
 def measure(pops, storage: str, steps: int, warmup: int) -> dict:
     """Initial state of every population, W untimed steps, then K timed steps, each step
     ending in the library's one host synchronisation.  Several populations (config 4) step
@@ -173,6 +186,10 @@ def measure(pops, storage: str, steps: int, warmup: int) -> dict:
                 "algorithmic_bytes_per_launch": int(remap_bytes / launches)}
     breakdown = {k: round(prof[k] / steps, 4) for k in ("remap_ms", "build_ms", "adjust_ms", "transform_ms")}
     breakdown["host_other_ms"] = round(ms_per_step - sum(breakdown.values()), 4)
+    ratio, src = profiled_traffic(" + ".join(p.name for p in pops), storage)
+    if ratio is not None:
+        roofline["traffic"] = int(ratio * remap_bytes / launches)
+        roofline["traffic_source"] = f"{src}: rocprofv3 FETCH_SIZE + WRITE_SIZE per launch, x{ratio:.3f} algorithmic"
     pct = {f"p{q}": round(float(np.percentile(step_ms, q)), 3) for q in (50, 90, 99)}
     return {"value": value, "ms_per_step": ms_per_step, "blocks_at_start": n_init, "roofline": roofline,
             "breakdown": breakdown, "step_ms_percentiles": pct}

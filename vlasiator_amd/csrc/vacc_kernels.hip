@@ -476,6 +476,16 @@ __device__ inline float pool_value(const Pool& pl, uint32_t b, int e) {
    return pl.scale[b] * dec16(pl.storage, static_cast<const uint16_t*>(pl.f)[(size_t)b * WID3 + e]);
 }
 
+// Row q (cells 4q..4q+3, i = 0..3 at j = q & 3, k = q >> 2) of block b, decoded to FP32.
+__device__ inline float4 pool_row(const Pool& pl, uint32_t b, int q) {
+   if (b == INVALID) return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+   if (pl.storage == ST_F32) return reinterpret_cast<const float4*>(pl.f)[(size_t)b * (WID3 / 4) + q];
+   const uint2 r = reinterpret_cast<const uint2*>(pl.f)[(size_t)b * (WID3 / 4) + q];
+   const float sc = pl.scale[b];
+   return make_float4(sc * dec16(pl.storage, r.x & 0xFFFFu), sc * dec16(pl.storage, r.x >> 16),
+                      sc * dec16(pl.storage, r.y & 0xFFFFu), sc * dec16(pl.storage, r.y >> 16));
+}
+
 template <int DIM>
 struct Pencil {
    int o;   // offset of the pencil's first cell in a block
@@ -820,8 +830,6 @@ __global__ __launch_bounds__(WG) void k_adjust(AdjustArgs a) {
    __shared__ uint32_t sh[4];
    __shared__ uint32_t carry;
    __shared__ double sloss;
-   __shared__ uint32_t nrem;
-   __shared__ uint32_t rem[WRITE ? 1 : WG];   // removed blocks among WG list entries (positions)
    if (WRITE && (*a.flags & VACC_STATUS_CAPACITY)) return;
    const int t = threadIdx.x;
    const uint32_t c = blockIdx.x;
@@ -869,7 +877,6 @@ __global__ __launch_bounds__(WG) void k_adjust(AdjustArgs a) {
    if (t == 0) {
       carry = 0;
       sloss = 0.0;
-      nrem = 0;
    }
    __syncthreads();
    {
@@ -1005,32 +1012,24 @@ __global__ __launch_bounds__(WG) void k_adjust(AdjustArgs a) {
          }
       } else {
          // removed blocks: no content and not kept; their mass goes to RHOLOSSADJUST
-         // (spatial_cell_cpu.cpp:227-267), summed one wave per block
-         for (uint32_t i0 = 0; i0 < n; i0 += WG) {
-            const uint32_t i = i0 + t;
-            bool removed = false;
-            if (i < n && w >= 0 && !a.content[off + i]) {
-               const uint32_t g = a.gid[off + i];
-               const int x = (int)(g % nx), y = (int)((g / nx) % ny), z = (int)(g / (nx * ny));
-               const int row = (z - mnz) * Ry + (y - mny) - r0;
-               if (row >= 0 && row < nr) {
-                  const int kx = x - mnx;
-                  removed = !(bm[row * W + (kx >> 5)] & (1u << (kx & 31)));
-               }
-            }
-            if (removed) rem[atomicAdd(&nrem, 1u)] = i;   // at most WG entries
-            __syncthreads();
-            const uint32_t nk = nrem;
-            const int lane = t & 63;
-            for (uint32_t k = t >> 6; k < nk; k += WG / 64) {   // whole waves: uniform trip count
-               const uint32_t sidx = a.src[off + rem[k]];
-               double v = (sidx != INVALID) ? (double)pool_value(a.pool, sidx, lane) : 0.0;
-               for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-               if (lane == 0) myloss += a.dv3 * v;
-            }
-            __syncthreads();
-            if (t == 0) nrem = 0;
-            __syncthreads();
+         // (spatial_cell_cpu.cpp:227-267), one thread per block, 16 row loads in flight
+         for (uint32_t i = t; i < n; i += WG) {
+            if (w < 0 || a.content[off + i]) continue;
+            const uint32_t g = a.gid[off + i];
+            const int x = (int)(g % nx), y = (int)((g / nx) % ny), z = (int)(g / (nx * ny));
+            const int row = (z - mnz) * Ry + (y - mny) - r0;
+            if (row < 0 || row >= nr) continue;
+            const int kx = x - mnx;
+            if (bm[row * W + (kx >> 5)] & (1u << (kx & 31))) continue;
+            const uint32_t sidx = a.src[off + i];
+            if (sidx == INVALID) continue;
+            float4 r[WID3 / 4];
+#pragma unroll
+            for (int q = 0; q < WID3 / 4; ++q) r[q] = pool_row(a.pool, sidx, q);
+            double sum = 0.0;
+#pragma unroll
+            for (int q = 0; q < WID3 / 4; ++q) sum += ((double)r[q].x + (double)r[q].y) + ((double)r[q].z + (double)r[q].w);
+            myloss += a.dv3 * sum;
          }
       }
       __syncthreads();
@@ -1065,16 +1064,6 @@ __global__ __launch_bounds__(WG) void k_content(const uint32_t* src, Pool pl, ui
 // ---------------------------------------------------------------------------
 // moments, transform, gather
 // ---------------------------------------------------------------------------
-
-// Row q (cells 4q..4q+3, i = 0..3 at j = q & 3, k = q >> 2) of block b, decoded to FP32.
-__device__ inline float4 pool_row(const Pool& pl, uint32_t b, int q) {
-   if (b == INVALID) return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-   if (pl.storage == ST_F32) return reinterpret_cast<const float4*>(pl.f)[(size_t)b * (WID3 / 4) + q];
-   const uint2 r = reinterpret_cast<const uint2*>(pl.f)[(size_t)b * (WID3 / 4) + q];
-   const float sc = pl.scale[b];
-   return make_float4(sc * dec16(pl.storage, r.x & 0xFFFFu), sc * dec16(pl.storage, r.x >> 16),
-                      sc * dec16(pl.storage, r.y & 0xFFFFu), sc * dec16(pl.storage, r.y >> 16));
-}
 
 __global__ __launch_bounds__(WG) void k_moments(MomentArgs a) {
    // One workgroup per cell; 16 lanes per block, each lane one float4 (i = 0..3 at fixed j, k),
