@@ -60,6 +60,7 @@ struct vacc_ctx {
    uint4* unit_rec = nullptr;
    uint4* col_rec = nullptr;
    uint32_t* perm = nullptr;
+   uint32_t* perm_cell = nullptr;
    uint32_t* n_units = nullptr;
    uint32_t* n_tgt = nullptr;
    uint32_t* unit_base = nullptr;
@@ -139,6 +140,8 @@ static void free_blocks(vacc_ctx* c) {
    if (c->unit_rec) (void)hipFree(c->unit_rec);
    if (c->col_rec) (void)hipFree(c->col_rec);
    if (c->perm) (void)hipFree(c->perm);
+   if (c->perm_cell) (void)hipFree(c->perm_cell);
+   c->perm_cell = nullptr;
    c->content = nullptr;
    c->sorted_src = nullptr;
    c->unit_rec = c->col_rec = nullptr;
@@ -203,7 +206,7 @@ static int ensure_blocks(vacc_ctx* c, size_t want) {
    void* f[2];
    float* sc[2] = {nullptr, nullptr};
    uint8_t* ct;
-   uint32_t *ss, *pm;
+   uint32_t *ss, *pm, *pc;
    uint4 *sr, *cr;
    for (int i = 0; i < 2; ++i) {
       HIPCHK(hipMalloc(&g[i], ncap * sizeof(uint32_t)));
@@ -216,6 +219,7 @@ static int ensure_blocks(vacc_ctx* c, size_t want) {
    HIPCHK(hipMalloc(&sr, ncap * sizeof(uint4)));
    HIPCHK(hipMalloc(&cr, ncap * sizeof(uint4)));
    HIPCHK(hipMalloc(&pm, ncap * sizeof(uint32_t)));
+   HIPCHK(hipMalloc(&pc, ncap * sizeof(uint32_t)));
    if (c->cap && c->nblocks) {
       // carry the current state: list entries and the whole current pool extent
       const size_t n = c->nblocks;
@@ -239,6 +243,7 @@ static int ensure_blocks(vacc_ctx* c, size_t want) {
    c->unit_rec = sr;
    c->col_rec = cr;
    c->perm = pm;
+   c->perm_cell = pc;
    c->cap = ncap;
    return 0;
 }
@@ -302,6 +307,7 @@ static int do_pass(vacc_ctx* c, int dim) {
       oa.unit_rec = c->unit_rec;
       oa.hist_cursor = c->len_hist + LEN_BINS;
       oa.perm = c->perm;
+      oa.perm_cell = c->perm_cell;
       oa.flags = c->flags;
       oa.ncells = c->ncells;
       launch_order(oa, c->stream);
@@ -311,6 +317,7 @@ static int do_pass(vacc_ctx* c, int dim) {
    ra.unit_base = c->unit_base;
    ra.out_off = c->off[lo];
    ra.perm = c->perm;
+   ra.perm_cell = c->perm_cell;
    ra.unit_rec = c->unit_rec;
    ra.col_rec = c->col_rec;
    ra.sorted_src = c->sorted_src;

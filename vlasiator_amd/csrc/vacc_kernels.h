@@ -64,6 +64,7 @@ struct OrderArgs {
    const uint4* unit_rec;
    uint32_t* hist_cursor;       // [LEN_BINS] insertion cursors
    uint32_t* perm;              // [total units] global unit indices, longest first
+   uint32_t* perm_cell;         // [total units] the cell of each perm entry
    const uint32_t* flags;
    uint32_t ncells;
 };
@@ -73,6 +74,7 @@ struct RemapArgs {
    const uint32_t* unit_base;   // [ncells+1] global work-unit index base
    const uint32_t* out_off;     // [ncells+1] output block offsets
    const uint32_t* perm;        // [total units] processing order
+   const uint32_t* perm_cell;   // [total units] cell of each entry of perm
    const uint4* unit_rec;
    const uint4* col_rec;
    const uint32_t* sorted_src;

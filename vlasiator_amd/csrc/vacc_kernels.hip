@@ -462,7 +462,9 @@ __global__ __launch_bounds__(WG) void k_order(OrderArgs a) {
    __syncthreads();
    for (uint32_t u = threadIdx.x; u < nu; u += WG) {
       const uint32_t b = len_bin(a.unit_rec[off + u].w);
-      a.perm[base_of[b] + atomicAdd(&cnt[b], 1u)] = base + u;
+      const uint32_t pos = base_of[b] + atomicAdd(&cnt[b], 1u);
+      a.perm[pos] = base + u;
+      a.perm_cell[pos] = c;
    }
 }
 
@@ -633,13 +635,7 @@ __global__ __launch_bounds__(WG) void k_remap(RemapArgs a, double i_dv) {
    const uint32_t gstride = gridDim.x * (WG / 16);
    for (uint32_t gi = blockIdx.x * (WG / 16) + (tid >> 4); gi < total; gi += gstride) {
       const uint32_t g = a.perm[gi];
-      uint32_t lo = 0, hi = a.ncells;
-      while (hi - lo > 1) {
-         const uint32_t mid = (lo + hi) >> 1;
-         if (a.unit_base[mid] <= g) lo = mid;
-         else hi = mid;
-      }
-      const uint32_t c = lo;
+      const uint32_t c = a.perm_cell[gi];
       const uint32_t cbase = a.cell_off[c];
       const uint4 sr = a.unit_rec[cbase + (g - a.unit_base[c])];
       const int b0 = (int)(sr.x & 0xFFFFu), b1 = (int)(sr.x >> 16);
