@@ -46,10 +46,10 @@ def log(msg):
 FAST = False
 
 
-def make_context(w: W.Workload, storage: str = "f32"):
+def make_context(w: W.Workload, storage: str = "f32", graph: bool = False):
     mesh = V.make_mesh(w.grid_length, w.vmin, w.vmax)
     sp = V.make_species(w.mass, w.charge, w.min_value, w.add_width)
-    op = V.make_options(w.method, fast_arith=FAST, storage=STORES[storage])
+    op = V.make_options(w.method, fast_arith=FAST, storage=STORES[storage], use_graph=graph)
     return V.Context(mesh, sp, op)
 
 
@@ -127,14 +127,15 @@ def profiled_traffic(workload: str, storage: str):
     return None, None
 
 
-def measure(pops, storage: str, steps: int, warmup: int) -> dict:
+def measure(pops, storage: str, steps: int, warmup: int, graph: bool = False) -> dict:
     """Initial state of every population, W untimed steps, then K timed steps, each step
     ending in the library's one host synchronisation.  Several populations (config 4) step
-    through vacc_accelerate_populations with the plasma-moment pivot."""
+    through vacc_accelerate_populations with the plasma-moment pivot.  graph=True replays
+    captured steps (one population only) and runs without the per-kernel event profile."""
     ctxs = []
     n_init = 0
     for w in pops:
-        ctx = make_context(w, storage)
+        ctx = make_context(w, storage, graph)
         t0 = time.perf_counter()
         ctx.init_peaks([W.to_vacc_peaks(c) for c in w.cells], W.boxes(w))
         n = ctx.total_blocks()
@@ -154,7 +155,8 @@ def measure(pops, storage: str, steps: int, warmup: int) -> dict:
         one_step(s)
     for c in ctxs:
         c.sync()
-        c.profile_begin()
+        if not graph:
+            c.profile_begin()
     updates = 0
     step_ms = []
     t0 = time.perf_counter()
@@ -167,6 +169,10 @@ def measure(pops, storage: str, steps: int, warmup: int) -> dict:
     for c in ctxs:
         status |= c.sync()
     elapsed = time.perf_counter() - t0
+    if graph:
+        for c in ctxs:
+            c.close()
+        return {"value": updates / elapsed, "ms_per_step": 1e3 * elapsed / steps}
     profs = [c.profile_read() for c in ctxs]
     for c in ctxs:
         c.close()
@@ -261,6 +267,10 @@ def main():
             forms[st] = {"value": fr["value"], "ms_per_step": fr["ms_per_step"], "steps": min(args.steps, 10),
                          "remap_GBps": fr["roofline"]["achieved"], "remap_frac": fr["roofline"]["frac"]}
         out["storage_forms"] = forms
+    if args.forms and len(pops) == 1:
+        # the same main form with each step replayed from a captured hipGraph
+        gr = measure(pops, args.storage, min(args.steps, 10), min(args.warmup, 3), graph=True)
+        out["graph_form"] = {"value": gr["value"], "ms_per_step": gr["ms_per_step"], "steps": min(args.steps, 10)}
 
     if args.plain_cells and sample_cells is not None:
         npl = min(args.plain_cells, len(sample_cells))

@@ -94,7 +94,10 @@ typedef struct vacc_options {
                                  (bit-identical f); 1: fused multiply-adds and hardware
                                  reciprocal/sqrt in the reconstruction and integration
                                  (f within a few ulp; block lists unchanged) */
-   int32_t reserved;
+   int32_t use_graph;         /* 1: vacc_step captures its launches in a hipGraph per state
+                                 variant (map order, list/pool parity, options) on first use
+                                 and replays it afterwards; results are identical.  Not used
+                                 while kernel profiling is on. */
 } vacc_options;
 
 /* Per-cell field input of the transform (cpu_acc_transform.cpp:71-115). */

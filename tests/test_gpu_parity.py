@@ -487,3 +487,33 @@ def test_spatial_neighbour_adjust_bit_exact(cfg0, corners):
     cases.assert_cells_equal(ctx.download_cells(), want, f"spatial adjust corners={corners}")
     assert np.allclose(loss, want_loss, rtol=1e-12, atol=0)
     ctx.close()
+
+
+def test_step_graph_replays_match_eager_steps():
+    """vacc_options.use_graph: 14 steps, so every captured variant (map order x pool parity)
+    is replayed, with eager work (moments, downloads) between replays; every compared state
+    equals the eager context's bit for bit."""
+    w = W.config(2, ncells=6)
+    cells = cases.build_cells(w)
+    fields = [V.make_fields(c.B, c.V, c.dt) for c in w.cells]
+    mesh = V.make_mesh(w.grid_length, w.vmin, w.vmax)
+    sp = V.make_species(w.mass, w.charge, w.min_value, w.add_width)
+    eager = V.Context(mesh, sp, V.make_options(w.method))
+    graph = V.Context(mesh, sp, V.make_options(w.method, use_graph=True))
+    for c in (eager, graph):
+        c.upload_cells(cells)
+        c.set_fields(fields)
+    n = len(cells)
+    for step in range(14):
+        _say(f"graph step {step}")
+        le, lg = np.zeros(n), np.zeros(n)
+        eager.step(step % 3, use_moments=True, adjust=True, rho_loss=le)
+        graph.step(step % 3, use_moments=True, adjust=True, rho_loss=lg)
+        assert np.array_equal(le, lg), f"step {step} loss"
+        assert eager.total_blocks() == graph.total_blocks()
+        if step % 4 == 3:   # eager work between replays
+            assert np.array_equal(eager.moments(n), graph.moments(n))
+        if step in (6, 13):
+            cases.assert_cells_equal(graph.download_cells(), eager.download_cells(), f"graph step {step}")
+    eager.close()
+    graph.close()

@@ -79,6 +79,19 @@ struct vacc_ctx {
    uint64_t* d_ext_off = nullptr;
    uint32_t* d_ext = nullptr;
    size_t nbr_cap = 0, ext_cap = 0, off_cap = 0, ext_off_cap = 0;
+   // captured steps (vacc_options.use_graph): one executable graph per state variant; gen
+   // changes whenever buffers move or the cell set changes, which retires every graph
+   struct StepGraph {
+      uint64_t gen;
+      uint32_t map_order;
+      int use_moments, adjust, lcur, fcur;
+      bool moments_in, content_in;
+      hipGraphExec_t exec;
+      int lcur_out, fcur_out;
+      bool moments_out, content_out;
+   };
+   std::vector<StepGraph> graphs;
+   uint64_t gen = 0;
    uint32_t* h_sum = nullptr;        // pinned host words: capacity flag, block total, status OR
    // event timing
    struct Rec {
@@ -125,7 +138,14 @@ struct ProfScope {
    }
 };
 
+static void drop_graphs(vacc_ctx* c) {
+   for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.exec);
+   c->graphs.clear();
+   c->gen++;
+}
+
 static void free_blocks(vacc_ctx* c) {
+   drop_graphs(c);
    for (int i = 0; i < 2; ++i) {
       if (c->gid[i]) (void)hipFree(c->gid[i]);
       if (c->src[i]) (void)hipFree(c->src[i]);
@@ -173,6 +193,7 @@ static void free_cells(vacc_ctx* c) {
 }
 
 static int alloc_cells(vacc_ctx* c, uint32_t n) {
+   drop_graphs(c);   // the cell count or per-cell buffers change
    if (n <= c->cell_cap && c->off[0]) return 0;
    free_cells(c);
    const size_t m = (size_t)n + 1;
@@ -496,6 +517,7 @@ int vacc_create(const vacc_mesh* mesh, const vacc_species* species, const vacc_o
 int vacc_destroy(vacc_ctx* c) {
    if (!c) return 0;
    if (c->stream) (void)hipStreamSynchronize(c->stream);
+   drop_graphs(c);
    free_blocks(c);
    free_cells(c);
    for (auto& r : c->recs) {
@@ -812,21 +834,74 @@ int vacc_adjust_blocks_spatial(vacc_ctx* c, const uint32_t* nbr_off, const uint3
    return rc;
 }
 
+// The launches of one step (transform, three passes, adjust) on the context stream.
+static int enqueue_step(vacc_ctx* c, uint32_t map_order, int32_t use_moments, int32_t adjust) {
+   enqueue_transform(c, map_order, use_moments ? 1 : 0);
+   for (int p = 0; p < 3; ++p) {
+      const int rc = do_pass(c, pass_dim((int)map_order, p));
+      if (rc) return rc;
+   }
+   if (adjust) {
+      const int rc = do_relist(c, c->sp.sparse_add_width_v, true);
+      if (rc) return rc;
+   }
+   return 0;
+}
+
+// Replay the captured step for the current state, capturing it first when needed.  The host
+// state transitions of the step (list/pool parity, validity flags) are recorded at capture
+// and applied on replay.
+static int graph_step(vacc_ctx* c, uint32_t map_order, int32_t use_moments, int32_t adjust) {
+   const int um = use_moments ? 1 : 0, ad = adjust ? 1 : 0;
+   for (const auto& g : c->graphs)
+      if (g.gen == c->gen && g.map_order == map_order && g.use_moments == um && g.adjust == ad && g.lcur == c->lcur &&
+          g.fcur == c->fcur && g.moments_in == c->moments_valid && g.content_in == c->content_valid) {
+         HIPCHK(hipGraphLaunch(g.exec, c->stream));
+         c->lcur = g.lcur_out;
+         c->fcur = g.fcur_out;
+         c->moments_valid = g.moments_out;
+         c->content_valid = g.content_out;
+         return 0;
+      }
+   vacc_ctx::StepGraph g{};
+   g.gen = c->gen;
+   g.map_order = map_order;
+   g.use_moments = um;
+   g.adjust = ad;
+   g.lcur = c->lcur;
+   g.fcur = c->fcur;
+   g.moments_in = c->moments_valid;
+   g.content_in = c->content_valid;
+   HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+   const int rc = enqueue_step(c, map_order, use_moments, adjust);
+   hipGraph_t graph = nullptr;
+   const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+   if (rc) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc;
+   }
+   if (e != hipSuccess) return set_err(VACC_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+   const hipError_t ei = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+   (void)hipGraphDestroy(graph);
+   if (ei != hipSuccess) return set_err(VACC_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+   g.lcur_out = c->lcur;
+   g.fcur_out = c->fcur;
+   g.moments_out = c->moments_valid;
+   g.content_out = c->content_valid;
+   c->graphs.push_back(g);
+   HIPCHK(hipGraphLaunch(g.exec, c->stream));
+   return 0;
+}
+
 int vacc_step(vacc_ctx* c, uint32_t map_order, int32_t use_moments, int32_t adjust, double* rho_loss) {
    if (!c || map_order > 2) return set_err(VACC_E_ARG, "bad argument");
    if (!c->ncells) return 0;
    if (!c->fields_valid) return set_err(VACC_E_STATE, "no cell fields: call vacc_set_fields first");
    int rc = reserve_for_step(c);
    if (rc) return rc;
-   enqueue_transform(c, map_order, use_moments ? 1 : 0);
-   for (int p = 0; p < 3; ++p) {
-      rc = do_pass(c, pass_dim((int)map_order, p));
-      if (rc) return rc;
-   }
-   if (adjust) {
-      rc = do_relist(c, c->sp.sparse_add_width_v, true);
-      if (rc) return rc;
-   }
+   rc = (c->opt.use_graph && !c->prof) ? graph_step(c, map_order, use_moments, adjust)
+                                       : enqueue_step(c, map_order, use_moments, adjust);
+   if (rc) return rc;
    rc = finish(c);
    if (rc < 0) return rc;
    if (adjust && rho_loss) HIPCHK(hipMemcpy(rho_loss, c->rho_loss, c->ncells * sizeof(double), hipMemcpyDeviceToHost));

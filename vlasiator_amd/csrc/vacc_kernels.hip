@@ -825,7 +825,7 @@ __global__ __launch_bounds__(WG) void k_adjust(AdjustArgs a) {
    __shared__ int sbox[6];
    __shared__ uint32_t sh[4];
    __shared__ uint32_t carry;
-   __shared__ double sloss;
+   __shared__ double wloss[WG / 64];
    if (WRITE && (*a.flags & VACC_STATUS_CAPACITY)) return;
    const int t = threadIdx.x;
    const uint32_t c = blockIdx.x;
@@ -870,10 +870,7 @@ __global__ __launch_bounds__(WG) void k_adjust(AdjustArgs a) {
       sbox[t] = INT_MAX;
       sbox[t + 3] = INT_MIN;
    }
-   if (t == 0) {
-      carry = 0;
-      sloss = 0.0;
-   }
+   if (t == 0) carry = 0;
    __syncthreads();
    {
       int lx = INT_MAX, ly = INT_MAX, lz = INT_MAX, hx = INT_MIN, hy = INT_MIN, hz = INT_MIN;
@@ -1033,11 +1030,16 @@ __global__ __launch_bounds__(WG) void k_adjust(AdjustArgs a) {
       __syncthreads();
    }
    if (!WRITE) {
-      if (myloss != 0.0) atomicAdd(&sloss, myloss);
+      // fixed-order reduction (wave tree, then waves in order): the loss is reproducible
+      double v = myloss;
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+      if ((t & 63) == 0) wloss[t >> 6] = v;
       __syncthreads();
       if (t == 0) {
+         double sum = 0.0;
+         for (int q = 0; q < WG / 64; ++q) sum += wloss[q];
          a.n_keep[c] = carry;
-         if (a.rho_loss) a.rho_loss[c] = sloss;
+         if (a.rho_loss) a.rho_loss[c] = sum;
       }
    }
 }
