@@ -44,6 +44,7 @@ def log(msg):
 
 
 FAST = False
+AUTOTUNE = True
 
 
 def make_context(w: W.Workload, storage: str = "f32", graph: bool = False):
@@ -133,6 +134,7 @@ def measure(pops, storage: str, steps: int, warmup: int, graph: bool = False) ->
     through vacc_accelerate_populations with the plasma-moment pivot.  graph=True replays
     captured steps (one population only) and runs without the per-kernel event profile."""
     ctxs = []
+    tuned = []
     n_init = 0
     for w in pops:
         ctx = make_context(w, storage, graph)
@@ -143,6 +145,8 @@ def measure(pops, storage: str, steps: int, warmup: int, graph: bool = False) ->
         log(f"[{storage}] {w.name}: initial state {n} blocks ({n * BYTES_PER_BLOCK[storage] / 1e9:.2f} GB) "
             f"in {time.perf_counter() - t0:.1f} s")
         ctx.set_fields([V.make_fields(c.B, c.V, c.dt) for c in w.cells])
+        if AUTOTUNE:
+            tuned.append(ctx.autotune())   # persistent remap grid per pass dimension (untimed)
         ctxs.append(ctx)
     adjust = pops[0].adjust_every_step
 
@@ -172,7 +176,7 @@ def measure(pops, storage: str, steps: int, warmup: int, graph: bool = False) ->
     if graph:
         for c in ctxs:
             c.close()
-        return {"value": updates / elapsed, "ms_per_step": 1e3 * elapsed / steps}
+        return {"value": updates / elapsed, "ms_per_step": 1e3 * elapsed / steps, "remap_wg_per_cu": tuned}
     profs = [c.profile_read() for c in ctxs]
     for c in ctxs:
         c.close()
@@ -198,7 +202,7 @@ def measure(pops, storage: str, steps: int, warmup: int, graph: bool = False) ->
         roofline["traffic_source"] = f"{src}: rocprofv3 FETCH_SIZE + WRITE_SIZE per launch, x{ratio:.3f} algorithmic"
     pct = {f"p{q}": round(float(np.percentile(step_ms, q)), 3) for q in (50, 90, 99)}
     return {"value": value, "ms_per_step": ms_per_step, "blocks_at_start": n_init, "roofline": roofline,
-            "breakdown": breakdown, "step_ms_percentiles": pct}
+            "breakdown": breakdown, "step_ms_percentiles": pct, "remap_wg_per_cu": tuned}
 
 
 def main():
@@ -214,11 +218,14 @@ def main():
     ap.add_argument("--plain-cells", type=int, default=128)
     ap.add_argument("--fast", action="store_true", help="fused-multiply-add remap arithmetic (fast_arith=1)")
     ap.add_argument("--storage", choices=list(STORES), default="f32", help="storage form of f in HBM")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="keep the default remap grid (128 workgroups per CU) instead of timing candidates")
     ap.add_argument("--no-forms", dest="forms", action="store_false",
                     help="skip the short runs of the other storage forms")
     args = ap.parse_args()
-    global FAST
+    global FAST, AUTOTUNE
     FAST = args.fast
+    AUTOTUNE = args.autotune
 
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
@@ -254,7 +261,8 @@ def main():
                       "method": ["PLM", "PPM", "PQM"][w.method], "storage": args.storage,
                       "arith": "fast" if FAST else "exact",
                       "adjust_every_step": w.adjust_every_step, "blocks_at_start": r["blocks_at_start"],
-                      "rotation_deg_per_step": w.rotation_deg},
+                      "rotation_deg_per_step": w.rotation_deg,
+                      "remap_wg_per_cu": r["remap_wg_per_cu"]},
            "roofline": r["roofline"], "breakdown_ms_per_step": r["breakdown"],
            "step_ms_percentiles": r["step_ms_percentiles"]}
     if args.forms:

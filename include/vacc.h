@@ -166,6 +166,12 @@ int vacc_adjust_blocks(vacc_ctx* ctx, double* rho_loss);
  * blocks than the reserve holds. */
 int vacc_adjust_blocks_spatial(vacc_ctx* ctx, const uint32_t* nbr_off, const uint32_t* nbr, const uint64_t* extra_off,
                                const uint32_t* extra_gids, double* rho_loss);
+/* Launch-shape autotuner (no counterpart in the reference): times the remap of each pass
+ * dimension on the current state with 8, 32, 64, 128, 256 and 512 workgroups per compute
+ * unit (twice each) and keeps the fastest per dimension for later steps.  The trial passes
+ * write scratch buffers only, so the state is unchanged; results of later steps do not depend
+ * on the choice.  mult_out (3 entries, may be NULL) receives the chosen multipliers. */
+int vacc_autotune(vacc_ctx* ctx, uint32_t* mult_out);
 /* FP64 moments per cell: out[4*c+0] = n, [1..3] = n*V (arch_moments.h:83-108).  They are
  * kept resident until f changes, so a following use_moments transform does not recompute
  * them. */

@@ -517,3 +517,27 @@ def test_step_graph_replays_match_eager_steps():
             cases.assert_cells_equal(graph.download_cells(), eager.download_cells(), f"graph step {step}")
     eager.close()
     graph.close()
+
+
+def test_autotune_keeps_state_and_results():
+    """vacc_autotune times trial remaps into scratch buffers: the state is unchanged, and
+    steps after tuning equal steps of an untuned context bit for bit."""
+    w = W.config(2, ncells=6)
+    cells = cases.build_cells(w)
+    fields = [V.make_fields(c.B, c.V, c.dt) for c in w.cells]
+    a, b = _ctx(w), _ctx(w)
+    for c in (a, b):
+        c.upload_cells(cells)
+        c.set_fields(fields)
+    _say("autotune")
+    mult = a.autotune()
+    assert all(m in (8, 32, 64, 128, 256, 512) for m in mult)
+    cases.assert_cells_equal(a.download_cells(), cells, "state after autotune")
+    for step in range(3):
+        la, lb = np.zeros(len(cells)), np.zeros(len(cells))
+        a.step(step % 3, use_moments=True, adjust=True, rho_loss=la)
+        b.step(step % 3, use_moments=True, adjust=True, rho_loss=lb)
+        assert np.array_equal(la, lb)
+    cases.assert_cells_equal(a.download_cells(), b.download_cells(), "tuned vs untuned")
+    a.close()
+    b.close()

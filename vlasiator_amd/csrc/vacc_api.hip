@@ -92,6 +92,7 @@ struct vacc_ctx {
    };
    std::vector<StepGraph> graphs;
    uint64_t gen = 0;
+   int remap_mult[3] = {128, 128, 128};   // remap workgroups per CU, per pass dimension (vacc_autotune)
    uint32_t* h_sum = nullptr;        // pinned host words: capacity flag, block total, status OR
    // event timing
    struct Rec {
@@ -288,7 +289,10 @@ static int reserve_for_step(vacc_ctx* c) {
 static float cell_f(const vacc_ctx* c, int d) { return (float)c->mi.cell[d]; }
 
 // One 1-D remap of every cell.
-static int do_pass(vacc_ctx* c, int dim) {
+// Column build and remap of one pass from (lcur, fcur) into the other list and pool; the
+// caller flips the parity.  grid_mult: persistent remap workgroups per CU (0: the tuned or
+// default value).
+static int enqueue_pass(vacc_ctx* c, int dim, int grid_mult) {
    const int li = c->lcur, lo = 1 - c->lcur, fi = c->fcur, fo = 1 - c->fcur;
    ColumnArgs ca{};
    ca.cell_off = c->off[li];
@@ -358,9 +362,17 @@ static int do_pass(vacc_ctx* c, int dim) {
    ra.min_value = c->sp.sparse_min_value;
    {
       ProfScope ps(c, 0);
-      launch_remap(dim, c->opt.method, c->opt.fast_arith != 0, ra, 0xFFFFFFFFu, c->stream);
+      launch_remap(dim, c->opt.method, c->opt.fast_arith != 0, ra, grid_mult ? grid_mult : c->remap_mult[dim],
+                   c->stream);
    }
    HIPCHK(hipGetLastError());
+   return 0;
+}
+
+static int do_pass(vacc_ctx* c, int dim) {
+   const int li = c->lcur, lo = 1 - c->lcur, fo = 1 - c->fcur;
+   const int rc = enqueue_pass(c, dim, 0);
+   if (rc) return rc;
    if (c->prof && c->nlog < PROF_LOG) {
       HIPCHK(hipMemcpyAsync(c->prof_counts + 2 * c->nlog, c->off[li] + c->ncells, sizeof(uint32_t),
                             hipMemcpyDeviceToDevice, c->stream));
@@ -832,6 +844,60 @@ int vacc_adjust_blocks_spatial(vacc_ctx* c, const uint32_t* nbr_off, const uint3
    if (rc < 0) return rc;
    if (rho_loss) HIPCHK(hipMemcpy(rho_loss, c->rho_loss, c->ncells * sizeof(double), hipMemcpyDeviceToHost));
    return rc;
+}
+
+int vacc_autotune(vacc_ctx* c, uint32_t* mult_out) {
+   if (!c) return set_err(VACC_E_ARG, "null context");
+   if (!c->ncells) return 0;
+   int rc = reserve_for_step(c);
+   if (rc) return rc;
+   if (c->fields_valid) enqueue_transform(c, 0, 0);   // intersections of the current fields
+   // the trial passes write the other list and pool (scratch) and the status words; keep the
+   // status words and leave the state as it was
+   uint32_t* saved = nullptr;
+   HIPCHK(hipMalloc(&saved, ((size_t)c->ncells + 1) * sizeof(uint32_t)));
+   HIPCHK(hipMemcpyAsync(saved, c->status, ((size_t)c->ncells + 1) * sizeof(uint32_t), hipMemcpyDeviceToDevice,
+                         c->stream));
+   const bool prof = c->prof;
+   c->prof = false;
+   static const int cand[] = {8, 32, 64, 128, 256, 512};
+   hipEvent_t e0, e1;
+   HIPCHK(hipEventCreate(&e0));
+   HIPCHK(hipEventCreate(&e1));
+   for (int dim = 0; dim < 3 && rc == 0; ++dim) {
+      float best = 1e30f;
+      int arg = c->remap_mult[dim];
+      for (int rep = 0; rep < 2 && rc == 0; ++rep)
+         for (int m : cand) {
+            (void)hipEventRecord(e0, c->stream);
+            rc = enqueue_pass(c, dim, m);
+            (void)hipEventRecord(e1, c->stream);
+            if (rc) break;
+            HIPCHK(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best) {
+               best = ms;
+               arg = m;
+            }
+         }
+      c->remap_mult[dim] = arg;
+   }
+   (void)hipEventDestroy(e0);
+   (void)hipEventDestroy(e1);
+   c->prof = prof;
+   HIPCHK(hipMemcpyAsync(c->status, saved, ((size_t)c->ncells + 1) * sizeof(uint32_t), hipMemcpyDeviceToDevice,
+                         c->stream));
+   HIPCHK(hipStreamSynchronize(c->stream));
+   HIPCHK(hipFree(saved));
+   if (rc) return rc;
+   c->content_valid = false;   // the trial passes wrote content flags of the scratch list
+   drop_graphs(c);             // captured steps hold the old launch shape
+   const int fl = finish(c);
+   if (fl < 0) return fl;
+   if (mult_out)
+      for (int d = 0; d < 3; ++d) mult_out[d] = (uint32_t)c->remap_mult[d];
+   return 0;
 }
 
 // The launches of one step (transform, three passes, adjust) on the context stream.

@@ -158,7 +158,8 @@ void launch_content(const uint32_t* src, const Pool& pool, uint8_t* content, uin
 void launch_columns(int dim, const ColumnArgs& a, hipStream_t s);
 void launch_scan(const ScanArgs& a, hipStream_t s);
 void launch_order(const OrderArgs& a, hipStream_t s);
-void launch_remap(int dim, int method, bool fast, const RemapArgs& a, uint32_t max_sets, hipStream_t s);
+// grid: grid_mult persistent workgroups per CU
+void launch_remap(int dim, int method, bool fast, const RemapArgs& a, int grid_mult, hipStream_t s);
 void launch_adjust(const AdjustArgs& a, bool write_phase, hipStream_t s);
 void launch_moments(const MomentArgs& a, hipStream_t s);
 // plasma pivot from the moments of up to MAX_POP populations holding the same cells
@@ -183,6 +184,6 @@ void launch_decode(const Pool& in, float* f, uint32_t n, hipStream_t s);
 void launch_iota(uint32_t* dst, uint32_t n, hipStream_t s);
 void launch_fill_u32(uint32_t* dst, uint32_t v, uint32_t n, hipStream_t s);
 
-int remap_grid_size();
+int device_cus();   // compute units of the current device
 
 }  // namespace vacc

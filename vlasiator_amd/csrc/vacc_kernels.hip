@@ -1256,16 +1256,15 @@ void launch_scan(const ScanArgs& a, hipStream_t s) {
    hipLaunchKernelGGL(k_scan2, dim3(1), dim3(1024), 0, s, a);
 }
 
-static int g_remap_grid = 0;
-int remap_grid_size() {
-   if (g_remap_grid == 0) {
+static int g_cus = 0;
+int device_cus() {
+   if (g_cus == 0) {
       int dev = 0, cus = 0;
       (void)hipGetDevice(&dev);
       (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-      if (cus <= 0) cus = 256;
-      g_remap_grid = cus * 4;
+      g_cus = (cus > 0) ? cus : 256;
    }
-   return g_remap_grid;
+   return g_cus;
 }
 
 template <int DIM, int METHOD>
@@ -1291,11 +1290,9 @@ static void remap_d(int method, const RemapArgs& a, uint32_t grid, double i_dv, 
    else remap_dm<DIM, VACC_PQM>(a, grid, i_dv, fast, s);
 }
 
-void launch_remap(int dim, int method, bool fast, const RemapArgs& a, uint32_t max_sets, hipStream_t s) {
-   if (a.ncells == 0 || max_sets == 0) return;
-   uint32_t grid = (uint32_t)remap_grid_size();
-   const uint64_t need = ((uint64_t)max_sets + (WG / 16) - 1) / (WG / 16);
-   if (need < grid) grid = (uint32_t)need;
+void launch_remap(int dim, int method, bool fast, const RemapArgs& a, int grid_mult, hipStream_t s) {
+   if (a.ncells == 0) return;
+   const uint32_t grid = (uint32_t)device_cus() * (uint32_t)std::max(grid_mult, 1);
    const double i_dv = 1.0 / (double)a.dv;   // cpu_acc_map.cpp:201, Real i_dv = 1.0/dv
    if (dim == 0) remap_d<0>(method, a, grid, i_dv, fast, s);
    else if (dim == 1) remap_d<1>(method, a, grid, i_dv, fast, s);

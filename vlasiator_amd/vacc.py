@@ -26,7 +26,7 @@ EXPORTS = [
     "vacc_compute_cell_intersections", "vacc_cell_intersections_host", "vacc_max_v_dt", "vacc_map_1d",
     "vacc_accelerate_cells", "vacc_adjust_blocks", "vacc_moments", "vacc_sync", "vacc_get_status",
     "vacc_init_peaks", "vacc_profile_begin", "vacc_profile_read", "vacc_set_fields", "vacc_step",
-    "vacc_accelerate_populations", "vacc_adjust_blocks_spatial",
+    "vacc_accelerate_populations", "vacc_adjust_blocks_spatial", "vacc_autotune",
 ]
 
 
@@ -106,6 +106,7 @@ def lib():
         L.vacc_step.argtypes = [vp, C.c_uint32, C.c_int32, C.c_int32, vp]
         L.vacc_accelerate_populations.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_int32, C.c_int32, vp]
         L.vacc_adjust_blocks_spatial.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.vacc_autotune.argtypes = [vp, vp]
         _lib = L
     return _lib
 
@@ -280,6 +281,12 @@ class Context:
         p = lambda a: None if a is None else _ptr(a)
         _check(lib().vacc_adjust_blocks_spatial(self.h, p(no), p(nb), p(eo), p(eg), _ptr(loss)))
         return loss[:ncells]
+
+    def autotune(self):
+        """Pick the persistent remap grid per pass dimension on the current state."""
+        out = np.zeros(3, np.uint32)
+        _check(lib().vacc_autotune(self.h, _ptr(out)))
+        return out.tolist()
 
     def moments(self, ncells):
         out = np.zeros(max(ncells, 1) * 4, np.float64)
