@@ -31,6 +31,8 @@
 namespace vacc {
 
 constexpr int WG = 256;
+constexpr int CWG = 512;         // k_columns workgroup: 8 waves share one cell's LDS bitmap
+constexpr int AWG = 512;         // k_adjust workgroup, likewise
 constexpr int BM_WORDS = 7168;   // LDS bitmap words per chunk of rows (LDS per workgroup <= 64 KiB)
 constexpr int ROW_MAX = 2048;    // rows per chunk (k_columns)
 
@@ -193,11 +195,11 @@ __device__ inline void column_extent(int ks, int ke, const SetGeom& g, float dv,
 // ---------------------------------------------------------------------------
 
 template <int DIM>
-__global__ __launch_bounds__(WG) void k_columns(ColumnArgs a) {
+__global__ __launch_bounds__(CWG) void k_columns(ColumnArgs a) {
    __shared__ uint32_t bm[BM_WORDS];
    __shared__ uint32_t rowv[4][ROW_MAX];
    __shared__ int sbox[6];
-   __shared__ uint32_t sh[4 * 4];
+   __shared__ uint32_t sh[4 * (CWG / 64)];
    __shared__ uint32_t carry[4];
    __shared__ uint32_t sstat;
    __shared__ uint32_t hist[LEN_BINS];
@@ -212,7 +214,7 @@ __global__ __launch_bounds__(WG) void k_columns(ColumnArgs a) {
       }
       return;
    }
-   for (int i = t; i < LEN_BINS; i += WG) hist[i] = 0;
+   for (int i = t; i < LEN_BINS; i += CWG) hist[i] = 0;
    if (t < 3) {
       sbox[t] = INT_MAX;
       sbox[t + 3] = INT_MIN;
@@ -222,7 +224,7 @@ __global__ __launch_bounds__(WG) void k_columns(ColumnArgs a) {
    __syncthreads();
    {
       int lo0 = INT_MAX, lo1 = INT_MAX, lok = INT_MAX, hi0 = INT_MIN, hi1 = INT_MIN, hik = INT_MIN;
-      for (uint32_t i = t; i < n; i += WG) {
+      for (uint32_t i = t; i < n; i += CWG) {
          int b0, b1, k;
          pass_coords<DIM>(a.gid[off + i], a.len, b0, b1, k);
          lo0 = min(lo0, b0); hi0 = max(hi0, b0);
@@ -248,9 +250,9 @@ __global__ __launch_bounds__(WG) void k_columns(ColumnArgs a) {
 
    for (int r0 = 0; r0 < rows; r0 += rpc) {
       const int nr = min(rpc, rows - r0);
-      for (int i = t; i < nr * W; i += WG) bm[i] = 0;
+      for (int i = t; i < nr * W; i += CWG) bm[i] = 0;
       __syncthreads();
-      for (uint32_t i = t; i < n; i += WG) {
+      for (uint32_t i = t; i < n; i += CWG) {
          int b0, b1, k;
          pass_coords<DIM>(a.gid[off + i], a.len, b0, b1, k);
          const int row = (b0 - mn0) * R1 + (b1 - mn1) - r0;
@@ -263,7 +265,7 @@ __global__ __launch_bounds__(WG) void k_columns(ColumnArgs a) {
       // per-row counts: blocks, columns, work units, target blocks.  The columns of a set are
       // separate work units unless the target ranges of two of them share a block, in which
       // case the set is one unit (its columns must add into the shared block in order).
-      for (int rr = t; rr < nr; rr += WG) {
+      for (int rr = t; rr < nr; rr += CWG) {
          const uint32_t* w = bm + rr * W;
          uint32_t nb = 0;
          for (int q = 0; q < W; ++q) nb += __popc(w[q]);
@@ -304,7 +306,7 @@ __global__ __launch_bounds__(WG) void k_columns(ColumnArgs a) {
       }
       __syncthreads();
       // exclusive scan of the four row quantities (contiguous chunk of rows per thread)
-      const int per = (nr + WG - 1) / WG;
+      const int per = (nr + CWG - 1) / CWG;
       const int rb = t * per, re = min(nr, rb + per);
       uint32_t s4[4] = {0, 0, 0, 0}, ex[4], tot[4];
       for (int i = rb; i < re; ++i)
@@ -322,7 +324,7 @@ __global__ __launch_bounds__(WG) void k_columns(ColumnArgs a) {
       }
       __syncthreads();
       // emit set and column records
-      for (int rr = t; rr < nr; rr += WG) {
+      for (int rr = t; rr < nr; rr += CWG) {
          const uint32_t* w = bm + rr * W;
          bool any = false;
          for (int q = 0; q < W; ++q) any |= (w[q] != 0);
@@ -375,7 +377,7 @@ __global__ __launch_bounds__(WG) void k_columns(ColumnArgs a) {
          }
       }
       // source index of every block in sorted (set, k) order
-      for (uint32_t i = t; i < n; i += WG) {
+      for (uint32_t i = t; i < n; i += CWG) {
          int b0, b1, k;
          pass_coords<DIM>(a.gid[off + i], a.len, b0, b1, k);
          const int row = (b0 - mn0) * R1 + (b1 - mn1) - r0;
@@ -398,7 +400,7 @@ __global__ __launch_bounds__(WG) void k_columns(ColumnArgs a) {
          atomicOr(&a.status[a.ncells], sstat);   // OR over all cells
       }
    }
-   for (int i = t; i < LEN_BINS; i += WG)
+   for (int i = t; i < LEN_BINS; i += CWG)
       if (hist[i]) atomicAdd(&a.len_hist[i], hist[i]);
 }
 
@@ -779,7 +781,7 @@ __device__ inline void dilate_box1(uint32_t* bm, int W, int Rx, int Ry, int Rz) 
    const int t = threadIdx.x;
    const int rows = Ry * Rz;
    const uint32_t last = ((Rx & 31) == 0) ? 0xFFFFFFFFu : ((1u << (Rx & 31)) - 1u);
-   for (int r = t; r < rows; r += WG) {            // x: one thread per row, words in order
+   for (int r = t; r < rows; r += (int)blockDim.x) {            // x: one thread per row, words in order
       uint32_t* row = bm + r * W;
       uint32_t prev = 0u, cur = row[0];
       for (int q = 0; q < W; ++q) {
@@ -792,7 +794,7 @@ __device__ inline void dilate_box1(uint32_t* bm, int W, int Rx, int Ry, int Rz) 
       }
    }
    __syncthreads();
-   for (int i = t; i < Rz * W; i += WG) {           // y: one thread per (z, word) column
+   for (int i = t; i < Rz * W; i += (int)blockDim.x) {           // y: one thread per (z, word) column
       const int z = i / W, q = i - z * W;
       uint32_t* col = bm + (size_t)z * Ry * W + q;
       uint32_t prev = 0u, cur = col[0];
@@ -804,7 +806,7 @@ __device__ inline void dilate_box1(uint32_t* bm, int W, int Rx, int Ry, int Rz) 
       }
    }
    __syncthreads();
-   for (int i = t; i < Ry * W; i += WG) {           // z: one thread per (y, word) column
+   for (int i = t; i < Ry * W; i += (int)blockDim.x) {           // z: one thread per (y, word) column
       uint32_t* col = bm + i;
       const int stride = Ry * W;
       uint32_t prev = 0u, cur = col[0];
@@ -819,13 +821,13 @@ __device__ inline void dilate_box1(uint32_t* bm, int W, int Rx, int Ry, int Rz) 
 }
 
 template <bool WRITE>
-__global__ __launch_bounds__(WG) void k_adjust(AdjustArgs a) {
+__global__ __launch_bounds__(AWG) void k_adjust(AdjustArgs a) {
    __shared__ uint32_t bm[BM_WORDS];
    __shared__ uint32_t wpre[WRITE ? BM_WORDS : 1];
    __shared__ int sbox[6];
-   __shared__ uint32_t sh[4];
+   __shared__ uint32_t sh[AWG / 64];
    __shared__ uint32_t carry;
-   __shared__ double wloss[WG / 64];
+   __shared__ double wloss[AWG / 64];
    if (WRITE && (*a.flags & VACC_STATUS_CAPACITY)) return;
    const int t = threadIdx.x;
    const uint32_t c = blockIdx.x;
@@ -843,11 +845,11 @@ __global__ __launch_bounds__(WG) void k_adjust(AdjustArgs a) {
       if (a.nbr_off)
          for (uint32_t k = a.nbr_off[c]; k < a.nbr_off[c + 1]; ++k) {
             const uint32_t o = a.cell_off[a.nbr[k]], m = a.cell_off[a.nbr[k] + 1] - o;
-            for (uint32_t i = t; i < m; i += WG)
+            for (uint32_t i = t; i < m; i += AWG)
                if (a.content[o + i]) fn(a.gid[o + i]);
          }
       if (a.ext_off)
-         for (uint64_t i = a.ext_off[c] + t; i < a.ext_off[c + 1]; i += WG) {
+         for (uint64_t i = a.ext_off[c] + t; i < a.ext_off[c + 1]; i += AWG) {
             const uint32_t g = a.ext_gid[i];
             if (g < ntot) fn(g);
          }
@@ -880,7 +882,7 @@ __global__ __launch_bounds__(WG) void k_adjust(AdjustArgs a) {
          ly = min(ly, y); hy = max(hy, y);
          lz = min(lz, z); hz = max(hz, z);
       };
-      for (uint32_t i = t; i < n; i += WG) grow(a.gid[off + i]);
+      for (uint32_t i = t; i < n; i += AWG) grow(a.gid[off + i]);
       for_spatial(grow);
       atomicMin(&sbox[0], lx); atomicMin(&sbox[1], ly); atomicMin(&sbox[2], lz);
       atomicMax(&sbox[3], hx); atomicMax(&sbox[4], hy); atomicMax(&sbox[5], hz);
@@ -904,10 +906,10 @@ __global__ __launch_bounds__(WG) void k_adjust(AdjustArgs a) {
    double myloss = 0.0;
    for (int r0 = 0; r0 < rows; r0 += rpc) {
       const int nr = min(rpc, rows - r0);
-      for (int i = t; i < nr * W; i += WG) bm[i] = 0;
+      for (int i = t; i < nr * W; i += AWG) bm[i] = 0;
       __syncthreads();
       if (separable) {
-         for (uint32_t i = t; i < n; i += WG) {
+         for (uint32_t i = t; i < n; i += AWG) {
             if (!a.content[off + i]) continue;
             const uint32_t g = a.gid[off + i];
             const int x = (int)(g % nx), y = (int)((g / nx) % ny), z = (int)(g / (nx * ny));
@@ -931,7 +933,7 @@ __global__ __launch_bounds__(WG) void k_adjust(AdjustArgs a) {
             atomicOr(&bm[row * W + (kx >> 5)], 1u << (kx & 31));
          }
       });
-      for (uint32_t i = t; i < n; i += WG) {
+      for (uint32_t i = t; i < n; i += AWG) {
          const uint32_t g = a.gid[off + i];
          const int x = (int)(g % nx), y = (int)((g / nx) % ny), z = (int)(g / (nx * ny));
          if (w < 0) {
@@ -962,7 +964,7 @@ __global__ __launch_bounds__(WG) void k_adjust(AdjustArgs a) {
       __syncthreads();
       // word prefix counts over the chunk (contiguous words per thread)
       const int nwords = nr * W;
-      const int per = (nwords + WG - 1) / WG;
+      const int per = (nwords + AWG - 1) / AWG;
       const int wb = t * per, we = min(nwords, wb + per);
       uint32_t s1[1] = {0}, ex1[1], tot1[1];
       for (int q = wb; q < we; ++q) s1[0] += __popc(bm[q]);
@@ -975,7 +977,7 @@ __global__ __launch_bounds__(WG) void k_adjust(AdjustArgs a) {
          }
          __syncthreads();
          const uint32_t obase = a.out_off[c];
-         for (int q = t; q < nwords; q += WG) {
+         for (int q = t; q < nwords; q += AWG) {
             uint32_t x = bm[q];
             uint32_t pos = wpre[q];
             const int row = q / W + r0;
@@ -990,7 +992,7 @@ __global__ __launch_bounds__(WG) void k_adjust(AdjustArgs a) {
             }
          }
          __syncthreads();
-         for (uint32_t i = t; i < n; i += WG) {
+         for (uint32_t i = t; i < n; i += AWG) {
             const uint32_t g = a.gid[off + i];
             const int x = (int)(g % nx), y = (int)((g / nx) % ny), z = (int)(g / (nx * ny));
             const int row = (z - mnz) * Ry + (y - mny) - r0;
@@ -1006,7 +1008,7 @@ __global__ __launch_bounds__(WG) void k_adjust(AdjustArgs a) {
       } else {
          // removed blocks: no content and not kept; their mass goes to RHOLOSSADJUST
          // (spatial_cell_cpu.cpp:227-267), one thread per block, 16 row loads in flight
-         for (uint32_t i = t; i < n; i += WG) {
+         for (uint32_t i = t; i < n; i += AWG) {
             if (w < 0 || a.content[off + i]) continue;
             const uint32_t g = a.gid[off + i];
             const int x = (int)(g % nx), y = (int)((g / nx) % ny), z = (int)(g / (nx * ny));
@@ -1037,7 +1039,7 @@ __global__ __launch_bounds__(WG) void k_adjust(AdjustArgs a) {
       __syncthreads();
       if (t == 0) {
          double sum = 0.0;
-         for (int q = 0; q < WG / 64; ++q) sum += wloss[q];
+         for (int q = 0; q < AWG / 64; ++q) sum += wloss[q];
          a.n_keep[c] = carry;
          if (a.rho_loss) a.rho_loss[c] = sum;
       }
@@ -1242,9 +1244,9 @@ __global__ void k_fill_u32(uint32_t* dst, uint32_t v, uint32_t n) {
 
 void launch_columns(int dim, const ColumnArgs& a, hipStream_t s) {
    if (a.ncells == 0) return;
-   if (dim == 0) hipLaunchKernelGGL(k_columns<0>, dim3(a.ncells), dim3(WG), 0, s, a);
-   else if (dim == 1) hipLaunchKernelGGL(k_columns<1>, dim3(a.ncells), dim3(WG), 0, s, a);
-   else hipLaunchKernelGGL(k_columns<2>, dim3(a.ncells), dim3(WG), 0, s, a);
+   if (dim == 0) hipLaunchKernelGGL(k_columns<0>, dim3(a.ncells), dim3(CWG), 0, s, a);
+   else if (dim == 1) hipLaunchKernelGGL(k_columns<1>, dim3(a.ncells), dim3(CWG), 0, s, a);
+   else hipLaunchKernelGGL(k_columns<2>, dim3(a.ncells), dim3(CWG), 0, s, a);
 }
 
 void launch_order(const OrderArgs& a, hipStream_t s) {
@@ -1301,8 +1303,8 @@ void launch_remap(int dim, int method, bool fast, const RemapArgs& a, int grid_m
 
 void launch_adjust(const AdjustArgs& a, bool write_phase, hipStream_t s) {
    if (a.ncells == 0) return;
-   if (write_phase) hipLaunchKernelGGL(k_adjust<true>, dim3(a.ncells), dim3(WG), 0, s, a);
-   else hipLaunchKernelGGL(k_adjust<false>, dim3(a.ncells), dim3(WG), 0, s, a);
+   if (write_phase) hipLaunchKernelGGL(k_adjust<true>, dim3(a.ncells), dim3(AWG), 0, s, a);
+   else hipLaunchKernelGGL(k_adjust<false>, dim3(a.ncells), dim3(AWG), 0, s, a);
 }
 
 // Workgroups for one wave per block, capped so that a dispatch stays far below 2^32 work-items.
