@@ -356,11 +356,12 @@ def test_fast_arith_within_tolerance(cfg, method):
     ctx.close()
 
 
-@pytest.mark.parametrize("per_cell", [0, 1])
-def test_cpp_host_adapter_driver(tmp_path, per_cell):
+@pytest.mark.parametrize("per_cell,adjust_mode", [(0, 1), (1, 1), (0, 2)])
+def test_cpp_host_adapter_driver(tmp_path, per_cell, adjust_mode):
     """The C++ host adapter (vlasiator_amd/host/vacc_host.hpp) through its example driver:
-    three steps of intersections + remaps (batched, or one cell per launch) + adjust, checked
-    bit for bit against the oracle run with the intersections the driver computed."""
+    three steps of intersections + remaps (batched, or one cell per launch) + adjust
+    (single-cell, or with ring spatial neighbours), checked bit for bit against the oracle run
+    with the intersections the driver computed."""
     import os
     import struct
     import subprocess
@@ -372,7 +373,7 @@ def test_cpp_host_adapter_driver(tmp_path, per_cell):
     steps = 3
     inp = tmp_path / "in.bin"
     with open(inp, "wb") as fp:
-        fp.write(b"VACC" + struct.pack("<IIiiii", 1, len(cells), O.PPM, steps, per_cell, 1))
+        fp.write(b"VACC" + struct.pack("<IIiiii", 1, len(cells), O.PPM, steps, per_cell, adjust_mode))
         fp.write(struct.pack("<3I", *w.grid_length) + struct.pack("<3d", *w.vmin) + struct.pack("<3d", *w.vmax))
         fp.write(struct.pack("<3di", w.mass, w.charge, w.min_value, w.add_width))
         for c, (g, f) in zip(w.cells, cells):
@@ -398,10 +399,17 @@ def test_cpp_host_adapter_driver(tmp_path, per_cell):
         got.append((g, f))
     m = cases.oracle_mesh(w)
     ref = cells
+    n = len(cells)
     for s in range(steps):
-        ref = [O.adjust(m, *O.accelerate_cell(m, g, f, isx[s][i], s % 3, O.PPM, w.min_value)[:2], w.min_value,
-                        w.add_width)[:2] for i, (g, f) in enumerate(ref)]
-    cases.assert_cells_equal(got, ref, "C++ driver")
+        acc = [O.accelerate_cell(m, g, f, isx[s][i], s % 3, O.PPM, w.min_value)[:2] for i, (g, f) in enumerate(ref)]
+        if adjust_mode == 1:
+            ref = [O.adjust(m, g, f, w.min_value, w.add_width)[:2] for g, f in acc]
+        else:
+            content = [g[O.content_flags(f, w.min_value).astype(bool)] for g, f in acc]
+            ref = [O.adjust_spatial(m, g, f, w.min_value, w.add_width,
+                                    np.concatenate([content[(i - 1) % n], content[(i + 1) % n]]))[:2]
+                   for i, (g, f) in enumerate(acc)]
+    cases.assert_cells_equal(got, ref, f"C++ driver adjust {adjust_mode}")
     # the host intersections agree with the oracle transform
     assert np.allclose(isx[0], cases.oracle_intersections(w, 0), rtol=0,
                        atol=1e-12 * 4e6)

@@ -1,7 +1,8 @@
 // acc_driver.cpp -- example C++ host program over the adapter (vacc_host.hpp): the per-step
 // loop of calculateAcceleration (vlasovmover.cpp:410-475) for a set of spatial cells of one
 // population: per cell the FP64 transform and intersections, then the three remaps (batched,
-// or one cell per launch), then the single-cell block adjust.
+// or one cell per launch), then the block adjust: single-cell (adjust = 1), or with spatial
+// neighbours (adjust = 2: the cells form a ring, cell i neighbouring i-1 and i+1).
 //
 //   vacc_acc_driver <input> <output>
 //
@@ -91,7 +92,12 @@ int main(int argc, char** argv) {
          rc = engine.accelerate_cells(ptrs, 0, map_order);
          if (rc > 0) status_or |= rc;
       }
-      if (rc >= 0 && do_adjust) rc = engine.adjust_blocks(ptrs, 0);
+      if (rc >= 0 && do_adjust == 1) rc = engine.adjust_blocks(ptrs, 0);
+      if (rc >= 0 && do_adjust == 2) {
+         std::vector<std::vector<uint32_t>> ring(ncells);
+         for (uint32_t i = 0; i < ncells; ++i) ring[i] = {(i + ncells - 1) % ncells, (i + 1) % ncells};
+         rc = engine.adjust_blocks(ptrs, 0, ring);
+      }
       if (rc < 0) {
          fprintf(stderr, "acceleration: %s\n", engine.last_error().c_str());
          return 1;

@@ -17,6 +17,11 @@
 //   adjust_blocks(cells, popID)
 //                       <- SpatialCell::adjustSingleCellVelocityBlocks(popID, true),
 //                          spatial_cells/spatial_cell_cpu.cpp:275
+//   adjust_blocks(cells, popID, neighbours)
+//                       <- adjustVelocityBlocks(mpiGrid, cells, true, popID), grid.cpp:775-814:
+//                          SpatialCell::adjust_velocity_blocks(spatial_neighbors, popID, true),
+//                          spatial_cell_cpu.cpp:179-273, with the neighbours of cells[i] given as
+//                          indices into cells
 // Errors are reported as in the C ABI: the functions return the library's status code
 // (0 ok, > 0 per-cell status bits, < 0 error) and never abort.
 #pragma once
@@ -86,6 +91,14 @@ public:
       return run(cells, popID, 0, true);
    }
 
+   // Sparsity adjust with spatial neighbours: blocks with content in a neighbour are kept
+   // (or added) as they are.  neighbours[i] lists indices into cells.
+   int adjust_blocks(std::vector<SpatialCell*>& cells, uint32_t popID,
+                     const std::vector<std::vector<uint32_t>>& neighbours) {
+      if (neighbours.size() != cells.size()) return VACC_E_ARG;
+      return run(cells, popID, 0, true, &neighbours);
+   }
+
 private:
    static vacc_cell_fields fields(const SpatialCell& sc, double dt) {
       vacc_cell_fields fl{};
@@ -111,7 +124,8 @@ private:
    }
 
    // upload the cells' populations, remap or adjust, and write the new state back
-   int run(std::vector<SpatialCell*>& cells, uint32_t popID, uint32_t map_order, bool adjust) {
+   int run(std::vector<SpatialCell*>& cells, uint32_t popID, uint32_t map_order, bool adjust,
+           const std::vector<std::vector<uint32_t>>* neighbours = nullptr) {
       vacc_ctx* c = nullptr;
       int rc = context(popID, &c);
       if (rc) return rc;
@@ -130,7 +144,18 @@ private:
       if ((rc = vacc_upload_cells(c, (uint32_t)n, off.data(), gids.data(), f.data())) < 0) return rc;
       int st = 0;
       std::vector<double> loss(n, 0.0);
-      if (adjust) {
+      if (adjust && neighbours) {
+         std::vector<uint32_t> noff(n + 1, 0), nbr;
+         for (size_t i = 0; i < n; ++i) {
+            for (uint32_t j : (*neighbours)[i]) {
+               if (j >= n) return VACC_E_ARG;
+               nbr.push_back(j);
+            }
+            noff[i + 1] = (uint32_t)nbr.size();
+         }
+         if (nbr.empty()) nbr.push_back(0);
+         st = vacc_adjust_blocks_spatial(c, noff.data(), nbr.data(), nullptr, nullptr, loss.data());
+      } else if (adjust) {
          st = vacc_adjust_blocks(c, loss.data());
       } else {
          if ((rc = vacc_set_intersections(c, isect.data())) < 0) return rc;
