@@ -47,10 +47,11 @@ FAST = False
 AUTOTUNE = True
 
 
-def make_context(w: W.Workload, storage: str = "f32", graph: bool = False):
+def make_context(w: W.Workload, storage: str = "f32", graph: bool = False, fast: bool | None = None):
     mesh = V.make_mesh(w.grid_length, w.vmin, w.vmax)
     sp = V.make_species(w.mass, w.charge, w.min_value, w.add_width)
-    op = V.make_options(w.method, fast_arith=FAST, storage=STORES[storage], use_graph=graph)
+    op = V.make_options(w.method, fast_arith=FAST if fast is None else fast, storage=STORES[storage],
+                        use_graph=graph)
     return V.Context(mesh, sp, op)
 
 
@@ -128,7 +129,7 @@ def profiled_traffic(workload: str, storage: str):
     return None, None
 
 
-def measure(pops, storage: str, steps: int, warmup: int, graph: bool = False) -> dict:
+def measure(pops, storage: str, steps: int, warmup: int, graph: bool = False, fast: bool | None = None) -> dict:
     """Initial state of every population, W untimed steps, then K timed steps, each step
     ending in the library's one host synchronisation.  Several populations (config 4) step
     through vacc_accelerate_populations with the plasma-moment pivot.  graph=True replays
@@ -137,7 +138,7 @@ def measure(pops, storage: str, steps: int, warmup: int, graph: bool = False) ->
     tuned = []
     n_init = 0
     for w in pops:
-        ctx = make_context(w, storage, graph)
+        ctx = make_context(w, storage, graph, fast)
         t0 = time.perf_counter()
         ctx.init_peaks([W.to_vacc_peaks(c) for c in w.cells], W.boxes(w))
         n = ctx.total_blocks()
@@ -274,6 +275,12 @@ def main():
             fr = measure(pops, st, min(args.steps, 10), min(args.warmup, 2))
             forms[st] = {"value": fr["value"], "ms_per_step": fr["ms_per_step"], "steps": min(args.steps, 10),
                          "remap_GBps": fr["roofline"]["achieved"], "remap_frac": fr["roofline"]["frac"]}
+        # fused-multiply-add arithmetic (tolerance parity instead of bit parity), FP32 and BF16
+        for st in ("f32", "bf16"):
+            fr = measure(pops, st, min(args.steps, 10), min(args.warmup, 2), fast=not FAST)
+            forms[f"{st}-{'exact' if FAST else 'fast'}"] = {
+                "value": fr["value"], "ms_per_step": fr["ms_per_step"], "steps": min(args.steps, 10),
+                "remap_GBps": fr["roofline"]["achieved"], "remap_frac": fr["roofline"]["frac"]}
         out["storage_forms"] = forms
     if args.forms and len(pops) == 1:
         # the same main form with each step replayed from a captured hipGraph

@@ -1,6 +1,6 @@
 """Storage forms against an FP64 run: the multi-peak PQM workload, 20 steps of (accelerate,
-adjust), with f stored as FP32, BF16 or FP16 on the GPU (FP32 arithmetic) and the oracle's
-Realf = double build as the FP64 run.  All runs use the same FP64 intersections.
+adjust), with f stored as FP32, BF16 or FP16 on the GPU (FP32 arithmetic, exact or with
+fused multiply-adds) and the oracle's Realf = double build as the FP64 run.  All runs use the same FP64 intersections.
 
 Per cell: L1 and L-inf of f relative to the FP64 run (blocks missing on one side count as
 zero), and the relative error of density n, bulk velocity |V - V64| / |V64| and scalar
@@ -79,11 +79,12 @@ def test_storage_forms_against_fp64_run():
             g, f, _ = O.adjust(m, g, f, w.min_value, w.add_width, fp64=True)
         ref.append((g, f))
     report = {"workload": w.name, "cells": NCELLS, "steps": STEPS}
-    for name, store in (("f32", V.STORE_F32), ("bf16", V.STORE_BF16), ("f16", V.STORE_F16)):
-        print(f"GPU run, storage {name}", flush=True)
+    for name, store, fast in (("f32", V.STORE_F32, False), ("bf16", V.STORE_BF16, False), ("f16", V.STORE_F16, False),
+                              ("f32-fast", V.STORE_F32, True), ("bf16-fast", V.STORE_BF16, True)):
+        print(f"GPU run, {name}", flush=True)
         ctx = V.Context(V.make_mesh(w.grid_length, w.vmin, w.vmax),
                         V.make_species(w.mass, w.charge, w.min_value, w.add_width),
-                        V.make_options(w.method, storage=store))
+                        V.make_options(w.method, storage=store, fast_arith=fast))
         ctx.upload_cells(cells)
         for s in range(STEPS):
             ctx.set_intersections(isx[s % 3])
@@ -97,6 +98,6 @@ def test_storage_forms_against_fp64_run():
     if path:
         with open(path, "w") as fh:
             json.dump(report, fh, indent=1)
-    for name in ("f32", "bf16", "f16"):
+    for name in ("f32", "bf16", "f16", "f32-fast", "bf16-fast"):
         assert report[name]["n"] <= 1e-6, (name, report[name])
-        assert report[name]["l1"] <= L1_BOUND[name], (name, report[name])
+        assert report[name]["l1"] <= L1_BOUND[name.split("-")[0]], (name, report[name])
