@@ -549,3 +549,42 @@ def test_autotune_keeps_state_and_results():
     cases.assert_cells_equal(a.download_cells(), b.download_cells(), "tuned vs untuned")
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("method", [O.PPM, O.PQM])
+def test_overlapping_column_targets_bit_exact(method):
+    """Column sets whose columns' target ranges share blocks (the remap keeps those blocks
+    open across the column boundary): every other block along z in a 6x6 block patch, and a
+    z-pass with a transverse shear of one cell per cell (di = dj = dv) so that the corner
+    spread of a set exceeds the one-block gap between its columns.  The intersection is
+    offset by -180 cells so that the sheared targets of the patch (block indices 20..25 in
+    x and y, i.e. a shear of 160..206 cells) stay inside the 200-cell z extent."""
+    w = W.config(0, ncells=1)
+    m = cases.oracle_mesh(w)
+    nx, ny, nz = w.grid_length
+    rng = np.random.default_rng(9)
+    gids = np.array([x + y * nx + z * nx * ny for x in range(20, 26) for y in range(20, 26)
+                     for z in range(10, 40, 2)], np.uint32)
+    f = (1e-12 * rng.random((gids.size, 64))).astype(np.float32)
+    dv = (w.vmax[2] - w.vmin[2]) / nz / 4
+    isx = np.zeros((1, 12))
+    isx[0, 0:4] = (w.vmin[0], 0.0, 0.0, 1.0)
+    isx[0, 4:8] = (w.vmin[1], 0.0, 0.0, 1.0)
+    isx[0, 8:12] = (w.vmin[2] + (0.3 - 180.0) * dv, dv, dv, 1.01 * dv)
+    _say(f"overlapping column targets method={method}")
+    ctx = _ctx(w, method)
+    ctx.upload_cells([(gids, f)])
+    ctx.set_intersections(isx)
+    assert ctx.map_1d(2) == 0
+    g2, f2, _ = O.map_1d(m, gids, f, isx[0, 8:12], 2, method, w.min_value)
+    got = ctx.download_cells()
+    cases.assert_cells_equal(got, [(g2, f2)], "overlapping targets")
+    # the case is meaningful: the mass stays inside the mesh, and remapping each column on its
+    # own yields more target blocks in total than the joint remap, i.e. columns of a set share
+    # target blocks
+    assert abs(f2.astype(np.float64).sum() / f.astype(np.float64).sum() - 1.0) < 1e-6
+    z = gids // (nx * ny)
+    alone = sum(O.map_1d(m, gids[z == k], f[z == k], isx[0, 8:12], 2, method, w.min_value)[0].size
+                for k in range(10, 40, 2))
+    assert alone > g2.size
+    ctx.close()

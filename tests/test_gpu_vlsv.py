@@ -1,0 +1,182 @@
+"""GPU checks of VLSV block I/O: libvacc writes a context's blocks into a VLSV file and
+loads them back (iowrite.cpp:141-318, ioread.cpp:371-700), against the oracle's numpy
+restatement of the same arrays.  Integer arrays and f are compared bit for bit; node
+coordinates too (same FP64 expression vmin + i * dv)."""
+import time
+
+import numpy as np
+import pytest
+
+import oracle as O
+from oracle import vlsv as OV
+from tests import cases
+from vlasiator_amd import vacc as V
+from vlasiator_amd import vlsv as L
+from vlasiator_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _ctx(w, storage=V.STORE_F32, grid_length=None, vmin=None, vmax=None):
+    mesh = V.make_mesh(grid_length or w.grid_length, vmin or w.vmin, vmax or w.vmax)
+    return V.Context(mesh, V.make_species(w.mass, w.charge, w.min_value, w.add_width),
+                     V.make_options(w.method, storage=storage))
+
+
+@pytest.fixture(scope="module")
+def stepped():
+    """Config-0 cells after one acceleration step and adjust (the oracle's result)."""
+    w = W.config(0, ncells=8)
+    cells = cases.build_cells(w)
+    isx = cases.oracle_intersections(w, 1)
+    m = cases.oracle_mesh(w)
+    out = []
+    for i, (g, f) in enumerate(cells):
+        g2, f2, _ = O.accelerate_cell(m, g, f, isx[i], 1, w.method, w.min_value)
+        out.append(O.adjust(m, g2, f2, w.min_value, w.add_width)[:2])
+    return w, [cases.sort_cell(g, f) for g, f in out]
+
+
+@pytest.mark.parametrize("storage", [V.STORE_F32, V.STORE_BF16])
+def test_write_velocity_distribution_matches_oracle_arrays(tmp_path, stepped, storage):
+    w, cells = stepped
+    _say(f"vlsv write storage={storage}")
+    ctx = _ctx(w, storage)
+    ctx.upload_cells(cells)
+    held = ctx.download_cells()     # what the context holds (requantized for BF16)
+    ids = np.arange(100, 100 + len(cells), dtype=np.uint64)
+    p = tmp_path / "restart.vlsv"
+    with L.Writer(str(p)) as wr:
+        wr.write_velocity_distribution(ctx, "proton", ids)
+    cases.assert_cells_equal(ctx.download_cells(), held, "state after write")
+    want = OV.velocity_distribution_arrays(w.grid_length, w.vmin, w.vmax, "proton", ids, held)
+    off, got = OV.read(p)
+    assert len(got) == len(want)
+    for (t1, a1, x1), (t2, a2, x2) in zip(got, want):
+        assert t1 == t2 and all(a1[k] == v for k, v in a2.items()), (t1, a1, a2)
+        assert x1.dtype == x2.dtype and np.array_equal(x1.reshape(x2.shape), x2), t1
+    # byte for byte the oracle's file of the same arrays, up to the footer
+    q = tmp_path / "oracle.vlsv"
+    OV.write(q, want)
+    assert p.read_bytes()[:off] == q.read_bytes()[:off]
+    ctx.close()
+
+
+@pytest.mark.parametrize("fdtype", [np.float32, np.float64])
+def test_read_block_data_matches_oracle(tmp_path, stepped, fdtype):
+    w, cells = stepped
+    rng = np.random.default_rng(11)
+    shuffled = []
+    for g, f in cells:
+        o = rng.permutation(len(g))
+        shuffled.append((g[o], f[o].astype(fdtype) * (1 + 1e-9 if fdtype == np.float64 else 1)))
+    ids = np.arange(1, len(cells) + 1, dtype=np.uint64)
+    p = tmp_path / "in.vlsv"
+    OV.write(p, OV.velocity_distribution_arrays(w.grid_length, w.vmin, w.vmax, "proton", ids, shuffled))
+    _say(f"vlsv read f as {np.dtype(fdtype).name}")
+    want_ids, want = OV.read_block_data(p, w.grid_length, w.vmin, w.vmax, "proton")
+    ctx = _ctx(w)
+    with L.Reader(str(p)) as r:
+        got_ids = r.read_block_data(ctx, "proton")
+    assert np.array_equal(got_ids, want_ids)
+    cases.assert_cells_equal(ctx.download_cells(), want, "read_block_data")
+    # a range of cells (one process's share)
+    with L.Reader(str(p)) as r:
+        got_ids = r.read_block_data(ctx, "proton", cell_offset=2, ncells=3)
+    want_ids, want = OV.read_block_data(p, w.grid_length, w.vmin, w.vmax, "proton", cell_offset=2, ncells=3)
+    assert np.array_equal(got_ids, want_ids)
+    cases.assert_cells_equal(ctx.download_cells(), want, "cell range")
+    ctx.close()
+
+
+def test_read_into_a_larger_velocity_mesh(tmp_path, stepped):
+    """A file written on the 50^3-block mesh read into a 60 x 58 x 64 mesh with the same
+    cell size, whose minimum lies 5, 3 and 9 blocks lower (ioread.cpp:520-611)."""
+    w, cells = stepped
+    dv = [OV.cell_size(w.vmin[d], w.vmax[d], w.grid_length[d]) for d in range(3)]
+    big_L = (60, 58, 64)
+    lo = (5, 3, 9)
+    big_min = tuple(w.vmin[d] - lo[d] * 4 * dv[d] for d in range(3))
+    big_max = tuple(big_min[d] + big_L[d] * 4 * dv[d] for d in range(3))
+    p = tmp_path / "small.vlsv"
+    ids = np.arange(1, len(cells) + 1, dtype=np.uint64)
+    OV.write(p, OV.velocity_distribution_arrays(w.grid_length, w.vmin, w.vmax, "proton", ids, cells))
+    want_ids, want = OV.read_block_data(p, big_L, big_min, big_max, "proton")
+    _say("vlsv read into larger mesh")
+    ctx = _ctx(w, grid_length=big_L, vmin=big_min, vmax=big_max)
+    with L.Reader(str(p)) as r:
+        r.read_block_data(ctx, "proton")
+    cases.assert_cells_equal(ctx.download_cells(), want, "renumbered")
+    ctx.close()
+    small = _ctx(w, grid_length=(40, 50, 50))
+    with L.Reader(str(p)) as r:
+        with pytest.raises(V.VaccError, match="shrink"):
+            r.read_block_data(small, "proton")
+    small.close()
+
+
+def test_restart_round_trip_then_step(tmp_path):
+    """Write, read into a fresh context, and one acceleration step on both: identical."""
+    w = W.config(0, ncells=6)
+    cells = cases.build_cells(w)
+    isx = cases.oracle_intersections(w, 2)
+    a = _ctx(w)
+    a.upload_cells(cells)
+    p = tmp_path / "rt.vlsv"
+    _say("vlsv restart round trip")
+    with L.Writer(str(p)) as wr:
+        wr.write_velocity_distribution(a, "proton")
+    b = _ctx(w)
+    with L.Reader(str(p)) as r:
+        r.read_block_data(b, "proton")
+    for c in (a, b):
+        c.set_intersections(isx)
+        c.accelerate(2)
+        c.adjust(len(cells))
+    cases.assert_cells_equal(b.download_cells(), a.download_cells(), "after step")
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("storage", [V.STORE_F32, V.STORE_F16])
+def test_multi_chunk_transfer(tmp_path, storage):
+    """300 000 blocks (more than two 131 072-block transfer chunks) out and back in."""
+    w = W.config(0, ncells=1)
+    rng = np.random.default_rng(12)
+    ncell, per = 60, 5000
+    gmax = int(np.prod(w.grid_length))
+    cells = []
+    for _ in range(ncell):
+        g = np.sort(rng.choice(gmax, per, replace=False)).astype(np.uint32)
+        cells.append((g, (rng.random((per, 64)) * 1e-12).astype(np.float32)))
+    a = _ctx(w, storage)
+    a.upload_cells(cells)
+    held = a.download_cells()
+    p = tmp_path / "big.vlsv"
+    _say(f"vlsv multi-chunk storage={storage}")
+    t0 = time.perf_counter()
+    with L.Writer(str(p)) as wr:
+        wr.write_velocity_distribution(a, "proton")
+    t1 = time.perf_counter()
+    b = _ctx(w, storage)
+    with L.Reader(str(p)) as r:
+        r.read_block_data(b, "proton")
+    t2 = time.perf_counter()
+    nbytes = ncell * per * 256
+    _say(f"write {nbytes / (t1 - t0) / 1e9:.2f} GB/s, read {nbytes / (t2 - t1) / 1e9:.2f} GB/s")
+    got = b.download_cells()
+    if storage == V.STORE_F32:
+        cases.assert_cells_equal(got, held, "multi-chunk")
+    else:
+        # re-encoding decoded blocks need not give the same codes; values stay within a code
+        # step of the block maximum (2^-11 for FP16)
+        for (g1, f1), (g2, f2) in zip(got, held):
+            assert np.array_equal(g1, g2)
+            tol = 2.0 ** -10 * np.abs(f2).max(axis=1, keepdims=True)
+            assert np.all(np.abs(f1 - f2) <= tol)
+    a.close()
+    b.close()

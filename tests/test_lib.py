@@ -10,21 +10,23 @@ from vlasiator_amd import vacc as V
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared_symbols():
-    txt = open(os.path.join(ROOT, "include", "vacc.h")).read()
+def declared_symbols(header="vacc.h"):
+    txt = open(os.path.join(ROOT, "include", header)).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
     return sorted(set(re.findall(r"\b(vacc_[a-z0-9_]+)\s*\(", txt)))
 
 
 def test_header_declares_bound_symbols():
+    from vlasiator_amd import vlsv
     assert sorted(V.EXPORTS) == declared_symbols()
+    assert sorted(vlsv.EXPORTS) == declared_symbols("vacc_vlsv.h")
 
 
 def test_library_exports_declared_symbols():
     if not os.path.exists(V.LIB_PATH):
         pytest.skip("libvacc.so not built")
     lib = V.lib()
-    for s in declared_symbols():
+    for s in declared_symbols() + declared_symbols("vacc_vlsv.h"):
         assert hasattr(lib, s), s
 
 

@@ -10,6 +10,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIB_DIR, "libvacc.so")
 SOURCES = ["vacc_kernels.hip", "vacc_init.hip", "vacc_api.hip"]
+CXX_SOURCES = ["vacc_vlsv.cpp"]   # host-only C++ (no HIP)
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("VACC_OFFLOAD_ARCH", "gfx950")
 
@@ -29,6 +30,14 @@ def build(verbose: bool = False) -> str:
         src = os.path.join(CSRC, s)
         obj = os.path.join(LIB_DIR, s.replace(".hip", ".o"))
         cmd = [HIPCC, *FLAGS, "-c", src, "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        objs.append(obj)
+    for s in CXX_SOURCES:
+        src = os.path.join(CSRC, s)
+        obj = os.path.join(LIB_DIR, s.replace(".cpp", ".o"))
+        cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-fPIC", "-Wall", "-c", src, "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)

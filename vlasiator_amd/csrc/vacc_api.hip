@@ -13,9 +13,11 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <string>
 #include <vector>
 
+#include "vacc_blocks_io.h"
 #include "vacc_common.h"
 #include "vacc_kernels.h"
 
@@ -94,6 +96,11 @@ struct vacc_ctx {
    uint64_t gen = 0;
    int remap_mult[3] = {128, 128, 128};   // remap workgroups per CU, per pass dimension (vacc_autotune)
    uint32_t* h_sum = nullptr;        // pinned host words: capacity flag, block total, status OR
+   // streamed block transfer (vacc_blocks_io.h): two pinned chunks, their copy events and,
+   // for the 16-bit forms, a device FP32 stage of two chunks; allocated on first use
+   float* io_pin[2] = {nullptr, nullptr};
+   float* io_dev = nullptr;
+   hipEvent_t io_ev[2] = {nullptr, nullptr};
    // event timing
    struct Rec {
       hipEvent_t a, b;
@@ -489,6 +496,177 @@ static int status_or(vacc_ctx* c) {
    return (int)(c->h_sum[2] & 0x7FFFFFFFu);
 }
 
+// First half of a load of new cells: checks the lists, sizes the buffers and copies the
+// offsets and GIDs; the caller then fills pool F[0] and calls end_load.
+static int begin_load(vacc_ctx* c, uint32_t ncells, const uint64_t* offsets, const uint32_t* gids) {
+   const uint64_t total = offsets[ncells];
+   if (total > 0xF0000000ull) return set_err(VACC_E_ARG, "too many blocks");
+   const uint64_t gmax = (uint64_t)c->mi.len[0] * c->mi.len[1] * c->mi.len[2];
+   std::vector<uint32_t> off32(ncells + 1);
+   for (uint32_t i = 0; i <= ncells; ++i) {
+      if (i && offsets[i] < offsets[i - 1]) return set_err(VACC_E_ARG, "offsets not ascending");
+      off32[i] = (uint32_t)offsets[i];
+   }
+   for (uint64_t i = 0; i < total; ++i)
+      if (gids[i] >= gmax) return set_err(VACC_E_ARG, "GID outside the velocity mesh");
+   int rc = alloc_cells(c, ncells);
+   if (rc) return rc;
+   HIPCHK(hipStreamSynchronize(c->stream));
+   c->ncells = ncells;
+   c->nblocks = 0;
+   c->lcur = 0;
+   c->fcur = 0;
+   const size_t want = total + total / 2 + 64 * (size_t)ncells + 4096;
+   if (want > c->cap) {
+      rc = ensure_blocks(c, want);
+      if (rc) return rc;
+   }
+   HIPCHK(hipMemcpyAsync(c->off[0], off32.data(), (ncells + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+   if (total) HIPCHK(hipMemcpyAsync(c->gid[0], gids, total * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+   HIPCHK(hipStreamSynchronize(c->stream));   // the caller's arrays may go away after return
+   return 0;
+}
+
+// A load that failed after begin_load leaves an empty context (no cells).
+static void abort_load(vacc_ctx* c) {
+   (void)hipStreamSynchronize(c->stream);
+   c->ncells = 0;
+   c->nblocks = 0;
+   (void)hipMemsetAsync(c->off[0], 0, sizeof(uint32_t), c->stream);
+   (void)hipStreamSynchronize(c->stream);
+   c->content_valid = c->moments_valid = c->fields_valid = false;
+}
+
+static int end_load(vacc_ctx* c, uint64_t total) {
+   launch_iota(c->src[0], (uint32_t)total, c->stream);
+   HIPCHK(hipMemsetAsync(c->status, 0, (c->ncells + 1) * sizeof(uint32_t), c->stream));
+   HIPCHK(hipMemsetAsync(c->flags, 0, sizeof(uint32_t), c->stream));
+   HIPCHK(hipStreamSynchronize(c->stream));
+   c->nblocks = total;
+   c->content_valid = false;
+   c->moments_valid = false;
+   c->fields_valid = false;
+   return 0;
+}
+
+// Streamed block transfer: chunk size in blocks (32 MiB of FP32 values per pinned buffer).
+static constexpr uint64_t IO_CHUNK = 1u << 17;
+
+static int ensure_io(vacc_ctx* c) {
+   for (int i = 0; i < 2; ++i) {
+      if (!c->io_pin[i]) HIPCHK(hipHostMalloc(&c->io_pin[i], IO_CHUNK * WID3 * sizeof(float), hipHostMallocDefault));
+      if (!c->io_ev[i]) HIPCHK(hipEventCreateWithFlags(&c->io_ev[i], hipEventDisableTiming));
+   }
+   if (c->opt.storage != VACC_STORE_F32 && !c->io_dev)
+      HIPCHK(hipMalloc(&c->io_dev, 2 * IO_CHUNK * WID3 * sizeof(float)));
+   return 0;
+}
+
+static Pool pool_at(const vacc_ctx* c, int i, uint64_t first) {
+   Pool p = pool(c, i);
+   p.f = (char*)p.f + first * WID3 * elem_bytes(c);
+   if (p.scale) p.scale += first;
+   return p;
+}
+
+namespace vacc {
+
+int set_error(int code, const std::string& msg) { return set_err(code, msg); }
+
+void ctx_mesh(const vacc_ctx* c, vacc_mesh* mesh, double cell[3]) {
+   *mesh = c->mesh;
+   for (int d = 0; d < 3; ++d) cell[d] = c->mi.cell[d];
+}
+
+int export_lists(vacc_ctx* c, std::vector<uint32_t>& counts, std::vector<uint32_t>& gids) {
+   int rc = refresh_count(c);
+   if (rc) return rc;
+   rc = do_relist(c, -1, false);    // ascending GID order, same blocks
+   if (rc) return rc;
+   rc = do_compact(c);
+   if (rc) return rc;
+   std::vector<uint32_t> off(c->ncells + 1);
+   HIPCHK(hipMemcpyAsync(off.data(), c->off[c->lcur], (c->ncells + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+   HIPCHK(hipStreamSynchronize(c->stream));
+   counts.resize(c->ncells);
+   for (uint32_t i = 0; i < c->ncells; ++i) counts[i] = off[i + 1] - off[i];
+   const size_t n = off[c->ncells];
+   gids.resize(n);
+   if (n) HIPCHK(hipMemcpy(gids.data(), c->gid[c->lcur], n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+   return 0;
+}
+
+// Chunk k goes to pinned buffer k % 2: the copy (and decode) of chunk k + 1 is issued before
+// the host consumes chunk k.
+int export_values(vacc_ctx* c, const BlockSink& sink) {
+   const uint64_t n = c->nblocks;
+   if (!n) return 0;
+   int rc = ensure_io(c);
+   if (rc) return rc;
+   const uint64_t nchunks = (n + IO_CHUNK - 1) / IO_CHUNK;
+   auto issue = [&](uint64_t k) -> int {
+      const uint64_t first = k * IO_CHUNK, m = std::min(IO_CHUNK, n - first);
+      float* dst = c->io_pin[k & 1];
+      if (c->opt.storage == VACC_STORE_F32) {
+         HIPCHK(hipMemcpyAsync(dst, (const float*)c->f[c->fcur] + first * WID3, m * WID3 * sizeof(float),
+                               hipMemcpyDeviceToHost, c->stream));
+      } else {
+         float* st = c->io_dev + (k & 1) * IO_CHUNK * WID3;
+         launch_decode(pool_at(c, c->fcur, first), st, (uint32_t)m, c->stream);
+         HIPCHK(hipGetLastError());
+         HIPCHK(hipMemcpyAsync(dst, st, m * WID3 * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+      }
+      HIPCHK(hipEventRecord(c->io_ev[k & 1], c->stream));
+      return 0;
+   };
+   if ((rc = issue(0))) return rc;
+   for (uint64_t k = 0; k < nchunks; ++k) {
+      if (k + 1 < nchunks && (rc = issue(k + 1))) return rc;
+      HIPCHK(hipEventSynchronize(c->io_ev[k & 1]));
+      const uint64_t first = k * IO_CHUNK;
+      if ((rc = sink(c->io_pin[k & 1], first, std::min(IO_CHUNK, n - first)))) {
+         (void)hipStreamSynchronize(c->stream);
+         return rc;
+      }
+   }
+   return 0;
+}
+
+// Chunk k is filled into pinned buffer k % 2 once the copy of chunk k - 2 out of it is done.
+int import_blocks(vacc_ctx* c, uint32_t ncells, const uint64_t* offsets, const uint32_t* gids,
+                  const BlockSource& source) {
+   if (!c || !offsets || (ncells && !gids)) return set_err(VACC_E_ARG, "null argument");
+   int rc = begin_load(c, ncells, offsets, gids);
+   if (rc) return rc;
+   const uint64_t n = offsets[ncells];
+   if (n) {
+      if ((rc = ensure_io(c))) return rc;
+      const uint64_t nchunks = (n + IO_CHUNK - 1) / IO_CHUNK;
+      for (uint64_t k = 0; k < nchunks; ++k) {
+         const uint64_t first = k * IO_CHUNK, m = std::min(IO_CHUNK, n - first);
+         float* buf = c->io_pin[k & 1];
+         if (k >= 2) HIPCHK(hipEventSynchronize(c->io_ev[k & 1]));
+         if ((rc = source(buf, first, m))) {
+            abort_load(c);
+            return rc;
+         }
+         if (c->opt.storage == VACC_STORE_F32) {
+            HIPCHK(hipMemcpyAsync((float*)c->f[0] + first * WID3, buf, m * WID3 * sizeof(float), hipMemcpyHostToDevice,
+                                  c->stream));
+         } else {
+            float* st = c->io_dev + (k & 1) * IO_CHUNK * WID3;
+            HIPCHK(hipMemcpyAsync(st, buf, m * WID3 * sizeof(float), hipMemcpyHostToDevice, c->stream));
+            launch_encode(st, pool_at(c, 0, first), (uint32_t)m, c->stream);
+            HIPCHK(hipGetLastError());
+         }
+         HIPCHK(hipEventRecord(c->io_ev[k & 1], c->stream));
+      }
+   }
+   return end_load(c, n);
+}
+
+}  // namespace vacc
+
 extern "C" {
 
 const char* vacc_last_error(void) { return g_err.c_str(); }
@@ -541,6 +719,11 @@ int vacc_destroy(vacc_ctx* c) {
    if (c->flags) (void)hipFree(c->flags);
    if (c->h_sum) (void)hipHostFree(c->h_sum);
    if (c->mom_ev) (void)hipEventDestroy(c->mom_ev);
+   for (int i = 0; i < 2; ++i) {
+      if (c->io_pin[i]) (void)hipHostFree(c->io_pin[i]);
+      if (c->io_ev[i]) (void)hipEventDestroy(c->io_ev[i]);
+   }
+   if (c->io_dev) (void)hipFree(c->io_dev);
    void* sp[] = {c->d_nbr_off, c->d_nbr, c->d_ext_off, c->d_ext};
    for (void* p : sp)
       if (p) (void)hipFree(p);
@@ -551,31 +734,10 @@ int vacc_destroy(vacc_ctx* c) {
 
 int vacc_upload_cells(vacc_ctx* c, uint32_t ncells, const uint64_t* offsets, const uint32_t* gids, const float* f) {
    if (!c || !offsets || (ncells && (!gids || !f))) return set_err(VACC_E_ARG, "null argument");
-   const uint64_t total = offsets[ncells];
-   if (total > 0xF0000000ull) return set_err(VACC_E_ARG, "too many blocks");
-   const uint64_t gmax = (uint64_t)c->mi.len[0] * c->mi.len[1] * c->mi.len[2];
-   std::vector<uint32_t> off32(ncells + 1);
-   for (uint32_t i = 0; i <= ncells; ++i) {
-      if (i && offsets[i] < offsets[i - 1]) return set_err(VACC_E_ARG, "offsets not ascending");
-      off32[i] = (uint32_t)offsets[i];
-   }
-   for (uint64_t i = 0; i < total; ++i)
-      if (gids[i] >= gmax) return set_err(VACC_E_ARG, "GID outside the velocity mesh");
-   int rc = alloc_cells(c, ncells);
+   int rc = begin_load(c, ncells, offsets, gids);
    if (rc) return rc;
-   HIPCHK(hipStreamSynchronize(c->stream));
-   c->ncells = ncells;
-   c->nblocks = 0;
-   c->lcur = 0;
-   c->fcur = 0;
-   const size_t want = total + total / 2 + 64 * (size_t)ncells + 4096;
-   if (want > c->cap) {
-      rc = ensure_blocks(c, want);
-      if (rc) return rc;
-   }
-   HIPCHK(hipMemcpyAsync(c->off[0], off32.data(), (ncells + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+   const uint64_t total = offsets[ncells];
    if (total) {
-      HIPCHK(hipMemcpyAsync(c->gid[0], gids, total * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
       if (c->opt.storage == VACC_STORE_F32) {
          HIPCHK(hipMemcpyAsync(c->f[0], f, total * WID3 * sizeof(float), hipMemcpyHostToDevice, c->stream));
       } else {
@@ -587,15 +749,7 @@ int vacc_upload_cells(vacc_ctx* c, uint32_t ncells, const uint64_t* offsets, con
          HIPCHK(hipFree(stage));
       }
    }
-   launch_iota(c->src[0], (uint32_t)total, c->stream);
-   HIPCHK(hipMemsetAsync(c->status, 0, (ncells + 1) * sizeof(uint32_t), c->stream));
-   HIPCHK(hipMemsetAsync(c->flags, 0, sizeof(uint32_t), c->stream));
-   HIPCHK(hipStreamSynchronize(c->stream));
-   c->nblocks = total;
-   c->content_valid = false;
-   c->moments_valid = false;
-   c->fields_valid = false;
-   return 0;
+   return end_load(c, total);
 }
 
 int vacc_total_blocks(vacc_ctx* c, uint64_t* total) {
