@@ -21,6 +21,7 @@ import argparse
 import json
 import os
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -29,6 +30,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 from vlasiator_amd import vacc as V  # noqa: E402
+from vlasiator_amd import vlsv as VL  # noqa: E402
 from vlasiator_amd import workloads as W  # noqa: E402
 
 METRIC = "velocity-cell updates/s per 3-D accel step; % HBM peak; FP32 vs BF16-store"
@@ -127,6 +129,46 @@ def profiled_traffic(workload: str, storage: str):
             if len(p) == 4 and p[0] == "traffic-ratio" and p[1] == workload and p[2] == storage:
                 return float(p[3]), os.path.relpath(path, ROOT)
     return None, None
+
+
+def io_form(w, ctx):
+    """VLSV block I/O of the state held by ctx (the sample cells' initial state): the restart
+    arrays written to a file under the temporary directory (page-cache writes, no fsync),
+    the same arrays written to /dev/null (HBM -> host streaming and the host-side packing
+    alone), and the file read back into a fresh context."""
+    nblocks = ctx.total_blocks()
+    nbytes = nblocks * 64 * 4
+    d = tempfile.mkdtemp(prefix="vacc_io_")
+    path = os.path.join(d, "restart.vlsv")
+    try:
+        t0 = time.perf_counter()
+        with VL.Writer(path) as wr:
+            wr.write_velocity_distribution(ctx, "proton")
+        t_write = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        with VL.Writer(os.devnull) as wr:
+            wr.write_velocity_distribution(ctx, "proton")
+        t_null = time.perf_counter() - t0
+        c2 = make_context(w)
+        t0 = time.perf_counter()
+        with VL.Reader(path) as rd:
+            rd.read_block_data(c2, "proton")
+        t_read = time.perf_counter() - t0
+        ok = c2.total_blocks() == nblocks
+        c2.close()
+        size = os.path.getsize(path)
+    finally:
+        if os.path.exists(path):
+            os.remove(path)
+        os.rmdir(d)
+    if not ok:
+        raise RuntimeError("VLSV read-back holds a different block count")
+    log(f"vlsv io: {nblocks} blocks, write {nbytes / t_write / 1e9:.2f} GB/s, to /dev/null "
+        f"{nbytes / t_null / 1e9:.2f} GB/s, read {nbytes / t_read / 1e9:.2f} GB/s")
+    return {"cells": len(w.cells), "blocks": nblocks, "f_bytes": nbytes, "file_bytes": size,
+            "write_GBps": round(nbytes / t_write / 1e9, 2), "write_devnull_GBps": round(nbytes / t_null / 1e9, 2),
+            "read_GBps": round(nbytes / t_read / 1e9, 2), "unit": "GB/s of f (FP32)",
+            "file_dir": "tempfile.gettempdir(), page cache"}
 
 
 def measure(pops, storage: str, steps: int, warmup: int, graph: bool = False, fast: bool | None = None) -> dict:
@@ -251,8 +293,10 @@ def main():
         sample_cells = sctx.download_cells()
         sctx.compute_intersections(fields[:sample_n], 0, use_moments=True)
         isect0 = sctx.get_intersections(sample_n)
+        io = io_form(sw, sctx) if args.forms else None
         sctx.close()
-
+    else:
+        io = None
     r = measure(pops, args.storage, args.steps, args.warmup)
     out = {"metric": METRIC, "value": r["value"], "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak",
@@ -287,6 +331,8 @@ def main():
         gr = measure(pops, args.storage, min(args.steps, 10), min(args.warmup, 3), graph=True)
         out["graph_form"] = {"value": gr["value"], "ms_per_step": gr["ms_per_step"], "steps": min(args.steps, 10)}
 
+    if io is not None:
+        out["vlsv_io"] = io
     if args.plain_cells and sample_cells is not None:
         npl = min(args.plain_cells, len(sample_cells))
         log(f"plain form: {npl} single-cell contexts")

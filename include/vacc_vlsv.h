@@ -20,7 +20,7 @@
  *   vacc_vlsv_read_block_data
  *        <- readBlockData(vlsv::ParallelReader&, meshName, fileCells, localCellStartOffset,
  *           localCells, mpiGrid) ioread.cpp:488 and _readBlockData ioread.cpp:371
- *           (velocity-mesh growth by GID renumbering, ioread.cpp:520-616; f read as float,
+ *           (velocity-mesh growth by GID renumbering, ioread.cpp:520-631; f read as float,
  *           double, int or uint of 4 or 8 bytes, ioread.cpp:667-700)
  *   vacc_vlsv_create / vacc_vlsv_write_array / vacc_vlsv_close
  *        <- vlsv::Writer::open / writeArray / close of the fmihpc/vlsv library as called at

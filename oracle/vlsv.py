@@ -126,7 +126,7 @@ def velocity_distribution_arrays(grid_length, vmin, vmax, pop, cell_ids, cells):
 def read_block_data(path, grid_length, vmin, vmax, pop, mesh_name=SPATIAL_MESH, cell_offset=0, ncells=None):
     """readBlockData + _readBlockData (ioread.cpp:371-700) for one process reading cells
     [cell_offset, cell_offset + ncells): returns (cell IDs, [(gids, f float32)]).
-    A smaller file velocity mesh is placed into ours by renumbering (ioread.cpp:520-611);
+    A smaller file velocity mesh is placed into ours by renumbering (ioread.cpp:520-631);
     errors raise ValueError."""
     _, E = read(path)
     bbox = [int(x) for x in find(E, "MESH_BBOX", mesh=pop).reshape(-1)[:6]]

@@ -95,7 +95,7 @@ def test_read_block_data_matches_oracle(tmp_path, stepped, fdtype):
 
 def test_read_into_a_larger_velocity_mesh(tmp_path, stepped):
     """A file written on the 50^3-block mesh read into a 60 x 58 x 64 mesh with the same
-    cell size, whose minimum lies 5, 3 and 9 blocks lower (ioread.cpp:520-611)."""
+    cell size, whose minimum lies 5, 3 and 9 blocks lower (ioread.cpp:520-631)."""
     w, cells = stepped
     dv = [OV.cell_size(w.vmin[d], w.vmax[d], w.grid_length[d]) for d in range(3)]
     big_L = (60, 58, 64)

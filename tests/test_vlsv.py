@@ -5,7 +5,7 @@ oracle, and the oracle's velocity-mesh renumbering against block coordinates.
 What pins what: the footer's tags, mandatory attributes and offset values follow
 doc/vlsv2/documentation.lyx:85-276 and are checked here with an independent XML parser
 (xml.etree); the array list and attributes follow iowrite.cpp:141-318; the renumbering
-of a smaller file velocity mesh (ioread.cpp:520-611) is pinned by the physical invariant
+of a smaller file velocity mesh (ioread.cpp:520-631) is pinned by the physical invariant
 it must keep -- every block keeps its velocity coordinates.  The two header words (the
 endianness word, then the footer offset) follow the public fmihpc/vlsv library, which the
 reference tree does not hold: parity unpinned against files that library wrote.
@@ -155,3 +155,21 @@ def test_oracle_renumbering_keeps_block_velocities(tmp_path):
     off_min = (big_min[0] + 2 * dv, big_min[1], big_min[2])   # 2 cells: not a block boundary
     with pytest.raises(ValueError, match="block boundary"):
         OV.read_block_data(p, big_L, off_min, tuple(off_min[d] + big_L[d] * 4 * dv for d in range(3)), "proton")
+
+
+def test_large_arrays_split_over_io_threads(tmp_path):
+    """Arrays above the 8 MiB split size go through concurrent positioned reads and writes."""
+    from vlasiator_amd import vlsv as L
+    rng = np.random.default_rng(6)
+    a = rng.random((170001, 64)).astype(np.float32)          # 43.5 MB, not a multiple of 4 KiB
+    b = rng.integers(0, 2 ** 32, 3000001, dtype=np.uint64)     # 24 MB
+    p = tmp_path / "big.vlsv"
+    with L.Writer(str(p)) as w:
+        w.write_array("SMALL", {"name": "s"}, np.arange(3, dtype=np.int32))
+        w.write_array("BLOCKVARIABLE", {"name": "a"}, a)
+        w.write_array("IDS", {"name": "b"}, b)
+    _, got = OV.read(p)
+    assert np.array_equal(got[1][2], a) and np.array_equal(got[2][2], b)
+    with L.Reader(str(p)) as r:
+        assert np.array_equal(r.read_array("BLOCKVARIABLE", {"name": "a"}), a)
+        assert np.array_equal(r.read_array("IDS", {"name": "b"}, 17, 2500000), b[17:2500017])

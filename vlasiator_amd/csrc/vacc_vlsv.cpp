@@ -15,6 +15,7 @@
 #include <cstring>
 #include <map>
 #include <string>
+#include <thread>
 #include <utility>
 #include <vector>
 
@@ -62,35 +63,52 @@ bool clean(const std::string& s) { return s.find_first_of("<>&\"") == std::strin
 
 int io_error(const char* what) { return set_error(VACC_E_ARG, std::string(what) + ": " + strerror(errno)); }
 
-int write_all(int fd, const void* p, size_t n) {
-   const char* b = static_cast<const char*>(p);
+// Positioned I/O of n bytes at off; returns 0, an errno value, or -1 at an early end of file.
+int pio(int fd, char* b, size_t n, uint64_t off, bool wr) {
    while (n) {
-      const ssize_t k = ::write(fd, b, n);
+      const ssize_t k = wr ? ::pwrite(fd, b, n, (off_t)off) : ::pread(fd, b, n, (off_t)off);
       if (k < 0) {
          if (errno == EINTR) continue;
-         return io_error("write failed");
+         return errno;
       }
-      b += k;
-      n -= (size_t)k;
-   }
-   return 0;
-}
-
-int pread_all(int fd, void* p, size_t n, uint64_t off) {
-   char* b = static_cast<char*>(p);
-   while (n) {
-      const ssize_t k = ::pread(fd, b, n, (off_t)off);
-      if (k < 0) {
-         if (errno == EINTR) continue;
-         return io_error("read failed");
-      }
-      if (k == 0) return set_error(VACC_E_ARG, "unexpected end of file");
+      if (k == 0) return -1;
       b += k;
       n -= (size_t)k;
       off += (uint64_t)k;
    }
    return 0;
 }
+
+// Large transfers are split over IO_THREADS concurrent calls: a page-cache copy on one thread
+// runs at a fraction of the host's memory bandwidth.
+constexpr int IO_THREADS = 4;
+constexpr size_t IO_SPLIT_MIN = size_t(8) << 20;
+
+int pio_all(int fd, void* p, size_t n, uint64_t off, bool wr) {
+   char* b = static_cast<char*>(p);
+   int err = 0;
+   if (n < IO_SPLIT_MIN) {
+      err = pio(fd, b, n, off, wr);
+   } else {
+      const size_t part = ((n + IO_THREADS - 1) / IO_THREADS + 4095) & ~size_t(4095);
+      int rcs[IO_THREADS] = {0};
+      std::thread ts[IO_THREADS];
+      for (int t = 1; t < IO_THREADS; ++t) {
+         const size_t a = std::min(n, t * part), e = std::min(n, (t + 1) * part);
+         if (a < e) ts[t] = std::thread([=, &rcs] { rcs[t] = pio(fd, b + a, e - a, off + a, wr); });
+      }
+      rcs[0] = pio(fd, b, std::min(n, part), off, wr);
+      for (int t = 1; t < IO_THREADS; ++t)
+         if (ts[t].joinable()) ts[t].join();
+      for (int t = 0; t < IO_THREADS && !err; ++t) err = rcs[t];
+   }
+   if (err == -1) return set_error(VACC_E_ARG, "unexpected end of file");
+   if (err) return set_error(VACC_E_ARG, std::string(wr ? "write failed: " : "read failed: ") + strerror(err));
+   return 0;
+}
+
+int pread_all(int fd, void* p, size_t n, uint64_t off) { return pio_all(fd, p, n, off, false); }
+int pwrite_all(int fd, const void* p, size_t n, uint64_t off) { return pio_all(fd, const_cast<void*>(p), n, off, true); }
 
 int attrs_from(uint32_t nattr, const char* const* attr, Attrs& out) {
    if (nattr && !attr) return set_error(VACC_E_ARG, "null attribute list");
@@ -171,7 +189,7 @@ int put_array(vacc_vlsv_writer* w, const char* tag, const Attrs& attrs, uint64_t
    if (rc) return rc;
    if (e.bytes() && !data) return set_error(VACC_E_ARG, "null array data");
    e.offset = w->offset;
-   if ((rc = write_all(w->fd, data, e.bytes()))) return rc;
+   if ((rc = pwrite_all(w->fd, data, e.bytes(), w->offset))) return rc;
    w->offset += e.bytes();
    w->entries.push_back(std::move(e));
    return 0;
@@ -343,7 +361,7 @@ int vacc_vlsv_create(const char* path, vacc_vlsv_writer** out) {
    const int fd = ::open(path, O_WRONLY | O_CREAT | O_TRUNC | O_CLOEXEC, 0644);
    if (fd < 0) return io_error("cannot create file");
    const uint64_t head[2] = {0, 0};   // endianness word (byte 0 = 0: little-endian), footer offset
-   int rc = write_all(fd, head, sizeof head);
+   int rc = pwrite_all(fd, head, sizeof head, 0);
    if (rc) {
       ::close(fd);
       return rc;
@@ -397,8 +415,8 @@ int vacc_vlsv_write_velocity_distribution(vacc_vlsv_writer* w, vacc_ctx* ctx, co
    Entry e;
    if ((rc = make_entry("BLOCKVARIABLE", cell_attrs, nblocks, VACC_WID3, VACC_VLSV_FLOAT, 4, e))) return rc;
    e.offset = w->offset;
-   rc = vacc::export_values(ctx, [&](const float* f, uint64_t, uint64_t n) {
-      return write_all(w->fd, f, n * VACC_WID3 * sizeof(float));
+   rc = vacc::export_values(ctx, [&](const float* f, uint64_t first, uint64_t n) {
+      return pwrite_all(w->fd, f, n * VACC_WID3 * sizeof(float), e.offset + first * VACC_WID3 * sizeof(float));
    });
    if (rc) return rc;
    w->offset += e.bytes();
@@ -409,9 +427,8 @@ int vacc_vlsv_write_velocity_distribution(vacc_vlsv_writer* w, vacc_ctx* ctx, co
 int vacc_vlsv_close(vacc_vlsv_writer* w) {
    if (!w) return set_error(VACC_E_ARG, "null writer");
    const std::string x = footer_text(w->entries);
-   int rc = write_all(w->fd, x.data(), x.size());
-   if (!rc && ::pwrite(w->fd, &w->offset, sizeof(uint64_t), 8) != (ssize_t)sizeof(uint64_t))
-      rc = io_error("cannot write the footer offset");
+   int rc = pwrite_all(w->fd, x.data(), x.size(), w->offset);
+   if (!rc) rc = pwrite_all(w->fd, &w->offset, sizeof(uint64_t), 8);
    if (::close(w->fd) != 0 && !rc) rc = io_error("close failed");
    delete w;
    return rc;
@@ -497,7 +514,7 @@ int vacc_vlsv_read_block_data(vacc_vlsv_reader* r, vacc_ctx* ctx, const char* po
    uint64_t shift[3] = {0, 0, 0};
    const bool renumber = bbox[0] != m.grid_length[0] || bbox[1] != m.grid_length[1] || bbox[2] != m.grid_length[2];
    if (renumber) {
-      // ioread.cpp:520-616: the file's velocity mesh must sit inside ours at a block offset
+      // ioread.cpp:520-631: the file's velocity mesh must sit inside ours at a block offset
       if (bbox[3] != VACC_WID || bbox[4] != VACC_WID || bbox[5] != VACC_WID)
          return set_error(VACC_E_ARG, "file velocity blocks are not 4x4x4 cells");
       for (int d = 0; d < 3; ++d)
@@ -550,7 +567,7 @@ int vacc_vlsv_read_block_data(vacc_vlsv_reader* r, vacc_ctx* ctx, const char* po
    std::vector<uint32_t> gids;
    if ((rc = read_as(r, *ide, start, nb, gids))) return rc;
    if (renumber) {
-      for (auto& g : gids) {   // ioread.cpp:599-611
+      for (auto& g : gids) {   // ioread.cpp:617-629
          const uint64_t x = g % bbox[0], y = (g / bbox[0]) % bbox[1], z = g / (bbox[0] * bbox[1]);
          if (z >= bbox[2]) return set_error(VACC_E_ARG, "GID outside the file's velocity mesh");
          g = (uint32_t)((x + shift[0]) + (y + shift[1]) * m.grid_length[0] +
