@@ -231,6 +231,8 @@ typedef struct vacc_profile {
    double adjust_ms;          /* adjust count + scan + write */
    uint64_t transform_launches;
    double transform_ms;       /* moments + transform/intersections */
+   uint64_t trans_launches;
+   double trans_ms;           /* spatial translation (k_trans, include/vacc_trans.h) */
 } vacc_profile;
 int vacc_profile_begin(vacc_ctx* ctx);
 int vacc_profile_read(vacc_ctx* ctx, vacc_profile* out);

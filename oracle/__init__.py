@@ -58,6 +58,7 @@ _f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
 _i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
 _u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
 _u16p = np.ctypeslib.ndpointer(np.uint16, flags="C_CONTIGUOUS")
+_u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
 
 
 def lib():
@@ -101,6 +102,9 @@ def lib():
         L.vo_block_csum.restype = C.c_float
         L.vo_block_csum.argtypes = [_f32p]
         L.vo_requantize.argtypes = [C.c_uint32, _f32p, C.c_int]
+        L.vo_ppm_nu_batch.argtypes = [_f32p, _f32p, C.c_int, C.c_float, _f32p]
+        L.vo_trans_map_1d.argtypes = [C.POINTER(Mesh), C.c_uint32, _u64p, _u32p, _f32p, C.c_uint32, _u32p, _u64p,
+                                      _u32p, _f32p, _f32p, C.c_int, C.c_float, C.c_float]
         _lib = L
     return _lib
 
@@ -135,6 +139,8 @@ def ref_lib():
         R = C.CDLL(REF_PATH)
         R.ref_coeff_batch.argtypes = [C.c_int, _f32p, C.c_int, C.c_float, _f32p]
         R.ref_coeff_batch.restype = C.c_int
+        R.ref_ppm_nu_batch.argtypes = [_f32p, _f32p, C.c_int, C.c_float, _f32p]
+        R.ref_ppm_nu_batch.restype = C.c_int
         _ref = R
     return _ref
 
@@ -169,6 +175,51 @@ def ref_coeff_batch(method: int, stencils: np.ndarray, threshold: float) -> np.n
     out = np.zeros((n, ncoef(method)), dtype=np.float32)
     R.ref_coeff_batch(method, s.reshape(-1), n, np.float32(threshold), out.reshape(-1))
     return out
+
+
+def ppm_nu_batch(h5, stencils: np.ndarray, threshold: float) -> np.ndarray:
+    """PPM on non-uniform widths h5 (5 values) for n stencils of 5 values (source at 2)."""
+    h = np.ascontiguousarray(h5, dtype=np.float32)
+    s = np.ascontiguousarray(stencils, dtype=np.float32)
+    out = np.zeros((s.shape[0], 3), dtype=np.float32)
+    lib().vo_ppm_nu_batch(h, s.reshape(-1), s.shape[0], np.float32(threshold), out.reshape(-1))
+    return out
+
+
+def ref_ppm_nu_batch(h5, stencils: np.ndarray, threshold: float) -> np.ndarray:
+    R = ref_lib()
+    if R is None:
+        raise RuntimeError("oracle/_ref not built")
+    h = np.ascontiguousarray(h5, dtype=np.float32)
+    s = np.ascontiguousarray(stencils, dtype=np.float32)
+    out = np.zeros((s.shape[0], 3), dtype=np.float32)
+    R.ref_ppm_nu_batch(h, s.reshape(-1), s.shape[0], np.float32(threshold), out.reshape(-1))
+    return out
+
+
+TRANS_STENCIL = 2
+INVALID = 0xFFFFFFFF
+
+
+def trans_map_1d(mesh: Mesh, cells, pencils, dim: int, dt: float, threshold: float):
+    """Spatial translation of all cells along dim (trans_map_1d_amr).  cells: [(gids, f)];
+    pencils: (lengths, starts, cell indices, dz, ratios) in processing order.  Returns the
+    cells with the same block lists and new f."""
+    plen, pstart, pcell, pdz, pratio = pencils
+    counts = [len(g) for g, _ in cells]
+    off = np.zeros(len(cells) + 1, np.uint64)
+    off[1:] = np.cumsum(counts)
+    g = (np.concatenate([np.asarray(g, np.uint32) for g, _ in cells]) if cells else np.zeros(0, np.uint32))
+    f = (np.concatenate([np.asarray(f, np.float32).reshape(-1, 64) for _, f in cells]) if cells
+         else np.zeros((0, 64), np.float32))
+    f = np.ascontiguousarray(f)
+    rc = lib().vo_trans_map_1d(C.byref(mesh), len(cells), off, np.ascontiguousarray(g), f.reshape(-1), len(plen),
+                               np.ascontiguousarray(plen, np.uint32), np.ascontiguousarray(pstart, np.uint64),
+                               np.ascontiguousarray(pcell, np.uint32), np.ascontiguousarray(pdz, np.float32),
+                               np.ascontiguousarray(pratio, np.float32), dim, np.float32(dt), np.float32(threshold))
+    if rc:
+        raise RuntimeError("vo_trans_map_1d failed")
+    return [(g[off[i]:off[i + 1]].copy(), f[off[i]:off[i + 1]].copy()) for i in range(len(cells))]
 
 
 def sort_columns(mesh: Mesh, gids: np.ndarray, dim: int):

@@ -12,6 +12,7 @@
 #include "vlasovsolver/cpu_1d_plm.hpp"
 #include "vlasovsolver/cpu_1d_ppm.hpp"
 #include "vlasovsolver/cpu_1d_pqm.hpp"
+#include "vlasovsolver/cpu_1d_ppm_nonuniform.hpp"
 
 extern "C" {
 
@@ -33,6 +34,29 @@ int ref_coeff_batch(int method, const float* stencils, int n, float threshold, f
       else compute_pqm_coeff(values, h8, 4, a, threshold);
       for (int l = 0; l < VECL && base + l < n; ++l)
          for (int c = 0; c < nc; ++c) out[nc * (base + l) + c] = a[c][l];
+   }
+   return VECL;
+}
+
+// PPM on non-uniform cell widths, as the spatial translation calls it
+// (cpu_trans_map_amr.cpp:100-103: compute_ppm_coeff_nonuniform(dz, values, h4, 2, a, thr)).
+// h5: the 5 widths of the stencil (shared by every lane of a Vec, as in the reference);
+// stencils: n x 5 values, the source cell at index 2.  out: n x 3.
+int ref_ppm_nu_batch(const float* h5, const float* stencils, int n, float threshold, float* out) {
+   Realf h[5];
+   for (int t = 0; t < 5; ++t) h[t] = h5[t];
+   for (int base = 0; base < n; base += VECL) {
+      Vec values[5];
+      for (int t = 0; t < 5; ++t) {
+         for (int l = 0; l < VECL; ++l) {
+            const int s = (base + l < n) ? base + l : base;
+            values[t].insert(l, stencils[5 * s + t]);
+         }
+      }
+      Vec a[3];
+      compute_ppm_coeff_nonuniform(h, values, h4, 2, a, threshold);
+      for (int l = 0; l < VECL && base + l < n; ++l)
+         for (int c = 0; c < 3; ++c) out[3 * (base + l) + c] = a[c][l];
    }
    return VECL;
 }

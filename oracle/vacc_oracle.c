@@ -1245,3 +1245,226 @@ void vo_requantize(uint32_t nblocks, float* f, int storage) {
    }
 }
 #endif /* !VO_REALF_DOUBLE */
+
+/* ------------------------------------------------------------------------ */
+/* Spatial translation (cpu_trans_map_amr.cpp, cpu_1d_ppm_nonuniform.hpp)    */
+/* ------------------------------------------------------------------------ */
+
+/* h4 left face value on non-uniform cell widths h, cpu_face_estimates.hpp:179-191
+ * (White et al. 2008 eq. 45).  The h-only sub-expressions are float arithmetic except the
+ * terms that involve 1.0 or 2.0, which are double; each double scalar multiplies the Vec in
+ * double and rounds into the lane (dm). */
+static float h4nu_face(const float* h, const float* u, int k) {
+   const float s4 = ((h[k - 2] + h[k - 1]) + h[k]) + h[k + 1];
+   const double D1 = 1.0 / (double)s4;
+   const float SA = ((h[k - 2] + h[k - 1]) * (h[k] + h[k + 1])) / (h[k - 1] + h[k]);
+   const float V1 = u[k - 1] * h[k] + u[k] * h[k - 1];
+   const double DA = 1.0 / (double)((h[k - 2] + h[k - 1]) + h[k]) + 1.0 / (double)((h[k - 1] + h[k]) + h[k + 1]);
+   const float A = dm(DA, SA * V1);
+   const float SB = (h[k] * (h[k] + h[k + 1])) / (((h[k - 2] + h[k - 1]) + h[k]) * (h[k - 2] + h[k - 1]));
+   const double DB = (double)h[k - 2] + 2.0 * (double)h[k - 1];
+   const float V2 = dm(DB, u[k - 1]) - u[k - 2] * h[k - 1];
+   const float B = SB * V2;
+   const float SC = (h[k - 1] * (h[k - 2] + h[k - 1])) / (((h[k - 1] + h[k]) + h[k + 1]) * (h[k] + h[k + 1]));
+   const double DC = 2.0 * (double)h[k] + (double)h[k + 1];
+   const float V3 = dm(DC, u[k]) - u[k + 1] * h[k];
+   const float C = SC * V3;
+   return dm(D1, (A + B) + C);
+}
+
+/* Filtered faces on non-uniform widths, cpu_face_estimates.hpp:327-372 (order h4) */
+static void ppm_nu_faces(const float* h, const float* v, int k, float thr, float* pm, float* pp) {
+   float fvl = h4nu_face(h, v, k);
+   float fvr = h4nu_face(h, v, k + 1);
+   float sabs, ssign;
+   if (thr > 0.0f) {
+      limiter_scaled(v, k, thr, &sabs, &ssign);
+   } else {
+      const float s = superbee(v[k - 1], v[k], v[k + 1]);
+      sabs = fabsf(s);
+      ssign = (s > 0.0f) ? 1.0f : -1.0f;
+   }
+   if (sabs == 0.0f) {
+      fvr = v[k];
+      fvl = v[k];
+   }
+   {
+      const float p = (v[k - 1] - fvl) * (fvl - v[k]);
+      if (p < 0.0f) fvl = v[k] - dm(0.5, ssign) * sabs;
+   }
+   {
+      const float p = (v[k + 1] - fvr) * (fvr - v[k]);
+      if (p < 0.0f) fvr = v[k] + dm(0.5, ssign) * sabs;
+   }
+   *pm = fvl;
+   *pp = fvr;
+}
+
+/* PPM coefficients on non-uniform widths, cpu_1d_ppm_nonuniform.hpp:40-62 */
+void vo_ppm_nu_coeff(const float* h, const float* v, int k, float thr, float a[3]) {
+   float m, p;
+   ppm_nu_faces(h, v, k, thr, &m, &p);
+   const float one_sixth = (float)(1.0 / 6.0);
+   {
+      const float d = p - m;
+      const float mid = v[k] - dm(0.5, m + p);
+      if (d * mid > (d * d) * one_sixth) m = 3.0f * v[k] - 2.0f * p;
+   }
+   {
+      const float d = p - m;
+      const float mid = v[k] - dm(0.5, m + p);
+      if (((-d) * d) * one_sixth > d * mid) p = 3.0f * v[k] - 2.0f * m;
+   }
+   a[0] = m;
+   a[1] = (dm(3.0, v[k]) - dm(2.0, m)) - p;
+   a[2] = (m + p) - dm(2.0, v[k]);
+}
+
+void vo_ppm_nu_batch(const float* h5, const float* stencils, int n, float thr, float* out) {
+   for (int i = 0; i < n; ++i) vo_ppm_nu_coeff(h5, stencils + 5 * i, 2, thr, out + 3 * i);
+}
+
+typedef struct { uint32_t gid, idx; } vo_gi;
+
+static int cmp_gi(const void* a, const void* b) {
+   const uint32_t x = ((const vo_gi*)a)->gid, y = ((const vo_gi*)b)->gid;
+   return (x > y) - (x < y);
+}
+
+/* Index of block gid in cell c (sorted table), or VO_INVALID. */
+static uint32_t find_block(const vo_gi* tab, const uint64_t* toff, uint32_t c, uint32_t gid) {
+   uint64_t lo = toff[c], hi = toff[c + 1];
+   while (lo < hi) {
+      const uint64_t mid = (lo + hi) / 2;
+      if (tab[mid].gid < gid) lo = mid + 1;
+      else hi = mid;
+   }
+   return (lo < toff[c + 1] && tab[lo].gid == gid) ? tab[lo].idx : VO_INVALID;
+}
+
+/* Translation of every cell along dimension dim by dt over the given pencils
+ * (trans_map_1d_amr, cpu_trans_map_amr.cpp:214-441, with propagatePencil :36-149 and
+ * copy_trans_block_data_amr :162-198).  Pencil p covers positions [pstart[p],
+ * pstart[p] + plen[p]) of pcell (cell index or VO_INVALID), pdz (sourceDZ) and pratio
+ * (targetRatios); its first and last VO_TRANS_STENCIL positions are stencil cells.
+ * Pencils come in the reference's processing order (bins, then pencils in a bin).
+ * The block lists do not change; f is replaced in place.  Cells that are a target
+ * (ratio > 0) at any pencil position start from zero and receive the contributions in
+ * pencil order, then source position order; other cells keep their values. */
+int vo_trans_map_1d(const vo_mesh* m, uint32_t ncells, const uint64_t* offsets, const uint32_t* gids, float* f,
+                    uint32_t npencils, const uint32_t* plen, const uint64_t* pstart, const uint32_t* pcell,
+                    const float* pdz, const float* pratio, int dim, float dt, float threshold) {
+   const int W = VO_TRANS_STENCIL;
+   const uint64_t nb = offsets[ncells];
+   /* per-cell sorted GID tables */
+   vo_gi* tab = (vo_gi*)malloc((nb ? nb : 1) * sizeof(vo_gi));
+   float* src = (float*)malloc((nb ? nb : 1) * VO_WID3 * sizeof(float));
+   uint8_t* is_target = (uint8_t*)calloc(ncells ? ncells : 1, 1);
+   if (!tab || !src || !is_target) {
+      free(tab);
+      free(src);
+      free(is_target);
+      return -1;
+   }
+   for (uint32_t c = 0; c < ncells; ++c) {
+      for (uint64_t b = offsets[c]; b < offsets[c + 1]; ++b) {
+         tab[b].gid = gids[b];
+         tab[b].idx = (uint32_t)b;
+      }
+      qsort(tab + offsets[c], offsets[c + 1] - offsets[c], sizeof(vo_gi), cmp_gi);
+   }
+   memcpy(src, f, nb * VO_WID3 * sizeof(float));
+   /* targetCellsInBin (cpu_trans_pencils.hpp:49, binPencils): cells written by any pencil */
+   for (uint32_t p = 0; p < npencils; ++p)
+      for (uint32_t q = 0; q < plen[p]; ++q) {
+         const uint32_t c = pcell[pstart[p] + q];
+         if (c != VO_INVALID && pratio[pstart[p] + q] > 0.0f) is_target[c] = 1;
+      }
+   for (uint32_t c = 0; c < ncells; ++c)
+      if (is_target[c]) memset(f + offsets[c] * VO_WID3, 0, (offsets[c + 1] - offsets[c]) * VO_WID3 * sizeof(float));
+   /* union of block GIDs over all cells */
+   uint32_t* uni = (uint32_t*)malloc((nb ? nb : 1) * sizeof(uint32_t));
+   for (uint64_t b = 0; b < nb; ++b) uni[b] = gids[b];
+   qsort(uni, nb, sizeof(uint32_t), cmp_u32);
+   uint64_t nu = 0;
+   for (uint64_t b = 0; b < nb; ++b)
+      if (nu == 0 || uni[nu - 1] != uni[b]) uni[nu++] = uni[b];
+   /* vcell_transpose, cpu_trans_map_amr.cpp:224-285 */
+   uint32_t to_id[3];
+   if (dim == 0) { to_id[0] = VO_WID2; to_id[1] = VO_WID; to_id[2] = 1; }
+   else if (dim == 1) { to_id[0] = 1; to_id[1] = VO_WID2; to_id[2] = VO_WID; }
+   else { to_id[0] = 1; to_id[1] = VO_WID; to_id[2] = VO_WID2; }
+   uint32_t vt[VO_WID3];
+   for (uint32_t k = 0; k < VO_WID; ++k)
+      for (uint32_t j = 0; j < VO_WID; ++j)
+         for (uint32_t i = 0; i < VO_WID; ++i) vt[i + j * VO_WID + k * VO_WID2] = i * to_id[0] + j * to_id[1] + k * to_id[2];
+   const float dvz = (float)vo_cell_size(m, dim);
+   const float vz_min = (float)m->vmin[dim];
+   uint32_t maxlen = 0;
+   for (uint32_t p = 0; p < npencils; ++p) maxlen = plen[p] > maxlen ? plen[p] : maxlen;
+   float* vals = (float*)malloc(((size_t)maxlen + 1) * VO_WID3 * sizeof(float));   /* [position][transposed cell] */
+   uint32_t* bidx = (uint32_t*)malloc(((size_t)maxlen + 1) * sizeof(uint32_t));
+   for (uint32_t p = 0; p < npencils; ++p) {
+      const uint32_t L = plen[p];
+      const uint64_t s0 = pstart[p];
+      const float* dz = pdz + s0;
+      const float* ratio = pratio + s0;
+      for (uint64_t gi = 0; gi < nu; ++gi) {
+         const uint32_t g = uni[gi];
+         int nonempty = 0;
+         for (uint32_t b = 0; b < L; ++b) {
+            const uint32_t c = pcell[s0 + b];
+            bidx[b] = (c == VO_INVALID) ? VO_INVALID : find_block(tab, offsets, c, g);
+            if (bidx[b] != VO_INVALID) {
+               ++nonempty;
+               for (int t = 0; t < VO_WID3; ++t) vals[b * VO_WID3 + t] = src[(uint64_t)bidx[b] * VO_WID3 + vt[t]];
+            } else {
+               for (int t = 0; t < VO_WID3; ++t) vals[b * VO_WID3 + t] = 0.0f;
+            }
+         }
+         if (!nonempty) continue;
+         const uint32_t bz = (dim == 0) ? g % m->len[0] : (dim == 1) ? (g / m->len[0]) % m->len[1] : g / (m->len[0] * m->len[1]);
+         for (int i = W; i < (int)L - W; ++i) {
+            const uint32_t tb[3] = {bidx[i - 1], bidx[i], bidx[i + 1]};
+            for (uint32_t k = 0; k < VO_WID; ++k) {
+               const float cell_vz = (float)(((double)(bz * VO_WID + k) + 0.5) * (double)dvz + (double)vz_min);
+               const float zt = (cell_vz * dt) / dz[i];
+               const int pos = zt > 0.0f;
+               const float z1 = pos ? (float)(1.0 - (double)zt) : 0.0f;
+               const float z2 = pos ? 1.0f : -zt;
+               for (uint32_t pv = 0; pv < VO_WID2 / 8; ++pv) {
+                  /* check_skip_remapping, cpu_trans_map_amr.cpp:18-25: 8 lanes x stencil */
+                  int any = 0;
+                  for (int s = -W; s <= W && !any; ++s)
+                     for (int l = 0; l < 8; ++l)
+                        if (vals[(i + s) * VO_WID3 + k * VO_WID2 + pv * 8 + l] > 0.0f) any = 1;
+                  if (!any) continue;
+                  for (int l = 0; l < 8; ++l) {
+                     const int t = k * VO_WID2 + pv * 8 + l;
+                     float u[5], h[5], a[3];
+                     for (int s = 0; s < 5; ++s) {
+                        u[s] = vals[(i - 2 + s) * VO_WID3 + t];
+                        h[s] = dz[i - 2 + s];
+                     }
+                     vo_ppm_nu_coeff(h, u, 2, threshold, a);
+                     const float ng = z2 * (a[0] + z2 * (a[1] + z2 * a[2])) - z1 * (a[0] + z1 * (a[1] + z1 * a[2]));
+                     if (ratio[i] != 0.0f && tb[1] != VO_INVALID)
+                        f[(uint64_t)tb[1] * VO_WID3 + vt[t]] += (u[2] - ng) * ratio[i];
+                     if (ratio[i + 1] != 0.0f && tb[2] != VO_INVALID)
+                        f[(uint64_t)tb[2] * VO_WID3 + vt[t]] += (pos ? (ng * dz[i]) / dz[i + 1] : 0.0f) * ratio[i + 1];
+                     if (ratio[i - 1] != 0.0f && tb[0] != VO_INVALID)
+                        f[(uint64_t)tb[0] * VO_WID3 + vt[t]] += (!pos ? (ng * dz[i]) / dz[i - 1] : 0.0f) * ratio[i - 1];
+                  }
+               }
+            }
+         }
+      }
+   }
+   free(vals);
+   free(bidx);
+   free(uni);
+   free(tab);
+   free(src);
+   free(is_target);
+   return 0;
+}

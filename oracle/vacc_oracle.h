@@ -91,6 +91,18 @@ void vo_content_flags(uint32_t n, const float* f, double min_value, uint8_t* fla
 /* ---------------- moments (arch_moments.h:83-108) ---------------- */
 void vo_moments(const vo_mesh* m, uint32_t n, const uint32_t* gids, const float* f, double out[4]);
 
+/* ---------------- spatial translation (TRANS_SEMILAG_PPM) ---------------- */
+#define VO_TRANS_STENCIL 2   /* VLASOV_STENCIL_WIDTH for PPM, definitions.h:78-84 */
+/* PPM coefficients on non-uniform widths h (cpu_1d_ppm_nonuniform.hpp:40-62); h and v are
+ * indexed like the stencil, source cell k. */
+void vo_ppm_nu_coeff(const float* h, const float* v, int k, float threshold, float a[3]);
+/* n stencils of 5 values (source cell at 2), one set of 5 widths: out n x 3 */
+void vo_ppm_nu_batch(const float* h5, const float* stencils, int n, float threshold, float* out);
+/* trans_map_1d_amr over the given pencils (cpu_trans_map_amr.cpp:214-441); f in place */
+int vo_trans_map_1d(const vo_mesh* m, uint32_t ncells, const uint64_t* offsets, const uint32_t* gids, float* f,
+                    uint32_t npencils, const uint32_t* plen, const uint64_t* pstart, const uint32_t* pcell,
+                    const float* pdz, const float* pratio, int dim, float dt, float threshold);
+
 /* ---------------- transform and intersections (FP64) ---------------- */
 /* cpu_acc_transform.cpp:66-153.  fields: B[3], dB[6] = {dBXdy,dBXdz,dBYdx,dBYdz,dBZdx,dBZdy},
  * V[3] bulk velocity, rhoq, EgradPe[3].  Output: 3x4 affine, row-major {L00 L01 L02 t0; ...}. */

@@ -49,6 +49,18 @@ def main():
         out[name] = oracle.ref_coeff_batch(m, s, THRESHOLD)
     np.savez_compressed(os.path.join(HERE, "reconstruction.npz"), **out)
     print("wrote", os.path.join(HERE, "reconstruction.npz"), s.shape)
+    # PPM on non-uniform widths (cpu_1d_ppm_nonuniform.hpp), as the spatial translation uses it:
+    # 5-cell stencils (source at 2), sets of widths (uniform, refined neighbours, random), and
+    # two thresholds (scaled and unscaled limiter branches)
+    rng = np.random.default_rng(11)
+    s5 = stencils(500, seed=13)[:, 2:7].copy()
+    widths = np.array([[1e6] * 5, [2e6, 2e6, 1e6, 1e6, 1e6], [1e6, 1e6, 1e6, 5e5, 5e5], [3.3e5] * 5]
+                      + [10 ** rng.uniform(4, 7, 5) for _ in range(4)], np.float32)
+    thresholds = np.array([THRESHOLD, 0.0])
+    coeffs = np.stack([np.stack([oracle.ref_ppm_nu_batch(h, s5, t) for t in thresholds]) for h in widths])
+    np.savez_compressed(os.path.join(HERE, "translation_ppm_nu.npz"), stencils=s5, widths=widths,
+                        thresholds=thresholds, coeffs=coeffs)
+    print("wrote", os.path.join(HERE, "translation_ppm_nu.npz"), coeffs.shape)
 
 
 if __name__ == "__main__":

@@ -1,0 +1,100 @@
+"""GPU parity of the spatial translation: libvacc's k_trans (through vacc_trans_map_1d) against
+the oracle's restatement of trans_map_1d_amr (cpu_trans_map_amr.cpp:214) on the same cells
+and pencils.  f is compared bit for bit; block lists do not change."""
+import numpy as np
+import pytest
+
+import oracle as O
+from tests import cases
+from vlasiator_amd import trans as T
+from vlasiator_amd import vacc as V
+from vlasiator_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+DIMS = (6, 5, 4)
+DX = (1.0e7, 8.0e6, 1.2e7)
+DT = 1.7                    # |v| <= 2e6 m/s: CFL <= 0.43
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _kinds(seed=3):
+    rng = np.random.default_rng(seed)
+    kind = np.zeros(int(np.prod(DIMS)), np.uint8)
+    x = np.arange(kind.size) % DIMS[0]
+    kind[x == 0] = T.CELL_INACTIVE
+    kind[x == 1] = T.CELL_BOUNDARY
+    kind[(x > 1) & (rng.random(kind.size) < 0.06)] = T.CELL_BOUNDARY
+    return kind
+
+
+@pytest.fixture(scope="module")
+def grid():
+    w = W.config(0, ncells=int(np.prod(DIMS)))
+    return w, cases.build_cells(w), cases.oracle_mesh(w), _kinds()
+
+
+def _ctx(w, storage=V.STORE_F32):
+    return V.Context(V.make_mesh(w.grid_length, w.vmin, w.vmax),
+                     V.make_species(w.mass, w.charge, w.min_value, w.add_width), V.make_options(w.method, storage=storage))
+
+
+@pytest.mark.parametrize("periodic", [(False, True, True), (True, False, True)])
+@pytest.mark.parametrize("dim", [0, 1, 2])
+def test_translation_bit_exact(grid, dim, periodic):
+    w, cells, m, kind = grid
+    pen = T.build_pencils_uniform(DIMS, periodic, kind, DX, dim)
+    _say(f"translation dim={dim} periodic={periodic} pencils={len(pen[0])}")
+    ctx = _ctx(w)
+    ctx.upload_cells(cells)
+    T.set_pencils(ctx, dim, pen)
+    T.trans_map_1d(ctx, dim, DT)
+    want = O.trans_map_1d(m, cells, pen, dim, DT, w.min_value)
+    cases.assert_cells_equal(ctx.download_cells(), want, f"dim {dim}")
+    ctx.close()
+
+
+def test_three_dimensions_after_an_acceleration_step(grid):
+    """Acceleration (lists left in column order), then x, y, z translation: the library
+    relists before translating; against the oracle run of the same sequence."""
+    w, cells, m, kind = grid
+    periodic = (False, True, True)
+    isx = cases.oracle_intersections(w, 1)
+    ctx = _ctx(w)
+    ctx.upload_cells(cells)
+    ctx.set_intersections(isx)
+    _say("accelerate + translate xyz")
+    ctx.accelerate(1)
+    want = [O.accelerate_cell(m, g, f, isx[i], 1, w.method, w.min_value)[:2] for i, (g, f) in enumerate(cells)]
+    for dim in range(3):
+        pen = T.build_pencils_uniform(DIMS, periodic, kind, DX, dim)
+        T.set_pencils(ctx, dim, pen)
+        T.trans_map_1d(ctx, dim, DT)
+        want = O.trans_map_1d(m, want, pen, dim, DT, w.min_value)
+    cases.assert_cells_equal(ctx.download_cells(), want, "xyz")
+    ctx.close()
+
+
+def test_translation_errors(grid):
+    w, cells, m, kind = grid
+    ctx = _ctx(w)
+    ctx.upload_cells(cells)
+    with pytest.raises(V.VaccError, match="no pencils"):
+        T.trans_map_1d(ctx, 0, DT)
+    pen = T.build_pencils_uniform(DIMS, (True, True, True), kind, DX, 0)
+    T.set_pencils(ctx, 0, pen)
+    ctx.upload_cells(cells[:10])              # a new cell set retires the pencils
+    with pytest.raises(V.VaccError, match="no pencils"):
+        T.trans_map_1d(ctx, 0, DT)
+    with pytest.raises(V.VaccError, match="outside the context"):
+        T.set_pencils(ctx, 0, pen)
+    ctx.close()
+    b = _ctx(w, V.STORE_BF16)
+    b.upload_cells(cells)
+    T.set_pencils(b, 0, pen)
+    with pytest.raises(V.VaccError, match="FP32"):
+        T.trans_map_1d(b, 0, DT)
+    b.close()

@@ -16,18 +16,23 @@ def declared_symbols(header="vacc.h"):
     return sorted(set(re.findall(r"\b(vacc_[a-z0-9_]+)\s*\(", txt)))
 
 
+HEADERS = ("vacc.h", "vacc_vlsv.h", "vacc_trans.h")
+
+
 def test_header_declares_bound_symbols():
-    from vlasiator_amd import vlsv
+    from vlasiator_amd import trans, vlsv
     assert sorted(V.EXPORTS) == declared_symbols()
     assert sorted(vlsv.EXPORTS) == declared_symbols("vacc_vlsv.h")
+    assert sorted(trans.EXPORTS) == declared_symbols("vacc_trans.h")
 
 
 def test_library_exports_declared_symbols():
     if not os.path.exists(V.LIB_PATH):
         pytest.skip("libvacc.so not built")
     lib = V.lib()
-    for s in declared_symbols() + declared_symbols("vacc_vlsv.h"):
-        assert hasattr(lib, s), s
+    for h in HEADERS:
+        for s in declared_symbols(h):
+            assert hasattr(lib, s), s
 
 
 def test_host_intersections_match_oracle():

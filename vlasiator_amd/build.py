@@ -9,8 +9,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIB_DIR, "libvacc.so")
-SOURCES = ["vacc_kernels.hip", "vacc_init.hip", "vacc_api.hip"]
-CXX_SOURCES = ["vacc_vlsv.cpp"]   # host-only C++ (no HIP)
+SOURCES = ["vacc_kernels.hip", "vacc_init.hip", "vacc_trans.hip", "vacc_api.hip"]
+CXX_SOURCES = ["vacc_vlsv.cpp", "vacc_pencils.cpp"]   # host-only C++ (no HIP)
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("VACC_OFFLOAD_ARCH", "gfx950")
 

@@ -17,6 +17,7 @@
 #include <string>
 #include <vector>
 
+#include "../../include/vacc_trans.h"
 #include "vacc_blocks_io.h"
 #include "vacc_common.h"
 #include "vacc_kernels.h"
@@ -90,7 +91,7 @@ struct vacc_ctx {
       bool moments_in, content_in;
       hipGraphExec_t exec;
       int lcur_out, fcur_out;
-      bool moments_out, content_out;
+      bool moments_out, content_out, sorted_out;
    };
    std::vector<StepGraph> graphs;
    uint64_t gen = 0;
@@ -101,10 +102,20 @@ struct vacc_ctx {
    float* io_pin[2] = {nullptr, nullptr};
    float* io_dev = nullptr;
    hipEvent_t io_ev[2] = {nullptr, nullptr};
+   // block lists in ascending GID order within each cell (after a block adjust or relist)
+   bool sorted = false;
+   uint64_t cell_epoch = 0;         // changes with every new cell set
+   // spatial translation schedule per dimension (vacc_trans_set_pencils)
+   struct TransDim {
+      TransCell* cells = nullptr;
+      TransEntry* entries = nullptr;
+      size_t cells_cap = 0, entries_cap = 0;
+      uint64_t epoch = ~0ull;
+   } trans[3];
    // event timing
    struct Rec {
       hipEvent_t a, b;
-      int kind;   // 0 remap, 1 build, 2 adjust, 3 transform
+      int kind;   // 0 remap, 1 build, 2 adjust, 3 transform, 4 translation
    };
    bool prof = false;
    std::vector<Rec> recs;
@@ -391,6 +402,7 @@ static int do_pass(vacc_ctx* c, int dim) {
    c->fcur = fo;
    c->content_valid = true;
    c->moments_valid = false;
+   c->sorted = false;   // column order of the pass
    return 0;
 }
 
@@ -461,6 +473,7 @@ static int do_relist(vacc_ctx* c, int add_width, bool want_loss, const SpatialIn
    HIPCHK(hipGetLastError());
    c->lcur = lo;
    c->content_valid = false;
+   c->sorted = true;
    if (add_width >= 0) c->moments_valid = false;   // blocks may have been removed
    return 0;
 }
@@ -516,6 +529,8 @@ static int begin_load(vacc_ctx* c, uint32_t ncells, const uint64_t* offsets, con
    c->nblocks = 0;
    c->lcur = 0;
    c->fcur = 0;
+   c->sorted = false;
+   c->cell_epoch++;
    const size_t want = total + total / 2 + 64 * (size_t)ncells + 4096;
    if (want > c->cap) {
       rc = ensure_blocks(c, want);
@@ -724,6 +739,10 @@ int vacc_destroy(vacc_ctx* c) {
       if (c->io_ev[i]) (void)hipEventDestroy(c->io_ev[i]);
    }
    if (c->io_dev) (void)hipFree(c->io_dev);
+   for (auto& t : c->trans) {
+      if (t.cells) (void)hipFree(t.cells);
+      if (t.entries) (void)hipFree(t.entries);
+   }
    void* sp[] = {c->d_nbr_off, c->d_nbr, c->d_ext_off, c->d_ext};
    for (void* p : sp)
       if (p) (void)hipFree(p);
@@ -1081,6 +1100,7 @@ static int graph_step(vacc_ctx* c, uint32_t map_order, int32_t use_moments, int3
          c->fcur = g.fcur_out;
          c->moments_valid = g.moments_out;
          c->content_valid = g.content_out;
+         c->sorted = g.sorted_out;
          return 0;
       }
    vacc_ctx::StepGraph g{};
@@ -1108,6 +1128,7 @@ static int graph_step(vacc_ctx* c, uint32_t map_order, int32_t use_moments, int3
    g.fcur_out = c->fcur;
    g.moments_out = c->moments_valid;
    g.content_out = c->content_valid;
+   g.sorted_out = c->sorted;
    c->graphs.push_back(g);
    HIPCHK(hipGraphLaunch(g.exec, c->stream));
    return 0;
@@ -1230,6 +1251,8 @@ int vacc_init_peaks(vacc_ctx* c, uint32_t ncells, uint32_t max_peaks, const vacc
    c->lcur = 0;
    c->fcur = 0;
    c->nblocks = 0;
+   c->sorted = false;
+   c->cell_epoch++;
    c->content_valid = false;
    c->moments_valid = false;
    c->fields_valid = false;
@@ -1327,6 +1350,7 @@ int vacc_profile_read(vacc_ctx* c, vacc_profile* out) {
       else if (r.kind == 1) { out->build_launches++; out->build_ms += ms; }
       else if (r.kind == 2) { out->adjust_launches++; out->adjust_ms += ms; }
       else if (r.kind == 3) { out->transform_launches++; out->transform_ms += ms; }
+      else if (r.kind == 4) { out->trans_launches++; out->trans_ms += ms; }
    }
    if (c->nlog) {
       std::vector<uint32_t> cnt(2 * c->nlog);
@@ -1352,6 +1376,156 @@ int vacc_get_status(vacc_ctx* c, uint32_t* status) {
       HIPCHK(hipMemcpy(status, c->status, c->ncells * sizeof(uint32_t), hipMemcpyDeviceToHost));
       HIPCHK(hipMemset(c->status, 0, (c->ncells + 1) * sizeof(uint32_t)));
    }
+   return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Spatial translation (include/vacc_trans.h)
+// ---------------------------------------------------------------------------
+
+// Width-only scalars of one h4 face on non-uniform widths, h[0..3] = widths at k-2..k+1,
+// with the float and double sub-expressions of cpu_face_estimates.hpp:179-191.
+static TransFace trans_face(const float* h) {
+   TransFace F{};
+   const float h0 = h[0], h1 = h[1], h2 = h[2], h3 = h[3];
+   F.D1 = 1.0 / (double)(((h0 + h1) + h2) + h3);
+   F.SA = ((h0 + h1) * (h2 + h3)) / (h1 + h2);
+   F.DA = 1.0 / (double)((h0 + h1) + h2) + 1.0 / (double)((h1 + h2) + h3);
+   F.SB = (h2 * (h2 + h3)) / (((h0 + h1) + h2) * (h0 + h1));
+   F.DB = (double)h0 + 2.0 * (double)h1;
+   F.SC = (h1 * (h0 + h1)) / (((h1 + h2) + h3) * (h2 + h3));
+   F.DC = 2.0 * (double)h2 + (double)h3;
+   F.h1 = h1;
+   F.h2 = h2;
+   return F;
+}
+
+int vacc_trans_set_pencils(vacc_ctx* c, uint32_t dim, const vacc_pencils* P) {
+   if (!c || !P) return set_err(VACC_E_ARG, "null argument");
+   if (dim > 2) return set_err(VACC_E_ARG, "dim must be 0, 1 or 2");
+   if (P->npencils && (!P->length || !P->start || !P->cell || !P->dz || !P->ratio))
+      return set_err(VACC_E_ARG, "null pencil array");
+   const int W = VACC_TRANS_STENCIL;
+   const uint32_t nc = c->ncells;
+   std::vector<std::vector<TransEntry>> ent(nc);
+   std::vector<std::vector<uint32_t>> tab(nc);
+   size_t total = 0;
+   // contributions in the reference's order: pencils as given (bins, then pencils of a bin),
+   // then source positions W .. L-W-1 (cpu_trans_map_amr.cpp:59-148)
+   for (uint32_t p = 0; p < P->npencils; ++p) {
+      const int L = (int)P->length[p];
+      if (L < 2 * W + 1) return set_err(VACC_E_ARG, "pencil shorter than 2*VACC_TRANS_STENCIL+1 positions");
+      const uint64_t s0 = P->start[p];
+      const uint32_t* pc = P->cell + s0;
+      const float* dz = P->dz + s0;
+      const float* ratio = P->ratio + s0;
+      for (int q = 0; q < L; ++q)
+         if (pc[q] != VACC_INVALID_ID && pc[q] >= nc) return set_err(VACC_E_ARG, "pencil cell index outside the context");
+      for (int i = W; i < L - W; ++i) {
+         const TransFace fl = trans_face(dz + i - 2), fr = trans_face(dz + i - 1);
+         for (int o : {-1, 0, 1}) {
+            const int q = i + o;
+            const uint32_t t = pc[q];
+            if (t == VACC_INVALID_ID || ratio[q] == 0.0f) continue;
+            TransEntry E{};
+            E.fl = fl;
+            E.fr = fr;
+            E.dz_i = dz[i];
+            E.dz_t = dz[q];
+            E.ratio = ratio[q];
+            E.kind = (o == 0) ? 1 : (o > 0 ? 0 : 2);
+            for (int st = 0; st < 5; ++st) {
+               const uint32_t sc = pc[i - 2 + st];
+               if (sc == VACC_INVALID_ID) {
+                  E.st[st] = 0xFF;
+                  continue;
+               }
+               auto& tb = tab[t];
+               size_t slot = std::find(tb.begin(), tb.end(), sc) - tb.begin();
+               if (slot == tb.size()) {
+                  if (tb.size() >= (size_t)TRANS_MAXN)
+                     return set_err(VACC_E_ARG, "a target cell's stencils span more than 12 cells");
+                  tb.push_back(sc);
+               }
+               E.st[st] = (uint8_t)slot;
+            }
+            ent[t].push_back(E);
+            ++total;
+         }
+      }
+   }
+   if (total > 0xFFFFFFFFull) return set_err(VACC_E_ARG, "too many contributions");
+   std::vector<TransCell> cells(nc);
+   std::vector<TransEntry> flat;
+   flat.reserve(total);
+   for (uint32_t t = 0; t < nc; ++t) {
+      TransCell& C = cells[t];
+      C.e0 = (uint32_t)flat.size();
+      C.ne = (uint32_t)ent[t].size();
+      C.nn = (uint32_t)tab[t].size();
+      for (size_t j = 0; j < tab[t].size(); ++j) C.nbr[j] = tab[t][j];
+      flat.insert(flat.end(), ent[t].begin(), ent[t].end());
+   }
+   auto& T = c->trans[dim];
+   HIPCHK(hipStreamSynchronize(c->stream));
+   if (T.cells_cap < cells.size() || !T.cells) {
+      if (T.cells) (void)hipFree(T.cells);
+      T.cells = nullptr;
+      T.cells_cap = 0;
+      HIPCHK(hipMalloc(&T.cells, std::max<size_t>(cells.size(), 1) * sizeof(TransCell)));
+      T.cells_cap = std::max<size_t>(cells.size(), 1);
+   }
+   if (T.entries_cap < flat.size() || !T.entries) {
+      if (T.entries) (void)hipFree(T.entries);
+      T.entries = nullptr;
+      T.entries_cap = 0;
+      HIPCHK(hipMalloc(&T.entries, std::max<size_t>(flat.size(), 1) * sizeof(TransEntry)));
+      T.entries_cap = std::max<size_t>(flat.size(), 1);
+   }
+   if (nc) HIPCHK(hipMemcpy(T.cells, cells.data(), nc * sizeof(TransCell), hipMemcpyHostToDevice));
+   if (!flat.empty()) HIPCHK(hipMemcpy(T.entries, flat.data(), flat.size() * sizeof(TransEntry), hipMemcpyHostToDevice));
+   T.epoch = c->cell_epoch;
+   return 0;
+}
+
+int vacc_trans_map_1d(vacc_ctx* c, uint32_t dim, double dt) {
+   if (!c) return set_err(VACC_E_ARG, "null context");
+   if (dim > 2) return set_err(VACC_E_ARG, "dim must be 0, 1 or 2");
+   const auto& T = c->trans[dim];
+   if (T.epoch != c->cell_epoch) return set_err(VACC_E_STATE, "no pencils installed for the current cells and dim");
+   if (c->opt.storage != VACC_STORE_F32) return set_err(VACC_E_ARG, "spatial translation needs FP32 storage");
+   if (!c->ncells) return 0;
+   drop_graphs(c);
+   int rc = 0;
+   if (!c->sorted && (rc = do_relist(c, -1, false))) return rc;   // ascending GIDs, same blocks
+   if ((rc = refresh_count(c))) return rc;
+   TransArgs ta{};
+   ta.off = c->off[c->lcur];
+   ta.gid = c->gid[c->lcur];
+   ta.src = c->src[c->lcur];
+   ta.fin = (const float*)c->f[c->fcur];
+   ta.fout = (float*)c->f[1 - c->fcur];
+   ta.cells = T.cells;
+   ta.entries = T.entries;
+   ta.ncells = c->ncells;
+   ta.nblocks = (uint32_t)c->nblocks;
+   ta.dim = (int)dim;
+   for (int d = 0; d < 3; ++d) ta.len[d] = c->mi.len[d];
+   ta.dvz = (float)c->mi.cell[dim];
+   ta.vz_min = (float)c->mi.vmin[dim];
+   ta.dt = (float)dt;
+   ta.thr = (float)c->sp.sparse_min_value;
+   ta.scale = (float)(1.0 / (double)ta.thr);
+   {
+      ProfScope ps(c, 4);
+      launch_trans(ta, c->stream);
+   }
+   HIPCHK(hipGetLastError());
+   launch_iota(c->src[c->lcur], (uint32_t)c->nblocks, c->stream);
+   c->fcur = 1 - c->fcur;
+   c->content_valid = false;
+   c->moments_valid = false;
+   HIPCHK(hipStreamSynchronize(c->stream));
    return 0;
 }
 

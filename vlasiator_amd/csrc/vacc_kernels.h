@@ -186,4 +186,60 @@ void launch_fill_u32(uint32_t* dst, uint32_t v, uint32_t n, hipStream_t s);
 
 int device_cus();   // compute units of the current device
 
+// ---------------------------------------------------------------------------
+// Spatial translation (vacc_trans.hip)
+// ---------------------------------------------------------------------------
+// Width-only scalars of the h4 face estimate on non-uniform widths for one face, with
+// h0..h3 = h[k-2..k+1] (cpu_face_estimates.hpp:179-191): the float and double
+// sub-expressions of the reference, evaluated once per face on the host.
+struct TransFace {
+   double D1;          // 1 / (h0 + h1 + h2 + h3)
+   double DA;          // 1 / (h0 + h1 + h2) + 1 / (h1 + h2 + h3)
+   double DB;          // h0 + 2 h1
+   double DC;          // 2 h2 + h3
+   float SA;           // (h0 + h1)(h2 + h3) / (h1 + h2)
+   float SB;           // h2 (h2 + h3) / ((h0 + h1 + h2)(h0 + h1))
+   float SC;           // h1 (h0 + h1) / ((h1 + h2 + h3)(h2 + h3))
+   float h1, h2;
+   float pad[3];
+};
+// One contribution to a target cell: from source position i of a pencil (stencil i-2..i+2)
+// into position i-1 (kind 2), i (kind 1) or i+1 (kind 0).
+constexpr int TRANS_MAXN = 12;   // distinct stencil cells per target cell
+struct TransEntry {
+   TransFace fl, fr;   // faces k = i and k = i + 1
+   float dz_i;         // source width
+   float dz_t;         // target width (kinds 0 and 2)
+   float ratio;        // target ratio
+   uint8_t st[5];      // stencil cells as slots of the target cell's table (0xFF: none)
+   uint8_t kind;
+   uint8_t pad[2];
+   uint32_t pad2;
+};
+struct TransCell {
+   uint32_t e0;        // first entry
+   uint32_t ne;        // entries (0: the cell is not written and keeps its values)
+   uint32_t nn;        // distinct stencil cells
+   uint32_t nbr[TRANS_MAXN];
+   uint32_t pad;
+};
+struct TransArgs {
+   const uint32_t* off;    // [ncells+1] list offsets (GIDs ascending within each cell)
+   const uint32_t* gid;
+   const uint32_t* src;    // list -> input pool index (INVALID = zero block)
+   const float* fin;
+   float* fout;            // compact output pool (block b of the list at b)
+   const TransCell* cells;
+   const TransEntry* entries;
+   uint32_t ncells;
+   uint32_t nblocks;
+   int dim;
+   uint32_t len[3];
+   float dvz, vz_min;      // (float) cell size and mesh minimum along dim
+   float dt;
+   float thr;              // reconstruction threshold
+   float scale;            // (float)(1 / thr)
+};
+void launch_trans(const TransArgs& a, hipStream_t s);
+
 }  // namespace vacc
