@@ -30,6 +30,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 from vlasiator_amd import vacc as V  # noqa: E402
+from vlasiator_amd import trans as TR  # noqa: E402
 from vlasiator_amd import vlsv as VL  # noqa: E402
 from vlasiator_amd import workloads as W  # noqa: E402
 
@@ -169,6 +170,88 @@ def io_form(w, ctx):
             "write_GBps": round(nbytes / t_write / 1e9, 2), "write_devnull_GBps": round(nbytes / t_null / 1e9, 2),
             "read_GBps": round(nbytes / t_read / 1e9, 2), "unit": "GB/s of f (FP32)",
             "file_dir": "tempfile.gettempdir(), page cache"}
+
+
+def grid_dims(n: int):
+    """A near-cubic spatial grid holding n cells (x fastest)."""
+    best = (n, 1, 1)
+    for a in range(1, n + 1):
+        if n % a:
+            continue
+        for b in range(1, n // a + 1):
+            if (n // a) % b:
+                continue
+            c = n // a // b
+            d = sorted((a, b, c), reverse=True)
+            if max(d) - min(d) < max(best) - min(best):
+                best = tuple(d)
+    return best
+
+
+TRANS_DX = 1.0e7    # m; with dt below, |v| dt / dx <= 0.45 on the +-3000 km/s mesh
+TRANS_DT = 1.5      # s
+
+
+def trans_form(w, steps: int, warmup: int, cpu_seconds: float):
+    """Spatial translation of the workload's initial state on a periodic grid of its cells:
+    one step is the x, y and z translation of every cell (vacc_trans_map_1d x 3).  Roofline of
+    k_trans: 256 B read + 256 B written per block (the stencil neighbours' re-reads are not
+    algorithmic).  CPU baseline: the oracle's restatement (one thread) of one dimension on
+    the first 8 x 8 x 4 cells of the grid with periodic pencils."""
+    dims = grid_dims(len(w.cells))
+    kind = np.zeros(len(w.cells), np.uint8)
+    ctx = make_context(w)
+    ctx.init_peaks([W.to_vacc_peaks(c) for c in w.cells], W.boxes(w))
+    nb = ctx.total_blocks()
+    pencils = [TR.build_pencils_uniform(dims, (True, True, True), kind, (TRANS_DX,) * 3, d) for d in range(3)]
+    for d in range(3):
+        TR.set_pencils(ctx, d, pencils[d])
+    for s in range(warmup):
+        for d in range(3):
+            TR.trans_map_1d(ctx, d, TRANS_DT)
+    ctx.sync()
+    ctx.profile_begin()
+    t0 = time.perf_counter()
+    for s in range(steps):
+        for d in range(3):
+            TR.trans_map_1d(ctx, d, TRANS_DT)
+    ctx.sync()
+    el = time.perf_counter() - t0
+    prof = ctx.profile_read()
+    nb_end = ctx.total_blocks()
+    sample = ctx.download_cells() if cpu_seconds > 0 else None
+    ctx.close()
+    launches = max(prof["trans_launches"], 1)
+    per_launch_ms = prof["trans_ms"] / launches
+    alg = nb * 512
+    achieved = alg / (per_launch_ms / 1e3) / 1e9
+    out = {"grid": list(dims), "cells": len(w.cells), "blocks": nb, "steps": steps, "dt_s": TRANS_DT,
+           "dx_m": TRANS_DX, "ms_per_step": 1e3 * el / steps, "value": 64 * nb * steps / el,
+           "unit": "cell-updates/s (x, y, z translation per step)",
+           "roofline": {"kernel": "k_trans", "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
+                        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "launches": prof["trans_launches"],
+                        "avg_launch_ms": round(per_launch_ms, 4), "algorithmic_bytes_per_launch": alg}}
+    assert nb_end == nb
+    log(f"translation: grid {dims}, {nb} blocks, {out['ms_per_step']:.2f} ms per x,y,z step, "
+        f"k_trans {per_launch_ms:.3f} ms/launch = {achieved:.0f} GB/s algorithmic")
+    if sample is not None:
+        import oracle as O
+        sd = tuple(min(a, b) for a, b in zip((8, 8, 4), dims))
+        ids = [x + dims[0] * (y + dims[1] * z) for z in range(sd[2]) for y in range(sd[1]) for x in range(sd[0])]
+        cells = [sample[i] for i in ids]
+        pen = TR.build_pencils_uniform(sd, (True, True, True), np.zeros(len(ids), np.uint8), (TRANS_DX,) * 3, 0)
+        m = O.make_mesh(w.grid_length, w.vmin, w.vmax)
+        blocks = sum(len(g) for g, _ in cells)
+        reps, t_cpu = 0, 0.0
+        while t_cpu < cpu_seconds:
+            t0 = time.perf_counter()
+            O.trans_map_1d(m, cells, pen, 0, TRANS_DT, w.min_value)
+            t_cpu += time.perf_counter() - t0
+            reps += 1
+        out["cpu_baseline"] = {"value": 64 * blocks * reps / t_cpu / 3, "unit": out["unit"], "cores": 1,
+                               "kind": "port", "sample": f"{len(ids)} cells ({blocks} blocks), x translation x {reps}, "
+                                                         f"{t_cpu:.1f} s; per x,y,z step = 3 x that time"}
+    return out
 
 
 def measure(pops, storage: str, steps: int, warmup: int, graph: bool = False, fast: bool | None = None) -> dict:
@@ -333,6 +416,8 @@ def main():
 
     if io is not None:
         out["vlsv_io"] = io
+    if args.forms and len(pops) == 1:
+        out["translation_form"] = trans_form(w, min(args.steps, 10), 2, 0.0 if args.no_cpu else 5.0)
     if args.plain_cells and sample_cells is not None:
         npl = min(args.plain_cells, len(sample_cells))
         log(f"plain form: {npl} single-cell contexts")

@@ -1409,6 +1409,8 @@ int vacc_trans_set_pencils(vacc_ctx* c, uint32_t dim, const vacc_pencils* P) {
    const uint32_t nc = c->ncells;
    std::vector<std::vector<TransEntry>> ent(nc);
    std::vector<std::vector<uint32_t>> tab(nc);
+   // per target cell: (pencil, target position) of each entry, for the canonical test
+   std::vector<std::vector<std::pair<uint32_t, int>>> where(nc);
    size_t total = 0;
    // contributions in the reference's order: pencils as given (bins, then pencils of a bin),
    // then source positions W .. L-W-1 (cpu_trans_map_amr.cpp:59-148)
@@ -1450,6 +1452,7 @@ int vacc_trans_set_pencils(vacc_ctx* c, uint32_t dim, const vacc_pencils* P) {
                E.st[st] = (uint8_t)slot;
             }
             ent[t].push_back(E);
+            where[t].emplace_back(p, q);
             ++total;
          }
       }
@@ -1464,6 +1467,23 @@ int vacc_trans_set_pencils(vacc_ctx* c, uint32_t dim, const vacc_pencils* P) {
       C.ne = (uint32_t)ent[t].size();
       C.nn = (uint32_t)tab[t].size();
       for (size_t j = 0; j < tab[t].size(); ++j) C.nbr[j] = tab[t][j];
+      // canonical: one target position q of one pencil receiving p1 from q-1, self from q and
+      // m1 from q+1 (entries already in that order); the stencil cells are then positions
+      // q-3 .. q+3, stored in position order so that entry e reads cells e .. e+4
+      const auto& wh = where[t];
+      if (C.ne == 3 && wh[0] == wh[1] && wh[1] == wh[2] && ent[t][0].kind == 0 && ent[t][1].kind == 1 &&
+          ent[t][2].kind == 2) {
+         const uint32_t p = wh[0].first;
+         const int q = wh[0].second;
+         const uint32_t* pc = P->cell + P->start[p];
+         bool ok = true;
+         for (int k = 0; k < 7; ++k) ok = ok && pc[q - 3 + k] != VACC_INVALID_ID;
+         if (ok) {
+            C.canonical = 1;
+            C.nn = 7;
+            for (int k = 0; k < 7; ++k) C.nbr[k] = pc[q - 3 + k];
+         }
+      }
       flat.insert(flat.end(), ent[t].begin(), ent[t].end());
    }
    auto& T = c->trans[dim];

@@ -214,14 +214,15 @@ struct TransEntry {
    uint8_t st[5];      // stencil cells as slots of the target cell's table (0xFF: none)
    uint8_t kind;
    uint8_t pad[2];
-   uint32_t pad2;
+   uint32_t pad2[3];   // 160 bytes: the kernel reads entries as 40 dwords
 };
 struct TransCell {
    uint32_t e0;        // first entry
    uint32_t ne;        // entries (0: the cell is not written and keeps its values)
-   uint32_t nn;        // distinct stencil cells
+   uint32_t nn;        // stencil cells in nbr
    uint32_t nbr[TRANS_MAXN];
-   uint32_t pad;
+   uint32_t canonical; // 1: the three entries p1 from q-1, self from q, m1 from q+1 of one
+                       // pencil, nbr[0..6] = the cells at positions q-3 .. q+3
 };
 struct TransArgs {
    const uint32_t* off;    // [ncells+1] list offsets (GIDs ascending within each cell)
