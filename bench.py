@@ -119,15 +119,16 @@ def plain_form(w, cells, isect, steps):
     return upd / dt, n0
 
 
-def profiled_traffic(workload: str, storage: str):
-    """HBM bytes per k_remap launch relative to the algorithmic bytes, as measured with
-    rocprofv3 FETCH_SIZE / WRITE_SIZE (copy-calibrated) for this workload and storage form
-    and recorded in a profiles/ summary ("traffic-ratio <workload> <storage> <ratio>")."""
+def profiled_traffic(workload: str, storage: str, key: str = "traffic-ratio"):
+    """HBM bytes per launch relative to the algorithmic bytes, as measured with rocprofv3
+    FETCH_SIZE / WRITE_SIZE (copy-calibrated) for this workload and storage form and recorded
+    in a profiles/ summary ("<key> <workload> <storage> <ratio>"; key traffic-ratio for
+    k_remap, traffic-ratio-k_trans for the translation)."""
     import glob
     for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
         for line in open(path):
             p = line.split()
-            if len(p) == 4 and p[0] == "traffic-ratio" and p[1] == workload and p[2] == storage:
+            if len(p) == 4 and p[0] == key and p[1] == workload and p[2] == storage:
                 return float(p[3]), os.path.relpath(path, ROOT)
     return None, None
 
@@ -232,6 +233,10 @@ def trans_form(w, steps: int, warmup: int, cpu_seconds: float):
                         "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "launches": prof["trans_launches"],
                         "avg_launch_ms": round(per_launch_ms, 4), "algorithmic_bytes_per_launch": alg}}
     assert nb_end == nb
+    ratio, src = profiled_traffic(w.name, "f32", key="traffic-ratio-k_trans")
+    if ratio is not None:
+        out["roofline"]["traffic"] = int(ratio * alg)
+        out["roofline"]["traffic_source"] = f"{src}: rocprofv3 FETCH_SIZE + WRITE_SIZE per launch, x{ratio:.3f} algorithmic"
     log(f"translation: grid {dims}, {nb} blocks, {out['ms_per_step']:.2f} ms per x,y,z step, "
         f"k_trans {per_launch_ms:.3f} ms/launch = {achieved:.0f} GB/s algorithmic")
     if sample is not None:
@@ -348,6 +353,8 @@ def main():
                     help="keep the default remap grid (128 workgroups per CU) instead of timing candidates")
     ap.add_argument("--no-forms", dest="forms", action="store_false",
                     help="skip the short runs of the other storage forms")
+    ap.add_argument("--translation-only", action="store_true",
+                    help="print only the spatial-translation measurement of the workload (one JSON line)")
     args = ap.parse_args()
     global FAST, AUTOTUNE
     FAST = args.fast
@@ -360,6 +367,9 @@ def main():
 
     pops = W.populations(args.config, ncells=args.cells or None)
     w = pops[0]
+    if args.translation_only:
+        print(json.dumps(trans_form(w, args.steps, args.warmup, 0.0 if args.no_cpu else args.cpu_seconds)), flush=True)
+        return
     ncells = len(w.cells)
     log(f"workload {w.name}: {ncells} cells, mesh {w.grid_length}, method {w.method}, storage {args.storage}")
     fields = [V.make_fields(c.B, c.V, c.dt) for c in w.cells]
