@@ -72,7 +72,9 @@ int vacc_trans_set_pencils(vacc_ctx* ctx, uint32_t dim, const vacc_pencils* penc
 
 /* Translates every cell of the context along dim by dt (converted to float, the reference's
  * Realf dt), over the installed pencils.  Threshold of the reconstruction: the species'
- * sparse_min_value.  FP32 storage only. */
+ * sparse_min_value.  With a 16-bit storage form the arithmetic is FP32 on the decoded
+ * blocks and every written block is encoded again; blocks that are not written keep their
+ * codes. */
 int vacc_trans_map_1d(vacc_ctx* ctx, uint32_t dim, double dt);
 
 #ifdef __cplusplus

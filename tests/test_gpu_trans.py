@@ -92,9 +92,58 @@ def test_translation_errors(grid):
     with pytest.raises(V.VaccError, match="outside the context"):
         T.set_pencils(ctx, 0, pen)
     ctx.close()
-    b = _ctx(w, V.STORE_BF16)
-    b.upload_cells(cells)
-    T.set_pencils(b, 0, pen)
-    with pytest.raises(V.VaccError, match="FP32"):
-        T.trans_map_1d(b, 0, DT)
-    b.close()
+
+
+@pytest.mark.parametrize("storage", [V.STORE_BF16, V.STORE_F16])
+def test_translation_16bit_storage(grid, storage):
+    """16-bit storage: the kernel computes in FP32 from the decoded blocks and encodes each
+    written block; against the oracle run on the decoded state with the written cells'
+    blocks requantized (cells that are not written keep their codes)."""
+    w, cells, m, kind = grid
+    st = {V.STORE_BF16: O.ST_BF16, V.STORE_F16: O.ST_F16}[storage]
+    periodic = (False, True, True)
+    ctx = _ctx(w, storage)
+    ctx.upload_cells(cells)
+    held = ctx.download_cells()
+    for dim in range(3):
+        pen = T.build_pencils_uniform(DIMS, periodic, kind, DX, dim)
+        written = set(int(c) for c, r in zip(pen[2], pen[4]) if r > 0)
+        _say(f"translation storage={storage} dim={dim}")
+        T.set_pencils(ctx, dim, pen)
+        T.trans_map_1d(ctx, dim, DT)
+        out = O.trans_map_1d(m, held, pen, dim, DT, w.min_value)
+        held = [(g, O.requantize(f, st) if i in written else f) for i, (g, f) in enumerate(out)]
+        cases.assert_cells_equal(ctx.download_cells(), held, f"storage {storage} dim {dim}")
+    ctx.close()
+
+
+def test_cells_written_by_two_pencils(grid):
+    """Refinement-style pencils (the bins of cpu_trans_pencils.hpp:48-50): two pencils share
+    their first two cells A, B (width 2 dx, target ratio 1/2 in each) and go on through their
+    own cells C, D (width dx, ratio 1).  A and B receive the contributions of both pencils in
+    pencil order; the general path of k_trans against the oracle."""
+    w, cells, m, kind = grid
+    six = cells[:6]                       # A, B, C1, D1, C2, D2
+    A, B, C1, D1, C2, D2 = range(6)
+    dx = 1.0e7
+    lengths, starts, ids, dz, ratio = [], [], [], [], []
+    for c, d in ((C1, D1), (C2, D2)):
+        p = [A, A, A, B, c, d, d, d]
+        starts.append(len(ids))
+        lengths.append(len(p))
+        ids += p
+        dz += [2 * dx if x in (A, B) else dx for x in p]
+        ratio += [0.0, 0.5, 0.5, 0.5, 1.0, 1.0, 1.0, 0.0]
+    pen = (np.array(lengths, np.uint32), np.array(starts, np.uint64), np.array(ids, np.uint32),
+           np.array(dz, np.float32), np.array(ratio, np.float32))
+    for dim in range(3):
+        _say(f"two-pencil cells dim={dim}")
+        ctx = _ctx(w)
+        ctx.upload_cells(six)
+        T.set_pencils(ctx, dim, pen)
+        T.trans_map_1d(ctx, dim, DT)
+        want = O.trans_map_1d(m, six, pen, dim, DT, w.min_value)
+        cases.assert_cells_equal(ctx.download_cells(), want, f"two pencils dim {dim}")
+        # the shared cells did receive mass from both pencils
+        assert not np.array_equal(want[A][1], six[A][1])
+        ctx.close()

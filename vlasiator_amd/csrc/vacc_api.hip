@@ -1425,7 +1425,9 @@ int vacc_trans_set_pencils(vacc_ctx* c, uint32_t dim, const vacc_pencils* P) {
          if (pc[q] != VACC_INVALID_ID && pc[q] >= nc) return set_err(VACC_E_ARG, "pencil cell index outside the context");
       for (int i = W; i < L - W; ++i) {
          const TransFace fl = trans_face(dz + i - 2), fr = trans_face(dz + i - 1);
-         for (int o : {-1, 0, 1}) {
+         // per source the reference adds self, then p1, then m1 (cpu_trans_map_amr.cpp:111-146);
+         // the order matters where one cell is the target at two of these positions
+         for (int o : {0, 1, -1}) {
             const int q = i + o;
             const uint32_t t = pc[q];
             if (t == VACC_INVALID_ID || ratio[q] == 0.0f) continue;
@@ -1513,7 +1515,6 @@ int vacc_trans_map_1d(vacc_ctx* c, uint32_t dim, double dt) {
    if (dim > 2) return set_err(VACC_E_ARG, "dim must be 0, 1 or 2");
    const auto& T = c->trans[dim];
    if (T.epoch != c->cell_epoch) return set_err(VACC_E_STATE, "no pencils installed for the current cells and dim");
-   if (c->opt.storage != VACC_STORE_F32) return set_err(VACC_E_ARG, "spatial translation needs FP32 storage");
    if (!c->ncells) return 0;
    drop_graphs(c);
    int rc = 0;
@@ -1523,8 +1524,8 @@ int vacc_trans_map_1d(vacc_ctx* c, uint32_t dim, double dt) {
    ta.off = c->off[c->lcur];
    ta.gid = c->gid[c->lcur];
    ta.src = c->src[c->lcur];
-   ta.fin = (const float*)c->f[c->fcur];
-   ta.fout = (float*)c->f[1 - c->fcur];
+   ta.in = pool(c, c->fcur);
+   ta.out = pool(c, 1 - c->fcur);
    ta.cells = T.cells;
    ta.entries = T.entries;
    ta.ncells = c->ncells;

@@ -228,8 +228,8 @@ struct TransArgs {
    const uint32_t* off;    // [ncells+1] list offsets (GIDs ascending within each cell)
    const uint32_t* gid;
    const uint32_t* src;    // list -> input pool index (INVALID = zero block)
-   const float* fin;
-   float* fout;            // compact output pool (block b of the list at b)
+   Pool in;
+   Pool out;               // compact output pool (block b of the list at b), same storage form
    const TransCell* cells;
    const TransEntry* entries;
    uint32_t ncells;

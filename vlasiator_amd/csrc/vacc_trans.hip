@@ -15,6 +15,7 @@
 // written keep their values.
 #include <hip/hip_runtime.h>
 
+#include "vacc_codec.h"
 #include "vacc_kernels.h"
 #include "vacc_recon.h"
 
@@ -90,6 +91,49 @@ __device__ inline float leaving(float zt, float c0, float c1, float c2) {
    return z2 * (c0 + z2 * (c1 + z2 * c2)) - z1 * (c0 + z1 * (c1 + z1 * c2));
 }
 
+// Value e of pool block idx: FP32, or scale * decoded 16-bit code (vacc_codec.h).
+template <int ST>
+__device__ inline float pool_val(const Pool& p, uint32_t idx, int e) {
+   if constexpr (ST == ST_F32) {
+      return static_cast<const float*>(p.f)[(size_t)idx * 64 + e];
+   } else {
+      return p.scale[idx] * Codec<ST>::dec(static_cast<const uint16_t*>(p.f)[(size_t)idx * 64 + e]);
+   }
+}
+
+// Writes block b: lane l holds the value of in-block index ab.  The 16-bit forms encode the
+// block with the canonical lane order (lane e = in-block index e), so the values are first
+// moved there: lane e reads the lane whose transposed index maps to e.
+template <int ST>
+__device__ inline void store_block(const Pool& p, uint32_t b, float x, int ab, int dim, int lane) {
+   if constexpr (ST == ST_F32) {
+      static_cast<float*>(p.f)[(size_t)b * 64 + ab] = x;
+   } else {
+      const int e = lane;
+      const int from = (dim == 0) ? (e >> 4) + 4 * ((e >> 2) & 3) + 16 * (e & 3)
+                       : (dim == 1) ? (e & 3) + 4 * (e >> 4) + 16 * ((e >> 2) & 3)
+                                    : e;
+      const float xe = __shfl(x, from, 64);
+      uint32_t code;
+      const float sc = encode_wave64<ST>(xe, code);
+      static_cast<uint16_t*>(p.f)[(size_t)b * 64 + e] = (uint16_t)code;
+      if (e == 0) p.scale[b] = sc;
+   }
+}
+
+// Copies block sb of the input pool (INVALID: a zero block) to block b of the output.
+template <int ST>
+__device__ inline void copy_block(const Pool& in, const Pool& out, uint32_t sb, uint32_t b, int lane) {
+   const bool ok = sb != VACC_INVALID_ID;
+   if constexpr (ST == ST_F32) {
+      static_cast<float*>(out.f)[(size_t)b * 64 + lane] = ok ? static_cast<const float*>(in.f)[(size_t)sb * 64 + lane] : 0.0f;
+   } else {
+      static_cast<uint16_t*>(out.f)[(size_t)b * 64 + lane] =
+          ok ? static_cast<const uint16_t*>(in.f)[(size_t)sb * 64 + lane] : (uint16_t)0;
+      if (lane == 0) out.scale[b] = ok ? in.scale[sb] : 0.0f;
+   }
+}
+
 // Largest t with off[t] <= b (the non-empty cell holding list block b).
 __device__ inline uint32_t cell_of(const uint32_t* off, uint32_t ncells, uint32_t b) {
    uint32_t lo = 0, hi = ncells - 1;
@@ -101,6 +145,7 @@ __device__ inline uint32_t cell_of(const uint32_t* off, uint32_t ncells, uint32_
    return lo;
 }
 
+template <int ST>
 __global__ __launch_bounds__(TWG) void k_trans(TransArgs a) {
    // per wave: a window of WIN consecutive entries (GID, pool index) of each stencil cell's
    // sorted block list, starting at the first entry not below the chunk's first GID
@@ -158,8 +203,7 @@ __global__ __launch_bounds__(TWG) void k_trans(TransArgs a) {
             }
          }
          if (ne == 0) {   // not written: the block keeps its values
-            const uint32_t sb = a.src[b];
-            a.fout[(size_t)b * 64 + lane] = (sb != VACC_INVALID_ID) ? a.fin[(size_t)sb * 64 + lane] : 0.0f;
+            copy_block<ST>(a.in, a.out, a.src[b], b, lane);
             continue;
          }
          const uint32_t g = (uint32_t)__builtin_amdgcn_readlane((int)gch, (int)(b - b0));
@@ -212,7 +256,7 @@ __global__ __launch_bounds__(TWG) void k_trans(TransArgs a) {
                const unsigned long long hit = __ballot(s_gid[wv][j][lane] == g);
                if (hit) {
                   const uint32_t idx = s_src[wv][j][__builtin_ctzll(hit)];
-                  if (idx != VACC_INVALID_ID) v[j] = a.fin[(size_t)idx * 64 + ab];
+                  if (idx != VACC_INVALID_ID) v[j] = pool_val<ST>(a.in, idx, ab);
                }
             }
          }
@@ -257,7 +301,7 @@ __global__ __launch_bounds__(TWG) void k_trans(TransArgs a) {
                const float ng = leaving(zt2, c0, c1, c2);
                acc = acc + (!(zt2 > 0.0f) ? (ng * dzi2) / E2.dz_t : 0.0f) * E2.ratio;
             }
-            a.fout[(size_t)b * 64 + ab] = acc;
+            store_block<ST>(a.out, b, acc, ab, a.dim, lane);
             continue;
          }
          for (uint32_t e = 0; e < ne; ++e) {
@@ -290,7 +334,7 @@ __global__ __launch_bounds__(TWG) void k_trans(TransArgs a) {
             }
             acc = acc + contrib;
          }
-         a.fout[(size_t)b * 64 + ab] = acc;
+         store_block<ST>(a.out, b, acc, ab, a.dim, lane);
       }
    }
 }
@@ -299,7 +343,9 @@ void launch_trans(const TransArgs& a, hipStream_t s) {
    if (a.nblocks == 0 || a.ncells == 0) return;
    const uint64_t chunks = ((uint64_t)a.nblocks + TCHUNK - 1) / TCHUNK;
    const uint64_t wgs = std::min<uint64_t>((chunks + (TWG / 64) - 1) / (TWG / 64), 1u << 20);
-   hipLaunchKernelGGL(k_trans, dim3((uint32_t)wgs), dim3(TWG), 0, s, a);
+   if (a.in.storage == ST_BF16) hipLaunchKernelGGL(k_trans<ST_BF16>, dim3((uint32_t)wgs), dim3(TWG), 0, s, a);
+   else if (a.in.storage == ST_F16) hipLaunchKernelGGL(k_trans<ST_F16>, dim3((uint32_t)wgs), dim3(TWG), 0, s, a);
+   else hipLaunchKernelGGL(k_trans<ST_F32>, dim3((uint32_t)wgs), dim3(TWG), 0, s, a);
 }
 
 }  // namespace vacc
