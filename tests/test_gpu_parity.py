@@ -356,12 +356,13 @@ def test_fast_arith_within_tolerance(cfg, method):
     ctx.close()
 
 
-@pytest.mark.parametrize("per_cell,adjust_mode", [(0, 1), (1, 1), (0, 2)])
+@pytest.mark.parametrize("per_cell,adjust_mode", [(0, 1), (1, 1), (0, 2), (0, 3)])
 def test_cpp_host_adapter_driver(tmp_path, per_cell, adjust_mode):
     """The C++ host adapter (vlasiator_amd/host/vacc_host.hpp) through its example driver:
     three steps of intersections + remaps (batched, or one cell per launch) + adjust
-    (single-cell, or with ring spatial neighbours), checked bit for bit against the oracle run
-    with the intersections the driver computed."""
+    (single-cell, or with ring spatial neighbours, or ring neighbours then translation along
+    the ring as a periodic pencil), then a VLSV write and read-back through the adapter;
+    checked bit for bit against the oracle run with the intersections the driver computed."""
     import os
     import struct
     import subprocess
@@ -409,6 +410,12 @@ def test_cpp_host_adapter_driver(tmp_path, per_cell, adjust_mode):
             ref = [O.adjust_spatial(m, g, f, w.min_value, w.add_width,
                                     np.concatenate([content[(i - 1) % n], content[(i + 1) % n]]))[:2]
                    for i, (g, f) in enumerate(acc)]
+        if adjust_mode == 3:
+            L = n + 4
+            pen = (np.array([L], np.uint32), np.array([0], np.uint64),
+                   np.array([(q + 2 * n - 2) % n for q in range(L)], np.uint32), np.full(L, 1.0e7, np.float32),
+                   np.array([1.0 if 1 <= q <= L - 2 else 0.0 for q in range(L)], np.float32))
+            ref = O.trans_map_1d(m, ref, pen, s % 3, 1.5, w.min_value)
     cases.assert_cells_equal(got, ref, f"C++ driver adjust {adjust_mode}")
     # the host intersections agree with the oracle transform
     assert np.allclose(isx[0], cases.oracle_intersections(w, 0), rtol=0,

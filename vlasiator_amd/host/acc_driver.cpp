@@ -2,7 +2,10 @@
 // loop of calculateAcceleration (vlasovmover.cpp:410-475) for a set of spatial cells of one
 // population: per cell the FP64 transform and intersections, then the three remaps (batched,
 // or one cell per launch), then the block adjust: single-cell (adjust = 1), or with spatial
-// neighbours (adjust = 2: the cells form a ring, cell i neighbouring i-1 and i+1).
+// neighbours (adjust = 2: the cells form a ring, cell i neighbouring i-1 and i+1), or with
+// spatial neighbours followed by the spatial translation along dimension step % 3 of the ring
+// as one periodic pencil (adjust = 3; cell width 1e7 m, dt 1.5 s).  At the end the cells'
+// blocks go through a VLSV file (<output>.vlsv, written and read back by the adapter).
 //
 //   vacc_acc_driver <input> <output>
 //
@@ -13,6 +16,7 @@
 //   f64 RHOLOSSADJUST; then i32 OR of the status codes.
 #include <cstdio>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "vacc_host.hpp"
@@ -93,13 +97,53 @@ int main(int argc, char** argv) {
          if (rc > 0) status_or |= rc;
       }
       if (rc >= 0 && do_adjust == 1) rc = engine.adjust_blocks(ptrs, 0);
-      if (rc >= 0 && do_adjust == 2) {
+      if (rc >= 0 && (do_adjust == 2 || do_adjust == 3)) {
          std::vector<std::vector<uint32_t>> ring(ncells);
          for (uint32_t i = 0; i < ncells; ++i) ring[i] = {(i + ncells - 1) % ncells, (i + 1) % ncells};
          rc = engine.adjust_blocks(ptrs, 0, ring);
       }
+      if (rc >= 0 && do_adjust == 3) {
+         // the ring as one periodic pencil: stencil cells n-2, n-1 | 0 .. n-1 | 0, 1
+         vacc_host::PencilSet ps;
+         const uint32_t L = ncells + 2 * VACC_TRANS_STENCIL;
+         ps.lengthOfPencils = {L};
+         ps.idsStart = {0};
+         for (uint32_t q = 0; q < L; ++q) {
+            ps.ids.push_back((q + 2 * ncells - VACC_TRANS_STENCIL) % ncells);
+            ps.sourceDZ.push_back(1.0e7f);
+            ps.targetRatios.push_back((q >= VACC_TRANS_STENCIL - 1 && q <= L - VACC_TRANS_STENCIL) ? 1.0f : 0.0f);
+         }
+         rc = engine.trans_map_1d(ptrs, ps, map_order, 1.5, 0);
+      }
       if (rc < 0) {
          fprintf(stderr, "acceleration: %s\n", engine.last_error().c_str());
+         return 1;
+      }
+   }
+   // through a VLSV restart file and back
+   {
+      const std::string path = std::string(argv[2]) + ".vlsv";
+      std::vector<uint64_t> ids(ncells);
+      for (uint32_t i = 0; i < ncells; ++i) ids[i] = 1000 + i;
+      vacc_vlsv_writer* w = nullptr;
+      if (vacc_vlsv_create(path.c_str(), &w) || engine.write_velocity_distribution(0, w, ptrs, ids, "proton") < 0 ||
+          vacc_vlsv_close(w)) {
+         fprintf(stderr, "vlsv write: %s\n", engine.last_error().c_str());
+         return 1;
+      }
+      for (auto& c : cells) {
+         c.populations[0].gids.clear();
+         c.populations[0].f.clear();
+      }
+      vacc_vlsv_reader* r = nullptr;
+      std::vector<uint64_t> got;
+      if (vacc_vlsv_open(path.c_str(), &r) || engine.read_block_data(r, "SpatialGrid", 0, ptrs, got, 0, "proton") < 0) {
+         fprintf(stderr, "vlsv read: %s\n", engine.last_error().c_str());
+         return 1;
+      }
+      vacc_vlsv_reader_close(r);
+      if (got != ids) {
+         fprintf(stderr, "vlsv read: cell ids differ\n");
          return 1;
       }
    }
