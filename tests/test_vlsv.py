@@ -173,3 +173,45 @@ def test_large_arrays_split_over_io_threads(tmp_path):
     with L.Reader(str(p)) as r:
         assert np.array_equal(r.read_array("BLOCKVARIABLE", {"name": "a"}), a)
         assert np.array_equal(r.read_array("IDS", {"name": "b"}, 17, 2500000), b[17:2500017])
+
+
+def test_library_reader_survives_corrupted_files(tmp_path):
+    """Random byte changes and truncations of a valid file: opening and reading either work or
+    fail with an error; the library never crashes and never reads outside the file."""
+    from vlasiator_amd import vlsv as L
+    arrs = _arrays(np.random.default_rng(8))
+    good = tmp_path / "good.vlsv"
+    OV.write(good, arrs)
+    raw = bytearray(good.read_bytes())
+    _, foot = struct.unpack_from("<QQ", raw, 0)
+    rng = np.random.default_rng(9)
+    bad = tmp_path / "bad.vlsv"
+    opened = failed = 0
+    for trial in range(400):
+        b = bytearray(raw)
+        kind = trial % 4
+        if kind == 0:      # bytes of the footer
+            for _ in range(rng.integers(1, 6)):
+                b[rng.integers(foot, len(b))] = rng.integers(0, 256)
+        elif kind == 1:    # footer digits (offsets and sizes)
+            digits = [i for i in range(foot, len(b)) if chr(b[i]).isdigit()]
+            for _ in range(rng.integers(1, 4)):
+                b[digits[rng.integers(len(digits))]] = ord(str(rng.integers(0, 10)))
+        elif kind == 2:    # the header words
+            struct.pack_into("<Q", b, 8 * int(rng.integers(0, 2)), int(rng.integers(0, 2 ** 63)))
+        else:              # truncation
+            b = b[:rng.integers(0, len(b))]
+        bad.write_bytes(bytes(b))
+        try:
+            r = L.Reader(str(bad))
+        except V.VaccError:
+            failed += 1
+            continue
+        opened += 1
+        with r:
+            for tag, attrs, a in arrs:
+                try:
+                    r.read_array(tag, attrs)
+                except V.VaccError:
+                    pass
+    assert opened > 0 and failed > 0

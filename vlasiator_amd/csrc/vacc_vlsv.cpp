@@ -323,6 +323,12 @@ const Entry* find(const vacc_vlsv_reader* r, const char* tag, const Attrs& attrs
    return nullptr;
 }
 
+// The array's byte size, or false when arraysize * vectorsize * datasize overflows.
+bool array_bytes(const Entry& e, uint64_t& n) {
+   uint64_t v;
+   return !__builtin_mul_overflow(e.arraysize, e.vectorsize, &v) && !__builtin_mul_overflow(v, e.datasize, &n);
+}
+
 int need(const vacc_vlsv_reader* r, const char* tag, const Attrs& attrs, const Entry*& e) {
    e = find(r, tag, attrs);
    if (!e) {
@@ -330,7 +336,9 @@ int need(const vacc_vlsv_reader* r, const char* tag, const Attrs& attrs, const E
       for (const auto& kv : attrs) a += " " + kv.first + "=" + kv.second;
       return set_error(VACC_E_ARG, std::string("no array ") + tag + a + " in the file");
    }
-   if (e->offset + e->bytes() > r->size) return set_error(VACC_E_ARG, std::string("array ") + tag + " beyond end of file");
+   uint64_t n = 0;
+   if (!array_bytes(*e, n) || e->offset > r->size || n > r->size - e->offset)
+      return set_error(VACC_E_ARG, std::string("array ") + tag + " beyond end of file");
    return 0;
 }
 
@@ -517,6 +525,7 @@ int vacc_vlsv_read_block_data(vacc_vlsv_reader* r, vacc_ctx* ctx, const char* po
       // ioread.cpp:520-631: the file's velocity mesh must sit inside ours at a block offset
       if (bbox[3] != VACC_WID || bbox[4] != VACC_WID || bbox[5] != VACC_WID)
          return set_error(VACC_E_ARG, "file velocity blocks are not 4x4x4 cells");
+      if (bbox[0] == 0 || bbox[1] == 0 || bbox[2] == 0) return set_error(VACC_E_ARG, "empty file velocity mesh");
       for (int d = 0; d < 3; ++d)
          if (m.grid_length[d] < bbox[d]) return set_error(VACC_E_ARG, "trying to shrink velocity space");
       static const char* const crd_tags[3] = {"MESH_NODE_CRDS_X", "MESH_NODE_CRDS_Y", "MESH_NODE_CRDS_Z"};
