@@ -277,29 +277,43 @@ __global__ __launch_bounds__(TWG) void k_trans(TransArgs a) {
             const TransEntry& E0 = s_ent[wv][0];
             const TransEntry& E1 = s_ent[wv][1];
             const TransEntry& E2 = s_ent[wv][2];
-            const float f0 = face_nu(E0.fl, v[0], v[1], v[2], v[3]);
             const float f1 = face_nu(E0.fr, v[1], v[2], v[3], v[4]);
             const float f2 = face_nu(E1.fr, v[2], v[3], v[4], v[5]);
-            const float f3 = face_nu(E2.fr, v[3], v[4], v[5], v[6]);
             const float dzi0 = E0.dz_i, dzi1 = E1.dz_i, dzi2 = E2.dz_i;
             const float zt0 = vdt / dzi0;
             const float zt1 = (dzi1 == dzi0) ? zt0 : vdt / dzi1;
             const float zt2 = (dzi2 == dzi1) ? zt1 : vdt / dzi2;
+            // p1 is select(zt0 > 0, .., 0) * ratio and m1 select(zt2 <= 0, .., 0) * ratio: where
+            // no lane of the block moves that way (a block's velocities along dim mostly share
+            // one sign) the contribution is +0 in every lane, so the source's reconstruction is
+            // skipped and the +0 is still added
+            const bool up0 = __ballot(zt0 > 0.0f) != 0ull;
+            const bool dn2 = __ballot(!(zt2 > 0.0f)) != 0ull;
             float c0, c1, c2;
-            if (do0) {
-               ppm_nu_coeffs(f0, f1, v[1], v[2], v[3], a.thr, a.scale, c0, c1, c2);
-               const float ng = leaving(zt0, c0, c1, c2);
-               acc = acc + ((zt0 > 0.0f) ? (ng * dzi0) / E0.dz_t : 0.0f) * E0.ratio;
+            if (up0) {
+               const float f0 = face_nu(E0.fl, v[0], v[1], v[2], v[3]);
+               if (do0) {
+                  ppm_nu_coeffs(f0, f1, v[1], v[2], v[3], a.thr, a.scale, c0, c1, c2);
+                  const float ng = leaving(zt0, c0, c1, c2);
+                  acc = acc + ((zt0 > 0.0f) ? (ng * dzi0) / E0.dz_t : 0.0f) * E0.ratio;
+               }
+            } else if (do0) {
+               acc = acc + 0.0f * E0.ratio;
             }
             if (do1) {
                ppm_nu_coeffs(f1, f2, v[2], v[3], v[4], a.thr, a.scale, c0, c1, c2);
                const float ng = leaving(zt1, c0, c1, c2);
                acc = acc + (v[3] - ng) * E1.ratio;
             }
-            if (do2) {
-               ppm_nu_coeffs(f2, f3, v[3], v[4], v[5], a.thr, a.scale, c0, c1, c2);
-               const float ng = leaving(zt2, c0, c1, c2);
-               acc = acc + (!(zt2 > 0.0f) ? (ng * dzi2) / E2.dz_t : 0.0f) * E2.ratio;
+            if (dn2) {
+               const float f3 = face_nu(E2.fr, v[3], v[4], v[5], v[6]);
+               if (do2) {
+                  ppm_nu_coeffs(f2, f3, v[3], v[4], v[5], a.thr, a.scale, c0, c1, c2);
+                  const float ng = leaving(zt2, c0, c1, c2);
+                  acc = acc + (!(zt2 > 0.0f) ? (ng * dzi2) / E2.dz_t : 0.0f) * E2.ratio;
+               }
+            } else if (do2) {
+               acc = acc + 0.0f * E2.ratio;
             }
             store_block<ST>(a.out, b, acc, ab, a.dim, lane);
             continue;
@@ -322,6 +336,11 @@ __global__ __launch_bounds__(TWG) void k_trans(TransArgs a) {
             if (((mask >> (lane & 56)) & 0xFFull) == 0ull) continue;
             const float zt = vdt / E.dz_i;
             const bool pos = zt > 0.0f;
+            const bool take = (E.kind == 0) ? pos : !pos;
+            if (E.kind != 1 && __ballot(take) == 0ull) {   // +0 in every lane (see the canonical path)
+               acc = acc + 0.0f * E.ratio;
+               continue;
+            }
             float c0, c1, c2;
             ppm_nu(E, u, a.thr, a.scale, c0, c1, c2);
             const float ng = leaving(zt, c0, c1, c2);
@@ -329,7 +348,6 @@ __global__ __launch_bounds__(TWG) void k_trans(TransArgs a) {
             if (E.kind == 1) {
                contrib = (u[2] - ng) * E.ratio;
             } else {
-               const bool take = (E.kind == 0) ? pos : !pos;
                contrib = (take ? (ng * E.dz_i) / E.dz_t : 0.0f) * E.ratio;
             }
             acc = acc + contrib;
