@@ -193,6 +193,25 @@ TRANS_DX = 1.0e7    # m; with dt below, |v| dt / dx <= 0.45 on the +-3000 km/s m
 TRANS_DT = 1.5      # s
 
 
+def trans_storage(w, storage: str, steps: int, pencils):
+    """x, y, z translation steps of the initial state in another storage form."""
+    ctx = make_context(w, storage)
+    ctx.init_peaks([W.to_vacc_peaks(c) for c in w.cells], W.boxes(w))
+    nb = ctx.total_blocks()
+    for d in range(3):
+        TR.set_pencils(ctx, d, pencils[d])
+        TR.trans_map_1d(ctx, d, TRANS_DT)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for s in range(steps):
+        for d in range(3):
+            TR.trans_map_1d(ctx, d, TRANS_DT)
+    ctx.sync()
+    el = time.perf_counter() - t0
+    ctx.close()
+    return {"ms_per_step": 1e3 * el / steps, "value": 64 * nb * steps / el, "steps": steps}
+
+
 def trans_form(w, steps: int, warmup: int, cpu_seconds: float):
     """Spatial translation of the workload's initial state on a periodic grid of its cells:
     one step is the x, y and z translation of every cell (vacc_trans_map_1d x 3).  Roofline of
@@ -237,6 +256,7 @@ def trans_form(w, steps: int, warmup: int, cpu_seconds: float):
     if ratio is not None:
         out["roofline"]["traffic"] = int(ratio * alg)
         out["roofline"]["traffic_source"] = f"{src}: rocprofv3 FETCH_SIZE + WRITE_SIZE per launch, x{ratio:.3f} algorithmic"
+    out["storage_forms"] = {st: trans_storage(w, st, min(steps, 3), pencils) for st in ("bf16",)}
     log(f"translation: grid {dims}, {nb} blocks, {out['ms_per_step']:.2f} ms per x,y,z step, "
         f"k_trans {per_launch_ms:.3f} ms/launch = {achieved:.0f} GB/s algorithmic")
     if sample is not None:
