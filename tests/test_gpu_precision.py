@@ -6,7 +6,7 @@ Per cell: L1 and L-inf of f relative to the FP64 run (blocks missing on one side
 zero), and the relative error of density n, bulk velocity |V - V64| / |V64| and scalar
 pressure P = (m/3) sum f |v - V|^2 dV^3 (arch_moments.h:123-150).
 
-Bounds asserted (the north-star precision contract): density within 1e-6 relative for every
+Bounds asserted (the precision contract of the storage forms): density within 1e-6 relative for every
 form.  The other figures are reported (set VACC_PRECISION_OUT to write them as JSON) and
 bounded loosely so that a broken codec fails: L1 of f <= 2e-4 (FP32), 3e-3 (FP16), 3e-2 (BF16).
 """

@@ -147,3 +147,30 @@ def test_cells_written_by_two_pencils(grid):
         # the shared cells did receive mass from both pencils
         assert not np.array_equal(want[A][1], six[A][1])
         ctx.close()
+
+
+def test_graph_steps_interleaved_with_translation(grid):
+    """Acceleration steps replayed from captured graphs (vacc_options.use_graph) alternating
+    with x, y, z translations: the translation keeps the captured variants (it moves no
+    buffer); every state equals that of an eager context doing the same."""
+    w, cells, m, kind = grid
+    fields = [V.make_fields(c.B, c.V, c.dt) for c in w.cells]
+    mesh = V.make_mesh(w.grid_length, w.vmin, w.vmax)
+    sp = V.make_species(w.mass, w.charge, w.min_value, w.add_width)
+    eager = V.Context(mesh, sp, V.make_options(w.method))
+    graph = V.Context(mesh, sp, V.make_options(w.method, use_graph=True))
+    pens = [T.build_pencils_uniform(DIMS, (True, True, True), kind, DX, d) for d in range(3)]
+    for c in (eager, graph):
+        c.upload_cells(cells)
+        c.set_fields(fields)
+        for d in range(3):
+            T.set_pencils(c, d, pens[d])
+    for step in range(8):
+        _say(f"graph + translation step {step}")
+        for c in (eager, graph):
+            c.step(step % 3, use_moments=True, adjust=True)
+            T.trans_map_1d(c, step % 3, DT)
+        if step in (3, 7):
+            cases.assert_cells_equal(graph.download_cells(), eager.download_cells(), f"step {step}")
+    eager.close()
+    graph.close()

@@ -1516,7 +1516,8 @@ int vacc_trans_map_1d(vacc_ctx* c, uint32_t dim, double dt) {
    const auto& T = c->trans[dim];
    if (T.epoch != c->cell_epoch) return set_err(VACC_E_STATE, "no pencils installed for the current cells and dim");
    if (!c->ncells) return 0;
-   drop_graphs(c);
+   // captured steps stay valid: no buffer moves, and the list/pool parity and validity flags
+   // that select a captured variant are updated below
    int rc = 0;
    if (!c->sorted && (rc = do_relist(c, -1, false))) return rc;   // ascending GIDs, same blocks
    if ((rc = refresh_count(c))) return rc;
