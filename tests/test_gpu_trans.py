@@ -174,3 +174,49 @@ def test_graph_steps_interleaved_with_translation(grid):
             cases.assert_cells_equal(graph.download_cells(), eager.download_cells(), f"step {step}")
     eager.close()
     graph.close()
+
+
+@pytest.mark.parametrize("periodic", [(True, True, True), (False, True, True)])
+def test_long_pencils_bit_exact(periodic):
+    """Pencils of 130 cells: the pencil walk takes them in segments of at most 58 positions
+    whose stencils overlap; boundary and inactive cells split some lines."""
+    dims = (130, 2, 1)
+    n = int(np.prod(dims))
+    w = W.config(0, ncells=n)
+    cells = cases.build_cells(w)
+    m = cases.oracle_mesh(w)
+    rng = np.random.default_rng(11)
+    kind = np.zeros(n, np.uint8)
+    kind[rng.random(n) < 0.03] = T.CELL_BOUNDARY
+    kind[67] = T.CELL_INACTIVE
+    dx = (1.0e7, 9.0e6, 1.1e7)
+    for dim in (0, 1):
+        pen = T.build_pencils_uniform(dims, periodic, kind, dx, dim)
+        _say(f"long pencils dim={dim} periodic={periodic} pencils={len(pen[0])} longest={int(pen[0].max())}")
+        ctx = _ctx(w)
+        ctx.upload_cells(cells)
+        T.set_pencils(ctx, dim, pen)
+        T.trans_map_1d(ctx, dim, DT)
+        want = O.trans_map_1d(m, cells, pen, dim, DT, w.min_value)
+        cases.assert_cells_equal(ctx.download_cells(), want, f"long dim {dim}")
+        ctx.close()
+
+
+def test_pencil_with_empty_positions(grid):
+    """A pencil position without a cell (VACC_INVALID_ID) reads as zero blocks and is never
+    written; its neighbours are still walked."""
+    w, cells, m, kind = grid
+    ten = cells[:10]
+    ids = [0, 1, 2, 3, 0xFFFFFFFF, 4, 5, 6, 7, 8, 9, 0xFFFFFFFF]
+    ratio = [0.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 0.0]
+    pen = (np.array([len(ids)], np.uint32), np.array([0], np.uint64), np.array(ids, np.uint32),
+           np.full(len(ids), 1.0e7, np.float32), np.array(ratio, np.float32))
+    for dim in range(3):
+        _say(f"empty positions dim={dim}")
+        ctx = _ctx(w)
+        ctx.upload_cells(ten)
+        T.set_pencils(ctx, dim, pen)
+        T.trans_map_1d(ctx, dim, DT)
+        want = O.trans_map_1d(m, ten, pen, dim, DT, w.min_value)
+        cases.assert_cells_equal(ctx.download_cells(), want, f"empty positions dim {dim}")
+        ctx.close()

@@ -109,7 +109,15 @@ struct vacc_ctx {
    struct TransDim {
       TransCell* cells = nullptr;
       TransEntry* entries = nullptr;
-      size_t cells_cap = 0, entries_cap = 0;
+      uint32_t* rest = nullptr;    // cells that are not walked
+      uint32_t* bounds = nullptr;  // walk share bounds, [nseg][wps + 1]
+      size_t bounds_cap = 0;
+      WalkSeg* seg = nullptr;      // pencil walk segments, their window positions and faces
+      WalkPos* pos = nullptr;
+      TransFace* face = nullptr;
+      size_t cells_cap = 0, entries_cap = 0, rest_cap = 0, seg_cap = 0, pos_cap = 0, face_cap = 0;
+      uint32_t nrest = 0;
+      uint32_t nseg = 0;
       uint64_t epoch = ~0ull;
    } trans[3];
    // event timing
@@ -682,6 +690,21 @@ int import_blocks(vacc_ctx* c, uint32_t ncells, const uint64_t* offsets, const u
 
 }  // namespace vacc
 
+// Device array p of at least n elements (contents not kept when it grows).
+template <class T>
+static int dev_grow(T*& p, size_t& cap, size_t n) {
+   if (p && cap >= n) return 0;
+   if (p) (void)hipFree(p);
+   p = nullptr;
+   cap = 0;
+   HIPCHK(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+   cap = std::max<size_t>(n, 1);
+   return 0;
+}
+
+// Waves of a pencil walk launch (all segments together), about.
+constexpr uint32_t WALK_WAVES = 32768;
+
 extern "C" {
 
 const char* vacc_last_error(void) { return g_err.c_str(); }
@@ -740,8 +763,9 @@ int vacc_destroy(vacc_ctx* c) {
    }
    if (c->io_dev) (void)hipFree(c->io_dev);
    for (auto& t : c->trans) {
-      if (t.cells) (void)hipFree(t.cells);
-      if (t.entries) (void)hipFree(t.entries);
+      void* tp[] = {t.cells, t.entries, t.rest, t.bounds, t.seg, t.pos, t.face};
+      for (void* p : tp)
+         if (p) (void)hipFree(p);
    }
    void* sp[] = {c->d_nbr_off, c->d_nbr, c->d_ext_off, c->d_ext};
    for (void* p : sp)
@@ -1409,7 +1433,7 @@ int vacc_trans_set_pencils(vacc_ctx* c, uint32_t dim, const vacc_pencils* P) {
    const uint32_t nc = c->ncells;
    std::vector<std::vector<TransEntry>> ent(nc);
    std::vector<std::vector<uint32_t>> tab(nc);
-   // per target cell: (pencil, target position) of each entry, for the canonical test
+   // per target cell: (pencil, target position) of each entry, for the walk test
    std::vector<std::vector<std::pair<uint32_t, int>>> where(nc);
    size_t total = 0;
    // contributions in the reference's order: pencils as given (bins, then pencils of a bin),
@@ -1460,52 +1484,132 @@ int vacc_trans_set_pencils(vacc_ctx* c, uint32_t dim, const vacc_pencils* P) {
       }
    }
    if (total > 0xFFFFFFFFull) return set_err(VACC_E_ARG, "too many contributions");
+   // walk candidates: every entry of the cell from one pencil, at one target position or at two
+   // that are at least three apart (no source reaches both), so that walking the sources in
+   // order adds the contributions in the reference's order
+   struct Cand {
+      uint32_t p = ~0u;
+      int q1 = -1, q2 = -1;
+   };
+   std::vector<Cand> cand(nc);
+   for (uint32_t t = 0; t < nc; ++t) {
+      const auto& wh = where[t];
+      if (wh.empty()) continue;
+      Cand C;
+      C.p = wh[0].first;
+      bool ok = true;
+      for (const auto& x : wh) {
+         if (x.first != C.p) ok = false;
+         else if (C.q1 < 0 || x.second == C.q1) C.q1 = x.second;
+         else if (C.q2 < 0 || x.second == C.q2) C.q2 = x.second;
+         else ok = false;
+      }
+      if (C.q2 >= 0) {
+         if (C.q2 < C.q1) std::swap(C.q1, C.q2);
+         ok = ok && C.q2 - C.q1 >= 3;
+      }
+      if (ok) cand[t] = C;
+   }
+   // segments: runs [qa, qb) of at most WALK_MAXN - 6 positions with the stencils of their
+   // sources smin..smax; a segment walks a candidate whose positions lie in smin-1 .. smax+1
+   // and whose contributions all come from sources smin..smax (a one-position cell: its
+   // position in [qa, qb)); two-position cells carry the first sum in one of two registers
+   std::vector<uint8_t> walk(nc, 0);
+   std::vector<WalkSeg> segs;
+   std::vector<WalkPos> wpos;
+   std::vector<TransFace> wface;
+   const int span = WALK_MAXN - 6;
+   std::vector<uint32_t> flags;
+   for (uint32_t p = 0; p < P->npencils; ++p) {
+      const int L = (int)P->length[p];
+      const uint64_t s0 = P->start[p];
+      const uint32_t* pc = P->cell + s0;
+      const float* dz = P->dz + s0;
+      const float* ratio = P->ratio + s0;
+      for (int qa = W; qa < L - W; qa += span) {
+         const int qb = std::min(qa + span, L - W);
+         const int smin = std::max(W, qa - 1), smax = std::min(L - W - 1, qb);
+         const int wa = smin - 2, n = smax - smin + 5;
+         // sources reaching position q lie in smin..smax
+         auto inside = [&](int q) {
+            return q >= smin - 1 && q <= smax + 1 && std::max(q - 1, W) >= smin && std::min(q + 1, L - W - 1) <= smax;
+         };
+         flags.assign((size_t)n, 0u);
+         int slot_end[2] = {-1, -1};
+         int nw = 0;
+         for (int q = smin - 1; q <= smax + 1; ++q) {
+            const uint32_t t = pc[q];
+            if (t == VACC_INVALID_ID || walk[t] || ratio[q] == 0.0f) continue;
+            const Cand& C = cand[t];
+            if (C.p != p || C.q1 != q) continue;
+            if (C.q2 < 0) {
+               if (q < qa || q >= qb) continue;
+               flags[q - wa] = WALK_W;
+               walk[t] = 1;
+               ++nw;
+            } else if (inside(C.q1) && inside(C.q2) && C.q2 >= smin + 2) {
+               // the second position's sum starts at source q2 - 1 >= smin + 1, after the
+               // first has ended (source q1 + 1); a slot is free once its last pair has ended
+               const int k = (slot_end[0] < q) ? 0 : (slot_end[1] < q) ? 1 : -1;
+               if (k < 0) continue;
+               flags[C.q1 - wa] = WALK_W | ((uint32_t)(k + 1) << WALK_SAVE_SHIFT);
+               flags[C.q2 - wa] = WALK_W | ((uint32_t)(k + 1) << WALK_INIT_SHIFT);
+               slot_end[k] = C.q2;
+               walk[t] = 1;
+               nw += 2;
+            }
+         }
+         if (!nw) continue;
+         WalkSeg S{};
+         S.pos0 = (uint32_t)wpos.size();
+         S.n = (uint32_t)n;
+         S.js = 2;
+         S.je = (uint32_t)(n - 3);
+         segs.push_back(S);
+         for (int k = 0; k < n; ++k) {
+            const int q = wa + k;
+            WalkPos X{};
+            X.cell = pc[q];
+            X.dz = dz[q];
+            X.ratio = ratio[q];
+            X.flags = flags[k];
+            wpos.push_back(X);
+            wface.push_back((k >= 2 && k <= n - 2) ? trans_face(dz + q - 2) : TransFace{});
+         }
+      }
+   }
+   if (wpos.size() > 0xFFFFFFFFull) return set_err(VACC_E_ARG, "too many pencil positions");
    std::vector<TransCell> cells(nc);
    std::vector<TransEntry> flat;
+   std::vector<uint32_t> rest;
    flat.reserve(total);
    for (uint32_t t = 0; t < nc; ++t) {
+      if (!walk[t]) rest.push_back(t);
       TransCell& C = cells[t];
       C.e0 = (uint32_t)flat.size();
       C.ne = (uint32_t)ent[t].size();
       C.nn = (uint32_t)tab[t].size();
       for (size_t j = 0; j < tab[t].size(); ++j) C.nbr[j] = tab[t][j];
-      // canonical: one target position q of one pencil receiving p1 from q-1, self from q and
-      // m1 from q+1 (entries already in that order); the stencil cells are then positions
-      // q-3 .. q+3, stored in position order so that entry e reads cells e .. e+4
-      const auto& wh = where[t];
-      if (C.ne == 3 && wh[0] == wh[1] && wh[1] == wh[2] && ent[t][0].kind == 0 && ent[t][1].kind == 1 &&
-          ent[t][2].kind == 2) {
-         const uint32_t p = wh[0].first;
-         const int q = wh[0].second;
-         const uint32_t* pc = P->cell + P->start[p];
-         bool ok = true;
-         for (int k = 0; k < 7; ++k) ok = ok && pc[q - 3 + k] != VACC_INVALID_ID;
-         if (ok) {
-            C.canonical = 1;
-            C.nn = 7;
-            for (int k = 0; k < 7; ++k) C.nbr[k] = pc[q - 3 + k];
-         }
-      }
+      C.walked = walk[t];
       flat.insert(flat.end(), ent[t].begin(), ent[t].end());
    }
    auto& T = c->trans[dim];
    HIPCHK(hipStreamSynchronize(c->stream));
-   if (T.cells_cap < cells.size() || !T.cells) {
-      if (T.cells) (void)hipFree(T.cells);
-      T.cells = nullptr;
-      T.cells_cap = 0;
-      HIPCHK(hipMalloc(&T.cells, std::max<size_t>(cells.size(), 1) * sizeof(TransCell)));
-      T.cells_cap = std::max<size_t>(cells.size(), 1);
-   }
-   if (T.entries_cap < flat.size() || !T.entries) {
-      if (T.entries) (void)hipFree(T.entries);
-      T.entries = nullptr;
-      T.entries_cap = 0;
-      HIPCHK(hipMalloc(&T.entries, std::max<size_t>(flat.size(), 1) * sizeof(TransEntry)));
-      T.entries_cap = std::max<size_t>(flat.size(), 1);
-   }
+   int rc = 0;
+   if ((rc = dev_grow(T.cells, T.cells_cap, cells.size())) || (rc = dev_grow(T.entries, T.entries_cap, flat.size())) ||
+       (rc = dev_grow(T.seg, T.seg_cap, segs.size())) || (rc = dev_grow(T.pos, T.pos_cap, wpos.size())) ||
+       (rc = dev_grow(T.face, T.face_cap, wface.size())) || (rc = dev_grow(T.rest, T.rest_cap, rest.size())))
+      return rc;
    if (nc) HIPCHK(hipMemcpy(T.cells, cells.data(), nc * sizeof(TransCell), hipMemcpyHostToDevice));
    if (!flat.empty()) HIPCHK(hipMemcpy(T.entries, flat.data(), flat.size() * sizeof(TransEntry), hipMemcpyHostToDevice));
+   if (!segs.empty()) {
+      HIPCHK(hipMemcpy(T.seg, segs.data(), segs.size() * sizeof(WalkSeg), hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(T.pos, wpos.data(), wpos.size() * sizeof(WalkPos), hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(T.face, wface.data(), wface.size() * sizeof(TransFace), hipMemcpyHostToDevice));
+   }
+   if (!rest.empty()) HIPCHK(hipMemcpy(T.rest, rest.data(), rest.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+   T.nrest = (uint32_t)rest.size();
+   T.nseg = (uint32_t)segs.size();
    T.epoch = c->cell_epoch;
    return 0;
 }
@@ -1513,7 +1617,7 @@ int vacc_trans_set_pencils(vacc_ctx* c, uint32_t dim, const vacc_pencils* P) {
 int vacc_trans_map_1d(vacc_ctx* c, uint32_t dim, double dt) {
    if (!c) return set_err(VACC_E_ARG, "null context");
    if (dim > 2) return set_err(VACC_E_ARG, "dim must be 0, 1 or 2");
-   const auto& T = c->trans[dim];
+   auto& T = c->trans[dim];
    if (T.epoch != c->cell_epoch) return set_err(VACC_E_STATE, "no pencils installed for the current cells and dim");
    if (!c->ncells) return 0;
    // captured steps stay valid: no buffer moves, and the list/pool parity and validity flags
@@ -1529,6 +1633,8 @@ int vacc_trans_map_1d(vacc_ctx* c, uint32_t dim, double dt) {
    ta.out = pool(c, 1 - c->fcur);
    ta.cells = T.cells;
    ta.entries = T.entries;
+   ta.rest = T.rest;
+   ta.nrest = T.nrest;
    ta.ncells = c->ncells;
    ta.nblocks = (uint32_t)c->nblocks;
    ta.dim = (int)dim;
@@ -1538,8 +1644,22 @@ int vacc_trans_map_1d(vacc_ctx* c, uint32_t dim, double dt) {
    ta.dt = (float)dt;
    ta.thr = (float)c->sp.sparse_min_value;
    ta.scale = (float)(1.0 / (double)ta.thr);
+   WalkArgs wa{};
+   wa.t = ta;
+   wa.seg = T.seg;
+   wa.pos = T.pos;
+   wa.face = T.face;
+   wa.nseg = T.nseg;
+   wa.nmesh = c->mi.len[0] * c->mi.len[1] * c->mi.len[2];
+   wa.wps = 1;   // a power of two, about WALK_WAVES in all, at most 1024 per segment
+   while (wa.wps < 1024 && (uint64_t)T.nseg * wa.wps * 2 <= WALK_WAVES) wa.wps *= 2;
+   wa.bin_shift = 0;
+   while (wa.nmesh && ((wa.nmesh - 1) >> wa.bin_shift) >= 4096u) ++wa.bin_shift;
+   if ((rc = dev_grow(T.bounds, T.bounds_cap, (size_t)T.nseg * (wa.wps + 1)))) return rc;
+   wa.bounds = T.bounds;
    {
       ProfScope ps(c, 4);
+      launch_trans_walk(wa, T.bounds, c->stream);
       launch_trans(ta, c->stream);
    }
    HIPCHK(hipGetLastError());

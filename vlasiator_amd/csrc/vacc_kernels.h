@@ -221,8 +221,27 @@ struct TransCell {
    uint32_t ne;        // entries (0: the cell is not written and keeps its values)
    uint32_t nn;        // stencil cells in nbr
    uint32_t nbr[TRANS_MAXN];
-   uint32_t canonical; // 1: the three entries p1 from q-1, self from q, m1 from q+1 of one
-                       // pencil, nbr[0..6] = the cells at positions q-3 .. q+3
+   uint32_t walked;    // 1: written by the pencil walk (k_trans_walk), skipped by k_trans
+};
+// Pencil walk: a segment of one pencil whose walked cells are all written by one wave per
+// velocity block.  A walked cell receives every contribution from this segment's sources,
+// at one window position or at two that are at least three apart (a periodic pencil's wrap).
+// Window positions 0..n-1 are consecutive pencil positions; sources are window positions
+// js..je, with js = 2 and je = n - 3; walked positions lie in js-1 .. je+1.
+constexpr int WALK_MAXN = 64;    // window positions per segment (one lane each)
+constexpr uint32_t WALK_W = 1u;           // flags: the position's cell is written by the segment
+constexpr uint32_t WALK_SAVE_SHIFT = 1;   // flags bits 1-2: finished sum saved in carry 1 or 2
+constexpr uint32_t WALK_INIT_SHIFT = 3;   // flags bits 3-4: sum starts from carry 1 or 2
+struct WalkSeg {
+   uint32_t pos0;      // first window position in the walk position and face arrays
+   uint32_t n;         // window positions
+   uint32_t js, je;    // first and last source
+};
+struct WalkPos {
+   uint32_t cell;      // context cell (VACC_INVALID_ID: none)
+   float dz;           // width along the dimension
+   float ratio;        // target ratio
+   uint32_t flags;     // WALK_W, save and init carry slots
 };
 struct TransArgs {
    const uint32_t* off;    // [ncells+1] list offsets (GIDs ascending within each cell)
@@ -232,6 +251,8 @@ struct TransArgs {
    Pool out;               // compact output pool (block b of the list at b), same storage form
    const TransCell* cells;
    const TransEntry* entries;
+   const uint32_t* rest;   // [nrest] cells that k_trans writes or copies (walked == 0)
+   uint32_t nrest;
    uint32_t ncells;
    uint32_t nblocks;
    int dim;
@@ -242,5 +263,18 @@ struct TransArgs {
    float scale;            // (float)(1 / thr)
 };
 void launch_trans(const TransArgs& a, hipStream_t s);
+struct WalkArgs {
+   TransArgs t;            // lists, pools and velocity constants (cells, entries, rest unused)
+   const WalkSeg* seg;
+   const WalkPos* pos;
+   const TransFace* face;  // left face of each window position (indexed like pos)
+   const uint32_t* bounds; // [nseg][wps + 1] GID bounds of each wave's share (k_walk_bounds)
+   uint32_t nseg;
+   uint32_t wps;           // waves per segment
+   uint32_t nmesh;         // velocity mesh blocks
+   uint32_t bin_shift;     // GID >> bin_shift < 4096: bins of the share histogram
+};
+// the share bounds are computed into `bounds` first ([nseg][wps + 1] words)
+void launch_trans_walk(const WalkArgs& a, uint32_t* bounds, hipStream_t s);
 
 }  // namespace vacc
