@@ -123,7 +123,7 @@ def profiled_traffic(workload: str, storage: str, key: str = "traffic-ratio"):
     """HBM bytes per launch relative to the algorithmic bytes, as measured with rocprofv3
     FETCH_SIZE / WRITE_SIZE (copy-calibrated) for this workload and storage form and recorded
     in a profiles/ summary ("<key> <workload> <storage> <ratio>"; key traffic-ratio for
-    k_remap, traffic-ratio-k_trans for the translation)."""
+    k_remap, traffic-ratio-k_trans_walk for the translation)."""
     import glob
     for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
         for line in open(path):
@@ -214,10 +214,11 @@ def trans_storage(w, storage: str, steps: int, pencils):
 
 def trans_form(w, steps: int, warmup: int, cpu_seconds: float):
     """Spatial translation of the workload's initial state on a periodic grid of its cells:
-    one step is the x, y and z translation of every cell (vacc_trans_map_1d x 3).  Roofline of
-    k_trans: 256 B read + 256 B written per block (the stencil neighbours' re-reads are not
-    algorithmic).  CPU baseline: the oracle's restatement (one thread) of one dimension on
-    the first 8 x 8 x 4 cells of the grid with periodic pencils."""
+    one step is the x, y and z translation of every cell (vacc_trans_map_1d x 3).  Every cell
+    of this grid is written by the pencil walk, so a launch is k_walk_bounds + k_trans_walk
+    (HIP events around both).  Roofline: 256 B read + 256 B written per block (the stencil
+    neighbours' re-reads are not algorithmic).  CPU baseline: the oracle's restatement (one
+    thread) of one dimension on the first 8 x 8 x 4 cells of the grid with periodic pencils."""
     dims = grid_dims(len(w.cells))
     kind = np.zeros(len(w.cells), np.uint8)
     ctx = make_context(w)
@@ -248,17 +249,17 @@ def trans_form(w, steps: int, warmup: int, cpu_seconds: float):
     out = {"grid": list(dims), "cells": len(w.cells), "blocks": nb, "steps": steps, "dt_s": TRANS_DT,
            "dx_m": TRANS_DX, "ms_per_step": 1e3 * el / steps, "value": 64 * nb * steps / el,
            "unit": "cell-updates/s (x, y, z translation per step)",
-           "roofline": {"kernel": "k_trans", "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
+           "roofline": {"kernel": "k_walk_bounds + k_trans_walk", "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS,
                         "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBPS, 4), "launches": prof["trans_launches"],
                         "avg_launch_ms": round(per_launch_ms, 4), "algorithmic_bytes_per_launch": alg}}
     assert nb_end == nb
-    ratio, src = profiled_traffic(w.name, "f32", key="traffic-ratio-k_trans")
+    ratio, src = profiled_traffic(w.name, "f32", key="traffic-ratio-k_trans_walk")
     if ratio is not None:
         out["roofline"]["traffic"] = int(ratio * alg)
         out["roofline"]["traffic_source"] = f"{src}: rocprofv3 FETCH_SIZE + WRITE_SIZE per launch, x{ratio:.3f} algorithmic"
     out["storage_forms"] = {st: trans_storage(w, st, min(steps, 3), pencils) for st in ("bf16",)}
     log(f"translation: grid {dims}, {nb} blocks, {out['ms_per_step']:.2f} ms per x,y,z step, "
-        f"k_trans {per_launch_ms:.3f} ms/launch = {achieved:.0f} GB/s algorithmic")
+        f"translation launch {per_launch_ms:.3f} ms = {achieved:.0f} GB/s algorithmic")
     if sample is not None:
         import oracle as O
         sd = tuple(min(a, b) for a, b in zip((8, 8, 4), dims))

@@ -232,7 +232,7 @@ typedef struct vacc_profile {
    uint64_t transform_launches;
    double transform_ms;       /* moments + transform/intersections */
    uint64_t trans_launches;
-   double trans_ms;           /* spatial translation (k_trans, include/vacc_trans.h) */
+   double trans_ms;           /* spatial translation (include/vacc_trans.h), all its kernels */
 } vacc_profile;
 int vacc_profile_begin(vacc_ctx* ctx);
 int vacc_profile_read(vacc_ctx* ctx, vacc_profile* out);

@@ -1,6 +1,7 @@
-"""GPU parity of the spatial translation: libvacc's k_trans (through vacc_trans_map_1d) against
-the oracle's restatement of trans_map_1d_amr (cpu_trans_map_amr.cpp:214) on the same cells
-and pencils.  f is compared bit for bit; block lists do not change."""
+"""GPU parity of the spatial translation: libvacc's kernels (k_trans_walk for the cells written
+from one pencil, k_trans for the others, through vacc_trans_map_1d) against the oracle's
+restatement of trans_map_1d_amr (cpu_trans_map_amr.cpp:214) on the same cells and pencils.
+f is compared bit for bit; block lists do not change."""
 import numpy as np
 import pytest
 
@@ -121,7 +122,7 @@ def test_cells_written_by_two_pencils(grid):
     """Refinement-style pencils (the bins of cpu_trans_pencils.hpp:48-50): two pencils share
     their first two cells A, B (width 2 dx, target ratio 1/2 in each) and go on through their
     own cells C, D (width dx, ratio 1).  A and B receive the contributions of both pencils in
-    pencil order; the general path of k_trans against the oracle."""
+    pencil order (k_trans), C1 through the pencil walk, against the oracle."""
     w, cells, m, kind = grid
     six = cells[:6]                       # A, B, C1, D1, C2, D2
     A, B, C1, D1, C2, D2 = range(6)
