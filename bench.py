@@ -33,6 +33,7 @@ from vlasiator_amd import vacc as V  # noqa: E402
 from vlasiator_amd import trans as TR  # noqa: E402
 from vlasiator_amd import vlsv as VL  # noqa: E402
 from vlasiator_amd import workloads as W  # noqa: E402
+from vlasiator_amd import hbm_probe  # noqa: E402
 
 METRIC = "velocity-cell updates/s per 3-D accel step; % HBM peak; FP32 vs BF16-store"
 HBM_PEAK_GBPS = 8000.0
@@ -424,6 +425,10 @@ def main():
                       "remap_wg_per_cu": r["remap_wg_per_cu"]},
            "roofline": r["roofline"], "breakdown_ms_per_step": r["breakdown"],
            "step_ms_percentiles": r["step_ms_percentiles"]}
+    # the achievable copy bandwidth of this GPU, the second roofline denominator
+    copy_gbps = hbm_probe.measure(2.0, 5)["copy_GBps"]
+    out["roofline"]["copy_GBps"] = round(copy_gbps, 1)
+    out["roofline"]["frac_of_copy"] = round(out["roofline"]["achieved"] / copy_gbps, 4)
     if args.forms:
         # the other storage forms on the same workload (same initial state, fewer steps)
         forms = {}
@@ -448,7 +453,8 @@ def main():
     if io is not None:
         out["vlsv_io"] = io
     if args.forms and len(pops) == 1:
-        out["translation_form"] = trans_form(w, min(args.steps, 10), 2, 0.0 if args.no_cpu else 5.0)
+        out["translation_form"] = tf = trans_form(w, min(args.steps, 10), 2, 0.0 if args.no_cpu else 5.0)
+        tf["roofline"]["frac_of_copy"] = round(tf["roofline"]["achieved"] / copy_gbps, 4)
     if args.plain_cells and sample_cells is not None:
         npl = min(args.plain_cells, len(sample_cells))
         log(f"plain form: {npl} single-cell contexts")
