@@ -221,3 +221,26 @@ def test_pencil_with_empty_positions(grid):
         want = O.trans_map_1d(m, ten, pen, dim, DT, w.min_value)
         cases.assert_cells_equal(ctx.download_cells(), want, f"empty positions dim {dim}")
         ctx.close()
+
+
+def test_shortest_pencils(grid):
+    """Pencils of 2*VACC_TRANS_STENCIL+1 positions (one propagated cell, five distinct cells):
+    the pencil walk's smallest window, one source; with ratios 1/2 on a second pencil that
+    shares the stencil cells but writes another cell."""
+    w, cells, m, kind = grid
+    seven = cells[:7]
+    ids = [0, 1, 2, 3, 4, 1, 2, 5, 3, 6]
+    ratio = [0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 0.0, 0.5, 0.0, 0.0]
+    dz = [1.0e7, 1.1e7, 0.9e7, 1.0e7, 1.2e7, 1.1e7, 0.9e7, 1.3e7, 1.0e7, 0.8e7]
+    pen = (np.array([5, 5], np.uint32), np.array([0, 5], np.uint64), np.array(ids, np.uint32),
+           np.array(dz, np.float32), np.array(ratio, np.float32))
+    for dim in range(3):
+        _say(f"shortest pencils dim={dim}")
+        ctx = _ctx(w)
+        ctx.upload_cells(seven)
+        T.set_pencils(ctx, dim, pen)
+        T.trans_map_1d(ctx, dim, DT)
+        want = O.trans_map_1d(m, seven, pen, dim, DT, w.min_value)
+        cases.assert_cells_equal(ctx.download_cells(), want, f"shortest dim {dim}")
+        assert not np.array_equal(want[2][1], seven[2][1])
+        ctx.close()
