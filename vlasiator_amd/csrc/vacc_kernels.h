@@ -214,7 +214,7 @@ struct TransEntry {
    uint8_t st[5];      // stencil cells as slots of the target cell's table (0xFF: none)
    uint8_t kind;
    uint8_t pad[2];
-   uint32_t pad2[3];   // 160 bytes: the kernel reads entries as 40 dwords
+   uint32_t pad2[3];   // 160 bytes, a multiple of 16
 };
 struct TransCell {
    uint32_t e0;        // first entry
