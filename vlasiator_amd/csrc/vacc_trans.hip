@@ -376,10 +376,10 @@ __global__ __launch_bounds__(TWG) void k_walk_bounds(WalkArgs w, uint32_t* bound
 // (at least three positions before it, so that no source reaches both), and ends in the
 // block or in a carry register.  Each cell thus receives its contributions in the order of
 // the reference's source loop: p1 from j-1, self from j, m1 from j+1, source after source.
-// FP32: seven waves per SIMD, 72 VGPRs without a spill (3.5 % faster than the six that 80
-// VGPRs allow); the 16-bit forms would spill at 72 and keep the compiler's choice.
+// Waves per SIMD: seven for FP32 (72 VGPRs, 3.5 % faster than the six that 80 VGPRs allow)
+// and six for the 16-bit forms (80 VGPRs instead of 85); neither spills, one more would.
 template <int ST>
-__global__ __launch_bounds__(TWG) __attribute__((amdgpu_waves_per_eu(ST == ST_F32 ? 7 : 1))) void k_trans_walk(
+__global__ __launch_bounds__(TWG) __attribute__((amdgpu_waves_per_eu(ST == ST_F32 ? 7 : 6))) void k_trans_walk(
     WalkArgs w) {
    // per wave: the current segment's faces (window positions 0..n-1)
    __shared__ TransFace s_face[TWG / 64][WALK_MAXN];
