@@ -244,3 +244,68 @@ def test_shortest_pencils(grid):
         cases.assert_cells_equal(ctx.download_cells(), want, f"shortest dim {dim}")
         assert not np.array_equal(want[2][1], seven[2][1])
         ctx.close()
+
+
+def _random_pencils(rng, ncells, npencils):
+    """Pencils of 5..24 positions over ncells cells: cells repeat within a pencil (next to each
+    other or apart) and across pencils, some positions are empty, ratios are 0, 1/2 or 1 and
+    widths vary by +-20 %."""
+    lengths, starts, ids, dz, ratio = [], [], [], [], []
+    for _ in range(npencils):
+        L = int(rng.integers(5, 25))
+        starts.append(len(ids))
+        lengths.append(L)
+        for q in range(L):
+            r = rng.random()
+            ids.append(0xFFFFFFFF if r < 0.05 else int(rng.integers(0, ncells)))
+            dz.append(float(1.0e7 * rng.uniform(0.8, 1.2)))
+            edge = q < 1 or q >= L - 1
+            ratio.append(0.0 if edge else float(rng.choice([0.0, 0.5, 1.0, 1.0])))
+    return (np.array(lengths, np.uint32), np.array(starts, np.uint64), np.array(ids, np.uint32),
+            np.array(dz, np.float32), np.array(ratio, np.float32))
+
+
+def _own_cell_pencils(rng, ncells, npencils):
+    """Pencils of 8..16 positions, each over its own ncells // npencils cells: a cell appears
+    once, twice (next to itself or apart) or more often in one pencil only, so that the pencil
+    walk takes the cells it can (one position, or two at least three apart) and k_trans the
+    others."""
+    per = ncells // npencils
+    lengths, starts, ids, dz, ratio = [], [], [], [], []
+    for p in range(npencils):
+        L = int(rng.integers(8, 17))
+        starts.append(len(ids))
+        lengths.append(L)
+        for q in range(L):
+            ids.append(p * per + int(rng.integers(0, per)))
+            dz.append(float(1.0e7 * rng.uniform(0.8, 1.2)))
+            ratio.append(0.0 if q < 1 or q >= L - 1 else float(rng.choice([0.0, 1.0, 1.0, 1.0])))
+    return (np.array(lengths, np.uint32), np.array(starts, np.uint64), np.array(ids, np.uint32),
+            np.array(dz, np.float32), np.array(ratio, np.float32))
+
+
+@pytest.mark.parametrize("kind_, seed", [("shared", 21), ("shared", 22), ("shared", 23), ("own", 31), ("own", 32),
+                                         ("own", 33)])
+def test_random_pencils_bit_exact(grid, kind_, seed):
+    """Random pencils: "shared" ones write cells at several positions and from several pencils,
+    with empty positions and ratios 0, 1/2, 1; "own" ones give each pencil its own cells, so
+    that many cells are written at one or two positions of one pencil.  Widths vary by
+    +-20 %.  The split between the pencil walk and k_trans, the walk's carries and its
+    segments, against the oracle in every dimension."""
+    w, cells, m, kind = grid
+    rng = np.random.default_rng(seed)
+    if kind_ == "shared":
+        sub = cells[:12]
+        pen = _random_pencils(rng, len(sub), 6)
+    else:
+        sub = cells[:24]
+        pen = _own_cell_pencils(rng, len(sub), 3)
+    for dim in range(3):
+        _say(f"random pencils {kind_} seed={seed} dim={dim} positions={len(pen[2])}")
+        ctx = _ctx(w)
+        ctx.upload_cells(sub)
+        T.set_pencils(ctx, dim, pen)
+        T.trans_map_1d(ctx, dim, DT)
+        want = O.trans_map_1d(m, sub, pen, dim, DT, w.min_value)
+        cases.assert_cells_equal(ctx.download_cells(), want, f"random {kind_} seed {seed} dim {dim}")
+        ctx.close()
