@@ -118,6 +118,7 @@ struct vacc_ctx {
       size_t cells_cap = 0, entries_cap = 0, rest_cap = 0, seg_cap = 0, pos_cap = 0, face_cap = 0;
       uint32_t nrest = 0;
       uint32_t nseg = 0;
+      uint32_t wps = 1;            // walk waves per segment
       uint64_t epoch = ~0ull;
    } trans[3];
    // event timing
@@ -1610,6 +1611,11 @@ int vacc_trans_set_pencils(vacc_ctx* c, uint32_t dim, const vacc_pencils* P) {
    if (!rest.empty()) HIPCHK(hipMemcpy(T.rest, rest.data(), rest.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
    T.nrest = (uint32_t)rest.size();
    T.nseg = (uint32_t)segs.size();
+   // walk waves per segment: a power of two, about WALK_WAVES in all, at most 1024 per segment;
+   // the share bounds of every wave are computed on the device at each call
+   T.wps = 1;
+   while (T.wps < 1024 && (uint64_t)T.nseg * T.wps * 2 <= WALK_WAVES) T.wps *= 2;
+   if ((rc = dev_grow(T.bounds, T.bounds_cap, (size_t)T.nseg * (T.wps + 1)))) return rc;
    T.epoch = c->cell_epoch;
    return 0;
 }
@@ -1617,7 +1623,7 @@ int vacc_trans_set_pencils(vacc_ctx* c, uint32_t dim, const vacc_pencils* P) {
 int vacc_trans_map_1d(vacc_ctx* c, uint32_t dim, double dt) {
    if (!c) return set_err(VACC_E_ARG, "null context");
    if (dim > 2) return set_err(VACC_E_ARG, "dim must be 0, 1 or 2");
-   auto& T = c->trans[dim];
+   const auto& T = c->trans[dim];
    if (T.epoch != c->cell_epoch) return set_err(VACC_E_STATE, "no pencils installed for the current cells and dim");
    if (!c->ncells) return 0;
    // captured steps stay valid: no buffer moves, and the list/pool parity and validity flags
@@ -1651,11 +1657,9 @@ int vacc_trans_map_1d(vacc_ctx* c, uint32_t dim, double dt) {
    wa.face = T.face;
    wa.nseg = T.nseg;
    wa.nmesh = c->mi.len[0] * c->mi.len[1] * c->mi.len[2];
-   wa.wps = 1;   // a power of two, about WALK_WAVES in all, at most 1024 per segment
-   while (wa.wps < 1024 && (uint64_t)T.nseg * wa.wps * 2 <= WALK_WAVES) wa.wps *= 2;
+   wa.wps = T.wps;
    wa.bin_shift = 0;
    while (wa.nmesh && ((wa.nmesh - 1) >> wa.bin_shift) >= 4096u) ++wa.bin_shift;
-   if ((rc = dev_grow(T.bounds, T.bounds_cap, (size_t)T.nseg * (wa.wps + 1)))) return rc;
    wa.bounds = T.bounds;
    {
       ProfScope ps(c, 4);
