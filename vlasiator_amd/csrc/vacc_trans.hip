@@ -1,25 +1,29 @@
 // vacc_trans.hip -- spatial translation kernels (trans_map_1d_amr, TRANS_SEMILAG_PPM).
 //
-// Two kernels write disjoint blocks of the output pool.  k_trans_walk (below) walks pencil
-// segments, one wave per velocity block, for the cells that receive contributions from one
-// pencil position only (the run cells of a pencil).  k_trans takes every other cell: cells
-// written at several pencil positions or by several pencils (periodic wrap, pencil ends,
-// refinement bins), and cells that are not written (copied).
+// Three kernels per call; the last two write disjoint blocks of a compact output pool.
+//   k_walk_bounds  splits each pencil segment's GID range into shares of equal block counts.
+//   k_trans_walk   walks pencil segments, one wave per velocity block, for the cells that
+//                  receive all their contributions from one pencil at one position, or at two
+//                  positions at least three apart (the run cells of a pencil, including a
+//                  periodic pencil's wrap cells).
+//   k_trans        takes every other cell: cells written at adjacent positions or by several
+//                  pencils (walls where the stencil repeats a cell, refinement bins), the wrap
+//                  cells of pencils longer than a walk segment, and cells that are not written
+//                  (copied).
 //
-// k_trans: one wave per target block, one lane per velocity cell: lane l holds the block's cell of
-// transposed index l = i + 4j + 16k, k along the translation dimension
-// (cpu_trans_map_amr.cpp:224-285), so the reference's 8-lane groups of check_skip_remapping
-// (VECL = 8, planes of 16 cells) are aligned groups of 8 lanes.  For target cell t the
-// host lists every contribution t receives (source position of a pencil, kind, widths,
-// ratio) in the reference's accumulation order, and the distinct cells of their stencils.
-// The host lists the cells k_trans writes or copies; a wave takes TCHUNK consecutive blocks
-// of one such cell (GIDs ascending within each cell).  For
-// each stencil cell j, a window of its sorted block list (GIDs and pool indices) is staged in
-// LDS from the first entry not below the current GID; a block's index in cell j is then one
-// compare per lane and a ballot, and the window moves on only when the GIDs pass its end, so
-// the one global load that depends on the lookup is the value itself.  The new values are accumulated in FP32
-// in the reference's order and written to a compact output pool; cells that are not
-// written keep their values.
+// In both writing kernels lane l holds the block's velocity cell of transposed index
+// l = i + 4j + 16k, k along the translation dimension (cpu_trans_map_amr.cpp:224-285), so
+// the reference's 8-lane groups of check_skip_remapping (VECL = 8, planes of 16 cells) are
+// aligned groups of 8 lanes; values are accumulated in FP32 in the reference's order.
+//
+// k_trans: for each of its cells the host lists every contribution the cell receives (source
+// position of a pencil, kind, widths, ratio) in the reference's accumulation order, and the
+// distinct cells of their stencils.  A wave takes TCHUNK consecutive blocks of one cell (GIDs
+// ascending within each cell).  For each stencil cell j, a window of its sorted block list
+// (GIDs and pool indices) is staged in LDS from the first entry not below the current GID; a
+// block's index in cell j is then one compare per lane and a ballot, and the window moves on
+// only when the GIDs pass its end, so the one global load that depends on the lookup is the
+// value itself.
 #include <hip/hip_runtime.h>
 
 #include "vacc_codec.h"
