@@ -691,6 +691,201 @@ int import_blocks(vacc_ctx* c, uint32_t ncells, const uint64_t* offsets, const u
 
 }  // namespace vacc
 
+// Width-only scalars of one h4 face on non-uniform widths, h[0..3] = widths at k-2..k+1,
+// with the float and double sub-expressions of cpu_face_estimates.hpp:179-191.
+static TransFace trans_face(const float* h) {
+   TransFace F{};
+   const float h0 = h[0], h1 = h[1], h2 = h[2], h3 = h[3];
+   F.D1 = 1.0 / (double)(((h0 + h1) + h2) + h3);
+   F.SA = ((h0 + h1) * (h2 + h3)) / (h1 + h2);
+   F.DA = 1.0 / (double)((h0 + h1) + h2) + 1.0 / (double)((h1 + h2) + h3);
+   F.SB = (h2 * (h2 + h3)) / (((h0 + h1) + h2) * (h0 + h1));
+   F.DB = (double)h0 + 2.0 * (double)h1;
+   F.SC = (h1 * (h0 + h1)) / (((h1 + h2) + h3) * (h2 + h3));
+   F.DC = 2.0 * (double)h2 + (double)h3;
+   F.h1 = h1;
+   F.h2 = h2;
+   return F;
+}
+
+// Host plan of one dimension's translation (vacc_trans_set_pencils): the per-cell schedule of
+// k_trans and the segments of the pencil walk.
+struct TransPlan {
+   std::vector<std::vector<TransEntry>> ent;                  // per cell, in accumulation order
+   std::vector<std::vector<uint32_t>> tab;                    // per cell: distinct stencil cells
+   std::vector<std::vector<std::pair<uint32_t, int>>> where;  // per entry: (pencil, target position)
+   size_t total = 0;                                          // entries
+   std::vector<uint8_t> walk;                                 // per cell: written by the walk
+   std::vector<WalkSeg> segs;
+   std::vector<WalkPos> wpos;
+   std::vector<TransFace> wface;
+};
+
+// Every contribution each cell receives, in the reference's order: pencils as given (bins,
+// then pencils of a bin), then source positions W .. L-W-1 (cpu_trans_map_amr.cpp:59-148).
+static int plan_schedule(const vacc_pencils* P, uint32_t nc, TransPlan& T) {
+   const int W = VACC_TRANS_STENCIL;
+   auto& ent = T.ent;
+   auto& tab = T.tab;
+   auto& where = T.where;
+   ent.assign(nc, {});
+   tab.assign(nc, {});
+   where.assign(nc, {});
+   size_t total = 0;
+   for (uint32_t p = 0; p < P->npencils; ++p) {
+      const int L = (int)P->length[p];
+      if (L < 2 * W + 1) return set_err(VACC_E_ARG, "pencil shorter than 2*VACC_TRANS_STENCIL+1 positions");
+      const uint64_t s0 = P->start[p];
+      const uint32_t* pc = P->cell + s0;
+      const float* dz = P->dz + s0;
+      const float* ratio = P->ratio + s0;
+      for (int q = 0; q < L; ++q)
+         if (pc[q] != VACC_INVALID_ID && pc[q] >= nc) return set_err(VACC_E_ARG, "pencil cell index outside the context");
+      for (int i = W; i < L - W; ++i) {
+         const TransFace fl = trans_face(dz + i - 2), fr = trans_face(dz + i - 1);
+         // per source the reference adds self, then p1, then m1 (cpu_trans_map_amr.cpp:111-146);
+         // the order matters where one cell is the target at two of these positions
+         for (int o : {0, 1, -1}) {
+            const int q = i + o;
+            const uint32_t t = pc[q];
+            if (t == VACC_INVALID_ID || ratio[q] == 0.0f) continue;
+            TransEntry E{};
+            E.fl = fl;
+            E.fr = fr;
+            E.dz_i = dz[i];
+            E.dz_t = dz[q];
+            E.ratio = ratio[q];
+            E.kind = (o == 0) ? 1 : (o > 0 ? 0 : 2);
+            for (int st = 0; st < 5; ++st) {
+               const uint32_t sc = pc[i - 2 + st];
+               if (sc == VACC_INVALID_ID) {
+                  E.st[st] = 0xFF;
+                  continue;
+               }
+               auto& tb = tab[t];
+               size_t slot = std::find(tb.begin(), tb.end(), sc) - tb.begin();
+               if (slot == tb.size()) {
+                  if (tb.size() >= (size_t)TRANS_MAXN)
+                     return set_err(VACC_E_ARG, "a target cell's stencils span more than 12 cells");
+                  tb.push_back(sc);
+               }
+               E.st[st] = (uint8_t)slot;
+            }
+            ent[t].push_back(E);
+            where[t].emplace_back(p, q);
+            ++total;
+         }
+      }
+   }
+   if (total > 0xFFFFFFFFull) return set_err(VACC_E_ARG, "too many contributions");
+   T.total = total;
+   return 0;
+}
+
+// The cells the pencil walk writes, and its segments (windows, flags, faces).
+static void plan_walk(const vacc_pencils* P, uint32_t nc, TransPlan& T) {
+   const int W = VACC_TRANS_STENCIL;
+   const auto& where = T.where;
+   // walk candidates: every entry of the cell from one pencil, at one target position or at two
+   // that are at least three apart (no source reaches both), so that walking the sources in
+   // order adds the contributions in the reference's order
+   struct Cand {
+      uint32_t p = ~0u;
+      int q1 = -1, q2 = -1;
+   };
+   std::vector<Cand> cand(nc);
+   for (uint32_t t = 0; t < nc; ++t) {
+      const auto& wh = where[t];
+      if (wh.empty()) continue;
+      Cand C;
+      C.p = wh[0].first;
+      bool ok = true;
+      for (const auto& x : wh) {
+         if (x.first != C.p) ok = false;
+         else if (C.q1 < 0 || x.second == C.q1) C.q1 = x.second;
+         else if (C.q2 < 0 || x.second == C.q2) C.q2 = x.second;
+         else ok = false;
+      }
+      if (C.q2 >= 0) {
+         if (C.q2 < C.q1) std::swap(C.q1, C.q2);
+         ok = ok && C.q2 - C.q1 >= 3;
+      }
+      if (ok) cand[t] = C;
+   }
+   // segments: runs [qa, qb) of at most WALK_MAXN - 6 positions with the stencils of their
+   // sources smin..smax; a segment walks a candidate whose positions lie in smin-1 .. smax+1
+   // and whose contributions all come from sources smin..smax (a one-position cell: its
+   // position in [qa, qb)); two-position cells carry the first sum in one of two registers
+   auto& walk = T.walk;
+   auto& segs = T.segs;
+   auto& wpos = T.wpos;
+   auto& wface = T.wface;
+   walk.assign(nc, 0);
+   segs.clear();
+   wpos.clear();
+   wface.clear();
+   const int span = WALK_MAXN - 6;
+   std::vector<uint32_t> flags;
+   for (uint32_t p = 0; p < P->npencils; ++p) {
+      const int L = (int)P->length[p];
+      const uint64_t s0 = P->start[p];
+      const uint32_t* pc = P->cell + s0;
+      const float* dz = P->dz + s0;
+      const float* ratio = P->ratio + s0;
+      for (int qa = W; qa < L - W; qa += span) {
+         const int qb = std::min(qa + span, L - W);
+         const int smin = std::max(W, qa - 1), smax = std::min(L - W - 1, qb);
+         const int wa = smin - 2, n = smax - smin + 5;
+         // sources reaching position q lie in smin..smax
+         auto inside = [&](int q) {
+            return q >= smin - 1 && q <= smax + 1 && std::max(q - 1, W) >= smin && std::min(q + 1, L - W - 1) <= smax;
+         };
+         flags.assign((size_t)n, 0u);
+         int slot_end[2] = {-1, -1};
+         int nw = 0;
+         for (int q = smin - 1; q <= smax + 1; ++q) {
+            const uint32_t t = pc[q];
+            if (t == VACC_INVALID_ID || walk[t] || ratio[q] == 0.0f) continue;
+            const Cand& C = cand[t];
+            if (C.p != p || C.q1 != q) continue;
+            if (C.q2 < 0) {
+               if (q < qa || q >= qb) continue;
+               flags[q - wa] = WALK_W;
+               walk[t] = 1;
+               ++nw;
+            } else if (inside(C.q1) && inside(C.q2) && C.q2 >= smin + 2) {
+               // the second position's sum starts at source q2 - 1 >= smin + 1, after the
+               // first has ended (source q1 + 1); a slot is free once its last pair has ended
+               const int k = (slot_end[0] < q) ? 0 : (slot_end[1] < q) ? 1 : -1;
+               if (k < 0) continue;
+               flags[C.q1 - wa] = WALK_W | ((uint32_t)(k + 1) << WALK_SAVE_SHIFT);
+               flags[C.q2 - wa] = WALK_W | ((uint32_t)(k + 1) << WALK_INIT_SHIFT);
+               slot_end[k] = C.q2;
+               walk[t] = 1;
+               nw += 2;
+            }
+         }
+         if (!nw) continue;
+         WalkSeg S{};
+         S.pos0 = (uint32_t)wpos.size();
+         S.n = (uint32_t)n;
+         S.js = 2;
+         S.je = (uint32_t)(n - 3);
+         segs.push_back(S);
+         for (int k = 0; k < n; ++k) {
+            const int q = wa + k;
+            WalkPos X{};
+            X.cell = pc[q];
+            X.dz = dz[q];
+            X.ratio = ratio[q];
+            X.flags = flags[k];
+            wpos.push_back(X);
+            wface.push_back((k >= 2 && k <= n - 2) ? trans_face(dz + q - 2) : TransFace{});
+         }
+      }
+   }
+}
+
 // Device array p of at least n elements (contents not kept when it grows).
 template <class T>
 static int dev_grow(T*& p, size_t& cap, size_t n) {
@@ -1408,177 +1603,24 @@ int vacc_get_status(vacc_ctx* c, uint32_t* status) {
 // Spatial translation (include/vacc_trans.h)
 // ---------------------------------------------------------------------------
 
-// Width-only scalars of one h4 face on non-uniform widths, h[0..3] = widths at k-2..k+1,
-// with the float and double sub-expressions of cpu_face_estimates.hpp:179-191.
-static TransFace trans_face(const float* h) {
-   TransFace F{};
-   const float h0 = h[0], h1 = h[1], h2 = h[2], h3 = h[3];
-   F.D1 = 1.0 / (double)(((h0 + h1) + h2) + h3);
-   F.SA = ((h0 + h1) * (h2 + h3)) / (h1 + h2);
-   F.DA = 1.0 / (double)((h0 + h1) + h2) + 1.0 / (double)((h1 + h2) + h3);
-   F.SB = (h2 * (h2 + h3)) / (((h0 + h1) + h2) * (h0 + h1));
-   F.DB = (double)h0 + 2.0 * (double)h1;
-   F.SC = (h1 * (h0 + h1)) / (((h1 + h2) + h3) * (h2 + h3));
-   F.DC = 2.0 * (double)h2 + (double)h3;
-   F.h1 = h1;
-   F.h2 = h2;
-   return F;
-}
 
 int vacc_trans_set_pencils(vacc_ctx* c, uint32_t dim, const vacc_pencils* P) {
    if (!c || !P) return set_err(VACC_E_ARG, "null argument");
    if (dim > 2) return set_err(VACC_E_ARG, "dim must be 0, 1 or 2");
    if (P->npencils && (!P->length || !P->start || !P->cell || !P->dz || !P->ratio))
       return set_err(VACC_E_ARG, "null pencil array");
-   const int W = VACC_TRANS_STENCIL;
    const uint32_t nc = c->ncells;
-   std::vector<std::vector<TransEntry>> ent(nc);
-   std::vector<std::vector<uint32_t>> tab(nc);
-   // per target cell: (pencil, target position) of each entry, for the walk test
-   std::vector<std::vector<std::pair<uint32_t, int>>> where(nc);
-   size_t total = 0;
-   // contributions in the reference's order: pencils as given (bins, then pencils of a bin),
-   // then source positions W .. L-W-1 (cpu_trans_map_amr.cpp:59-148)
-   for (uint32_t p = 0; p < P->npencils; ++p) {
-      const int L = (int)P->length[p];
-      if (L < 2 * W + 1) return set_err(VACC_E_ARG, "pencil shorter than 2*VACC_TRANS_STENCIL+1 positions");
-      const uint64_t s0 = P->start[p];
-      const uint32_t* pc = P->cell + s0;
-      const float* dz = P->dz + s0;
-      const float* ratio = P->ratio + s0;
-      for (int q = 0; q < L; ++q)
-         if (pc[q] != VACC_INVALID_ID && pc[q] >= nc) return set_err(VACC_E_ARG, "pencil cell index outside the context");
-      for (int i = W; i < L - W; ++i) {
-         const TransFace fl = trans_face(dz + i - 2), fr = trans_face(dz + i - 1);
-         // per source the reference adds self, then p1, then m1 (cpu_trans_map_amr.cpp:111-146);
-         // the order matters where one cell is the target at two of these positions
-         for (int o : {0, 1, -1}) {
-            const int q = i + o;
-            const uint32_t t = pc[q];
-            if (t == VACC_INVALID_ID || ratio[q] == 0.0f) continue;
-            TransEntry E{};
-            E.fl = fl;
-            E.fr = fr;
-            E.dz_i = dz[i];
-            E.dz_t = dz[q];
-            E.ratio = ratio[q];
-            E.kind = (o == 0) ? 1 : (o > 0 ? 0 : 2);
-            for (int st = 0; st < 5; ++st) {
-               const uint32_t sc = pc[i - 2 + st];
-               if (sc == VACC_INVALID_ID) {
-                  E.st[st] = 0xFF;
-                  continue;
-               }
-               auto& tb = tab[t];
-               size_t slot = std::find(tb.begin(), tb.end(), sc) - tb.begin();
-               if (slot == tb.size()) {
-                  if (tb.size() >= (size_t)TRANS_MAXN)
-                     return set_err(VACC_E_ARG, "a target cell's stencils span more than 12 cells");
-                  tb.push_back(sc);
-               }
-               E.st[st] = (uint8_t)slot;
-            }
-            ent[t].push_back(E);
-            where[t].emplace_back(p, q);
-            ++total;
-         }
-      }
-   }
-   if (total > 0xFFFFFFFFull) return set_err(VACC_E_ARG, "too many contributions");
-   // walk candidates: every entry of the cell from one pencil, at one target position or at two
-   // that are at least three apart (no source reaches both), so that walking the sources in
-   // order adds the contributions in the reference's order
-   struct Cand {
-      uint32_t p = ~0u;
-      int q1 = -1, q2 = -1;
-   };
-   std::vector<Cand> cand(nc);
-   for (uint32_t t = 0; t < nc; ++t) {
-      const auto& wh = where[t];
-      if (wh.empty()) continue;
-      Cand C;
-      C.p = wh[0].first;
-      bool ok = true;
-      for (const auto& x : wh) {
-         if (x.first != C.p) ok = false;
-         else if (C.q1 < 0 || x.second == C.q1) C.q1 = x.second;
-         else if (C.q2 < 0 || x.second == C.q2) C.q2 = x.second;
-         else ok = false;
-      }
-      if (C.q2 >= 0) {
-         if (C.q2 < C.q1) std::swap(C.q1, C.q2);
-         ok = ok && C.q2 - C.q1 >= 3;
-      }
-      if (ok) cand[t] = C;
-   }
-   // segments: runs [qa, qb) of at most WALK_MAXN - 6 positions with the stencils of their
-   // sources smin..smax; a segment walks a candidate whose positions lie in smin-1 .. smax+1
-   // and whose contributions all come from sources smin..smax (a one-position cell: its
-   // position in [qa, qb)); two-position cells carry the first sum in one of two registers
-   std::vector<uint8_t> walk(nc, 0);
-   std::vector<WalkSeg> segs;
-   std::vector<WalkPos> wpos;
-   std::vector<TransFace> wface;
-   const int span = WALK_MAXN - 6;
-   std::vector<uint32_t> flags;
-   for (uint32_t p = 0; p < P->npencils; ++p) {
-      const int L = (int)P->length[p];
-      const uint64_t s0 = P->start[p];
-      const uint32_t* pc = P->cell + s0;
-      const float* dz = P->dz + s0;
-      const float* ratio = P->ratio + s0;
-      for (int qa = W; qa < L - W; qa += span) {
-         const int qb = std::min(qa + span, L - W);
-         const int smin = std::max(W, qa - 1), smax = std::min(L - W - 1, qb);
-         const int wa = smin - 2, n = smax - smin + 5;
-         // sources reaching position q lie in smin..smax
-         auto inside = [&](int q) {
-            return q >= smin - 1 && q <= smax + 1 && std::max(q - 1, W) >= smin && std::min(q + 1, L - W - 1) <= smax;
-         };
-         flags.assign((size_t)n, 0u);
-         int slot_end[2] = {-1, -1};
-         int nw = 0;
-         for (int q = smin - 1; q <= smax + 1; ++q) {
-            const uint32_t t = pc[q];
-            if (t == VACC_INVALID_ID || walk[t] || ratio[q] == 0.0f) continue;
-            const Cand& C = cand[t];
-            if (C.p != p || C.q1 != q) continue;
-            if (C.q2 < 0) {
-               if (q < qa || q >= qb) continue;
-               flags[q - wa] = WALK_W;
-               walk[t] = 1;
-               ++nw;
-            } else if (inside(C.q1) && inside(C.q2) && C.q2 >= smin + 2) {
-               // the second position's sum starts at source q2 - 1 >= smin + 1, after the
-               // first has ended (source q1 + 1); a slot is free once its last pair has ended
-               const int k = (slot_end[0] < q) ? 0 : (slot_end[1] < q) ? 1 : -1;
-               if (k < 0) continue;
-               flags[C.q1 - wa] = WALK_W | ((uint32_t)(k + 1) << WALK_SAVE_SHIFT);
-               flags[C.q2 - wa] = WALK_W | ((uint32_t)(k + 1) << WALK_INIT_SHIFT);
-               slot_end[k] = C.q2;
-               walk[t] = 1;
-               nw += 2;
-            }
-         }
-         if (!nw) continue;
-         WalkSeg S{};
-         S.pos0 = (uint32_t)wpos.size();
-         S.n = (uint32_t)n;
-         S.js = 2;
-         S.je = (uint32_t)(n - 3);
-         segs.push_back(S);
-         for (int k = 0; k < n; ++k) {
-            const int q = wa + k;
-            WalkPos X{};
-            X.cell = pc[q];
-            X.dz = dz[q];
-            X.ratio = ratio[q];
-            X.flags = flags[k];
-            wpos.push_back(X);
-            wface.push_back((k >= 2 && k <= n - 2) ? trans_face(dz + q - 2) : TransFace{});
-         }
-      }
-   }
+   TransPlan plan;
+   int rc = 0;
+   if ((rc = plan_schedule(P, nc, plan))) return rc;
+   plan_walk(P, nc, plan);
+   const auto& ent = plan.ent;
+   const auto& tab = plan.tab;
+   const auto& walk = plan.walk;
+   const auto& segs = plan.segs;
+   const auto& wpos = plan.wpos;
+   const auto& wface = plan.wface;
+   const size_t total = plan.total;
    if (wpos.size() > 0xFFFFFFFFull) return set_err(VACC_E_ARG, "too many pencil positions");
    std::vector<TransCell> cells(nc);
    std::vector<TransEntry> flat;
@@ -1596,7 +1638,6 @@ int vacc_trans_set_pencils(vacc_ctx* c, uint32_t dim, const vacc_pencils* P) {
    }
    auto& T = c->trans[dim];
    HIPCHK(hipStreamSynchronize(c->stream));
-   int rc = 0;
    if ((rc = dev_grow(T.cells, T.cells_cap, cells.size())) || (rc = dev_grow(T.entries, T.entries_cap, flat.size())) ||
        (rc = dev_grow(T.seg, T.seg_cap, segs.size())) || (rc = dev_grow(T.pos, T.pos_cap, wpos.size())) ||
        (rc = dev_grow(T.face, T.face_cap, wface.size())) || (rc = dev_grow(T.rest, T.rest_cap, rest.size())))
