@@ -309,3 +309,44 @@ def test_random_pencils_bit_exact(grid, kind_, seed):
         want = O.trans_map_1d(m, sub, pen, dim, DT, w.min_value)
         cases.assert_cells_equal(ctx.download_cells(), want, f"random {kind_} seed {seed} dim {dim}")
         ctx.close()
+
+
+def test_coarse_cells_shared_by_finer_pencils(grid):
+    """Refinement: coarse cells (width 2 dx) that two pencils of finer cells pass, each once at
+    a run position with target ratio 1/2, and a third pencil that writes one of them and a
+    fine cell again.
+    The pencil walk takes them in launch levels (a later pencil's sum starts from the block
+    the earlier one stored); FP32 and BF16 (where k_trans takes them) against the oracle."""
+    w, cells, m, kind = grid
+    sub = cells[:14]
+    C1, C2 = 0, 1
+    dx = 1.0e7
+    # the third pencil writes C1 and cell 4 again (after the first two); every written cell's
+    # stencils span at most 12 cells, so that k_trans can take them in the 16-bit form
+    rows = [[2, 3, 4, C1, 5, C2, 6, 7], [8, 9, 10, C1, 11, C2, 12, 13], [3, 9, C1, 4, 10, 5]]
+    lengths, starts, ids, dz, ratio = [], [], [], [], []
+    for r in rows:
+        starts.append(len(ids))
+        lengths.append(len(r))
+        for q, c in enumerate(r):
+            ids.append(c)
+            dz.append(2 * dx if c in (C1, C2) else dx)
+            edge = q < 2 or q >= len(r) - 2
+            ratio.append(0.0 if edge else (0.5 if c in (C1, C2) else 1.0))
+    pen = (np.array(lengths, np.uint32), np.array(starts, np.uint64), np.array(ids, np.uint32),
+           np.array(dz, np.float32), np.array(ratio, np.float32))
+    for storage in (V.STORE_F32, V.STORE_BF16):
+        st = {V.STORE_F32: None, V.STORE_BF16: O.ST_BF16}[storage]
+        for dim in range(3):
+            _say(f"coarse cells storage={storage} dim={dim}")
+            ctx = _ctx(w, storage)
+            ctx.upload_cells(sub)
+            held = ctx.download_cells()
+            T.set_pencils(ctx, dim, pen)
+            T.trans_map_1d(ctx, dim, DT)
+            out = O.trans_map_1d(m, held, pen, dim, DT, w.min_value)
+            written = set(int(c) for c, r in zip(pen[2], pen[4]) if r > 0)
+            want = out if st is None else [(g, O.requantize(f, st) if i in written else f) for i, (g, f) in enumerate(out)]
+            cases.assert_cells_equal(ctx.download_cells(), want, f"coarse storage {storage} dim {dim}")
+            assert not np.array_equal(out[C1][1], held[C1][1])
+            ctx.close()

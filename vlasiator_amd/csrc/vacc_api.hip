@@ -119,6 +119,7 @@ struct vacc_ctx {
       uint32_t nrest = 0;
       uint32_t nseg = 0;
       uint32_t wps = 1;            // walk waves per segment
+      std::vector<uint32_t> level_first;   // walk launch levels: first segment of each, and the end
       uint64_t epoch = ~0ull;
    } trans[3];
    // event timing
@@ -716,7 +717,8 @@ struct TransPlan {
    std::vector<std::vector<std::pair<uint32_t, int>>> where;  // per entry: (pencil, target position)
    size_t total = 0;                                          // entries
    std::vector<uint8_t> walk;                                 // per cell: written by the walk
-   std::vector<WalkSeg> segs;
+   std::vector<WalkSeg> segs;                                 // ordered by level
+   std::vector<uint32_t> level_first;                         // [levels + 1] first segment of each level
    std::vector<WalkPos> wpos;
    std::vector<TransFace> wface;
 };
@@ -782,8 +784,10 @@ static int plan_schedule(const vacc_pencils* P, uint32_t nc, TransPlan& T) {
    return 0;
 }
 
-// The cells the pencil walk writes, and its segments (windows, flags, faces).
-static void plan_walk(const vacc_pencils* P, uint32_t nc, TransPlan& T) {
+// The cells the pencil walk writes, and its segments (windows, flags, faces) in launch levels.
+// levels: cells that several pencils write may be walked (FP32 storage; the 16-bit forms would
+// encode the partial sums between levels).
+static void plan_walk(const vacc_pencils* P, uint32_t nc, bool levels, TransPlan& T) {
    const int W = VACC_TRANS_STENCIL;
    const auto& where = T.where;
    // walk candidates: every entry of the cell from one pencil, at one target position or at two
@@ -812,6 +816,42 @@ static void plan_walk(const vacc_pencils* P, uint32_t nc, TransPlan& T) {
       }
       if (ok) cand[t] = C;
    }
+   // cells written by several pencils, at one run position of each (a coarse cell that the
+   // pencils of finer cells pass): each pencil's part is walked in a launch level above those
+   // of the earlier pencils that write the cell, starting from the sum they stored
+   std::vector<std::vector<std::pair<uint32_t, int>>> multi(nc);   // (pencil, position) per part
+   std::vector<std::vector<std::pair<uint32_t, int>>> pmulti(P->npencils);   // (cell, part) per pencil
+   if (levels) {
+      for (uint32_t t = 0; t < nc; ++t) {
+         const auto& wh = where[t];
+         if (wh.empty() || cand[t].p != ~0u) continue;
+         std::vector<std::pair<uint32_t, int>> parts;
+         bool ok = true;
+         for (const auto& x : wh) {
+            const int L = (int)P->length[x.first];
+            if (!parts.empty() && parts.back().first == x.first) ok = ok && parts.back().second == x.second;
+            else if (!parts.empty() && x.first < parts.back().first) ok = false;
+            else parts.push_back(x);
+            ok = ok && x.second >= W && x.second <= L - W - 1;
+         }
+         if (!ok) continue;
+         for (int k = 1; k < (int)parts.size(); ++k) ok = ok && parts[k].first != parts[k - 1].first;
+         if (!ok) continue;
+         multi[t] = parts;
+         for (int k = 0; k < (int)parts.size(); ++k) pmulti[parts[k].first].emplace_back(t, k);
+      }
+   }
+   std::vector<uint32_t> plevel(P->npencils, 0), last(nc, 0);
+   uint32_t nlevels = 1;
+   for (uint32_t p = 0; p < P->npencils; ++p) {
+      uint32_t lv = 0;
+      for (const auto& x : pmulti[p])
+         if (x.second > 0) lv = std::max(lv, last[x.first] + 1);
+      for (const auto& x : pmulti[p]) last[x.first] = lv;
+      plevel[p] = lv;
+      nlevels = std::max(nlevels, lv + 1);
+   }
+   std::vector<uint32_t> seglevel;
    // segments: runs [qa, qb) of at most WALK_MAXN - 6 positions with the stencils of their
    // sources smin..smax; a segment walks a candidate whose positions lie in smin-1 .. smax+1
    // and whose contributions all come from sources smin..smax (a one-position cell: its
@@ -821,6 +861,8 @@ static void plan_walk(const vacc_pencils* P, uint32_t nc, TransPlan& T) {
    auto& wpos = T.wpos;
    auto& wface = T.wface;
    walk.assign(nc, 0);
+   for (uint32_t t = 0; t < nc; ++t)
+      if (!multi[t].empty()) walk[t] = 1;
    segs.clear();
    wpos.clear();
    wface.clear();
@@ -845,7 +887,17 @@ static void plan_walk(const vacc_pencils* P, uint32_t nc, TransPlan& T) {
          int nw = 0;
          for (int q = smin - 1; q <= smax + 1; ++q) {
             const uint32_t t = pc[q];
-            if (t == VACC_INVALID_ID || walk[t] || ratio[q] == 0.0f) continue;
+            if (t == VACC_INVALID_ID || ratio[q] == 0.0f) continue;
+            if (!multi[t].empty()) {   // a part of a cell written by several pencils
+               if (q < qa || q >= qb) continue;
+               for (int k = 0; k < (int)multi[t].size(); ++k)
+                  if (multi[t][k].first == p && multi[t][k].second == q) {
+                     flags[q - wa] = WALK_W | (k > 0 ? WALK_INIT_STORED << WALK_INIT_SHIFT : 0u);
+                     ++nw;
+                  }
+               continue;
+            }
+            if (walk[t]) continue;
             const Cand& C = cand[t];
             if (C.p != p || C.q1 != q) continue;
             if (C.q2 < 0) {
@@ -872,6 +924,7 @@ static void plan_walk(const vacc_pencils* P, uint32_t nc, TransPlan& T) {
          S.js = 2;
          S.je = (uint32_t)(n - 3);
          segs.push_back(S);
+         seglevel.push_back(plevel[p]);
          for (int k = 0; k < n; ++k) {
             const int q = wa + k;
             WalkPos X{};
@@ -884,6 +937,19 @@ static void plan_walk(const vacc_pencils* P, uint32_t nc, TransPlan& T) {
          }
       }
    }
+   // segments in level order (stable), and the first segment of each level
+   std::vector<uint32_t> order(segs.size());
+   for (uint32_t k = 0; k < (uint32_t)order.size(); ++k) order[k] = k;
+   std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return seglevel[a] < seglevel[b]; });
+   std::vector<WalkSeg> sorted(segs.size());
+   for (size_t k = 0; k < order.size(); ++k) sorted[k] = segs[order[k]];
+   segs.swap(sorted);
+   T.level_first.assign(nlevels + 1, 0);
+   for (uint32_t k = 0, lv = 0; lv <= nlevels; ++lv) {
+      while (k < order.size() && seglevel[order[k]] < lv) ++k;
+      T.level_first[lv] = k;
+   }
+   T.level_first[nlevels] = (uint32_t)segs.size();
 }
 
 // Device array p of at least n elements (contents not kept when it grows).
@@ -1613,7 +1679,7 @@ int vacc_trans_set_pencils(vacc_ctx* c, uint32_t dim, const vacc_pencils* P) {
    TransPlan plan;
    int rc = 0;
    if ((rc = plan_schedule(P, nc, plan))) return rc;
-   plan_walk(P, nc, plan);
+   plan_walk(P, nc, c->opt.storage == VACC_STORE_F32, plan);
    const auto& ent = plan.ent;
    const auto& tab = plan.tab;
    const auto& walk = plan.walk;
@@ -1652,6 +1718,7 @@ int vacc_trans_set_pencils(vacc_ctx* c, uint32_t dim, const vacc_pencils* P) {
    if (!rest.empty()) HIPCHK(hipMemcpy(T.rest, rest.data(), rest.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
    T.nrest = (uint32_t)rest.size();
    T.nseg = (uint32_t)segs.size();
+   T.level_first = plan.level_first;
    // walk waves per segment: a power of two, about WALK_WAVES in all, at most 1024 per segment;
    // the share bounds of every wave are computed on the device at each call
    T.wps = 1;
@@ -1701,10 +1768,18 @@ int vacc_trans_map_1d(vacc_ctx* c, uint32_t dim, double dt) {
    wa.wps = T.wps;
    wa.bin_shift = 0;
    while (wa.nmesh && ((wa.nmesh - 1) >> wa.bin_shift) >= 4096u) ++wa.bin_shift;
-   wa.bounds = T.bounds;
    {
       ProfScope ps(c, 4);
-      launch_trans_walk(wa, T.bounds, c->stream);
+      for (size_t lv = 0; lv + 1 < T.level_first.size(); ++lv) {   // launch levels in order
+         const uint32_t first = T.level_first[lv], count = T.level_first[lv + 1] - first;
+         if (!count) continue;
+         WalkArgs wl = wa;
+         wl.seg = T.seg + first;
+         wl.nseg = count;
+         uint32_t* bounds = T.bounds + (size_t)first * (T.wps + 1);
+         wl.bounds = bounds;
+         launch_trans_walk(wl, bounds, c->stream);
+      }
       launch_trans(ta, c->stream);
    }
    HIPCHK(hipGetLastError());

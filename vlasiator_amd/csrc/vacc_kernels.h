@@ -224,14 +224,17 @@ struct TransCell {
    uint32_t walked;    // 1: written by the pencil walk (k_trans_walk), skipped by k_trans
 };
 // Pencil walk: a segment of one pencil whose walked cells are all written by one wave per
-// velocity block.  A walked cell receives every contribution from this segment's sources,
-// at one window position or at two that are at least three apart (a periodic pencil's wrap).
+// velocity block.  A walked cell receives every contribution of this pencil from this
+// segment's sources, at one window position or at two that are at least three apart (a
+// periodic pencil's wrap); a cell that several pencils write at one run position each is
+// walked in launch levels, pencil after pencil.
 // Window positions 0..n-1 are consecutive pencil positions; sources are window positions
 // js..je, with js = 2 and je = n - 3; walked positions lie in js-1 .. je+1.
 constexpr int WALK_MAXN = 64;    // window positions per segment (one lane each)
 constexpr uint32_t WALK_W = 1u;           // flags: the position's cell is written by the segment
 constexpr uint32_t WALK_SAVE_SHIFT = 1;   // flags bits 1-2: finished sum saved in carry 1 or 2
-constexpr uint32_t WALK_INIT_SHIFT = 3;   // flags bits 3-4: sum starts from carry 1 or 2
+constexpr uint32_t WALK_INIT_SHIFT = 3;   // flags bits 3-4: sum starts from carry 1 or 2, or
+constexpr uint32_t WALK_INIT_STORED = 3;  // from the block an earlier launch level stored (FP32)
 struct WalkSeg {
    uint32_t pos0;      // first window position in the walk position and face arrays
    uint32_t n;         // window positions

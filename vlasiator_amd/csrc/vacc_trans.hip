@@ -380,8 +380,11 @@ __global__ __launch_bounds__(TWG) void k_walk_bounds(WalkArgs w, uint32_t* bound
 // (at least three positions before it, so that no source reaches both), and ends in the
 // block or in a carry register.  Each cell thus receives its contributions in the order of
 // the reference's source loop: p1 from j-1, self from j, m1 from j+1, source after source.
-// Waves per SIMD: seven for FP32 (72 VGPRs, 3.5 % faster than the six that 80 VGPRs allow)
-// and six for the 16-bit forms (80 VGPRs instead of 85); neither spills, one more would.
+// A cell that several pencils write (one run position each) is walked in launch levels in
+// pencil order; a later level's sum starts from the block the earlier one stored (FP32).
+// Waves per SIMD: seven for FP32 (72 VGPRs with 12 bytes per lane spilled, 5 % faster than
+// six waves at 80 VGPRs without a spill) and six for the 16-bit forms (80 VGPRs instead of 85,
+// no spill).
 template <int ST>
 __global__ __launch_bounds__(TWG) __attribute__((amdgpu_waves_per_eu(ST == ST_F32 ? 7 : 6))) void k_trans_walk(
     WalkArgs w) {
@@ -430,6 +433,7 @@ __global__ __launch_bounds__(TWG) __attribute__((amdgpu_waves_per_eu(ST == ST_F3
       const unsigned long long save2 = __ballot(((pf >> WALK_SAVE_SHIFT) & 3u) == 2u);
       const unsigned long long init1 = __ballot(((pf >> WALK_INIT_SHIFT) & 3u) == 1u);
       const unsigned long long init2 = __ballot(((pf >> WALK_INIT_SHIFT) & 3u) == 2u);
+      const unsigned long long inits = __ballot(((pf >> WALK_INIT_SHIFT) & 3u) == WALK_INIT_STORED);
       const int js = (int)S.js, je = (int)S.je, n = (int)S.n;
       const unsigned long long smask = ((je >= 63) ? ~0ull : ((1ull << (je + 1)) - 1ull)) & ~((1ull << js) - 1ull);
       for (;;) {
@@ -474,6 +478,11 @@ __global__ __launch_bounds__(TWG) __attribute__((amdgpu_waves_per_eu(ST == ST_F3
          float c1 = 0.0f, c2 = 0.0f;   // carry registers
          // start of window position q's sum: zero or a carry register
          auto start = [&](int q) -> float {
+            if constexpr (ST == ST_F32) {
+               // the sum an earlier launch level stored into this block
+               if (((inits & tmask) >> q) & 1ull)
+                  return static_cast<const float*>(a.out.f)[(size_t)__builtin_amdgcn_readlane((int)ob, q) * 64 + ab];
+            }
             return ((init1 >> q) & 1ull) ? c1 : ((init2 >> q) & 1ull) ? c2 : 0.0f;
          };
          // end of window position q's sum x: into a carry register or the block
