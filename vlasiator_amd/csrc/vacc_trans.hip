@@ -5,11 +5,12 @@
 //   k_trans_walk   walks pencil segments, one wave per velocity block, for the cells that
 //                  receive all their contributions from one pencil at one position, or at two
 //                  positions at least three apart (the run cells of a pencil, including a
-//                  periodic pencil's wrap cells).
-//   k_trans        takes every other cell: cells written at adjacent positions or by several
-//                  pencils (walls where the stencil repeats a cell, refinement bins), the wrap
-//                  cells of pencils longer than a walk segment, and cells that are not written
-//                  (copied).
+//                  periodic pencil's wrap cells); with FP32 storage also for cells that
+//                  several pencils write at one run position each, in launch levels.
+//   k_trans        takes every other cell: cells written at adjacent positions (walls where
+//                  the stencil repeats a cell), the wrap cells of pencils longer than a walk
+//                  segment, cells of several pencils in the 16-bit forms, and cells that are
+//                  not written (copied).
 //
 // In both writing kernels lane l holds the block's velocity cell of transposed index
 // l = i + 4j + 16k, k along the translation dimension (cpu_trans_map_amr.cpp:224-285), so
