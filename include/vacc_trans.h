@@ -66,8 +66,10 @@ int vacc_build_pencils_uniform(const uint32_t dims[3], const int32_t periodic[3]
                                uint32_t* length, uint64_t* start, uint32_t* cell, float* dz, float* ratio);
 
 /* Installs the pencils of dimension dim for the context's cells (kept until replaced or the
- * cell set changes).  Errors if a pencil is shorter than 2*VACC_TRANS_STENCIL+1 or a cell
- * index is outside the context. */
+ * cell set changes).  Errors if a pencil is shorter than 2*VACC_TRANS_STENCIL+1, a cell
+ * index is outside the context, or a written cell whose contributions come from several
+ * positions of a pencil, or from several pencils in a 16-bit storage form, has stencils that
+ * span more than 12 distinct cells. */
 int vacc_trans_set_pencils(vacc_ctx* ctx, uint32_t dim, const vacc_pencils* pencils);
 
 /* Translates every cell of the context along dim by dt (converted to float, the reference's

@@ -350,3 +350,37 @@ def test_coarse_cells_shared_by_finer_pencils(grid):
             cases.assert_cells_equal(ctx.download_cells(), want, f"coarse storage {storage} dim {dim}")
             assert not np.array_equal(out[C1][1], held[C1][1])
             ctx.close()
+
+
+def test_wide_stencils_of_coarse_cells():
+    """A coarse cell passed by three pencils of its own finer cells has stencils over 14 cells:
+    the pencil walk takes it (FP32), the per-cell schedule of the 16-bit forms cannot."""
+    w = W.config(0, ncells=18)
+    sub = cases.build_cells(w)
+    m = cases.oracle_mesh(w)
+    C = 0
+    rows = [[1, 2, 3, C, 4, 5, 6], [7, 8, 9, C, 10, 11, 12], [13, 14, C, 15, 16, 17]]
+    lengths, starts, ids, dz, ratio = [], [], [], [], []
+    for r in rows:
+        starts.append(len(ids))
+        lengths.append(len(r))
+        for q, c in enumerate(r):
+            ids.append(c)
+            dz.append(2.0e7 if c == C else 1.0e7)
+            ratio.append(0.0 if (q < 2 or q >= len(r) - 2) else (1.0 / 3.0 if c == C else 1.0))
+    pen = (np.array(lengths, np.uint32), np.array(starts, np.uint64), np.array(ids, np.uint32),
+           np.array(dz, np.float32), np.array(ratio, np.float32))
+    for dim in range(3):
+        _say(f"wide stencils dim={dim}")
+        ctx = _ctx(w)
+        ctx.upload_cells(sub)
+        T.set_pencils(ctx, dim, pen)
+        T.trans_map_1d(ctx, dim, DT)
+        want = O.trans_map_1d(m, sub, pen, dim, DT, w.min_value)
+        cases.assert_cells_equal(ctx.download_cells(), want, f"wide dim {dim}")
+        ctx.close()
+    ctx = _ctx(w, V.STORE_BF16)
+    ctx.upload_cells(sub)
+    with pytest.raises(V.VaccError, match="more than 12 cells"):
+        T.set_pencils(ctx, 0, pen)
+    ctx.close()

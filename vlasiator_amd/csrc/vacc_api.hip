@@ -716,6 +716,7 @@ struct TransPlan {
    std::vector<std::vector<uint32_t>> tab;                    // per cell: distinct stencil cells
    std::vector<std::vector<std::pair<uint32_t, int>>> where;  // per entry: (pencil, target position)
    size_t total = 0;                                          // entries
+   std::vector<uint8_t> wide;                                 // per cell: stencils span > TRANS_MAXN cells
    std::vector<uint8_t> walk;                                 // per cell: written by the walk
    std::vector<WalkSeg> segs;                                 // ordered by level
    std::vector<uint32_t> level_first;                         // [levels + 1] first segment of each level
@@ -733,6 +734,7 @@ static int plan_schedule(const vacc_pencils* P, uint32_t nc, TransPlan& T) {
    ent.assign(nc, {});
    tab.assign(nc, {});
    where.assign(nc, {});
+   T.wide.assign(nc, 0);
    size_t total = 0;
    for (uint32_t p = 0; p < P->npencils; ++p) {
       const int L = (int)P->length[p];
@@ -767,8 +769,11 @@ static int plan_schedule(const vacc_pencils* P, uint32_t nc, TransPlan& T) {
                auto& tb = tab[t];
                size_t slot = std::find(tb.begin(), tb.end(), sc) - tb.begin();
                if (slot == tb.size()) {
-                  if (tb.size() >= (size_t)TRANS_MAXN)
-                     return set_err(VACC_E_ARG, "a target cell's stencils span more than 12 cells");
+                  if (tb.size() >= (size_t)TRANS_MAXN) {   // k_trans cannot take this cell
+                     T.wide[t] = 1;
+                     E.st[st] = 0xFF;
+                     continue;
+                  }
                   tb.push_back(sc);
                }
                E.st[st] = (uint8_t)slot;
@@ -1680,6 +1685,8 @@ int vacc_trans_set_pencils(vacc_ctx* c, uint32_t dim, const vacc_pencils* P) {
    int rc = 0;
    if ((rc = plan_schedule(P, nc, plan))) return rc;
    plan_walk(P, nc, c->opt.storage == VACC_STORE_F32, plan);
+   for (uint32_t t = 0; t < nc; ++t)   // k_trans reads at most TRANS_MAXN stencil cells per cell
+      if (plan.wide[t] && !plan.walk[t]) return set_err(VACC_E_ARG, "a target cell's stencils span more than 12 cells");
    const auto& ent = plan.ent;
    const auto& tab = plan.tab;
    const auto& walk = plan.walk;
