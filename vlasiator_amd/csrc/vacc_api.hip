@@ -825,7 +825,6 @@ static void plan_walk(const vacc_pencils* P, uint32_t nc, bool levels, TransPlan
    // pencils of finer cells pass): each pencil's part is walked in a launch level above those
    // of the earlier pencils that write the cell, starting from the sum they stored
    std::vector<std::vector<std::pair<uint32_t, int>>> multi(nc);   // (pencil, position) per part
-   std::vector<std::vector<std::pair<uint32_t, int>>> pmulti(P->npencils);   // (cell, part) per pencil
    if (levels) {
       for (uint32_t t = 0; t < nc; ++t) {
          const auto& wh = where[t];
@@ -843,20 +842,33 @@ static void plan_walk(const vacc_pencils* P, uint32_t nc, bool levels, TransPlan
          for (int k = 1; k < (int)parts.size(); ++k) ok = ok && parts[k].first != parts[k - 1].first;
          if (!ok) continue;
          multi[t] = parts;
-         for (int k = 0; k < (int)parts.size(); ++k) pmulti[parts[k].first].emplace_back(t, k);
       }
    }
-   std::vector<uint32_t> plevel(P->npencils, 0), last(nc, 0);
-   uint32_t nlevels = 1;
-   for (uint32_t p = 0; p < P->npencils; ++p) {
-      uint32_t lv = 0;
-      for (const auto& x : pmulti[p])
-         if (x.second > 0) lv = std::max(lv, last[x.first] + 1);
-      for (const auto& x : pmulti[p]) last[x.first] = lv;
-      plevel[p] = lv;
-      nlevels = std::max(nlevels, lv + 1);
+   const int span = WALK_MAXN - 6;
+   // a cell at two positions of one pencil that no segment holds together (the wrap cells of
+   // a periodic pencil longer than a segment): its first part is stored by the segment at
+   // the first position, its second starts from that block in a later level
+   if (levels) {
+      for (uint32_t t = 0; t < nc; ++t) {
+         const Cand& C = cand[t];
+         if (C.p == ~0u || C.q2 < 0) continue;
+         const int L = (int)P->length[C.p];
+         bool fits = false;
+         for (int qa = W; qa < L - W && !fits; qa += span) {
+            const int qb = std::min(qa + span, L - W);
+            const int smin = std::max(W, qa - 1), smax = std::min(L - W - 1, qb);
+            auto inside = [&](int q) {
+               return q >= smin - 1 && q <= smax + 1 && std::max(q - 1, W) >= smin && std::min(q + 1, L - W - 1) <= smax;
+            };
+            fits = inside(C.q1) && inside(C.q2) && C.q2 >= smin + 2;
+         }
+         if (fits) continue;
+         multi[t] = {{C.p, C.q1}, {C.p, C.q2}};
+         cand[t] = Cand{};
+      }
    }
-   std::vector<uint32_t> seglevel;
+   std::vector<uint32_t> last(nc, 0), seglevel;
+   uint32_t nlevels = 1;
    // segments: runs [qa, qb) of at most WALK_MAXN - 6 positions with the stencils of their
    // sources smin..smax; a segment walks a candidate whose positions lie in smin-1 .. smax+1
    // and whose contributions all come from sources smin..smax (a one-position cell: its
@@ -871,7 +883,6 @@ static void plan_walk(const vacc_pencils* P, uint32_t nc, bool levels, TransPlan
    segs.clear();
    wpos.clear();
    wface.clear();
-   const int span = WALK_MAXN - 6;
    std::vector<uint32_t> flags;
    for (uint32_t p = 0; p < P->npencils; ++p) {
       const int L = (int)P->length[p];
@@ -890,14 +901,17 @@ static void plan_walk(const vacc_pencils* P, uint32_t nc, bool levels, TransPlan
          flags.assign((size_t)n, 0u);
          int slot_end[2] = {-1, -1};
          int nw = 0;
+         std::vector<std::pair<uint32_t, int>> parts;   // (cell, part) stored parts of this segment
          for (int q = smin - 1; q <= smax + 1; ++q) {
             const uint32_t t = pc[q];
             if (t == VACC_INVALID_ID || ratio[q] == 0.0f) continue;
-            if (!multi[t].empty()) {   // a part of a cell written by several pencils
-               if (q < qa || q >= qb) continue;
+            if (!multi[t].empty()) {   // a stored part: its run position, or a pencil end's
+               const bool mine = (q >= qa && q < qb) || (q == W - 1 && qa == W) || (q == L - W && qb == L - W);
+               if (!mine) continue;
                for (int k = 0; k < (int)multi[t].size(); ++k)
                   if (multi[t][k].first == p && multi[t][k].second == q) {
                      flags[q - wa] = WALK_W | (k > 0 ? WALK_INIT_STORED << WALK_INIT_SHIFT : 0u);
+                     parts.emplace_back(t, k);
                      ++nw;
                   }
                continue;
@@ -929,7 +943,13 @@ static void plan_walk(const vacc_pencils* P, uint32_t nc, bool levels, TransPlan
          S.js = 2;
          S.je = (uint32_t)(n - 3);
          segs.push_back(S);
-         seglevel.push_back(plevel[p]);
+         // launch level: above the level of every earlier part of its stored parts' cells
+         uint32_t lv = 0;
+         for (const auto& x : parts)
+            if (x.second > 0) lv = std::max(lv, last[x.first] + 1);
+         for (const auto& x : parts) last[x.first] = lv;
+         seglevel.push_back(lv);
+         nlevels = std::max(nlevels, lv + 1);
          for (int k = 0; k < n; ++k) {
             const int q = wa + k;
             WalkPos X{};

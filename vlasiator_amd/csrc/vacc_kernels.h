@@ -226,8 +226,8 @@ struct TransCell {
 // Pencil walk: a segment of one pencil whose walked cells are all written by one wave per
 // velocity block.  A walked cell receives every contribution of this pencil from this
 // segment's sources, at one window position or at two that are at least three apart (a
-// periodic pencil's wrap); a cell that several pencils write at one run position each is
-// walked in launch levels, pencil after pencil.
+// periodic pencil's wrap); a cell that several pencils write at one run position each, or
+// whose two positions lie in different segments, is walked in launch levels, part after part.
 // Window positions 0..n-1 are consecutive pencil positions; sources are window positions
 // js..je, with js = 2 and je = n - 3; walked positions lie in js-1 .. je+1.
 constexpr int WALK_MAXN = 64;    // window positions per segment (one lane each)

@@ -5,12 +5,13 @@
 //   k_trans_walk   walks pencil segments, one wave per velocity block, for the cells that
 //                  receive all their contributions from one pencil at one position, or at two
 //                  positions at least three apart (the run cells of a pencil, including a
-//                  periodic pencil's wrap cells); with FP32 storage also for cells that
-//                  several pencils write at one run position each, in launch levels.
+//                  periodic pencil's wrap cells); with FP32 storage also, in launch levels,
+//                  for cells that several pencils write at one run position each and for the
+//                  wrap cells of pencils longer than a walk segment.
 //   k_trans        takes every other cell: cells written at adjacent positions (walls where
-//                  the stencil repeats a cell), the wrap cells of pencils longer than a walk
-//                  segment, cells of several pencils in the 16-bit forms, and cells that are
-//                  not written (copied).
+//                  the stencil repeats a cell), cells of several pencils and the wrap cells
+//                  of long pencils in the 16-bit forms, and cells that are not written
+//                  (copied).
 //
 // In both writing kernels lane l holds the block's velocity cell of transposed index
 // l = i + 4j + 16k, k along the translation dimension (cpu_trans_map_amr.cpp:224-285), so
@@ -381,8 +382,9 @@ __global__ __launch_bounds__(TWG) void k_walk_bounds(WalkArgs w, uint32_t* bound
 // (at least three positions before it, so that no source reaches both), and ends in the
 // block or in a carry register.  Each cell thus receives its contributions in the order of
 // the reference's source loop: p1 from j-1, self from j, m1 from j+1, source after source.
-// A cell that several pencils write (one run position each) is walked in launch levels in
-// pencil order; a later level's sum starts from the block the earlier one stored (FP32).
+// A cell that several pencils write (one run position each), or whose two positions lie in
+// different segments of one pencil, is walked in launch levels in the reference's order; a
+// later level's sum starts from the block the earlier one stored (FP32).
 // Waves per SIMD: seven for FP32 (72 VGPRs with 12 bytes per lane spilled, 5 % faster than
 // six waves at 80 VGPRs without a spill) and six for the 16-bit forms (80 VGPRs instead of 85,
 // no spill).
@@ -500,10 +502,11 @@ __global__ __launch_bounds__(TWG) __attribute__((amdgpu_waves_per_eu(ST == ST_F3
          const unsigned long long reach = (up == ~0ull) ? (tmask | (tmask >> 1))
                                           : (up == 0ull) ? (tmask | (tmask << 1))
                                                          : (tmask | (tmask << 1) | (tmask >> 1));
-         // position je+1 (the second position of a wrap cell) is only reached from je, which
-         // stays in so that the carried sum is stored even when it receives nothing; js-1 only
-         // saves into a carry, which stays zero when it receives nothing
-         unsigned long long rel = (reach & smask) | (((tmask >> (je + 1)) & 1ull) << je);
+         // positions js-1 and je+1 (pencil ends: the parts of wrap cells) are only reached from
+         // js and je, which stay in so that their sums are always stored or carried, also when
+         // they receive nothing
+         unsigned long long rel = (reach & smask) | (((tmask >> (je + 1)) & 1ull) << je) |
+                                  (((tmask >> (js - 1)) & 1ull) << js);
          float u0 = 0.0f, u1 = 0.0f, u2 = 0.0f, u3 = 0.0f, u4 = 0.0f, nx = 0.0f, fl = 0.0f;
          float am = 0.0f, a0 = 0.0f;   // sums of positions j-1, j
          float dzc = __builtin_nanf(""), zt = 0.0f;   // zt = vdt / dzc
