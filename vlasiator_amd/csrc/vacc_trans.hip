@@ -480,7 +480,9 @@ __global__ __launch_bounds__(TWG) __attribute__((amdgpu_waves_per_eu(ST == ST_F3
          };
          float c1 = 0.0f, c2 = 0.0f;   // carry registers
          // start of window position q's sum: zero or a carry register
+         const unsigned long long inited = init1 | init2 | inits;
          auto start = [&](int q) -> float {
+            if (!((inited >> q) & 1ull)) return 0.0f;   // the common case, a uniform branch
             if constexpr (ST == ST_F32) {
                // the sum an earlier launch level stored into this block
                if (((inits & tmask) >> q) & 1ull)
