@@ -106,32 +106,45 @@ def test_identity_transform_copies_f():
             assert np.all(fs[np.searchsorted(gs, extra)] == 0)   # added target blocks stay zero
 
 
-def test_integer_cell_shift_is_exact():
+@pytest.mark.parametrize("method", [O.PPM, O.PQM])
+def test_integer_cell_shift_is_exact(method):
+    """A Lagrangian grid shifted by a whole number of cells (dk = dv, I = vmin + n dv) moves
+    every value by n cells.  The output holds exactly the blocks that the shifted cells of the
+    source blocks reach (no shift is a multiple of 4, so no block is touched by only the
+    conservative end of a column's extent), every shifted value sits in its target cell, and
+    every other cell of the output is zero."""
     g, f = _maxwellian_cell()
     m = _mesh()
     dv = 4e6 / 200
+    nx = 50
     for dim in range(3):
-        for shift in (1, 3, -2):
+        stride = [1, 4, 16][dim]
+        bstride = [1, nx, nx * nx][dim]
+        for shift in (1, 3, -2, -5):
             isx = np.zeros(4)
-            # Lagrangian cell gk covers source cell gk+shift
+            # Lagrangian cell gk covers source cell gk+shift; the step along the mapped axis
+            # is stored at 1+dim (intersection_x_di, _y_dj, _z_dk; cpu_acc_map.cpp:151-176)
             isx[0] = -2e6 + shift * dv
-            isx[3] = dv
-            g2, f2, _ = O.map_1d(m, g, f, isx, dim, O.PPM, 1e-15)
-            src = {int(a): b for a, b in zip(g, f)}
-            dst = {int(a): b for a, b in zip(g2, f2)}
-            nx = 50
-            stride = [1, 4, 16][dim]
-            bstride = [1, nx, nx * nx][dim]
-            for gid, vals in src.items():
+            isx[1 + dim] = dv
+            g2, f2, _ = O.map_1d(m, g, f, isx, dim, method, 1e-15)
+            dst = {int(a): i for i, a in enumerate(g2)}
+            assert len(dst) == len(g2)
+            want = {}
+            for gid, vals in zip(g.tolist(), f):
                 bi = [gid % nx, (gid // nx) % nx, gid // (nx * nx)][dim]
                 for e in range(64):
                     k = [(e & 3), (e >> 2) & 3, e >> 4][dim]
                     gk = 4 * bi + k - shift
+                    if not 0 <= gk < 4 * nx:
+                        continue                      # shifted out of the mesh
                     tb, tk = divmod(gk, 4)
                     tg = gid + (tb - bi) * bstride
-                    te = e + (tk - k) * stride
-                    if tg in dst:
-                        assert abs(dst[tg][te] - vals[e]) <= 1e-5 * abs(vals[e])
+                    want.setdefault(tg, np.zeros(64, np.float64))[e + (tk - k) * stride] = vals[e]
+            assert sorted(want) == sorted(dst), (dim, shift)
+            tol = 1e-5 if method == O.PPM else 1e-4   # rounding of F(1) = sum of the coefficients
+            for tg, wv in want.items():
+                got = f2[dst[tg]].astype(np.float64)
+                assert np.all(np.abs(got - wv) <= tol * np.abs(wv)), (dim, shift, tg)
 
 
 def test_mass_conserved_by_rotation_step():
@@ -150,6 +163,85 @@ def test_mass_conserved_by_rotation_step():
             assert abs(m1[0] / m0[0] - 1) < 1e-5
             # the pivot (bulk velocity) is invariant under the rotation about it
             assert np.allclose(m1[1:] / m1[0], m0[1:] / m0[0], atol=2e2)
+
+
+def _rodrigues(axis, th):
+    a = np.asarray(axis, float) / np.linalg.norm(axis)
+    K = np.array([[0, -a[2], a[1]], [a[2], 0, -a[0]], [-a[1], a[0], 0]])
+    return np.eye(3) + math.sin(th) * K + (1 - math.cos(th)) * K @ K
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_rotation_moves_offpivot_centroid_to_closed_form(axis):
+    """A narrow Maxwellian (T = 1e5 K, thermal speed 1.4 dv) centred 4e5 m/s off the pivot P
+    perpendicular to B (and 1e5 m/s along it) is rotated by one 5-degree step.  dv/dt =
+    (q/m)(v - P) x B turns v - P about B by theta = -2 pi dt / T_g for a positive charge, so
+    the first moment must land at P + R(theta)(V0 - P).
+
+    Tolerance 0.02 dv (400 m/s): the measured miss is below 0.0025 dv for every axis, map order
+    and method.  A wrong rotation sense lands 2 |V0 - P| sin|theta| = 3.06 dv away, an
+    intersection of the wrong map order 0.13 dv away, and swapped di/dj lose the cell."""
+    m = _mesh()
+    dv = 4e6 / 200
+    P = np.array([1e5, -1.2e5, 6e4])
+    Bh = np.zeros(3)
+    Bh[axis] = 1.0
+    p1, p2 = [q for q in range(3) if q != axis]
+    d = np.zeros(3)
+    d[p1], d[p2], d[axis] = 4e5 * math.cos(0.5), 4e5 * math.sin(0.5), 1e5
+    g, f = O.init_cell(m, [O.make_peak(0, 1e6, tuple(P + d), 1e5)], W.MASS_PROTON, 1e-15, 1)
+    B = tuple(1e-8 * Bh)
+    Tg = W.gyro_period(B)
+    dt = Tg * 5 / 360
+    T = O.transform(O.make_fields(B, tuple(P)), W.MASS_PROTON, W.CHARGE, dt, 1.6e-19)
+    m0 = O.moments(m, g, f)
+    c0 = m0[1:] / m0[0]
+    want = P + _rodrigues(Bh, -2 * math.pi * dt / Tg) @ (c0 - P)
+    assert np.linalg.norm(want - c0) > 1.7 * dv     # the step moves the centroid by 1.75 dv
+    for mo in range(3):
+        isx = O.intersections(m, T, mo)
+        for method in (O.PPM, O.PQM):
+            g2, f2, _ = O.accelerate_cell(m, g, f, isx, mo, method, 1e-15)
+            m1 = O.moments(m, g2, f2)
+            assert abs(m1[0] / m0[0] - 1) < 1e-6
+            assert np.abs(m1[1:] / m1[0] - want).max() < 0.02 * dv, (mo, method)
+
+
+@pytest.mark.parametrize("deg", [5.0, 20.0])
+def test_intersections_b_parallel_z_closed_form(deg):
+    """B along +z, pivot 0: the backward map is the rotation R by phi = 2 pi dt / T_g about z
+    (c = cos phi, s = sin phi).  Intersecting its Lagrangian planes and lines with the
+    Euclidean grid by hand (mesh minimum m on every axis, h = dv/2) gives, as (I, step per i,
+    step per j, step per k) of the mapped coordinate:
+      1st pass along x: ((m - s(m+h))/c, dv/c, -s dv/c, 0)
+      1st pass along y: ((m + s(m+h))/c, s dv/c, dv/c, 0)
+      2nd pass along y, 3rd pass along y: (c m + s(m+h), s dv, c dv, 0)
+      2nd pass along x: as the 1st pass along x
+      3rd pass along x: (c m - s(m+h), c dv, -s dv, 0)
+      any pass along z: (m, 0, 0, dv)
+    (cpu_acc_intersections.cpp:132-231 first, :246-358 second, :376-450 third)."""
+    m = _mesh()
+    dv = 4e6 / 200
+    mn, h = -2e6, dv / 2
+    B = (0.0, 0.0, 1e-8)
+    Tg = W.gyro_period(B)
+    dt = Tg * deg / 360
+    T = O.transform(O.make_fields(B, (0, 0, 0)), W.MASS_PROTON, W.CHARGE, dt, 1.6e-19)
+    phi = 2 * math.pi * dt / Tg
+    c, s = math.cos(phi), math.sin(phi)
+    x1 = ((mn - s * (mn + h)) / c, dv / c, -s * dv / c, 0.0)
+    y23 = (c * mn + s * (mn + h), s * dv, c * dv, 0.0)
+    z = (mn, 0.0, 0.0, dv)
+    cf = {(0, 0): x1, (0, 1): ((mn + s * (mn + h)) / c, s * dv / c, dv / c, 0.0), (0, 2): z,
+          (1, 1): y23, (1, 2): z, (1, 0): x1,
+          (2, 2): z, (2, 0): (c * mn - s * (mn + h), c * dv, -s * dv, 0.0), (2, 1): y23}
+    orders = [(0, 1, 2), (1, 2, 0), (2, 0, 1)]
+    for mo in range(3):
+        isx = O.intersections(m, T, mo).reshape(3, 4)
+        for pos, dim in enumerate(orders[mo]):
+            w = np.array(cf[(pos, dim)])
+            assert np.abs(isx[dim, 0] - w[0]) < 1e-6, (mo, pos, dim)
+            assert np.all(np.abs(isx[dim, 1:] - w[1:]) < 1e-9 * dv), (mo, pos, dim)
 
 
 def test_transform_closed_form():
@@ -285,3 +377,79 @@ def test_batched_driver_matches_per_cell():
         g1, f1, _ = O.accelerate_cell(cases.oracle_mesh(w), gc, fc, isx[i], 1, O.PPM, 1e-15)
         assert np.array_equal(go[oo[i]:oo[i + 1]], g1)
         assert np.array_equal(fo[oo[i]:oo[i + 1]], f1)
+
+
+def _brute_map_1d(mesh_len, vmin, dv, g, f, isx4, dim, method, thr):
+    """Target values of one map_1d as FP64 overlap integrals.  Target cell gk of the line with
+    transverse global cell indices (t0, t1) departs from [L0 + gk dk, L0 + (gk+1) dk], with
+    L0 = I + t0 s_t0 + t1 s_t1 and dk = s_dim (s_q = isx4[1+q], the step of the mapped
+    coordinate per cell along axis q).  Each source cell contributes the integral of its
+    reconstruction (coefficients from coeff_batch, which the reference's own headers pin) over
+    the overlap.  Returns {(t0, t1): {gk: value}}."""
+    nx = mesh_len
+    nc = 4 * nx
+    bidx = np.stack([g % nx, (g // nx) % nx, g // (nx * nx)], axis=1)
+    lines = {}
+    for b in range(len(g)):
+        for e in range(64):
+            c = [4 * bidx[b, 0] + (e & 3), 4 * bidx[b, 1] + ((e >> 2) & 3), 4 * bidx[b, 2] + (e >> 4)]
+            key = tuple(int(c[q]) for q in range(3) if q != dim)
+            lines.setdefault(key, np.zeros(nc))[c[dim]] = f[b, e]
+    tq = [q for q in range(3) if q != dim]
+    dk = isx4[1 + dim]
+    out = {}
+    for key, line in lines.items():
+        L0 = isx4[0] + key[0] * isx4[1 + tq[0]] + key[1] * isx4[1 + tq[1]]
+        srcs = sorted({int(k + o) for k in np.nonzero(line)[0] for o in range(-4, 5) if 0 <= k + o < nc})
+        pad = np.concatenate([np.zeros(4), line, np.zeros(4)]).astype(np.float32)
+        a = O.coeff_batch(method, np.stack([pad[k:k + 9] for k in srcs]), thr).astype(np.float64)
+        tgt = {}
+        for k, ak in zip(srcs, a):
+            vs = vmin + k * dv
+            for gk in range(math.floor((vs - L0) / dk), math.floor((vs + dv - L0) / dk) + 1):
+                xl = min(max((L0 + gk * dk - vs) / dv, 0.0), 1.0)
+                xr = min(max((L0 + (gk + 1) * dk - vs) / dv, 0.0), 1.0)
+                if xr > xl:
+                    Fr = xr * sum(ak[i] * xr ** i for i in range(len(ak)))
+                    Fl = xl * sum(ak[i] * xl ** i for i in range(len(ak)))
+                    tgt[gk] = tgt.get(gk, 0.0) + (Fr - Fl)
+        out[key] = tgt
+    return out
+
+
+@pytest.mark.parametrize("method", [O.PPM, O.PQM])
+def test_map_1d_matches_brute_force_overlaps(method):
+    """Every pass of a 20-degree rotation of a cell, against independent FP64 overlap integrals
+    (cpu_acc_map.cpp:533-628).  Bound 3e-5 of the cell's maximum: the reference keeps the
+    Lagrangian coordinates in FP32, whose ulp near |v| = 2e6 m/s is 0.25 m/s = 1.25e-5 dv
+    (measured: at most 8e-6).  A wrong target index, sign or dropped segment moves O(1) of a
+    value.  No mass may land outside the output blocks."""
+    m = _mesh()
+    dv, nx = 4e6 / 200, 50
+    g, f = _maxwellian_cell(T=1.5e5)
+    B = (3e-9, -4e-9, 1e-8)
+    T = O.transform(O.make_fields(B, (1e5, -2e5, 5e4)), W.MASS_PROTON, W.CHARGE,
+                    W.gyro_period(B) * 20 / 360, 1.6e-19)
+    isx = O.intersections(m, T, 0)
+    fmax = float(np.abs(f).max())
+    for dim in range(3):
+        isx4 = isx[4 * dim:4 * dim + 4]
+        g2, f2, _ = O.map_1d(m, g, f, isx4, dim, method, 1e-15)
+        bf = _brute_map_1d(nx, -2e6, dv, g, f, isx4, dim, method, 1e-15)
+        got = {}
+        for b, gid in enumerate(g2.tolist()):
+            bi = (gid % nx, (gid // nx) % nx, gid // (nx * nx))
+            for e in range(64):
+                c = [4 * bi[0] + (e & 3), 4 * bi[1] + ((e >> 2) & 3), 4 * bi[2] + (e >> 4)]
+                got[(tuple(c[q] for q in range(3) if q != dim), c[dim])] = float(f2[b, e])
+        outside = 0.0
+        for key, tg in bf.items():
+            for gk, val in tg.items():
+                if (key, gk) in got:
+                    assert abs(got[(key, gk)] - val) <= 3e-5 * fmax, (dim, key, gk)
+                else:
+                    outside += abs(val)
+        assert outside == 0.0
+        for (key, gk), val in got.items():
+            if gk not in bf.get(key, {}):
+                assert val == 0.0

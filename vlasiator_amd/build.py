@@ -48,13 +48,6 @@ def build(verbose: bool = False) -> str:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
-    # VALU issue-rate probe (a program of its own, not part of the library)
-    probe = os.path.join(LIB_DIR, "vacc_valu_probe")
-    cmd = [HIPCC, *FLAGS, os.path.join(CSRC, "valu_probe.hip"), "-o", probe + ".tmp"]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    subprocess.run(cmd, check=True)
-    os.replace(probe + ".tmp", probe)
     # example C++ host program over the adapter (vlasiator_amd/host)
     drv = os.path.join(LIB_DIR, "vacc_acc_driver")
     cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall", "-I", os.path.join(HERE, "..", "include"),
