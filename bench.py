@@ -64,6 +64,28 @@ def run_step(ctx, step, adjust):
     return ctx.step(step % 3, use_moments=True, adjust=adjust)
 
 
+def host_cpus():
+    """The host's CPU model and the threads the CPU baseline may use: the process's CPU share
+    as the environment states it (OMP_NUM_THREADS, which the GPU host sets to its per-GPU
+    share), else every CPU in the process's affinity mask."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    allowed = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        share, rule = min(int(env), allowed), "OMP_NUM_THREADS (the host's CPU share of this process)"
+    else:
+        share, rule = allowed, "every CPU of the process's affinity mask"
+    return {"model": model, "visible": os.cpu_count(), "allowed": allowed, "share": share, "rule": rule}
+
+
 def cpu_baseline(w, sample_cells, isect, target_s, nthreads):
     """Oracle (C restatement of the reference CPU path) on a bounded sample of cells."""
     import oracle as O
@@ -462,9 +484,12 @@ def main():
         out["plain_form"] = {"value": rate_plain, "unit": "cell-updates/s", "cells": npl,
                              "batched_over_plain": r["value"] / rate_plain}
     if not args.no_cpu:
-        nthreads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        log(f"cpu baseline: {nthreads} threads, ~{args.cpu_seconds} s")
+        host = host_cpus()
+        nthreads = args.cpu_threads or host["share"]
+        log(f"cpu baseline: {nthreads} threads on {host['model']}, ~{args.cpu_seconds} s")
         out["cpu_baseline"] = cpu_baseline(w, sample_cells, isect0, args.cpu_seconds, nthreads)
+        out["cpu_baseline"].update({"cpu_model": host["model"], "host_cpus": host["visible"],
+                                    "cpus_allowed": host["allowed"], "threads_rule": host["rule"]})
     print(json.dumps(out), flush=True)
 
 

@@ -187,6 +187,28 @@ int vacc_set_fields(vacc_ctx* ctx, const vacc_cell_fields* fields);
  * adjust != 0, the sparsity adjust (adjustVelocityBlocks, :473).  rho_loss as in
  * vacc_adjust_blocks (may be NULL).  Returns the OR of the per-cell status bits. */
 int vacc_step(vacc_ctx* ctx, uint32_t map_order, int32_t use_moments, int32_t adjust, double* rho_loss);
+/* Acceleration subcycling of one population (calculateAcceleration(mpiGrid, dt),
+ * vlasovsolver/vlasovmover.cpp:481-565, and its subcycle step :410-475).
+ * vacc_subcycle_count: per cell max_v_dt = |T_gyro| * max_rotation_deg / 360 from the resident
+ * fields' B (updateAccelerationMaxdt, cpu_acc_transform.cpp:41-55) and the subcycle count
+ * max(ceil(|dt| / max_v_dt), 1) (getAccelerationSubcycles, vlasovmover.cpp:398-401), kept
+ * resident; *nmax receives the maximum over the context's cells.  Across processes the caller
+ * takes the maximum of nmax (the reference's MPI_Allreduce, vlasovmover.cpp:544) and runs that
+ * many steps.
+ * vacc_subcycle_step: subcycle `step` (0-based) of every cell with step < its count, the others
+ * unchanged: moments pivot (use_moments != 0), transform and intersections with the cell's
+ * subcycle dt -- max_v_dt, or what is left of |dt| on its last subcycle, with the sign of dt
+ * (vlasovmover.cpp:429-456) -- the three remaps in map_order (the same order for every
+ * subcycle of a step, :424-426), and, when adjust != 0, the single-cell sparsity adjust of
+ * every cell (:472-474, and the final adjust :556; adjusting a cell that was not accelerated
+ * changes nothing).  rho_loss as in vacc_adjust_blocks.  One host synchronisation.
+ * vacc_accelerate_subcycled: count, then every step with adjust; *nsub (may be NULL) receives
+ * the step count, rho_loss (may be NULL) the removed mass summed over the steps. */
+int vacc_subcycle_count(vacc_ctx* ctx, double dt, uint32_t* nmax);
+int vacc_subcycle_step(vacc_ctx* ctx, uint32_t step, uint32_t map_order, int32_t use_moments, int32_t adjust,
+                       double* rho_loss);
+int vacc_accelerate_subcycled(vacc_ctx* ctx, double dt, uint32_t map_order, int32_t use_moments, uint32_t* nsub,
+                              double* rho_loss);
 /* One acceleration step of several populations of the same cells (one context each, npop <=
  * 8, e.g. protons and He2+): calculateAcceleration(mpiGrid, dt), vlasovmover.cpp:481-565, with
  * one subcycle per population.  Populations are advanced in order; with use_moments != 0 each

@@ -12,12 +12,12 @@ static void run(const float* col, int n, float thr, float* out) {
    for (int k = 0; k < n; ++k) rc.coeffs(col + 4 + k, thr, scale, out + R::NC * k);
 }
 
-// col: n + 8 values, the n source cells at col[4 .. n+3].  fast selects the fused variants.
-extern "C" int recon_stream(int method, int fast, const float* col, int n, float thr, float* out) {
+// col: n + 8 values, the n source cells at col[4 .. n+3].
+extern "C" int recon_stream(int method, const float* col, int n, float thr, float* out) {
    using namespace vacc;
-   if (method == VACC_PLM) fast ? run<ReconFast<VACC_PLM>>(col, n, thr, out) : run<Recon<VACC_PLM>>(col, n, thr, out);
-   else if (method == VACC_PPM) fast ? run<ReconFast<VACC_PPM>>(col, n, thr, out) : run<Recon<VACC_PPM>>(col, n, thr, out);
-   else fast ? run<ReconFast<VACC_PQM>>(col, n, thr, out) : run<Recon<VACC_PQM>>(col, n, thr, out);
+   if (method == VACC_PLM) run<Recon<VACC_PLM>>(col, n, thr, out);
+   else if (method == VACC_PPM) run<Recon<VACC_PPM>>(col, n, thr, out);
+   else run<Recon<VACC_PQM>>(col, n, thr, out);
    return 0;
 }
 

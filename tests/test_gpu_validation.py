@@ -1,0 +1,86 @@
+"""GPU tests of load validation: a load whose lists break the block-list contract is rejected
+with VACC_E_ARG before anything changes, so the context keeps the cells it held.  The contract
+is VelocityMesh::push_back's (spatial_cells/velocity_mesh_cpu.h:428-444: no invalid GID, no GID
+twice in one cell) plus packed offsets that start at 0 and ascend."""
+import numpy as np
+import pytest
+
+import oracle as O
+from tests import cases
+from vlasiator_amd import vacc as V
+from vlasiator_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _ctx(w, storage=V.STORE_F32):
+    return V.Context(V.make_mesh(w.grid_length, w.vmin, w.vmax),
+                     V.make_species(w.mass, w.charge, w.min_value, w.add_width),
+                     V.make_options(w.method, storage=storage))
+
+
+def _packed(cells):
+    off = np.zeros(len(cells) + 1, np.uint64)
+    off[1:] = np.cumsum([len(g) for g, _ in cells])
+    return off, np.concatenate([g for g, _ in cells]).astype(np.uint32), np.concatenate([f for _, f in cells])
+
+
+@pytest.mark.parametrize("storage", [V.STORE_F32, V.STORE_BF16])
+def test_rejected_loads_keep_the_context(storage):
+    w = W.config(0, ncells=3)
+    cells = cases.build_cells(w)
+    print(f"load validation storage={storage} blocks={[len(g) for g, _ in cells]}", flush=True)
+    ctx = _ctx(w, storage)
+    ctx.upload_cells(cells)
+    before = ctx.download_cells()
+    off, g, f = _packed(cells)
+    nmesh = int(np.prod(w.grid_length))
+
+    bad_cases = []
+    dup = g.copy()
+    dup[int(off[1]) + 5] = dup[int(off[1]) + 2]           # a GID twice in cell 1
+    bad_cases.append(("repeated GID", off, dup, "repeated"))
+    out = g.copy()
+    out[int(off[2])] = nmesh                              # outside the mesh
+    bad_cases.append(("GID outside the mesh", off, out, "outside"))
+    inv = g.copy()
+    inv[0] = 0xFFFFFFFF                                   # the invalid GID
+    bad_cases.append(("invalid GID", off, inv, "outside"))
+    shifted = off.copy()
+    shifted[0] = 1
+    bad_cases.append(("offsets[0] != 0", shifted, g, "offsets[0]"))
+    desc = off.copy()
+    desc[2] = desc[1] - 1
+    bad_cases.append(("descending offsets", desc, g, "ascending"))
+    for name, o, gg, msg in bad_cases:
+        with pytest.raises(V.VaccError, match=msg):
+            ctx.upload(o, gg, f)
+        cases.assert_cells_equal(ctx.download_cells(), before, f"after rejected load ({name})")
+    # the same GID in different cells is allowed
+    same = [(cells[0][0], cells[0][1]), (cells[0][0], cells[0][1] * 2)]
+    ctx.upload_cells(same)
+    got = ctx.download_cells()
+    assert np.array_equal(np.sort(got[1][0]), np.sort(cells[0][0]))
+    ctx.close()
+
+
+def test_step_after_rejected_load_matches_oracle():
+    """A rejected load leaves intersections and fields as they were: the next acceleration of
+    the kept cells equals the oracle's."""
+    w = W.config(0, ncells=2)
+    cells = cases.build_cells(w)
+    isx = cases.oracle_intersections(w, 1)
+    ctx = _ctx(w)
+    ctx.upload_cells(cells)
+    ctx.set_intersections(isx)
+    off, g, f = _packed(cells)
+    dup = g.copy()
+    dup[1] = dup[0]
+    with pytest.raises(V.VaccError):
+        ctx.upload(off, dup, f)
+    ctx.accelerate(1)
+    got = ctx.download_cells()
+    m = cases.oracle_mesh(w)
+    want = [O.accelerate_cell(m, gc, fc, isx[i], 1, w.method, w.min_value)[:2] for i, (gc, fc) in enumerate(cells)]
+    cases.assert_cells_equal(got, want, "step after a rejected load")
+    ctx.close()

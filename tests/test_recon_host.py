@@ -29,7 +29,7 @@ def lib():
                         "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
                         SRC, "-o", OUT], check=True, capture_output=True)
     L = C.CDLL(OUT)
-    L.recon_stream.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_float, C.c_void_p]
+    L.recon_stream.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_float, C.c_void_p]
     return L
 
 
@@ -60,33 +60,10 @@ def test_streamed_device_reconstruction_matches_oracle(lib, method):
     for col in columns(31 + method):
         n = col.size - 8
         got = np.zeros(n * nc, np.float32)
-        lib.recon_stream(method, 0, col.ctypes.data, n, np.float32(1e-15), got.ctypes.data)
+        lib.recon_stream(method, col.ctypes.data, n, np.float32(1e-15), got.ctypes.data)
         st = np.stack([col[k:k + 9] for k in range(n)])
         want = O.coeff_batch(method, st, 1e-15).reshape(-1)
         assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
-
-
-@pytest.mark.parametrize("method", [O.PLM, O.PPM, O.PQM])
-def test_fast_reconstruction_close_to_oracle(lib, method):
-    """The fused variants integrate to the cell average and stay within float rounding of the
-    exact coefficients except where a limiter/filter decision flips on rounding."""
-    nc = O.ncoef(method)
-    total, flipped = 0, 0
-    for col in columns(77 + method):
-        n = col.size - 8
-        got = np.zeros(n * nc, np.float32)
-        lib.recon_stream(method, 1, col.ctypes.data, n, np.float32(1e-15), got.ctypes.data)
-        got = got.reshape(n, nc).astype(np.float64)
-        st = np.stack([col[k:k + 9] for k in range(n)])
-        want = O.coeff_batch(method, st, 1e-15).astype(np.float64)
-        scale = np.maximum(np.abs(st).max(axis=1), 1e-30)[:, None]
-        err = np.abs(got - want) / scale
-        # integral of the reconstruction over the cell = cell average
-        assert np.all(np.abs(got.sum(axis=1) - st[:, 4]) <= 2e-5 * scale[:, 0])
-        close = np.all(err <= 1e-4, axis=1)
-        total += n
-        flipped += int((~close).sum())
-    assert flipped <= 0.02 * total
 
 
 def test_div3_equals_ieee_division_for_every_float(lib):

@@ -75,6 +75,17 @@ struct vacc_ctx {
    double* moments = nullptr;
    double* pivot = nullptr;          // [4*ncells] plasma pivot of a multi-population step
    double* rho_loss = nullptr;
+   // subcycling (vacc_subcycle_count / vacc_subcycle_step): per cell max_v_dt, subcycle count,
+   // the current subcycle's dt and whether the cell takes part in it
+   double* sub_maxdt = nullptr;
+   double* sub_dt = nullptr;
+   uint32_t* sub_n = nullptr;
+   uint8_t* active = nullptr;
+   uint32_t* sub_nmax = nullptr;     // device word: maximum subcycle count over the cells
+   bool sub_valid = false;           // sub_maxdt / sub_n hold the counts of the last vacc_subcycle_count
+   bool subcycling = false;          // launches inside vacc_subcycle_step use sub_dt and active
+   double sub_dt_total = 0.0;
+   uint32_t sub_max = 0;
    hipEvent_t mom_ev = nullptr;      // moments of this population are ready (multi-population step)
    // spatial-neighbour adjust inputs (device copies, grown on demand)
    uint32_t* d_nbr_off = nullptr;
@@ -213,6 +224,13 @@ static void free_cells(vacc_ctx* c) {
    if (c->moments) (void)hipFree(c->moments);
    if (c->pivot) (void)hipFree(c->pivot);
    if (c->rho_loss) (void)hipFree(c->rho_loss);
+   void* subs[] = {c->sub_maxdt, c->sub_dt, c->sub_n, c->active, c->sub_nmax};
+   for (void* p : subs)
+      if (p) (void)hipFree(p);
+   c->sub_maxdt = c->sub_dt = nullptr;
+   c->sub_n = c->sub_nmax = nullptr;
+   c->active = nullptr;
+   c->sub_valid = false;
    c->isect = nullptr;
    c->fields = nullptr;
    c->moments = nullptr;
@@ -239,6 +257,11 @@ static int alloc_cells(vacc_ctx* c, uint32_t n) {
    HIPCHK(hipMalloc(&c->moments, m * 4 * sizeof(double)));
    HIPCHK(hipMalloc(&c->pivot, m * 4 * sizeof(double)));
    HIPCHK(hipMalloc(&c->rho_loss, m * sizeof(double)));
+   HIPCHK(hipMalloc(&c->sub_maxdt, m * sizeof(double)));
+   HIPCHK(hipMalloc(&c->sub_dt, m * sizeof(double)));
+   HIPCHK(hipMalloc(&c->sub_n, m * sizeof(uint32_t)));
+   HIPCHK(hipMalloc(&c->active, m));
+   HIPCHK(hipMalloc(&c->sub_nmax, sizeof(uint32_t)));
    c->cell_cap = n;
    return 0;
 }
@@ -247,39 +270,55 @@ static size_t elem_bytes(const vacc_ctx* c) { return c->opt.storage == VACC_STOR
 
 static Pool pool(const vacc_ctx* c, int i) { return Pool{c->f[i], c->scl[i], c->opt.storage}; }
 
-// (Re)allocate block storage for `want` blocks, preserving the current list and pool.
+// (Re)allocate block storage for `want` blocks, preserving the current list and pool.  On
+// failure the new buffers are freed and the old ones are kept.
 static int ensure_blocks(vacc_ctx* c, size_t want) {
    if (want <= c->cap && c->gid[0]) return 0;
    if (want > 0xF0000000ull) return set_err(VACC_E_NOMEM, "block count exceeds 32-bit list indices");
    const size_t ncap = std::max<size_t>(want, 1024);
-   uint32_t *g[2], *s[2];
-   void* f[2];
+   uint32_t *g[2] = {nullptr, nullptr}, *s[2] = {nullptr, nullptr};
+   void* f[2] = {nullptr, nullptr};
    float* sc[2] = {nullptr, nullptr};
-   uint8_t* ct;
-   uint32_t *ss, *pm, *pc;
-   uint4 *sr, *cr;
-   for (int i = 0; i < 2; ++i) {
-      HIPCHK(hipMalloc(&g[i], ncap * sizeof(uint32_t)));
-      HIPCHK(hipMalloc(&s[i], ncap * sizeof(uint32_t)));
-      HIPCHK(hipMalloc(&f[i], ncap * WID3 * elem_bytes(c)));
-      if (c->opt.storage != VACC_STORE_F32) HIPCHK(hipMalloc(&sc[i], ncap * sizeof(float)));
+   uint8_t* ct = nullptr;
+   uint32_t *ss = nullptr, *pm = nullptr, *pc = nullptr;
+   uint4 *sr = nullptr, *cr = nullptr;
+   auto release = [&]() {
+      void* all[] = {g[0], g[1], s[0], s[1], f[0], f[1], sc[0], sc[1], ct, ss, pm, pc, sr, cr};
+      for (void* p : all)
+         if (p) (void)hipFree(p);
+   };
+   hipError_t e = hipSuccess;
+   for (int i = 0; i < 2 && e == hipSuccess; ++i) {
+      e = hipMalloc(&g[i], ncap * sizeof(uint32_t));
+      if (e == hipSuccess) e = hipMalloc(&s[i], ncap * sizeof(uint32_t));
+      if (e == hipSuccess) e = hipMalloc(&f[i], ncap * WID3 * elem_bytes(c));
+      if (e == hipSuccess && c->opt.storage != VACC_STORE_F32) e = hipMalloc(&sc[i], ncap * sizeof(float));
    }
-   HIPCHK(hipMalloc(&ct, ncap));
-   HIPCHK(hipMalloc(&ss, ncap * sizeof(uint32_t)));
-   HIPCHK(hipMalloc(&sr, ncap * sizeof(uint4)));
-   HIPCHK(hipMalloc(&cr, ncap * sizeof(uint4)));
-   HIPCHK(hipMalloc(&pm, ncap * sizeof(uint32_t)));
-   HIPCHK(hipMalloc(&pc, ncap * sizeof(uint32_t)));
-   if (c->cap && c->nblocks) {
+   if (e == hipSuccess) e = hipMalloc(&ct, ncap);
+   if (e == hipSuccess) e = hipMalloc(&ss, ncap * sizeof(uint32_t));
+   if (e == hipSuccess) e = hipMalloc(&sr, ncap * sizeof(uint4));
+   if (e == hipSuccess) e = hipMalloc(&cr, ncap * sizeof(uint4));
+   if (e == hipSuccess) e = hipMalloc(&pm, ncap * sizeof(uint32_t));
+   if (e == hipSuccess) e = hipMalloc(&pc, ncap * sizeof(uint32_t));
+   if (e == hipSuccess && c->cap && c->nblocks) {
       // carry the current state: list entries and the whole current pool extent
       const size_t n = c->nblocks;
-      HIPCHK(hipMemcpyAsync(g[c->lcur], c->gid[c->lcur], n * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
-      HIPCHK(hipMemcpyAsync(s[c->lcur], c->src[c->lcur], n * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream));
-      HIPCHK(hipMemcpyAsync(f[c->fcur], c->f[c->fcur], c->cap * WID3 * elem_bytes(c), hipMemcpyDeviceToDevice, c->stream));
-      if (sc[c->fcur])
-         HIPCHK(hipMemcpyAsync(sc[c->fcur], c->scl[c->fcur], c->cap * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
-      HIPCHK(hipMemcpyAsync(ct, c->content, n, hipMemcpyDeviceToDevice, c->stream));
-      HIPCHK(hipStreamSynchronize(c->stream));
+      e = hipMemcpyAsync(g[c->lcur], c->gid[c->lcur], n * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream);
+      if (e == hipSuccess)
+         e = hipMemcpyAsync(s[c->lcur], c->src[c->lcur], n * sizeof(uint32_t), hipMemcpyDeviceToDevice, c->stream);
+      if (e == hipSuccess)
+         e = hipMemcpyAsync(f[c->fcur], c->f[c->fcur], c->cap * WID3 * elem_bytes(c), hipMemcpyDeviceToDevice, c->stream);
+      if (e == hipSuccess && sc[c->fcur])
+         e = hipMemcpyAsync(sc[c->fcur], c->scl[c->fcur], c->cap * sizeof(float), hipMemcpyDeviceToDevice, c->stream);
+      if (e == hipSuccess) e = hipMemcpyAsync(ct, c->content, n, hipMemcpyDeviceToDevice, c->stream);
+      const hipError_t e2 = hipStreamSynchronize(c->stream);
+      if (e == hipSuccess) e = e2;
+   }
+   if (e != hipSuccess) {
+      release();
+      (void)hipGetLastError();
+      return set_err(e == hipErrorOutOfMemory ? VACC_E_NOMEM : VACC_E_HIP,
+                     std::string("block storage for ") + std::to_string(ncap) + " blocks: " + hipGetErrorString(e));
    }
    free_blocks(c);
    for (int i = 0; i < 2; ++i) {
@@ -339,6 +378,7 @@ static int enqueue_pass(vacc_ctx* c, int dim, int grid_mult) {
    ca.dv = cell_f(c, dim);                 // Realf dv = cellSize (cpu_acc_map.cpp:147)
    ca.vmin = (float)c->mi.vmin[dim];       // Realf v_min (cpu_acc_map.cpp:148)
    ca.wall_margin = c->opt.wall_margin;
+   ca.active = c->subcycling ? c->active : nullptr;
    {
       ProfScope ps(c, 1);
       launch_fill_u32(c->len_hist, 0u, LEN_BINS, c->stream);
@@ -388,6 +428,8 @@ static int enqueue_pass(vacc_ctx* c, int dim, int grid_mult) {
    ra.vmin = ca.vmin;
    ra.threshold = (float)c->sp.sparse_min_value;   // Realf threshold argument of compute_*_coeff
    ra.min_value = c->sp.sparse_min_value;
+   ra.dv_d = c->mi.cell[dim];
+   ra.vmin_d = c->mi.vmin[dim];
    {
       ProfScope ps(c, 0);
       launch_remap(dim, c->opt.method, c->opt.fast_arith != 0, ra, grid_mult ? grid_mult : c->remap_mult[dim],
@@ -519,58 +561,123 @@ static int status_or(vacc_ctx* c) {
    return (int)(c->h_sum[2] & 0x7FFFFFFFu);
 }
 
-// First half of a load of new cells: checks the lists, sizes the buffers and copies the
-// offsets and GIDs; the caller then fills pool F[0] and calls end_load.
-static int begin_load(vacc_ctx* c, uint32_t ncells, const uint64_t* offsets, const uint32_t* gids) {
+// Block capacity a load of `total` blocks in `ncells` cells allocates: one step's growth (new
+// target blocks and adjust halo blocks) on top of the blocks themselves.
+static size_t load_capacity(uint64_t total, uint32_t ncells) {
+   return (size_t)total + (size_t)total / 2 + 64 * (size_t)ncells + 4096;
+}
+
+// Checks of a load's lists, made before any state changes, so that a rejected load keeps the
+// context as it was: offsets start at 0 and ascend, every GID lies in the mesh and appears
+// once in its cell (VelocityMesh::push_back rejects invalid and repeated GIDs,
+// spatial_cells/velocity_mesh_cpu.h:428-444), and the blocks with their head-room fit the
+// 32-bit list indices.  Repeats are found with one bit per mesh block, cleared after each cell.
+static int check_load(vacc_ctx* c, uint32_t ncells, const uint64_t* offsets, const uint32_t* gids) {
+   if (offsets[0] != 0) return set_err(VACC_E_ARG, "offsets[0] must be 0");
+   for (uint32_t i = 1; i <= ncells; ++i)
+      if (offsets[i] < offsets[i - 1]) return set_err(VACC_E_ARG, "offsets not ascending");
    const uint64_t total = offsets[ncells];
-   if (total > 0xF0000000ull) return set_err(VACC_E_ARG, "too many blocks");
+   if (load_capacity(total, ncells) > 0xF0000000ull)
+      return set_err(VACC_E_NOMEM, "block count with its growth reserve exceeds 32-bit list indices");
    const uint64_t gmax = (uint64_t)c->mi.len[0] * c->mi.len[1] * c->mi.len[2];
-   std::vector<uint32_t> off32(ncells + 1);
-   for (uint32_t i = 0; i <= ncells; ++i) {
-      if (i && offsets[i] < offsets[i - 1]) return set_err(VACC_E_ARG, "offsets not ascending");
-      off32[i] = (uint32_t)offsets[i];
+   std::vector<uint64_t> seen((gmax + 63) / 64, 0ull);
+   for (uint32_t i = 0; i < ncells; ++i) {
+      const uint64_t b = offsets[i], e = offsets[i + 1];
+      uint64_t j = b;
+      int rc = 0;
+      for (; j < e; ++j) {
+         const uint32_t g = gids[j];
+         if (g >= gmax) {
+            rc = set_err(VACC_E_ARG, "GID outside the velocity mesh in cell " + std::to_string(i));
+            break;
+         }
+         uint64_t& w = seen[g >> 6];
+         const uint64_t bit = 1ull << (g & 63);
+         if (w & bit) {
+            rc = set_err(VACC_E_ARG, "GID " + std::to_string(g) + " repeated in cell " + std::to_string(i));
+            break;
+         }
+         w |= bit;
+      }
+      for (uint64_t q = b; q < j; ++q) seen[gids[q] >> 6] = 0ull;
+      if (rc) return rc;
    }
-   for (uint64_t i = 0; i < total; ++i)
-      if (gids[i] >= gmax) return set_err(VACC_E_ARG, "GID outside the velocity mesh");
-   int rc = alloc_cells(c, ncells);
+   return 0;
+}
+
+// A load that failed after its checks passed leaves an empty context (no cells): nothing
+// that a later call could run on refers to the half-written lists.
+static void abort_load(vacc_ctx* c) {
+   (void)hipStreamSynchronize(c->stream);
+   drop_graphs(c);
+   c->ncells = 0;
+   c->nblocks = 0;
+   c->lcur = c->fcur = 0;
+   c->sorted = false;
+   c->cell_epoch++;
+   if (c->off[0]) {
+      (void)hipMemsetAsync(c->off[0], 0, sizeof(uint32_t), c->stream);
+      (void)hipStreamSynchronize(c->stream);
+   }
+   c->content_valid = c->moments_valid = c->fields_valid = c->sub_valid = false;
+}
+
+// First half of a load of new cells: checks the lists, sizes the buffers and copies the
+// offsets and GIDs; the caller then fills pool F[0] and calls end_load, or abort_load when
+// that fails.
+static int begin_load(vacc_ctx* c, uint32_t ncells, const uint64_t* offsets, const uint32_t* gids) {
+   int rc = check_load(c, ncells, offsets, gids);
    if (rc) return rc;
-   HIPCHK(hipStreamSynchronize(c->stream));
+   const uint64_t total = offsets[ncells];
+   std::vector<uint32_t> off32(ncells + 1);
+   for (uint32_t i = 0; i <= ncells; ++i) off32[i] = (uint32_t)offsets[i];
+   // from here on a failure empties the context
+   c->content_valid = c->moments_valid = c->fields_valid = c->sub_valid = false;
+   rc = alloc_cells(c, ncells);
+   if (!rc) {
+      const size_t want = load_capacity(total, ncells);
+      if (want > c->cap) {
+         c->nblocks = 0;   // nothing of the old state is carried into the new buffers
+         rc = ensure_blocks(c, want);
+      }
+   }
+   if (rc) {
+      abort_load(c);
+      return rc;
+   }
    c->ncells = ncells;
    c->nblocks = 0;
    c->lcur = 0;
    c->fcur = 0;
    c->sorted = false;
    c->cell_epoch++;
-   const size_t want = total + total / 2 + 64 * (size_t)ncells + 4096;
-   if (want > c->cap) {
-      rc = ensure_blocks(c, want);
-      if (rc) return rc;
+   hipError_t e = hipMemcpyAsync(c->off[0], off32.data(), (ncells + 1) * sizeof(uint32_t), hipMemcpyHostToDevice,
+                                 c->stream);
+   if (e == hipSuccess && total)
+      e = hipMemcpyAsync(c->gid[0], gids, total * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream);
+   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);   // the caller's arrays may go away after return
+   if (e != hipSuccess) {
+      abort_load(c);
+      return set_err(VACC_E_HIP, std::string("load of the block lists: ") + hipGetErrorString(e));
    }
-   HIPCHK(hipMemcpyAsync(c->off[0], off32.data(), (ncells + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
-   if (total) HIPCHK(hipMemcpyAsync(c->gid[0], gids, total * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
-   HIPCHK(hipStreamSynchronize(c->stream));   // the caller's arrays may go away after return
    return 0;
-}
-
-// A load that failed after begin_load leaves an empty context (no cells).
-static void abort_load(vacc_ctx* c) {
-   (void)hipStreamSynchronize(c->stream);
-   c->ncells = 0;
-   c->nblocks = 0;
-   (void)hipMemsetAsync(c->off[0], 0, sizeof(uint32_t), c->stream);
-   (void)hipStreamSynchronize(c->stream);
-   c->content_valid = c->moments_valid = c->fields_valid = false;
 }
 
 static int end_load(vacc_ctx* c, uint64_t total) {
    launch_iota(c->src[0], (uint32_t)total, c->stream);
-   HIPCHK(hipMemsetAsync(c->status, 0, (c->ncells + 1) * sizeof(uint32_t), c->stream));
-   HIPCHK(hipMemsetAsync(c->flags, 0, sizeof(uint32_t), c->stream));
-   HIPCHK(hipStreamSynchronize(c->stream));
+   hipError_t e = hipGetLastError();
+   if (e == hipSuccess) e = hipMemsetAsync(c->status, 0, (c->ncells + 1) * sizeof(uint32_t), c->stream);
+   if (e == hipSuccess) e = hipMemsetAsync(c->flags, 0, sizeof(uint32_t), c->stream);
+   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+   if (e != hipSuccess) {
+      abort_load(c);
+      return set_err(VACC_E_HIP, std::string("end of load: ") + hipGetErrorString(e));
+   }
    c->nblocks = total;
    c->content_valid = false;
    c->moments_valid = false;
    c->fields_valid = false;
+   c->sub_valid = false;
    return 0;
 }
 
@@ -992,6 +1099,103 @@ static int dev_grow(T*& p, size_t& cap, size_t n) {
 // Waves of a pencil walk launch (all segments together), about.
 constexpr uint32_t WALK_WAVES = 32768;
 
+// Device temporaries of one call, freed when it returns (on every path).
+struct DevTemps {
+   std::vector<void*> p;
+   template <class T>
+   hipError_t alloc(T** out, size_t bytes) {
+      void* q = nullptr;
+      const hipError_t e = hipMalloc(&q, bytes);
+      if (e == hipSuccess) p.push_back(q);
+      *out = static_cast<T*>(q);
+      return e;
+   }
+   ~DevTemps() {
+      for (void* q : p) (void)hipFree(q);
+   }
+};
+
+// The load of vacc_init_peaks once its arguments are checked; a failure leaves the caller
+// to empty the context.
+static int init_peaks_load(vacc_ctx* c, uint32_t ncells, uint32_t max_peaks, const vacc_peak* peaks,
+                           const int32_t* npeaks, const int32_t* bbox, const std::vector<uint32_t>& bm_off) {
+   int rc = alloc_cells(c, ncells);
+   if (rc) return rc;
+   HIPCHK(hipStreamSynchronize(c->stream));
+   c->ncells = ncells;
+   c->lcur = 0;
+   c->fcur = 0;
+   c->nblocks = 0;
+   c->sorted = false;
+   c->cell_epoch++;
+   c->content_valid = false;
+   c->moments_valid = false;
+   c->fields_valid = false;
+   c->sub_valid = false;
+   if (!ncells) return 0;
+   DevTemps tmp;
+   vacc_peak* d_pk = nullptr;
+   int32_t *d_np = nullptr, *d_bb = nullptr;
+   uint32_t *d_bmo = nullptr, *d_bm = nullptr;
+   float* stage = nullptr;
+   HIPCHK(tmp.alloc(&d_pk, (size_t)ncells * max_peaks * sizeof(vacc_peak)));
+   HIPCHK(tmp.alloc(&d_np, ncells * sizeof(int32_t)));
+   HIPCHK(tmp.alloc(&d_bb, (size_t)ncells * 6 * sizeof(int32_t)));
+   HIPCHK(tmp.alloc(&d_bmo, (ncells + 1) * sizeof(uint32_t)));
+   HIPCHK(tmp.alloc(&d_bm, std::max<size_t>(bm_off[ncells], 1) * sizeof(uint32_t)));
+   HIPCHK(hipMemcpy(d_pk, peaks, (size_t)ncells * max_peaks * sizeof(vacc_peak), hipMemcpyHostToDevice));
+   HIPCHK(hipMemcpy(d_np, npeaks, ncells * sizeof(int32_t), hipMemcpyHostToDevice));
+   HIPCHK(hipMemcpy(d_bb, bbox, (size_t)ncells * 6 * sizeof(int32_t), hipMemcpyHostToDevice));
+   HIPCHK(hipMemcpy(d_bmo, bm_off.data(), (ncells + 1) * sizeof(uint32_t), hipMemcpyHostToDevice));
+   InitArgs ia{};
+   ia.peaks = d_pk;
+   ia.npeaks = d_np;
+   ia.bbox = d_bb;
+   ia.bm_off = d_bmo;
+   ia.content_bm = d_bm;
+   ia.n_keep = c->n_tgt;
+   ia.ncells = ncells;
+   ia.max_peaks = max_peaks;
+   for (int d = 0; d < 3; ++d) {
+      ia.len[d] = c->mi.len[d];
+      ia.vmin[d] = c->mi.vmin[d];
+      ia.cell[d] = c->mi.cell[d];
+   }
+   ia.mass = c->sp.mass;
+   ia.min_value = c->sp.sparse_min_value;
+   ia.add_width = c->sp.sparse_add_width_v;
+   launch_init(ia, false, c->stream);
+   ScanArgs sa{};
+   sa.a = c->n_tgt;
+   sa.a_off = c->off[0];
+   sa.n = ncells;
+   launch_scan(sa, c->stream);
+   HIPCHK(hipGetLastError());
+   rc = refresh_count(c);
+   if (rc) return rc;
+   const size_t total = c->nblocks;
+   c->nblocks = 0;   // nothing to carry over
+   rc = ensure_blocks(c, total + total / 2 + 64 * (size_t)ncells + 4096);
+   if (rc) return rc;
+   ia.out_off = c->off[0];
+   ia.gid_out = c->gid[0];
+   ia.src_out = c->src[0];
+   if (c->opt.storage == VACC_STORE_F32) {
+      ia.f_out = static_cast<float*>(c->f[0]);
+   } else {
+      HIPCHK(tmp.alloc(&stage, std::max<size_t>(total, 1) * WID3 * sizeof(float)));
+      ia.f_out = stage;
+   }
+   launch_init(ia, true, c->stream);
+   if (stage) launch_encode(stage, pool(c, 0), (uint32_t)total, c->stream);
+   HIPCHK(hipGetLastError());
+   HIPCHK(hipMemsetAsync(c->status, 0, (ncells + 1) * sizeof(uint32_t), c->stream));
+   HIPCHK(hipMemsetAsync(c->flags, 0, sizeof(uint32_t), c->stream));
+   HIPCHK(hipStreamSynchronize(c->stream));
+   c->nblocks = total;
+   return 0;
+}
+
 extern "C" {
 
 const char* vacc_last_error(void) { return g_err.c_str(); }
@@ -1068,15 +1272,30 @@ int vacc_upload_cells(vacc_ctx* c, uint32_t ncells, const uint64_t* offsets, con
    if (rc) return rc;
    const uint64_t total = offsets[ncells];
    if (total) {
+      hipError_t e = hipSuccess;
       if (c->opt.storage == VACC_STORE_F32) {
-         HIPCHK(hipMemcpyAsync(c->f[0], f, total * WID3 * sizeof(float), hipMemcpyHostToDevice, c->stream));
+         e = hipMemcpyAsync(c->f[0], f, total * WID3 * sizeof(float), hipMemcpyHostToDevice, c->stream);
       } else {
-         float* stage = nullptr;
-         HIPCHK(hipMalloc(&stage, total * WID3 * sizeof(float)));
-         HIPCHK(hipMemcpyAsync(stage, f, total * WID3 * sizeof(float), hipMemcpyHostToDevice, c->stream));
-         launch_encode(stage, pool(c, 0), (uint32_t)total, c->stream);
-         HIPCHK(hipStreamSynchronize(c->stream));
-         HIPCHK(hipFree(stage));
+         // 16-bit forms: FP32 chunks go through the two device stages of the streamed
+         // transfer and are encoded into the pool (stream order keeps the stages safe)
+         if ((rc = ensure_io(c))) {
+            abort_load(c);
+            return rc;
+         }
+         for (uint64_t first = 0; first < total && e == hipSuccess; first += IO_CHUNK) {
+            const uint64_t m = std::min(IO_CHUNK, total - first);
+            float* st = c->io_dev + ((first / IO_CHUNK) & 1) * IO_CHUNK * WID3;
+            e = hipMemcpyAsync(st, f + first * WID3, m * WID3 * sizeof(float), hipMemcpyHostToDevice, c->stream);
+            if (e == hipSuccess) {
+               launch_encode(st, pool_at(c, 0, first), (uint32_t)m, c->stream);
+               e = hipGetLastError();
+            }
+         }
+      }
+      if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+      if (e != hipSuccess) {
+         abort_load(c);
+         return set_err(VACC_E_HIP, std::string("upload of f: ") + hipGetErrorString(e));
       }
    }
    return end_load(c, total);
@@ -1119,12 +1338,16 @@ int vacc_download_cells(vacc_ctx* c, uint64_t* offsets, uint32_t* gids, float* f
       if (c->opt.storage == VACC_STORE_F32) {
          HIPCHK(hipMemcpy(f, c->f[c->fcur], n * WID3 * sizeof(float), hipMemcpyDeviceToHost));
       } else {
-         float* stage = nullptr;
-         HIPCHK(hipMalloc(&stage, n * WID3 * sizeof(float)));
-         launch_decode(pool(c, c->fcur), stage, (uint32_t)n, c->stream);
-         HIPCHK(hipMemcpyAsync(f, stage, n * WID3 * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+         // decode chunk by chunk into the two device stages of the streamed transfer
+         if ((rc = ensure_io(c))) return rc;
+         for (uint64_t first = 0; first < n; first += IO_CHUNK) {
+            const uint64_t m = std::min<uint64_t>(IO_CHUNK, n - first);
+            float* st = c->io_dev + ((first / IO_CHUNK) & 1) * IO_CHUNK * WID3;
+            launch_decode(pool_at(c, c->fcur, first), st, (uint32_t)m, c->stream);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipMemcpyAsync(f + first * WID3, st, m * WID3 * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+         }
          HIPCHK(hipStreamSynchronize(c->stream));
-         HIPCHK(hipFree(stage));
       }
    }
    return 0;
@@ -1183,7 +1406,8 @@ static void enqueue_transform(vacc_ctx* c, uint32_t map_order, int mode) {
    if (mode == 1) enqueue_moments(c);
    ProfScope ps(c, 3);
    const double* mo = (mode == 1) ? c->moments : (mode == 2) ? c->pivot : nullptr;
-   launch_transform(c->mi, c->fields, mo, mode, c->sp.mass, c->sp.charge, c->opt.hall_minimum_rhoq,
+   launch_transform(c->mi, c->fields, c->subcycling ? c->sub_dt : nullptr, mo, mode, c->sp.mass, c->sp.charge,
+                    c->opt.hall_minimum_rhoq,
                     c->opt.ohm_gradpe_term, (int)map_order, c->isect, c->ncells, c->stream);
 }
 
@@ -1193,6 +1417,7 @@ static int upload_fields(vacc_ctx* c, const vacc_cell_fields* fields) {
    HIPCHK(hipMemcpyAsync(c->fields, tin.data(), tin.size() * sizeof(TransformIn), hipMemcpyHostToDevice, c->stream));
    HIPCHK(hipStreamSynchronize(c->stream));   // tin is released on return
    c->fields_valid = true;
+   c->sub_valid = false;
    return 0;
 }
 
@@ -1460,6 +1685,64 @@ int vacc_step(vacc_ctx* c, uint32_t map_order, int32_t use_moments, int32_t adju
    return rc;
 }
 
+int vacc_subcycle_count(vacc_ctx* c, double dt, uint32_t* nmax) {
+   if (!c || !nmax || !std::isfinite(dt)) return set_err(VACC_E_ARG, "bad argument");
+   *nmax = 0;
+   if (!c->ncells) return 0;
+   if (!c->fields_valid) return set_err(VACC_E_STATE, "no cell fields: call vacc_set_fields first");
+   if (!(c->opt.max_rotation_deg > 0.0)) return set_err(VACC_E_ARG, "max_rotation_deg must be positive");
+   launch_fill_u32(c->sub_nmax, 0u, 1, c->stream);
+   launch_subcycle_count(c->fields, c->ncells, c->sp.mass, c->sp.charge, c->opt.max_rotation_deg, dt, c->sub_maxdt,
+                         c->sub_n, c->sub_nmax, c->stream);
+   HIPCHK(hipGetLastError());
+   HIPCHK(hipMemcpyAsync(c->h_sum + 3, c->sub_nmax, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+   HIPCHK(hipStreamSynchronize(c->stream));
+   c->sub_max = c->h_sum[3];
+   c->sub_dt_total = dt;
+   c->sub_valid = true;
+   *nmax = c->sub_max;
+   return 0;
+}
+
+int vacc_subcycle_step(vacc_ctx* c, uint32_t step, uint32_t map_order, int32_t use_moments, int32_t adjust,
+                       double* rho_loss) {
+   if (!c || map_order > 2) return set_err(VACC_E_ARG, "bad argument");
+   if (!c->ncells) return 0;
+   if (!c->fields_valid || !c->sub_valid)
+      return set_err(VACC_E_STATE, "no subcycle counts for the current fields: call vacc_subcycle_count first");
+   int rc = reserve_for_step(c);
+   if (rc) return rc;
+   launch_subcycle_step(c->sub_maxdt, c->sub_n, c->ncells, step, c->sub_dt_total, c->sub_dt, c->active, c->stream);
+   HIPCHK(hipGetLastError());
+   c->subcycling = true;
+   rc = enqueue_step(c, map_order, use_moments, adjust);
+   c->subcycling = false;
+   if (rc) return rc;
+   rc = finish(c);
+   if (rc < 0) return rc;
+   if (adjust && rho_loss) HIPCHK(hipMemcpy(rho_loss, c->rho_loss, c->ncells * sizeof(double), hipMemcpyDeviceToHost));
+   return rc;
+}
+
+int vacc_accelerate_subcycled(vacc_ctx* c, double dt, uint32_t map_order, int32_t use_moments, uint32_t* nsub,
+                              double* rho_loss) {
+   if (!c || map_order > 2) return set_err(VACC_E_ARG, "bad argument");
+   uint32_t n = 0;
+   int rc = vacc_subcycle_count(c, dt, &n);
+   if (rc) return rc;
+   if (nsub) *nsub = n;
+   std::vector<double> loss(rho_loss ? c->ncells : 0, 0.0), part(loss.size());
+   if (rho_loss) std::fill(rho_loss, rho_loss + c->ncells, 0.0);
+   int st = 0;
+   for (uint32_t s = 0; s < n; ++s) {
+      rc = vacc_subcycle_step(c, s, map_order, use_moments, 1, rho_loss ? part.data() : nullptr);
+      if (rc < 0) return rc;
+      st |= rc;
+      for (size_t i = 0; i < part.size(); ++i) rho_loss[i] += part[i];
+   }
+   return st;
+}
+
 int vacc_accelerate_populations(vacc_ctx* const* ctxs, uint32_t npop, uint32_t map_order, int32_t use_moments,
                                 int32_t adjust, double* const* rho_loss) {
    if (!ctxs || npop == 0 || npop > (uint32_t)MAX_POP || map_order > 2) return set_err(VACC_E_ARG, "bad argument");
@@ -1555,85 +1838,10 @@ int vacc_init_peaks(vacc_ctx* c, uint32_t ncells, uint32_t max_peaks, const vacc
       }
       bm_off[i + 1] = bm_off[i] + words;
    }
-   int rc = alloc_cells(c, ncells);
-   if (rc) return rc;
-   HIPCHK(hipStreamSynchronize(c->stream));
-   c->ncells = ncells;
-   c->lcur = 0;
-   c->fcur = 0;
-   c->nblocks = 0;
-   c->sorted = false;
-   c->cell_epoch++;
-   c->content_valid = false;
-   c->moments_valid = false;
-   c->fields_valid = false;
-   if (!ncells) return 0;
-   vacc_peak* d_pk = nullptr;
-   int32_t *d_np = nullptr, *d_bb = nullptr;
-   uint32_t *d_bmo = nullptr, *d_bm = nullptr;
-   HIPCHK(hipMalloc(&d_pk, (size_t)ncells * max_peaks * sizeof(vacc_peak)));
-   HIPCHK(hipMalloc(&d_np, ncells * sizeof(int32_t)));
-   HIPCHK(hipMalloc(&d_bb, (size_t)ncells * 6 * sizeof(int32_t)));
-   HIPCHK(hipMalloc(&d_bmo, (ncells + 1) * sizeof(uint32_t)));
-   HIPCHK(hipMalloc(&d_bm, std::max<size_t>(bm_off[ncells], 1) * sizeof(uint32_t)));
-   HIPCHK(hipMemcpy(d_pk, peaks, (size_t)ncells * max_peaks * sizeof(vacc_peak), hipMemcpyHostToDevice));
-   HIPCHK(hipMemcpy(d_np, npeaks, ncells * sizeof(int32_t), hipMemcpyHostToDevice));
-   HIPCHK(hipMemcpy(d_bb, bbox, (size_t)ncells * 6 * sizeof(int32_t), hipMemcpyHostToDevice));
-   HIPCHK(hipMemcpy(d_bmo, bm_off.data(), (ncells + 1) * sizeof(uint32_t), hipMemcpyHostToDevice));
-   InitArgs ia{};
-   ia.peaks = d_pk;
-   ia.npeaks = d_np;
-   ia.bbox = d_bb;
-   ia.bm_off = d_bmo;
-   ia.content_bm = d_bm;
-   ia.n_keep = c->n_tgt;
-   ia.ncells = ncells;
-   ia.max_peaks = max_peaks;
-   for (int d = 0; d < 3; ++d) {
-      ia.len[d] = c->mi.len[d];
-      ia.vmin[d] = c->mi.vmin[d];
-      ia.cell[d] = c->mi.cell[d];
-   }
-   ia.mass = c->sp.mass;
-   ia.min_value = c->sp.sparse_min_value;
-   ia.add_width = c->sp.sparse_add_width_v;
-   launch_init(ia, false, c->stream);
-   ScanArgs sa{};
-   sa.a = c->n_tgt;
-   sa.a_off = c->off[0];
-   sa.n = ncells;
-   launch_scan(sa, c->stream);
-   HIPCHK(hipGetLastError());
-   rc = refresh_count(c);
-   if (rc) return rc;
-   const size_t total = c->nblocks;
-   c->nblocks = 0;   // nothing to carry over
-   rc = ensure_blocks(c, total + total / 2 + 64 * (size_t)ncells + 4096);
-   if (rc) return rc;
-   ia.out_off = c->off[0];
-   ia.gid_out = c->gid[0];
-   ia.src_out = c->src[0];
-   float* stage = nullptr;
-   if (c->opt.storage == VACC_STORE_F32) {
-      ia.f_out = static_cast<float*>(c->f[0]);
-   } else {
-      HIPCHK(hipMalloc(&stage, std::max<size_t>(total, 1) * WID3 * sizeof(float)));
-      ia.f_out = stage;
-   }
-   launch_init(ia, true, c->stream);
-   if (stage) launch_encode(stage, pool(c, 0), (uint32_t)total, c->stream);
-   HIPCHK(hipGetLastError());
-   HIPCHK(hipMemsetAsync(c->status, 0, (ncells + 1) * sizeof(uint32_t), c->stream));
-   HIPCHK(hipMemsetAsync(c->flags, 0, sizeof(uint32_t), c->stream));
-   HIPCHK(hipStreamSynchronize(c->stream));
-   c->nblocks = total;
-   if (stage) (void)hipFree(stage);
-   (void)hipFree(d_pk);
-   (void)hipFree(d_np);
-   (void)hipFree(d_bb);
-   (void)hipFree(d_bmo);
-   (void)hipFree(d_bm);
-   return 0;
+   c->content_valid = c->moments_valid = c->fields_valid = c->sub_valid = false;
+   const int rc = init_peaks_load(c, ncells, max_peaks, peaks, npeaks, bbox, bm_off);
+   if (rc) abort_load(c);
+   return rc;
 }
 
 int vacc_profile_begin(vacc_ctx* c) {

@@ -17,7 +17,7 @@ struct Pool {
 // share a block), 16 bytes.
 //   x: b0 | b1 << 16   transverse block indices in the pass frame
 //   y: first column (cell-local)
-//   z: number of columns | COPY_FLAG
+//   z: number of columns | COPY_FLAG | EXACT_FLAG
 //   w: source blocks of the unit (its work)
 // Column record, 16 bytes.
 //   x: first block index along the pass axis | length << 16
@@ -25,6 +25,12 @@ struct Pool {
 //   z: first target block | last target block << 16
 //   w: output slot of the first target block (cell-local)
 constexpr uint32_t COPY_FLAG = 0x80000000u;
+// The tolerance-form remap (k_remap_lean) leaves a unit to the exact kernel when one of its
+// columns starts with every lane of an 8-lane group at or below the Lagrangian base (the
+// reference's maximum-lane rule, cpu_acc_map.cpp:515-529, then bounds the deposits by the
+// group's first lane), or when its cell's deposits over a source block could outrun the ring.
+constexpr uint32_t EXACT_FLAG = 0x40000000u;
+constexpr uint32_t UNIT_NCOL_MASK = 0x3FFFFFFFu;
 constexpr int LEN_BINS = 256;   // length bins of the work-unit ordering
 
 struct ColumnArgs {
@@ -44,6 +50,7 @@ struct ColumnArgs {
    float dv;                   // (float) cell size along the pass axis
    float vmin;                 // (float) mesh minimum along the pass axis
    int wall_margin;
+   const uint8_t* active;      // [ncells] or null: cells with 0 pass through unchanged (subcycling)
 };
 
 struct ScanArgs {
@@ -90,6 +97,8 @@ struct RemapArgs {
    float dv, vmin;
    float threshold;             // (float) sparse min value, reconstruction threshold
    double min_value;            // content test threshold
+   double dv_d, vmin_d;         // cell size and mesh minimum along the pass axis (FP64)
+   int only_flagged;            // exact kernel: 1 = take only the units marked EXACT_FLAG
 };
 
 struct AdjustArgs {
@@ -159,7 +168,7 @@ void launch_columns(int dim, const ColumnArgs& a, hipStream_t s);
 void launch_scan(const ScanArgs& a, hipStream_t s);
 void launch_order(const OrderArgs& a, hipStream_t s);
 // grid: grid_mult persistent workgroups per CU
-void launch_remap(int dim, int method, bool fast, const RemapArgs& a, int grid_mult, hipStream_t s);
+void launch_remap(int dim, int method, bool fast, RemapArgs a, int grid_mult, hipStream_t s);
 void launch_adjust(const AdjustArgs& a, bool write_phase, hipStream_t s);
 void launch_moments(const MomentArgs& a, hipStream_t s);
 // plasma pivot from the moments of up to MAX_POP populations holding the same cells
@@ -174,7 +183,18 @@ struct PlasmaArgs {
 };
 void launch_plasma(const PlasmaArgs& a, hipStream_t s);
 // use_moments: 0 fields V and rhoq; 1 this population's moments; 2 plasma pivot buffer
-void launch_transform(const MeshInfo& m, const TransformIn* fields, const double* moments, int use_moments,
+// Subcycles of one acceleration step (vlasovmover.cpp:398-456): per cell max_v_dt from the
+// resident fields' B (cpu_acc_transform.cpp:41-55) and the subcycle count
+// max(ceil(|dt| / max_v_dt), 1); the context's maximum goes to *nmax (atomicMax).
+void launch_subcycle_count(const TransformIn* fields, uint32_t ncells, double mass, double charge,
+                           double max_rotation_deg, double dt, double* maxdt, uint32_t* nsub, uint32_t* nmax,
+                           hipStream_t s);
+// Subcycle `step`: active[c] = step < nsub[c]; subdt[c] = max_v_dt, or what is left of |dt| on
+// the cell's last subcycle, with the sign of dt (0 for inactive cells).
+void launch_subcycle_step(const double* maxdt, const uint32_t* nsub, uint32_t ncells, uint32_t step, double dt,
+                          double* subdt, uint8_t* active, hipStream_t s);
+// dt_override: [ncells] or null; replaces fields[c].dt.
+void launch_transform(const MeshInfo& m, const TransformIn* fields, const double* dt_override, const double* moments, int use_moments,
                       double mass, double charge, double hall_min_rhoq, int ohm_gradpe, int map_order,
                       double* isect_out, uint32_t ncells, hipStream_t s);
 void launch_gather(const uint32_t* src, const Pool& in, const Pool& out, uint32_t n, hipStream_t s);

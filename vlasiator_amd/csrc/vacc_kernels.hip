@@ -22,6 +22,7 @@
 // rounded.  The file is compiled with -ffp-contract=off.
 #include "vacc_kernels.h"
 #include "vacc_recon.h"
+#include "vacc_lean.h"
 #include "vacc_codec.h"
 
 #include <float.h>
@@ -195,6 +196,29 @@ __device__ inline void column_extent(int ks, int ke, const SetGeom& g, float dv,
    wallhit = (fk < wall) || (fk >= maxlen - wall) || (lk < wall) || (lk >= maxlen - wall);
 }
 
+// The reference's lanes of minimum and maximum Lagrangian index (cpu_acc_map.cpp:515-529)
+// start from DBL_MIN, so a group of 8 lanes whose Lagrangian indices at the column start are
+// all <= 0 takes its first lane as the maximum.  True when that happens to either group of the
+// column's 16 pencils (the expressions are k_remap's, so the signs agree exactly).
+__device__ inline bool max_lane_rule_binds(int b0, int b1, int ks, float I, float di, float dj, float dv, float vmin) {
+   const float vr = (float)(uint32_t)(WID * ks) * dv + vmin;
+   bool none_pos[2] = {true, true};
+#pragma unroll
+   for (int jp = 0; jp < WID; ++jp)
+#pragma unroll
+      for (int ip = 0; ip < WID; ++ip) {
+         const float imin = (I + (float)(uint32_t)(b0 * WID + ip) * di) + (float)(uint32_t)(b1 * WID + jp) * dj;
+         if (vr - imin > 0.0f) none_pos[jp >> 1] = false;
+      }
+   return none_pos[0] || none_pos[1];
+}
+
+// Deposits of one source block (4 cells) in the tolerance-form remap reach at most
+// 4 dv/dk + 2 cells past a lane's open target, and the 16 lanes of a unit are spread over
+// (3|di| + 3|dj|)/dk + 1 cells; with block alignment the ring of RING cells holds them when
+// (4 dv + 3|di| + 3|dj|)/dk <= LEAN_SPAN.
+constexpr float LEAN_SPAN = 20.0f;
+
 // ---------------------------------------------------------------------------
 // k_columns
 // ---------------------------------------------------------------------------
@@ -249,9 +273,12 @@ __global__ __launch_bounds__(CWG) void k_columns(ColumnArgs a) {
    const int rpc = min(ROW_MAX, BM_WORDS / W);
    float I, di, dj, dk;
    pass_isect<DIM>(a.isect + 12 * (size_t)c, I, di, dj, dk);
-   const bool copy = pass_is_copy(a.dv, di, dj, dk);
+   // a cell outside this subcycle passes through unchanged (no status)
+   const bool idle = a.active && !a.active[c];
+   const bool copy = idle || pass_is_copy(a.dv, di, dj, dk);
+   const bool lean_wide = !copy && !((4.0f * a.dv + 3.0f * fabsf(di) + 3.0f * fabsf(dj)) / dk <= LEAN_SPAN);
    const int maxlen = (int)a.len[DIM];
-   uint32_t mystat = copy ? VACC_STATUS_DK : 0u;
+   uint32_t mystat = (copy && !idle) ? VACC_STATUS_DK : 0u;
 
    for (int r0 = 0; r0 < rows; r0 += rpc) {
       const int nr = min(rpc, rows - r0);
@@ -368,15 +395,21 @@ __global__ __launch_bounds__(CWG) void k_columns(ColumnArgs a) {
             ++ncol;
             p = e + 1;
          }
-         // work units: {b0 | b1 << 16, first column, columns | COPY_FLAG, source blocks}
+         // work units: {b0 | b1 << 16, first column, columns | COPY_FLAG | EXACT_FLAG, source blocks}
          const uint32_t bb = (uint32_t)b0 | ((uint32_t)b1 << 16), cf = copy ? COPY_FLAG : 0u;
+         auto exact_col = [&](uint32_t j) {
+            return !copy && max_lane_rule_binds(b0, b1, (int)(a.col_rec[off + cb + j].x & 0xFFFFu), I, di, dj, a.dv, a.vmin);
+         };
          if (overlap) {
-            a.unit_rec[off + ui] = make_uint4(bb, cb, ncol | cf, bpos);
+            bool ex = lean_wide;
+            for (uint32_t j = 0; j < ncol && !ex; ++j) ex = exact_col(j);
+            a.unit_rec[off + ui] = make_uint4(bb, cb, ncol | cf | (ex ? EXACT_FLAG : 0u), bpos);
             atomicAdd(&hist[len_bin(bpos)], 1u);
          } else {
             for (uint32_t j = 0; j < ncol; ++j) {
                const uint32_t len = a.col_rec[off + cb + j].x >> 16;
-               a.unit_rec[off + ui + j] = make_uint4(bb, cb + j, 1u | cf, len);
+               const uint32_t ef = (lean_wide || exact_col(j)) ? EXACT_FLAG : 0u;
+               a.unit_rec[off + ui + j] = make_uint4(bb, cb + j, 1u | cf | ef, len);
                atomicAdd(&hist[len_bin(len)], 1u);
             }
          }
@@ -625,7 +658,7 @@ __device__ inline void copy_block(const RemapArgs& a, int l16, int setlane0, int
    }
 }
 
-template <int DIM, int METHOD, bool FAST, int ST>
+template <int DIM, int METHOD, int ST>
 __global__ __launch_bounds__(WG) void k_remap(RemapArgs a, double i_dv) {
    __shared__ float ring[RING][WG];
    if (*a.flags & VACC_STATUS_CAPACITY) return;
@@ -636,7 +669,6 @@ __global__ __launch_bounds__(WG) void k_remap(RemapArgs a, double i_dv) {
    const int o = (DIM == 0) ? (ip * WID2 + jp * WID) : (DIM == 1) ? (jp * WID2 + ip) : (jp * WID + ip);
    const float thr = a.threshold;
    const float scale = (float)(1.0 / (double)thr);
-   const float i_dv_f = (float)i_dv;
    const double mv = a.min_value;
    const uint32_t total = a.unit_base[a.ncells];
    const uint32_t gstride = gridDim.x * (WG / 16);
@@ -647,7 +679,8 @@ __global__ __launch_bounds__(WG) void k_remap(RemapArgs a, double i_dv) {
       const uint4 sr = a.unit_rec[cbase + (g - a.unit_base[c])];
       const int b0 = (int)(sr.x & 0xFFFFu), b1 = (int)(sr.x >> 16);
       const uint32_t col0 = sr.y;
-      const uint32_t ncol = sr.z & ~COPY_FLAG;
+      if (a.only_flagged && !(sr.z & EXACT_FLAG)) continue;   // the tolerance form took it
+      const uint32_t ncol = sr.z & UNIT_NCOL_MASK;
       const bool copy = (sr.z & COPY_FLAG) != 0u;
       const uint32_t obase = a.out_off[c];
       float I, di, dj, dk;
@@ -699,7 +732,7 @@ __global__ __launch_bounds__(WG) void k_remap(RemapArgs a, double i_dv) {
          int next_flush = fk;
          int next_open = (carry_open >= 0) ? carry_open : fk;
          const int lo_gk = WID * fk, hi_gk = WID * lk + WID - 1;
-         typename std::conditional<FAST, ReconFast<METHOD>, Recon<METHOD>>::type rc;
+         Recon<METHOD> rc;
          {
             const float v0[12] = {w0[0], w0[1], w0[2], w0[3], w1[0], w1[1], w1[2], w1[3], w2[0], w2[1], w2[2], w2[3]};
             rc.start(v0 + 4, scale);
@@ -744,12 +777,8 @@ __global__ __launch_bounds__(WG) void k_remap(RemapArgs a, double i_dv) {
                   const float mx = fmaxf(y, v_l);
                   const float mn = fminf(mx, v_r);
                   const float tdl = tdr;
-                  if (FAST) {
-                     tdr = integral_fast<METHOD>((mn - v_l) * i_dv_f, cf);
-                  } else {
-                     const float vn = (float)((double)(mn - v_l) * i_dv);
-                     tdr = integral<METHOD>(vn, cf);
-                  }
+                  const float vn = (float)((double)(mn - v_l) * i_dv);
+                  tdr = integral<METHOD>(vn, cf);
                   float& r = ring[gk & (RING - 1)][tid];
                   r = r + (tdr - tdl);
                }
@@ -761,6 +790,156 @@ __global__ __launch_bounds__(WG) void k_remap(RemapArgs a, double i_dv) {
                w2[e] = w3[e];
             }
          }
+         while (next_flush < next_open && next_flush < nfk) {
+            flush_block<DIM, ST>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), mv);
+            ++next_flush;
+         }
+         for (int tb = next_open; tb <= lk && tb < nfk; ++tb) zero_block<DIM, ST>(a, l16, setlane0, o, slotb + (tb - fk), mv);
+         carry_open = (next_open > nfk) ? next_open : -1;
+         prev_end = max(prev_end, lk);
+      }
+   }
+}
+
+// ---------------------------------------------------------------------------
+// k_remap_lean: the tolerance form of the remap (options.fast_arith = 1)
+// ---------------------------------------------------------------------------
+// The work units, column records, target ring and block flushes of k_remap, with the
+// arithmetic of vacc_lean.h per source cell: reconstructions in deviations from the cell
+// average and the flux-form deposit (one integral per Lagrangian boundary, the cell average
+// for the rest of the cell), from the FP64 intersections.  Ring opening and block flushing
+// run once per source block on unit-wide bounds (DPP row reductions: a unit is one 16-lane
+// row).  Units marked EXACT_FLAG are left to k_remap (only_flagged = 1).
+
+template <int DIM>
+__device__ inline void pass_isect_d(const double* is, double& I, double& di, double& dj, double& dk) {
+   I = is[4 * DIM + 0];
+   const double a = is[4 * DIM + 1], b = is[4 * DIM + 2], c = is[4 * DIM + 3];
+   if (DIM == 0) { di = c; dj = b; dk = a; }
+   else if (DIM == 1) { di = a; dj = c; dk = b; }
+   else { di = a; dj = b; dk = c; }
+}
+
+// maximum / minimum over the 16 lanes of the caller's DPP row, in every lane of the row
+__device__ inline int row_max16(int x) {
+   x = max(x, __builtin_amdgcn_mov_dpp(x, 0x128, 0xF, 0xF, false));   // row_ror:8
+   x = max(x, __builtin_amdgcn_mov_dpp(x, 0x124, 0xF, 0xF, false));   // row_ror:4
+   x = max(x, __builtin_amdgcn_mov_dpp(x, 0x122, 0xF, 0xF, false));   // row_ror:2
+   return max(x, __builtin_amdgcn_mov_dpp(x, 0x121, 0xF, 0xF, false));   // row_ror:1
+}
+__device__ inline int row_min16(int x) {
+   x = min(x, __builtin_amdgcn_mov_dpp(x, 0x128, 0xF, 0xF, false));
+   x = min(x, __builtin_amdgcn_mov_dpp(x, 0x124, 0xF, 0xF, false));
+   x = min(x, __builtin_amdgcn_mov_dpp(x, 0x122, 0xF, 0xF, false));
+   return min(x, __builtin_amdgcn_mov_dpp(x, 0x121, 0xF, 0xF, false));
+}
+
+template <int DIM, int METHOD, int ST>
+__global__ __launch_bounds__(WG) void k_remap_lean(RemapArgs a) {
+   using R = ReconLean<METHOD>;
+   constexpr int NC = R::NC;
+   __shared__ float ring[RING][WG];
+   if (*a.flags & VACC_STATUS_CAPACITY) return;
+   const int tid = threadIdx.x;
+   const int l16 = tid & 15;
+   const int ip = l16 & 3, jp = l16 >> 2;
+   const int setlane0 = (tid & 63) & ~15;
+   const int o = (DIM == 0) ? (ip * WID2 + jp * WID) : (DIM == 1) ? (jp * WID2 + ip) : (jp * WID + ip);
+   const double mv = a.min_value;
+   const uint32_t total = a.unit_base[a.ncells];
+   const uint32_t gstride = gridDim.x * (WG / 16);
+   for (uint32_t gi = blockIdx.x * (WG / 16) + (tid >> 4); gi < total; gi += gstride) {
+      const uint32_t g = a.perm[gi];
+      const uint32_t c = a.perm_cell[gi];
+      const uint32_t cbase = a.cell_off[c];
+      const uint4 sr = a.unit_rec[cbase + (g - a.unit_base[c])];
+      if (sr.z & EXACT_FLAG) continue;   // taken by k_remap
+      const int b0 = (int)(sr.x & 0xFFFFu), b1 = (int)(sr.x >> 16);
+      const uint32_t col0 = sr.y;
+      const uint32_t ncol = sr.z & UNIT_NCOL_MASK;
+      const bool copy = (sr.z & COPY_FLAG) != 0u;
+      const uint32_t obase = a.out_off[c];
+      double I, di, dj, dk;
+      pass_isect_d<DIM>(a.isect + 12 * (size_t)c, I, di, dj, dk);
+      // Lagrangian base of this lane's pencil and the boundary spacing in cells
+      const double imin = I + (double)(b0 * WID + ip) * di + (double)(b1 * WID + jp) * dj;
+      const double rdk = 1.0 / dk;
+      const float r = (float)(dk / a.dv_d), inv_r = (float)(a.dv_d * rdk);
+      int prev_end = -1;
+      int carry_open = -1;
+      for (uint32_t ci = 0; ci < ncol; ++ci) {
+         const uint4 cr = a.col_rec[cbase + col0 + ci];
+         const int ks = (int)(cr.x & 0xFFFFu), len = (int)(cr.x >> 16);
+         const int fk = (int)(cr.z & 0xFFFFu), lk = (int)(cr.z >> 16);
+         const int nfk = (ci + 1 < ncol) ? (int)(a.col_rec[cbase + col0 + ci + 1].z & 0xFFFFu) : INT_MAX;
+         const uint32_t* srcs = a.sorted_src + cbase + cr.y;
+         const size_t slotb = (size_t)obase + cr.w;   // output slot of block fk
+         for (int tb = max(fk, prev_end + 1) + l16; tb <= lk; tb += 16) {
+            const size_t slot = slotb + (size_t)(tb - fk);
+            a.gid_out[slot] = pass_gid<DIM>(b0, b1, tb, a.len);
+            a.src_out[slot] = (uint32_t)slot;
+         }
+         if (copy) {
+            for (int b = 0; b < len; ++b) copy_block<DIM, ST>(a, l16, setlane0, o, srcs[b], slotb + b, mv);
+            prev_end = max(prev_end, lk);
+            continue;
+         }
+         const int lo = WID * fk, hi = WID * lk + WID - 1;
+         auto emit = [&](int gt, float m) {
+            if (gt >= lo && gt <= hi) atomicAdd(&ring[gt & (RING - 1)][tid], m);
+         };
+         Flux<NC> fx;
+         fx.begin((a.vmin_d + (double)(WID * ks) * a.dv_d - imin) * rdk, r);
+         float w0[4] = {0.0f, 0.0f, 0.0f, 0.0f}, w1[4], w2[4];
+         load_pencil<DIM, ST>(a.in, srcs[0], o, w1);
+         if (len > 1) load_pencil<DIM, ST>(a.in, srcs[1], o, w2);
+         else w2[0] = w2[1] = w2[2] = w2[3] = 0.0f;
+         int next_flush = fk;
+         int next_open = (carry_open >= 0) ? carry_open : fk;
+         R rc;
+         {
+            const float v0[12] = {w0[0], w0[1], w0[2], w0[3], w1[0], w1[1], w1[2], w1[3], w2[0], w2[1], w2[2], w2[3]};
+            rc.start(v0 + 4);
+         }
+         for (int b = 0; b < len; ++b) {
+            float w3[4];
+            if (b + 2 < len) load_pencil<DIM, ST>(a.in, srcs[b + 2], o, w3);
+            else w3[0] = w3[1] = w3[2] = w3[3] = 0.0f;
+            // targets the block's 4 cells can reach (one cell of margin for rounding), and the
+            // lowest target any lane still accumulates
+            const int reach = fx.g + ((fx.t < 4.0f) ? (int)((4.0f - fx.t) * inv_r) + 1 : 0);
+            const int openLim = row_max16(min(reach, hi));
+            const int flushLim = row_min16(fx.g - 1);
+            while (next_open <= lk && WID * next_open <= openLim) {
+               if (next_open - next_flush >= RING / WID) {   // ring full: not for LEAN_SPAN cells
+                  atomicOr(&a.status[c], VACC_STATUS_DK);
+                  atomicOr(&a.status[a.ncells], VACC_STATUS_DK);
+                  flush_block<DIM, ST>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), mv);
+                  ++next_flush;
+               }
+#pragma unroll
+               for (int e = 0; e < 4; ++e) ring[(WID * next_open + e) & (RING - 1)][tid] = 0.0f;
+               ++next_open;
+            }
+            while (next_flush < next_open && next_flush < nfk && WID * next_flush + WID - 1 < flushLim) {
+               flush_block<DIM, ST>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), mv);
+               ++next_flush;
+            }
+            const float v[12] = {w0[0], w0[1], w0[2], w0[3], w1[0], w1[1], w1[2], w1[3], w2[0], w2[1], w2[2], w2[3]};
+#pragma unroll
+            for (int kk = 0; kk < WID; ++kk) {
+               float cf[NC];
+               rc.coeffs(v + 4 + kk, cf);
+               fx.cell(cf, v[4 + kk], emit);
+            }
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+               w0[e] = w1[e];
+               w1[e] = w2[e];
+               w2[e] = w3[e];
+            }
+         }
+         fx.end(emit);
          while (next_flush < next_open && next_flush < nfk) {
             flush_block<DIM, ST>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), mv);
             ++next_flush;
@@ -1135,12 +1314,13 @@ __global__ __launch_bounds__(WG) void k_moments(MomentArgs a) {
    }
 }
 
-__global__ void k_transform(MeshInfo m, const TransformIn* fields, const double* moments, int use_moments,
-                            double mass, double charge, double hall_min_rhoq, int ohm_gradpe, int map_order,
-                            double* isect_out, uint32_t ncells) {
+__global__ void k_transform(MeshInfo m, const TransformIn* fields, const double* dt_override, const double* moments,
+                            int use_moments, double mass, double charge, double hall_min_rhoq, int ohm_gradpe,
+                            int map_order, double* isect_out, uint32_t ncells) {
    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
    if (c >= ncells) return;
    TransformIn in = fields[c];
+   if (dt_override) in.dt = dt_override[c];
    if (use_moments == 1) {
       // one population: VX_V = (nVx * m) / (n * m), RHOQ_V = n * q (arch_moments.cpp:419-440)
       const double* mo = moments + 4 * (size_t)c;
@@ -1274,20 +1454,26 @@ int device_cus() {
    return g_cus;
 }
 
+template <int DIM, int METHOD, int ST>
+static void remap_dms(RemapArgs a, uint32_t grid, double i_dv, bool fast, hipStream_t s) {
+   if (!fast) {
+      a.only_flagged = 0;
+      hipLaunchKernelGGL((k_remap<DIM, METHOD, ST>), dim3(grid), dim3(WG), 0, s, a, i_dv);
+      return;
+   }
+   // tolerance form: the lean kernel, then the exact kernel over the few units it leaves
+   hipLaunchKernelGGL((k_remap_lean<DIM, METHOD, ST>), dim3(grid), dim3(WG), 0, s, a);
+   a.only_flagged = 1;
+   hipLaunchKernelGGL((k_remap<DIM, METHOD, ST>), dim3((uint32_t)device_cus()), dim3(WG), 0, s, a, i_dv);
+}
+
 template <int DIM, int METHOD>
 static void remap_dm(const RemapArgs& a, uint32_t grid, double i_dv, bool fast, hipStream_t s) {
    // the source and target pools share one storage form
    const int st = a.in.storage;
-   if (st == ST_F32) {
-      if (fast) hipLaunchKernelGGL((k_remap<DIM, METHOD, true, ST_F32>), dim3(grid), dim3(WG), 0, s, a, i_dv);
-      else hipLaunchKernelGGL((k_remap<DIM, METHOD, false, ST_F32>), dim3(grid), dim3(WG), 0, s, a, i_dv);
-   } else if (st == ST_BF16) {
-      if (fast) hipLaunchKernelGGL((k_remap<DIM, METHOD, true, ST_BF16>), dim3(grid), dim3(WG), 0, s, a, i_dv);
-      else hipLaunchKernelGGL((k_remap<DIM, METHOD, false, ST_BF16>), dim3(grid), dim3(WG), 0, s, a, i_dv);
-   } else {
-      if (fast) hipLaunchKernelGGL((k_remap<DIM, METHOD, true, ST_F16>), dim3(grid), dim3(WG), 0, s, a, i_dv);
-      else hipLaunchKernelGGL((k_remap<DIM, METHOD, false, ST_F16>), dim3(grid), dim3(WG), 0, s, a, i_dv);
-   }
+   if (st == ST_F32) remap_dms<DIM, METHOD, ST_F32>(a, grid, i_dv, fast, s);
+   else if (st == ST_BF16) remap_dms<DIM, METHOD, ST_BF16>(a, grid, i_dv, fast, s);
+   else remap_dms<DIM, METHOD, ST_F16>(a, grid, i_dv, fast, s);
 }
 
 template <int DIM>
@@ -1297,7 +1483,7 @@ static void remap_d(int method, const RemapArgs& a, uint32_t grid, double i_dv, 
    else remap_dm<DIM, VACC_PQM>(a, grid, i_dv, fast, s);
 }
 
-void launch_remap(int dim, int method, bool fast, const RemapArgs& a, int grid_mult, hipStream_t s) {
+void launch_remap(int dim, int method, bool fast, RemapArgs a, int grid_mult, hipStream_t s) {
    if (a.ncells == 0) return;
    const uint32_t grid = (uint32_t)device_cus() * (uint32_t)std::max(grid_mult, 1);
    const double i_dv = 1.0 / (double)a.dv;   // cpu_acc_map.cpp:201, Real i_dv = 1.0/dv
@@ -1329,12 +1515,56 @@ void launch_moments(const MomentArgs& a, hipStream_t s) {
    hipLaunchKernelGGL(k_moments, dim3(a.ncells), dim3(WG), 0, s, a);
 }
 
-void launch_transform(const MeshInfo& m, const TransformIn* fields, const double* moments, int use_moments,
-                      double mass, double charge, double hall_min_rhoq, int ohm_gradpe, int map_order,
+void launch_transform(const MeshInfo& m, const TransformIn* fields, const double* dt_override, const double* moments,
+                      int use_moments, double mass, double charge, double hall_min_rhoq, int ohm_gradpe, int map_order,
                       double* isect_out, uint32_t ncells, hipStream_t s) {
    if (ncells == 0) return;
-   hipLaunchKernelGGL(k_transform, dim3((ncells + 127) / 128), dim3(128), 0, s, m, fields, moments, use_moments,
-                      mass, charge, hall_min_rhoq, ohm_gradpe, map_order, isect_out, ncells);
+   hipLaunchKernelGGL(k_transform, dim3((ncells + 127) / 128), dim3(128), 0, s, m, fields, dt_override, moments,
+                      use_moments, mass, charge, hall_min_rhoq, ohm_gradpe, map_order, isect_out, ncells);
+}
+
+// max_v_dt per cell (cpu_acc_transform.cpp:41-55) and subcycles (vlasovmover.cpp:398-401)
+__global__ void k_subcycle_count(const TransformIn* fields, uint32_t ncells, double mass, double charge,
+                                 double max_rotation_deg, double dt, double* maxdt, uint32_t* nsub, uint32_t* nmax) {
+   const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+   if (c >= ncells) return;
+   const double* B = fields[c].B;
+   const double B_mag = sqrt(B[0] * B[0] + B[1] * B[1] + B[2] * B[2]) + 1e-30;
+   const double gyro_period = 2 * PI * mass / (charge * B_mag);
+   const double m = fabs(gyro_period) * (max_rotation_deg / 360.0);
+   const double q = ceil(fabs(dt) / m);
+   const uint32_t n = (q >= 1.0) ? (q < 4294967295.0 ? (uint32_t)q : 0xFFFFFFFFu) : 1u;
+   maxdt[c] = m;
+   nsub[c] = n;
+   atomicMax(nmax, n);
+}
+
+// subcycle dt of step `step` (vlasovmover.cpp:429-456)
+__global__ void k_subcycle_step(const double* maxdt, const uint32_t* nsub, uint32_t ncells, uint32_t step, double dt,
+                                double* subdt, uint8_t* active) {
+   const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+   if (c >= ncells) return;
+   const double m = maxdt[c];
+   double d = ((double)(step + 1) * m > fabs(dt)) ? fmax(fabs(dt) - (double)step * m, 0.0) : m;
+   if (dt < 0) d = -d;
+   const bool on = step < nsub[c];
+   active[c] = on ? 1 : 0;
+   subdt[c] = on ? d : 0.0;
+}
+
+void launch_subcycle_count(const TransformIn* fields, uint32_t ncells, double mass, double charge,
+                           double max_rotation_deg, double dt, double* maxdt, uint32_t* nsub, uint32_t* nmax,
+                           hipStream_t s) {
+   if (ncells == 0) return;
+   hipLaunchKernelGGL(k_subcycle_count, dim3((ncells + 127) / 128), dim3(128), 0, s, fields, ncells, mass, charge,
+                      max_rotation_deg, dt, maxdt, nsub, nmax);
+}
+
+void launch_subcycle_step(const double* maxdt, const uint32_t* nsub, uint32_t ncells, uint32_t step, double dt,
+                          double* subdt, uint8_t* active, hipStream_t s) {
+   if (ncells == 0) return;
+   hipLaunchKernelGGL(k_subcycle_step, dim3((ncells + 127) / 128), dim3(128), 0, s, maxdt, nsub, ncells, step, dt,
+                      subdt, active);
 }
 
 void launch_gather(const uint32_t* src, const Pool& in, const Pool& out, uint32_t n, hipStream_t s) {

@@ -263,203 +263,4 @@ VACC_HD float integral(float x, const float* c) {
 }
 
 
-// ---------------------------------------------------------------------------
-// Fast variants (options.fast_arith = 1): the same reconstructions with fused multiply-adds,
-// reciprocal multiplies instead of IEEE divisions and float instead of double literals.
-// Results differ from the reference's rounding by a few ulp of the coefficients; the block
-// bookkeeping (extents, Lagrangian cell indices) is unchanged.
-// ---------------------------------------------------------------------------
-
-// hardware reciprocal / square root (about 1 ulp) on the device; plain IEEE on the host
-VACC_HD float fast_rcp(float x) {
-#if defined(__HIP_DEVICE_COMPILE__)
-   return __builtin_amdgcn_rcpf(x);
-#else
-   return 1.0f / x;
-#endif
-}
-VACC_HD float fast_sqrt(float x) {
-#if defined(__HIP_DEVICE_COMPILE__)
-   return __builtin_amdgcn_sqrtf(x);
-#else
-   return sqrtf(x);
-#endif
-}
-
-VACC_HD float face_h4_fast(const float* v) {
-   return (fmaf(7.0f, v[-1] + v[0], -v[-2]) - v[1]) * (1.0f / 12.0f);
-}
-VACC_HD float face_h8_fast(const float* v) {
-   float s = fmaf(533.0f, v[-1] + v[0], -3.0f * (v[-4] + v[3]));
-   s = fmaf(29.0f, v[-3] + v[2], s);
-   s = fmaf(-139.0f, v[-2] + v[1], s);
-   return s * (1.0f / 840.0f);
-}
-VACC_HD float deriv_h7_fast(const float* v) {
-   float s = fmaf(7175.0f, v[0] - v[-1], 9.0f * (v[-4] - v[3]));
-   s = fmaf(119.0f, v[2] - v[-3], s);
-   s = fmaf(889.0f, v[-2] - v[1], s);
-   return s * (1.0f / 5040.0f);
-}
-
-template <int METHOD>
-struct ReconFast;
-
-template <>
-struct ReconFast<VACC_PLM> {
-   static constexpr int NC = 2;
-   float s0, b;
-   VACC_HD void start(const float* v, float scale) {
-      s0 = v[0] * scale;
-      b = s0 - v[-1] * scale;
-   }
-   VACC_HD void coeffs(const float* v, float thr, float scale, float* c) {
-      const float s1 = v[1] * scale;
-      const float a = s1 - s0;
-      const float d = superbee_ab(a, b) * thr;
-      c[0] = fmaf(-0.5f, d, v[0]);
-      c[1] = 0.5f * d;
-      s0 = s1;
-      b = a;
-   }
-};
-
-template <>
-struct ReconFast<VACC_PPM> {
-   static constexpr int NC = 3;
-   float fl_raw, s0, b;
-   VACC_HD void start(const float* v, float scale) {
-      fl_raw = face_h4_fast(v);
-      s0 = v[0] * scale;
-      b = s0 - v[-1] * scale;
-   }
-   VACC_HD void coeffs(const float* v, float thr, float scale, float* c) {
-      const float fr_raw = face_h4_fast(v + 1);
-      const float s1 = v[1] * scale;
-      const float a = s1 - s0;
-      const float sl = superbee_ab(a, b);
-      const float sabs = fabsf(sl) * thr;
-      const float hs = (sl > 0.0f) ? 0.5f * sabs : -0.5f * sabs;
-      const float vk = v[0];
-      float m = fl_raw, p = fr_raw;
-      fl_raw = fr_raw;
-      s0 = s1;
-      b = a;
-      if (sabs == 0.0f) {
-         p = vk;
-         m = vk;
-      }
-      if ((v[-1] - m) * (m - vk) < 0.0f) m = vk - hs;
-      if ((v[1] - p) * (p - vk) < 0.0f) p = vk + hs;
-      const float one_sixth = 1.0f / 6.0f;
-      {
-         const float d = p - m;
-         const float mid = fmaf(-0.5f, m + p, vk);
-         if (d * mid > (d * d) * one_sixth) m = fmaf(3.0f, vk, -2.0f * p);
-      }
-      {
-         const float d = p - m;
-         const float mid = fmaf(-0.5f, m + p, vk);
-         if (-(d * d) * one_sixth > d * mid) p = fmaf(3.0f, vk, -2.0f * m);
-      }
-      c[0] = m;
-      c[1] = fmaf(3.0f, vk, -fmaf(2.0f, m, p));
-      c[2] = fmaf(-2.0f, vk, m + p);
-   }
-};
-
-template <>
-struct ReconFast<VACC_PQM> {
-   static constexpr int NC = 5;
-   float fvl_raw, fdl_raw, s0, b;
-   VACC_HD void start(const float* v, float scale) {
-      fvl_raw = face_h8_fast(v);
-      fdl_raw = deriv_h7_fast(v);
-      s0 = v[0] * scale;
-      b = s0 - v[-1] * scale;
-   }
-   VACC_HD void coeffs(const float* v, float thr, float scale, float* c) {
-      const float fvr_raw = face_h8_fast(v + 1), fdr_raw = deriv_h7_fast(v + 1);
-      const float s1 = v[1] * scale;
-      const float a = s1 - s0;
-      const float sl = superbee_ab(a, b);
-      const float sabs = fabsf(sl) * thr;
-      const float ssgn = (sl > 0.0f) ? 1.0f : -1.0f;
-      const float vk = v[0];
-      float fvl = fvl_raw, fvr = fvr_raw, fdl = fdl_raw, fdr = fdr_raw;
-      fvl_raw = fvr_raw;
-      fdl_raw = fdr_raw;
-      s0 = s1;
-      b = a;
-      if (sabs == 0.0f) {
-         fvr = vk;
-         fvl = vk;
-         fdl = 0.0f;
-         fdr = 0.0f;
-      }
-      const float hs = 0.5f * ssgn * sabs, ds = ssgn * sabs;
-      if ((v[-1] - fvl) * (fvl - vk) < 0.0f || ssgn * fdl < 0.0f) {
-         fvl = vk - hs;
-         fdl = ds;
-      }
-      if ((v[1] - fvr) * (fvr - vk) < 0.0f || ssgn * fdr < 0.0f) {
-         fvr = vk + hs;
-         fdr = ds;
-      }
-      const float b0 = fmaf(60.0f, vk, fmaf(-24.0f, fvr, fmaf(-36.0f, fvl, fmaf(3.0f, fdr, -9.0f * fdl))));
-      const float b1 = fmaf(-360.0f, vk, fmaf(36.0f, fdl, fmaf(-24.0f, fdr, fmaf(168.0f, fvr, 192.0f * fvl))));
-      const float b2 = fmaf(360.0f, vk, fmaf(30.0f, fdr - fdl, -180.0f * (fvl + fvr)));
-      const float disc = fmaf(b1, b1, -4.0f * b0 * b2);
-      const float sq = (disc < 0.0f) ? fmaf(200.0f, b2, b1) : fast_sqrt(disc);
-      const float r2b = 0.5f * fast_rcp(b2);
-      const float root1 = (sq - b1) * r2b;
-      const float root2 = -(b1 + sq) * r2b;
-      const float psl = 2.0f * (vk - v[-1]);
-      const float psr = 2.0f * (v[1] - vk);
-      const float ssg = psl + psr;
-      const float c2 = 0.5f * b1, c3 = b2 * (1.0f / 3.0f);
-      const float r1s = (root1 >= 0.0f && root1 <= 1.0f) ? fmaf(root1, fmaf(root1, fmaf(root1, c3, c2), b0), fdl) : ssg;
-      const float r2s = (root2 >= 0.0f && root2 <= 1.0f) ? fmaf(root2, fmaf(root2, fmaf(root2, c3, c2), b0), fdl) : ssg;
-      if (r1s * ssg < 0.0f || r2s * ssg < 0.0f) {
-         if (fabsf(psl) <= fabsf(psr)) {
-            fdl = (10.0f * vk - 2.0f * fvr - 8.0f * fvl) * (1.0f / 3.0f);
-            fdr = fmaf(-10.0f, vk, fmaf(6.0f, fvr, 4.0f * fvl));
-            if (ssg * fdl < 0.0f) {
-               fdl = 0.0f;
-               fvr = fmaf(5.0f, vk, -4.0f * fvl);
-               fdr = 20.0f * (vk - fvl);
-            } else if (ssg * fdr < 0.0f) {
-               fdr = 0.0f;
-               fvl = 0.5f * fmaf(5.0f, vk, -3.0f * fvr);
-               fdl = (10.0f / 3.0f) * (fvr - vk);
-            }
-         } else {
-            fdl = fmaf(10.0f, vk, fmaf(-6.0f, fvl, -4.0f * fvr));
-            fdr = (-10.0f * vk + 2.0f * fvl + 8.0f * fvr) * (1.0f / 3.0f);
-            if (ssg * fdl < 0.0f) {
-               fdl = 0.0f;
-               fvr = 0.5f * fmaf(5.0f, vk, -3.0f * fvl);
-               fdr = (10.0f / 3.0f) * (vk - fvl);
-            } else if (ssg * fdr < 0.0f) {
-               fdr = 0.0f;
-               fvl = fmaf(5.0f, vk, -4.0f * fvr);
-               fdl = 20.0f * (fvr - vk);
-            }
-         }
-      }
-      c[0] = fvl;
-      c[1] = 0.5f * fdl;
-      c[2] = fmaf(10.0f, vk, fmaf(-4.0f, fvr, fmaf(-6.0f, fvl, 0.5f * fmaf(-3.0f, fdl, fdr))));
-      c[3] = fmaf(-15.0f, vk, fmaf(1.5f, fdl, fmaf(7.0f, fvr, fmaf(8.0f, fvl, -fdr))));
-      c[4] = fmaf(6.0f, vk, fmaf(0.5f, fdr - fdl, -3.0f * (fvl + fvr)));
-   }
-};
-
-template <int METHOD>
-VACC_HD float integral_fast(float x, const float* c) {
-   if (METHOD == VACC_PLM) return x * fmaf(x, c[1], c[0]);
-   if (METHOD == VACC_PPM) return x * fmaf(x, fmaf(x, c[2], c[1]), c[0]);
-   return x * fmaf(x, fmaf(x, fmaf(x, fmaf(x, c[4], c[3]), c[2]), c[1]), c[0]);
-}
-
 }  // namespace vacc

@@ -27,6 +27,7 @@ EXPORTS = [
     "vacc_accelerate_cells", "vacc_adjust_blocks", "vacc_moments", "vacc_sync", "vacc_get_status",
     "vacc_init_peaks", "vacc_profile_begin", "vacc_profile_read", "vacc_set_fields", "vacc_step",
     "vacc_accelerate_populations", "vacc_adjust_blocks_spatial", "vacc_autotune",
+    "vacc_subcycle_count", "vacc_subcycle_step", "vacc_accelerate_subcycled",
 ]
 
 
@@ -107,6 +108,9 @@ def lib():
         L.vacc_accelerate_populations.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_int32, C.c_int32, vp]
         L.vacc_adjust_blocks_spatial.argtypes = [vp, vp, vp, vp, vp, vp]
         L.vacc_autotune.argtypes = [vp, vp]
+        L.vacc_subcycle_count.argtypes = [vp, C.c_double, C.POINTER(C.c_uint32)]
+        L.vacc_subcycle_step.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_int32, C.c_int32, vp]
+        L.vacc_accelerate_subcycled.argtypes = [vp, C.c_double, C.c_uint32, C.c_int32, C.POINTER(C.c_uint32), vp]
         _lib = L
     return _lib
 
@@ -251,6 +255,26 @@ class Context:
         rho_loss: optional float64 array of ncells entries that receives the removed mass."""
         ptr = None if rho_loss is None else _ptr(rho_loss)
         return _check(lib().vacc_step(self.h, int(map_order), 1 if use_moments else 0, 1 if adjust else 0, ptr))
+
+    def subcycle_count(self, dt: float) -> int:
+        """Per-cell subcycle counts of a step of length dt (kept resident); returns the maximum."""
+        n = C.c_uint32()
+        _check(lib().vacc_subcycle_count(self.h, float(dt), C.byref(n)))
+        return n.value
+
+    def subcycle_step(self, step: int, map_order: int, use_moments: bool = True, adjust: bool = True,
+                      rho_loss=None) -> int:
+        ptr = None if rho_loss is None else _ptr(rho_loss)
+        return _check(lib().vacc_subcycle_step(self.h, int(step), int(map_order), 1 if use_moments else 0,
+                                               1 if adjust else 0, ptr))
+
+    def accelerate_subcycled(self, dt: float, map_order: int, use_moments: bool = True, rho_loss=None):
+        """Every subcycle of a step of length dt; returns (status, number of subcycles)."""
+        n = C.c_uint32()
+        ptr = None if rho_loss is None else _ptr(rho_loss)
+        st = _check(lib().vacc_accelerate_subcycled(self.h, float(dt), int(map_order), 1 if use_moments else 0,
+                                                    C.byref(n), ptr))
+        return st, n.value
 
     # -- work --------------------------------------------------------------------
     def map_1d(self, dim: int) -> int:
