@@ -74,12 +74,25 @@ typedef struct vacc_mesh {
    double max_limits[3];
 } vacc_mesh;
 
-/* Particle species and sparsity parameters (particle_species.h, object_wrapper.cpp:49-56). */
+/* Particle species and sparsity parameters (particle_species.h:45-57, object_wrapper.cpp:120-134). */
 typedef struct vacc_species {
    double mass;
    double charge;
-   double sparse_min_value;      /* <pop>_sparse.minValue */
-   int32_t sparse_add_width_v;   /* <pop>_sparse.blockAddWidthV */
+   double sparse_min_value;          /* <pop>_sparse.minValue */
+   int32_t sparse_add_width_v;       /* <pop>_sparse.blockAddWidthV */
+   /* <pop>_sparse.dynamicAlgorithm: 0 the static minValue; 1 or 2 a per-cell threshold set at
+    * every adjust (SpatialCell::updateSparseMinValue, spatial_cell_cpu.cpp:699-727), linear
+    * in the cell's density (1) or block count (2) through (dynamicBulkValue1,
+    * dynamicMinValue1) and (dynamicBulkValue2, dynamicMinValue2), clamped to
+    * [dynamicMinValue1, dynamicMinValue2].  The threshold is the content test of the adjust
+    * and of the remap's written blocks, and the reconstruction threshold of the next remaps
+    * (cpu_acc_map.cpp:539-547).  The density is that of the population's current f. */
+   int32_t sparse_dynamic_algorithm;
+   double sparse_dynamic_bulk1, sparse_dynamic_bulk2;   /* dynamicBulkValue1, dynamicBulkValue2 */
+   double sparse_dynamic_min1, sparse_dynamic_min2;     /* dynamicMinValue1, dynamicMinValue2 */
+   /* <pop>_sparse.conserve_mass: after every adjust the cell's f is scaled by (sum of f before)
+    * / (sum after), block_adjust_cpu.cpp:96-113 (the 16-bit forms scale each block's scale) */
+   int32_t sparse_conserve_mass;
    int32_t reserved;
 } vacc_species;
 

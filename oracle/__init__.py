@@ -291,6 +291,27 @@ def adjust(mesh: Mesh, gids, f, min_value: float, add_width: int = 1, fp64: bool
         cap = nout.value + 16
 
 
+def sparse_min_value(algorithm: int, min_value: float, bulk1: float, bulk2: float, min1: float, min2: float,
+                     rho: float, nblocks: int) -> float:
+    """updateSparseMinValue (spatial_cell_cpu.cpp:699-727)."""
+    L = lib()
+    L.vo_sparse_min_value.restype = C.c_double
+    L.vo_sparse_min_value.argtypes = [C.c_int] + [C.c_double] * 6 + [C.c_uint32]
+    return L.vo_sparse_min_value(int(algorithm), min_value, bulk1, bulk2, min1, min2, float(rho), int(nblocks))
+
+
+def conserve_mass(f_pre, f_post, fp64: bool = False) -> np.ndarray:
+    """sparse_conserve_mass rescale (block_adjust_cpu.cpp:96-113): returns f_post * pre/post."""
+    L, dt = _flib(fp64)
+    a = np.ascontiguousarray(f_pre, dtype=dt).reshape(-1)
+    b = np.array(f_post, dtype=dt).reshape(-1)
+    fn = L.vo_conserve_mass
+    fn.argtypes = [C.c_uint32, np.ctypeslib.ndpointer(dt, flags="C_CONTIGUOUS"), C.c_uint32,
+                   np.ctypeslib.ndpointer(dt, flags="C_CONTIGUOUS")]
+    fn(a.size // WID3, a, b.size // WID3, b)
+    return b.reshape(-1, WID3)
+
+
 def adjust_spatial(mesh: Mesh, gids, f, min_value: float, add_width: int, extra_gids):
     """Adjust with the spatial neighbours' content GIDs kept as they are."""
     g = np.ascontiguousarray(gids, dtype=np.uint32)

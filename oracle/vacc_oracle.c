@@ -729,6 +729,28 @@ int vo_adjust_spatial(const vo_mesh* m, uint32_t n_in, const uint32_t* gids, con
    return 0;
 }
 
+/* spatial_cell_cpu.cpp:699-727 */
+double vo_sparse_min_value(int algorithm, double min_value, double bulk1, double bulk2, double min1, double min2,
+                           double rho, uint32_t nblocks) {
+   if (algorithm != 1 && algorithm != 2) return min_value;
+   const double k = (min2 - min1) / (bulk2 - bulk1);
+   const double b = min1 - k * bulk1;
+   const double x = (algorithm == 1) ? rho : (double)nblocks;
+   const double y = k * x + b;
+   if (y < min1) return min1;
+   if (y > min2) return min2;
+   return y;
+}
+
+/* block_adjust_cpu.cpp:96-113 */
+void vo_conserve_mass(uint32_t n_pre, const float* f_pre, uint32_t n_post, float* f_post) {
+   double pre = 0.0, post = 0.0;
+   for (size_t i = 0; i < (size_t)n_pre * VO_WID3; ++i) pre += f_pre[i];
+   for (size_t i = 0; i < (size_t)n_post * VO_WID3; ++i) post += f_post[i];
+   if (post != 0.0)
+      for (size_t i = 0; i < (size_t)n_post * VO_WID3; ++i) f_post[i] = (float)((double)f_post[i] * (pre / post));
+}
+
 int vo_adjust(const vo_mesh* m, uint32_t n_in, const uint32_t* gids, const float* f,
               double min_value, int add_width,
               uint32_t cap, uint32_t* n_out, uint32_t* gids_out, float* f_out, double* rho_loss) {

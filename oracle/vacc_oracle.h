@@ -85,6 +85,15 @@ int vo_adjust(const vo_mesh* m, uint32_t n_in, const uint32_t* gids, const float
 int vo_adjust_spatial(const vo_mesh* m, uint32_t n_in, const uint32_t* gids, const float* f,
                       double min_value, int add_width, uint32_t n_extra, const uint32_t* extra_gids,
                       uint32_t cap, uint32_t* n_out, uint32_t* gids_out, float* f_out, double* rho_loss);
+/* Sparsity threshold of a cell, SpatialCell::updateSparseMinValue (spatial_cell_cpu.cpp:699-727):
+ * algorithm 0 -> min_value; 1 (x = rho) or 2 (x = nblocks) -> k x + b on the line through
+ * (bulk1, min1) and (bulk2, min2), below min1 -> min1, above min2 -> min2. */
+double vo_sparse_min_value(int algorithm, double min_value, double bulk1, double bulk2, double min1, double min2,
+                           double rho, uint32_t nblocks);
+/* sparse_conserve_mass (block_adjust_cpu.cpp:96-113): pre = sum of f_pre, post = sum of
+ * f_post (Real = double sums in block order), and, if post != 0, f_post *= pre / post with
+ * each product rounded to Realf. */
+void vo_conserve_mass(uint32_t n_pre, const float* f_pre, uint32_t n_post, float* f_post);
 /* content flag per block: any f >= min_value (spatial_cell_cpu.cpp:290-316) */
 void vo_content_flags(uint32_t n, const float* f, double min_value, uint8_t* flags);
 

@@ -453,3 +453,34 @@ def test_map_1d_matches_brute_force_overlaps(method):
         for (key, gk), val in got.items():
             if gk not in bf.get(key, {}):
                 assert val == 0.0
+
+
+def test_sparse_min_value_closed_forms():
+    """updateSparseMinValue (spatial_cell_cpu.cpp:699-727): the line through (bulk1, min1) and
+    (bulk2, min2), clamped to [min1, min2]; algorithm 0 keeps minValue."""
+    b1, b2, m1, m2 = 1e5, 1e7, 1e-16, 1e-14
+    for alg in (1, 2):
+        f = lambda x: O.sparse_min_value(alg, 1e-15, b1, b2, m1, m2, x if alg == 1 else 0.0,
+                                         int(x) if alg == 2 else 0)
+        assert f(b1) == pytest.approx(m1, rel=1e-12)
+        assert f(b2) == pytest.approx(m2, rel=1e-12)
+        assert f(0.5 * (b1 + b2)) == pytest.approx(0.5 * (m1 + m2), rel=1e-12)
+        assert f(b1 / 2) == m1 and f(b2 * 3) == m2
+    assert O.sparse_min_value(0, 1e-15, b1, b2, m1, m2, 5e6, 1000) == 1e-15
+
+
+def test_conserve_mass_rescale():
+    """sparse_conserve_mass (block_adjust_cpu.cpp:96-113): the kept blocks are scaled by
+    pre/post, so their sum returns to the pre-adjust sum; post = 0 leaves them unchanged."""
+    g, f = _maxwellian_cell()
+    m = _mesh()
+    thr = 3e-13   # a high threshold removes tail blocks and their mass
+    g2, f2, loss = O.adjust(m, g, f, thr, 1)
+    assert loss > 0
+    f3 = O.conserve_mass(f, f2)
+    pre = f.astype(np.float64).sum()
+    assert abs(f3.astype(np.float64).sum() / pre - 1) < 1e-7
+    ratio = f3[f2 > 0] / f2[f2 > 0]
+    assert np.allclose(ratio, pre / f2.astype(np.float64).sum(), rtol=1e-7)
+    z = np.zeros((3, 64), np.float32)
+    assert np.array_equal(O.conserve_mass(f, z), z)

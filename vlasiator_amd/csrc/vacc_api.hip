@@ -82,6 +82,10 @@ struct vacc_ctx {
    uint32_t* sub_n = nullptr;
    uint8_t* active = nullptr;
    uint32_t* sub_nmax = nullptr;     // device word: maximum subcycle count over the cells
+   // per-cell sparsity threshold (velocityBlockMinValue) and the sums of sparse_conserve_mass
+   double* minval = nullptr;
+   double* csum_pre = nullptr;
+   double* csum_post = nullptr;
    bool sub_valid = false;           // sub_maxdt / sub_n hold the counts of the last vacc_subcycle_count
    bool subcycling = false;          // launches inside vacc_subcycle_step use sub_dt and active
    double sub_dt_total = 0.0;
@@ -224,11 +228,12 @@ static void free_cells(vacc_ctx* c) {
    if (c->moments) (void)hipFree(c->moments);
    if (c->pivot) (void)hipFree(c->pivot);
    if (c->rho_loss) (void)hipFree(c->rho_loss);
-   void* subs[] = {c->sub_maxdt, c->sub_dt, c->sub_n, c->active, c->sub_nmax};
+   void* subs[] = {c->sub_maxdt, c->sub_dt, c->sub_n, c->active, c->sub_nmax, c->minval, c->csum_pre, c->csum_post};
    for (void* p : subs)
       if (p) (void)hipFree(p);
    c->sub_maxdt = c->sub_dt = nullptr;
    c->sub_n = c->sub_nmax = nullptr;
+   c->minval = c->csum_pre = c->csum_post = nullptr;
    c->active = nullptr;
    c->sub_valid = false;
    c->isect = nullptr;
@@ -262,6 +267,9 @@ static int alloc_cells(vacc_ctx* c, uint32_t n) {
    HIPCHK(hipMalloc(&c->sub_n, m * sizeof(uint32_t)));
    HIPCHK(hipMalloc(&c->active, m));
    HIPCHK(hipMalloc(&c->sub_nmax, sizeof(uint32_t)));
+   HIPCHK(hipMalloc(&c->minval, m * sizeof(double)));
+   HIPCHK(hipMalloc(&c->csum_pre, m * sizeof(double)));
+   HIPCHK(hipMalloc(&c->csum_post, m * sizeof(double)));
    c->cell_cap = n;
    return 0;
 }
@@ -428,6 +436,7 @@ static int enqueue_pass(vacc_ctx* c, int dim, int grid_mult) {
    ra.vmin = ca.vmin;
    ra.threshold = (float)c->sp.sparse_min_value;   // Realf threshold argument of compute_*_coeff
    ra.min_value = c->sp.sparse_min_value;
+   ra.minval = c->minval;
    ra.dv_d = c->mi.cell[dim];
    ra.vmin_d = c->mi.vmin[dim];
    {
@@ -468,13 +477,27 @@ struct SpatialIn {
 // Rebuild the list: add_width >= 0 is the sparsity adjust, add_width < 0 a re-sort by GID.
 // With spatial-neighbour inputs the kept count can exceed the reserve: the total is read back
 // after the count phase and the block storage grown before the write phase.
+static void enqueue_moments(vacc_ctx* c);
+
 static int do_relist(vacc_ctx* c, int add_width, bool want_loss, const SpatialIn* sp = nullptr) {
    ProfScope ps(c, add_width >= 0 ? 2 : 4);
    const int li = c->lcur, lo = 1 - c->lcur;
+   const bool dynamic = c->sp.sparse_dynamic_algorithm == 1 || c->sp.sparse_dynamic_algorithm == 2;
+   if (add_width >= 0 && dynamic) {
+      // update_velocity_block_content_lists (block_adjust_cpu.cpp:46-49): the cell's threshold
+      // first (density of the current f, or its block count), then the content flags with it
+      if (c->sp.sparse_dynamic_algorithm == 1) enqueue_moments(c);
+      SparseArgs spa{c->sp.sparse_dynamic_algorithm, c->sp.sparse_min_value, c->sp.sparse_dynamic_bulk1,
+                     c->sp.sparse_dynamic_bulk2, c->sp.sparse_dynamic_min1, c->sp.sparse_dynamic_min2};
+      launch_sparse_min(spa, c->moments, c->off[li], c->minval, c->ncells, c->stream);
+      c->content_valid = false;
+   }
    if (add_width >= 0 && !c->content_valid) {
-      launch_content(c->src[li], pool(c, c->fcur), c->content, (uint32_t)c->nblocks, c->sp.sparse_min_value, c->stream);
+      launch_content_cells(c->off[li], c->src[li], pool(c, c->fcur), c->minval, c->content, c->ncells, c->stream);
       c->content_valid = true;
    }
+   const bool conserve = add_width >= 0 && c->sp.sparse_conserve_mass;
+   if (conserve) launch_cell_sums(c->off[li], c->src[li], pool(c, c->fcur), c->csum_pre, c->ncells, c->stream);
    AdjustArgs aa{};
    aa.cell_off = c->off[li];
    aa.gid = c->gid[li];
@@ -522,6 +545,10 @@ static int do_relist(vacc_ctx* c, int add_width, bool want_loss, const SpatialIn
    aa.gid_out = c->gid[lo];
    aa.src_out = c->src[lo];
    launch_adjust(aa, true, c->stream);
+   if (conserve) {   // block_adjust_cpu.cpp:96-113
+      launch_cell_sums(c->off[lo], c->src[lo], pool(c, c->fcur), c->csum_post, c->ncells, c->stream);
+      launch_cell_scale(c->off[lo], c->src[lo], pool(c, c->fcur), c->csum_pre, c->csum_post, c->ncells, c->stream);
+   }
    HIPCHK(hipGetLastError());
    c->lcur = lo;
    c->content_valid = false;
@@ -665,6 +692,8 @@ static int begin_load(vacc_ctx* c, uint32_t ncells, const uint64_t* offsets, con
 
 static int end_load(vacc_ctx* c, uint64_t total) {
    launch_iota(c->src[0], (uint32_t)total, c->stream);
+   // a new cell starts from the species' minValue (spatial_cell_cpu.cpp:73)
+   launch_fill_f64(c->minval, c->sp.sparse_min_value, c->ncells, c->stream);
    hipError_t e = hipGetLastError();
    if (e == hipSuccess) e = hipMemsetAsync(c->status, 0, (c->ncells + 1) * sizeof(uint32_t), c->stream);
    if (e == hipSuccess) e = hipMemsetAsync(c->flags, 0, sizeof(uint32_t), c->stream);
@@ -1161,6 +1190,7 @@ static int init_peaks_load(vacc_ctx* c, uint32_t ncells, uint32_t max_peaks, con
       ia.vmin[d] = c->mi.vmin[d];
       ia.cell[d] = c->mi.cell[d];
    }
+   launch_fill_f64(c->minval, c->sp.sparse_min_value, ncells, c->stream);
    ia.mass = c->sp.mass;
    ia.min_value = c->sp.sparse_min_value;
    ia.add_width = c->sp.sparse_add_width_v;
@@ -1209,6 +1239,10 @@ int vacc_create(const vacc_mesh* mesh, const vacc_species* species, const vacc_o
    }
    if (opts->method < VACC_PLM || opts->method > VACC_PQM) return set_err(VACC_E_ARG, "unknown method");
    if (opts->storage < VACC_STORE_F32 || opts->storage > VACC_STORE_F16) return set_err(VACC_E_ARG, "unknown storage");
+   if (species->sparse_dynamic_algorithm < 0 || species->sparse_dynamic_algorithm > 2)
+      return set_err(VACC_E_ARG, "sparse_dynamic_algorithm must be 0, 1 or 2");
+   if (species->sparse_dynamic_algorithm != 0 && !(species->sparse_dynamic_bulk2 != species->sparse_dynamic_bulk1))
+      return set_err(VACC_E_ARG, "sparse_dynamic_bulk1 and _bulk2 must differ");
    if (species->sparse_add_width_v < 0 || species->sparse_add_width_v > 8)
       return set_err(VACC_E_ARG, "sparse_add_width_v must be in [0, 8]");
    int ndev = 0;

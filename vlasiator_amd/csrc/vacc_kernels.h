@@ -97,6 +97,7 @@ struct RemapArgs {
    float dv, vmin;
    float threshold;             // (float) sparse min value, reconstruction threshold
    double min_value;            // content test threshold
+   const double* minval;        // [ncells] sparsity threshold per cell (velocityBlockMinValue)
    double dv_d, vmin_d;         // cell size and mesh minimum along the pass axis (FP64)
    int only_flagged;            // exact kernel: 1 = take only the units marked EXACT_FLAG
 };
@@ -193,6 +194,25 @@ void launch_subcycle_count(const TransformIn* fields, uint32_t ncells, double ma
 // the cell's last subcycle, with the sign of dt (0 for inactive cells).
 void launch_subcycle_step(const double* maxdt, const uint32_t* nsub, uint32_t ncells, uint32_t step, double dt,
                           double* subdt, uint8_t* active, hipStream_t s);
+// Sparsity threshold of every cell, SpatialCell::updateSparseMinValue (spatial_cell_cpu.cpp:699-727):
+// algorithm 1 takes x = the density moments[4c], 2 the cell's block count from off[].
+struct SparseArgs {
+   int algorithm;
+   double min_value, bulk1, bulk2, min1, min2;
+};
+void launch_sparse_min(const SparseArgs& p, const double* moments, const uint32_t* off, double* minval,
+                       uint32_t ncells, hipStream_t s);
+// content flag of every block of a list with the cell's threshold (one workgroup per cell)
+void launch_content_cells(const uint32_t* off, const uint32_t* src, const Pool& pool, const double* minval,
+                          uint8_t* content, uint32_t ncells, hipStream_t s);
+// FP64 sum of f over each cell's blocks
+void launch_cell_sums(const uint32_t* off, const uint32_t* src, const Pool& pool, double* out, uint32_t ncells,
+                      hipStream_t s);
+// sparse_conserve_mass (block_adjust_cpu.cpp:96-113): f *= pre[c] / post[c] where post != 0,
+// each product rounded to FP32 (16-bit forms: the block scale)
+void launch_cell_scale(const uint32_t* off, const uint32_t* src, const Pool& pool, const double* pre,
+                       const double* post, uint32_t ncells, hipStream_t s);
+void launch_fill_f64(double* dst, double v, uint32_t n, hipStream_t s);
 // dt_override: [ncells] or null; replaces fields[c].dt.
 void launch_transform(const MeshInfo& m, const TransformIn* fields, const double* dt_override, const double* moments, int use_moments,
                       double mass, double charge, double hall_min_rhoq, int ohm_gradpe, int map_order,

@@ -667,9 +667,6 @@ __global__ __launch_bounds__(WG) void k_remap(RemapArgs a, double i_dv) {
    const int ip = l16 & 3, jp = l16 >> 2;
    const int setlane0 = (tid & 63) & ~15;
    const int o = (DIM == 0) ? (ip * WID2 + jp * WID) : (DIM == 1) ? (jp * WID2 + ip) : (jp * WID + ip);
-   const float thr = a.threshold;
-   const float scale = (float)(1.0 / (double)thr);
-   const double mv = a.min_value;
    const uint32_t total = a.unit_base[a.ncells];
    const uint32_t gstride = gridDim.x * (WG / 16);
    for (uint32_t gi = blockIdx.x * (WG / 16) + (tid >> 4); gi < total; gi += gstride) {
@@ -680,6 +677,11 @@ __global__ __launch_bounds__(WG) void k_remap(RemapArgs a, double i_dv) {
       const int b0 = (int)(sr.x & 0xFFFFu), b1 = (int)(sr.x >> 16);
       const uint32_t col0 = sr.y;
       if (a.only_flagged && !(sr.z & EXACT_FLAG)) continue;   // the tolerance form took it
+      // the cell's sparsity threshold: the content test, and the reconstruction threshold as
+      // the Realf argument of compute_*_coeff (cpu_acc_map.cpp:539-547)
+      const double mv = a.minval[c];
+      const float thr = (float)mv;
+      const float scale = (float)(1.0 / (double)thr);
       const uint32_t ncol = sr.z & UNIT_NCOL_MASK;
       const bool copy = (sr.z & COPY_FLAG) != 0u;
       const uint32_t obase = a.out_off[c];
@@ -845,7 +847,6 @@ __global__ __launch_bounds__(WG) void k_remap_lean(RemapArgs a) {
    const int ip = l16 & 3, jp = l16 >> 2;
    const int setlane0 = (tid & 63) & ~15;
    const int o = (DIM == 0) ? (ip * WID2 + jp * WID) : (DIM == 1) ? (jp * WID2 + ip) : (jp * WID + ip);
-   const double mv = a.min_value;
    const uint32_t total = a.unit_base[a.ncells];
    const uint32_t gstride = gridDim.x * (WG / 16);
    for (uint32_t gi = blockIdx.x * (WG / 16) + (tid >> 4); gi < total; gi += gstride) {
@@ -854,6 +855,7 @@ __global__ __launch_bounds__(WG) void k_remap_lean(RemapArgs a) {
       const uint32_t cbase = a.cell_off[c];
       const uint4 sr = a.unit_rec[cbase + (g - a.unit_base[c])];
       if (sr.z & EXACT_FLAG) continue;   // taken by k_remap
+      const double mv = a.minval[c];
       const int b0 = (int)(sr.x & 0xFFFFu), b1 = (int)(sr.x >> 16);
       const uint32_t col0 = sr.y;
       const uint32_t ncol = sr.z & UNIT_NCOL_MASK;
@@ -1245,6 +1247,75 @@ __global__ __launch_bounds__(WG) void k_content(const uint32_t* src, Pool pl, ui
    }
 }
 
+// spatial_cell_cpu.cpp:699-727
+__global__ void k_sparse_min(SparseArgs p, const double* moments, const uint32_t* off, double* minval,
+                             uint32_t ncells) {
+   const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+   if (c >= ncells) return;
+   double v = p.min_value;
+   if (p.algorithm == 1 || p.algorithm == 2) {
+      const double k = (p.min2 - p.min1) / (p.bulk2 - p.bulk1);
+      const double b = p.min1 - k * p.bulk1;
+      const double x = (p.algorithm == 1) ? moments[4 * (size_t)c] : (double)(off[c + 1] - off[c]);
+      const double y = k * x + b;
+      v = (y < p.min1) ? p.min1 : (y > p.min2) ? p.min2 : y;
+   }
+   minval[c] = v;
+}
+
+// content flags with the cell's threshold: one workgroup per cell, one wave per block
+__global__ __launch_bounds__(WG) void k_content_cells(const uint32_t* off, const uint32_t* src, Pool pl,
+                                                      const double* minval, uint8_t* content) {
+   const uint32_t c = blockIdx.x;
+   const int lane = threadIdx.x & 63;
+   const double mv = minval[c];
+   for (uint32_t b = off[c] + (threadIdx.x >> 6); b < off[c + 1]; b += WG / 64) {
+      const uint32_t s = src[b];
+      const float x = (s == INVALID) ? 0.0f : pool_value(pl, s, lane);
+      const unsigned long long bal = __ballot((double)x >= mv);
+      if (lane == 0) content[b] = (uint8_t)(bal != 0ull);
+   }
+}
+
+// FP64 sum of each cell's f: one workgroup per cell, one wave per block
+__global__ __launch_bounds__(WG) void k_cell_sums(const uint32_t* off, const uint32_t* src, Pool pl, double* out) {
+   __shared__ double part[WG / 64];
+   const uint32_t c = blockIdx.x;
+   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+   double acc = 0.0;
+   for (uint32_t b = off[c] + w; b < off[c + 1]; b += WG / 64) {
+      const uint32_t s = src[b];
+      acc += (s == INVALID) ? 0.0 : (double)pool_value(pl, s, lane);
+   }
+   for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+   if (lane == 0) part[w] = acc;
+   __syncthreads();
+   if (threadIdx.x == 0) {
+      double t = 0.0;
+      for (int q = 0; q < WG / 64; ++q) t += part[q];
+      out[c] = t;
+   }
+}
+
+// f *= pre / post per cell (block_adjust_cpu.cpp:108-112); one workgroup per cell
+__global__ __launch_bounds__(WG) void k_cell_scale(const uint32_t* off, const uint32_t* src, Pool pl,
+                                                   const double* pre, const double* post) {
+   const uint32_t c = blockIdx.x;
+   if (post[c] == 0.0) return;
+   const double r = pre[c] / post[c];
+   const int lane = threadIdx.x & 63;
+   for (uint32_t b = off[c] + (threadIdx.x >> 6); b < off[c + 1]; b += WG / 64) {
+      const uint32_t s = src[b];
+      if (s == INVALID) continue;
+      if (pl.storage == ST_F32) {
+         float* f = static_cast<float*>(pl.f) + (size_t)s * WID3 + lane;
+         *f = (float)((double)*f * r);
+      } else if (lane == 0) {
+         pl.scale[s] = (float)((double)pl.scale[s] * r);
+      }
+   }
+}
+
 // ---------------------------------------------------------------------------
 // moments, transform, gather
 // ---------------------------------------------------------------------------
@@ -1596,6 +1667,40 @@ void launch_decode(const Pool& in, float* f, uint32_t n, hipStream_t s) {
 void launch_iota(uint32_t* dst, uint32_t n, hipStream_t s) {
    if (n == 0) return;
    hipLaunchKernelGGL(k_iota, dim3((n + 255) / 256), dim3(256), 0, s, dst, n);
+}
+
+__global__ void k_fill_f64(double* dst, double v, uint32_t n) {
+   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+   if (i < n) dst[i] = v;
+}
+
+void launch_fill_f64(double* dst, double v, uint32_t n, hipStream_t s) {
+   if (n == 0) return;
+   hipLaunchKernelGGL(k_fill_f64, dim3((n + 255) / 256), dim3(256), 0, s, dst, v, n);
+}
+
+void launch_sparse_min(const SparseArgs& p, const double* moments, const uint32_t* off, double* minval,
+                       uint32_t ncells, hipStream_t s) {
+   if (ncells == 0) return;
+   hipLaunchKernelGGL(k_sparse_min, dim3((ncells + 127) / 128), dim3(128), 0, s, p, moments, off, minval, ncells);
+}
+
+void launch_content_cells(const uint32_t* off, const uint32_t* src, const Pool& pool, const double* minval,
+                          uint8_t* content, uint32_t ncells, hipStream_t s) {
+   if (ncells == 0) return;
+   hipLaunchKernelGGL(k_content_cells, dim3(ncells), dim3(WG), 0, s, off, src, pool, minval, content);
+}
+
+void launch_cell_sums(const uint32_t* off, const uint32_t* src, const Pool& pool, double* out, uint32_t ncells,
+                      hipStream_t s) {
+   if (ncells == 0) return;
+   hipLaunchKernelGGL(k_cell_sums, dim3(ncells), dim3(WG), 0, s, off, src, pool, out);
+}
+
+void launch_cell_scale(const uint32_t* off, const uint32_t* src, const Pool& pool, const double* pre,
+                       const double* post, uint32_t ncells, hipStream_t s) {
+   if (ncells == 0) return;
+   hipLaunchKernelGGL(k_cell_scale, dim3(ncells), dim3(WG), 0, s, off, src, pool, pre, post);
 }
 
 void launch_fill_u32(uint32_t* dst, uint32_t v, uint32_t n, hipStream_t s) {

@@ -41,7 +41,10 @@ class Mesh(C.Structure):
 
 class Species(C.Structure):
     _fields_ = [("mass", C.c_double), ("charge", C.c_double), ("sparse_min_value", C.c_double),
-                ("sparse_add_width_v", C.c_int32), ("reserved", C.c_int32)]
+                ("sparse_add_width_v", C.c_int32), ("sparse_dynamic_algorithm", C.c_int32),
+                ("sparse_dynamic_bulk1", C.c_double), ("sparse_dynamic_bulk2", C.c_double),
+                ("sparse_dynamic_min1", C.c_double), ("sparse_dynamic_min2", C.c_double),
+                ("sparse_conserve_mass", C.c_int32), ("reserved", C.c_int32)]
 
 
 class Options(C.Structure):
@@ -134,10 +137,15 @@ def make_mesh(grid_length, vmin, vmax) -> Mesh:
     return m
 
 
-def make_species(mass, charge, min_value=1e-15, add_width=1) -> Species:
+def make_species(mass, charge, min_value=1e-15, add_width=1, dynamic=0, bulk=(0.0, 1.0), dyn_min=(0.0, 0.0),
+                 conserve_mass=False) -> Species:
     s = Species()
     s.mass, s.charge, s.sparse_min_value = float(mass), float(charge), float(min_value)
     s.sparse_add_width_v = int(add_width)
+    s.sparse_dynamic_algorithm = int(dynamic)
+    s.sparse_dynamic_bulk1, s.sparse_dynamic_bulk2 = float(bulk[0]), float(bulk[1])
+    s.sparse_dynamic_min1, s.sparse_dynamic_min2 = float(dyn_min[0]), float(dyn_min[1])
+    s.sparse_conserve_mass = 1 if conserve_mass else 0
     return s
 
 
