@@ -190,6 +190,12 @@ int vacc_autotune(vacc_ctx* ctx, uint32_t* mult_out);
  * them. */
 int vacc_moments(vacc_ctx* ctx, double* out);
 
+/* Pressure tensor per cell (the second moments, arch_moments.h:123-150 with
+ * arch_moments.cpp:474-489): out[6*c+0..5] = mass * sum f (v - V)_a (v - V)_b dV^3 for
+ * (a,b) = xx, yy, zz, yz, xz, xy (P_11, P_22, P_33, P_23, P_13, P_12), FP64, v at cell centres.
+ * pivot: 3 doubles per cell, the bulk velocity over all species (VX_V, VY_V, VZ_V), or NULL
+ * for this population's own bulk velocity from its first moments. */
+int vacc_pressure(vacc_ctx* ctx, const double* pivot, double* out);
 /* Make the per-cell transform inputs resident (ncells entries; the dt of each entry is the
  * cell's subcycle dt).  Replaced by the next call; cleared by an upload. */
 int vacc_set_fields(vacc_ctx* ctx, const vacc_cell_fields* fields);

@@ -291,6 +291,17 @@ def adjust(mesh: Mesh, gids, f, min_value: float, add_width: int = 1, fp64: bool
         cap = nout.value + 16
 
 
+def pressure(mesh: Mesh, gids, f, V, mass: float) -> np.ndarray:
+    """Pressure tensor (xx, yy, zz, yz, xz, xy) of one cell about V (arch_moments.h:123-150)."""
+    L = lib()
+    L.vo_pressure.argtypes = [C.POINTER(Mesh), C.c_uint32, _u32p, _f32p, _f64p, C.c_double, _f64p]
+    g = np.ascontiguousarray(gids, np.uint32)
+    ff = np.ascontiguousarray(f, np.float32).reshape(-1)
+    out = np.zeros(6)
+    L.vo_pressure(C.byref(mesh), g.size, g, ff, np.ascontiguousarray(V, np.float64), float(mass), out)
+    return out
+
+
 def sparse_min_value(algorithm: int, min_value: float, bulk1: float, bulk2: float, min1: float, min2: float,
                      rho: float, nblocks: int) -> float:
     """updateSparseMinValue (spatial_cell_cpu.cpp:699-727)."""

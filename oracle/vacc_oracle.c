@@ -787,6 +787,33 @@ void vo_moments(const vo_mesh* m, uint32_t n, const uint32_t* gids, const float*
    out[0] = s[0]; out[1] = s[1]; out[2] = s[2]; out[3] = s[3];
 }
 
+/* arch_moments.h:123-150 */
+void vo_pressure(const vo_mesh* m, uint32_t n, const uint32_t* gids, const float* f, const double V[3],
+                 double mass, double out[6]) {
+   const double dv[3] = {vo_cell_size(m, 0), vo_cell_size(m, 1), vo_cell_size(m, 2)};
+   const double DV3 = dv[0] * dv[1] * dv[2];
+   double s[6] = {0, 0, 0, 0, 0, 0};
+   for (uint32_t b = 0; b < n; ++b) {
+      uint32_t idx[3];
+      block_indices(m, gids[b], idx);
+      for (int k = 0; k < VO_WID; ++k)
+         for (int j = 0; j < VO_WID; ++j)
+            for (int i = 0; i < VO_WID; ++i) {
+               const double a = f[(size_t)b * VO_WID3 + i + VO_WID * j + VO_WID2 * k];
+               const double dx = (m->vmin[0] + (idx[0] * VO_WID + i + 0.5) * dv[0]) - V[0];
+               const double dy = (m->vmin[1] + (idx[1] * VO_WID + j + 0.5) * dv[1]) - V[1];
+               const double dz = (m->vmin[2] + (idx[2] * VO_WID + k + 0.5) * dv[2]) - V[2];
+               s[0] += a * dx * dx * DV3;
+               s[1] += a * dy * dy * DV3;
+               s[2] += a * dz * dz * DV3;
+               s[3] += a * dy * dz * DV3;
+               s[4] += a * dx * dz * DV3;
+               s[5] += a * dx * dy * DV3;
+            }
+   }
+   for (int q = 0; q < 6; ++q) out[q] = mass * s[q];
+}
+
 /* ------------------------------------------------------------------------ */
 /* Transform (cpu_acc_transform.cpp:66-153) and intersections                */
 /* (cpu_acc_intersections.cpp:44-450), FP64                                   */

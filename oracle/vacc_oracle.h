@@ -99,6 +99,10 @@ void vo_content_flags(uint32_t n, const float* f, double min_value, uint8_t* fla
 
 /* ---------------- moments (arch_moments.h:83-108) ---------------- */
 void vo_moments(const vo_mesh* m, uint32_t n, const uint32_t* gids, const float* f, double out[4]);
+/* second moments (arch_moments.h:123-150): out = mass * sum f (v - V)_a (v - V)_b DV3 for
+ * xx, yy, zz, yz, xz, xy, v at cell centres, summed per value in block order */
+void vo_pressure(const vo_mesh* m, uint32_t n, const uint32_t* gids, const float* f, const double V[3],
+                 double mass, double out[6]);
 
 /* ---------------- spatial translation (TRANS_SEMILAG_PPM) ---------------- */
 #define VO_TRANS_STENCIL 2   /* VLASOV_STENCIL_WIDTH for PPM, definitions.h:78-84 */

@@ -610,3 +610,25 @@ def test_overlapping_column_targets_bit_exact(method):
                 for k in range(10, 40, 2))
     assert alone > g2.size
     ctx.close()
+
+
+def test_pressure_tensor_matches_oracle():
+    """vacc_pressure (FP64 second moments, arch_moments.h:123-150) against the oracle, about
+    the population's own bulk velocity and about a given pivot; sums in another order, so
+    within 1e-11 of the diagonal."""
+    w = W.config(2, ncells=4)
+    cells = cases.build_cells(w)
+    m = cases.oracle_mesh(w)
+    ctx = _ctx(w)
+    ctx.upload_cells(cells)
+    piv = np.array([[1e4 * i, -2e4, 3e4] for i in range(len(cells))])
+    for pivot in (None, piv):
+        _say(f"pressure pivot={'own' if pivot is None else 'given'}")
+        got = ctx.pressure(len(cells), pivot)
+        for i, (g, f) in enumerate(cells):
+            mo = O.moments(m, g, f)
+            V = mo[1:] / mo[0] if pivot is None else pivot[i]
+            want = O.pressure(m, g, f, V, w.mass)
+            scale = np.abs(want[:3]).max()
+            assert np.all(np.abs(got[i] - want) <= 1e-11 * scale), (i, got[i], want)
+    ctx.close()

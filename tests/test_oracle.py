@@ -484,3 +484,22 @@ def test_conserve_mass_rescale():
     assert np.allclose(ratio, pre / f2.astype(np.float64).sum(), rtol=1e-7)
     z = np.zeros((3, 64), np.float32)
     assert np.array_equal(O.conserve_mass(f, z), z)
+
+
+def test_pressure_of_maxwellian():
+    """Second moments of a sampled Maxwellian about its bulk velocity: P_aa = n k_B T on the
+    diagonal, 0 off it (the velocity mesh resolves the thermal speed by ~2.5 cells)."""
+    kB = 1.3806503e-23
+    T, V, n = 4e5, (1e5, -2e5, 5e4), 1e6
+    g, f = _maxwellian_cell(T=T, V=V)
+    m = _mesh()
+    mo = O.moments(m, g, f)
+    P = O.pressure(m, g, f, mo[1:] / mo[0], W.MASS_PROTON)
+    p0 = n * kB * T
+    assert np.allclose(P[:3], p0, rtol=3e-3)
+    assert np.all(np.abs(P[3:]) < 1e-3 * p0)
+    # about a pivot shifted by u along x: P_xx grows by m n u^2, the rest is unchanged
+    u = 5e4
+    P2 = O.pressure(m, g, f, mo[1:] / mo[0] + np.array([u, 0, 0]), W.MASS_PROTON)
+    assert P2[0] - P[0] == pytest.approx(W.MASS_PROTON * mo[0] * u * u, rel=1e-9)
+    assert np.allclose(P2[1:3], P[1:3], rtol=1e-12)

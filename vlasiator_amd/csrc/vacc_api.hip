@@ -1854,6 +1854,43 @@ int vacc_moments(vacc_ctx* c, double* out) {
    return 0;
 }
 
+int vacc_pressure(vacc_ctx* c, const double* pivot, double* out) {
+   if (!c || (!out && c->ncells)) return set_err(VACC_E_ARG, "null argument");
+   if (!c->ncells) return 0;
+   const size_t n = c->ncells;
+   std::vector<double> piv(3 * n);
+   if (pivot) {
+      std::copy(pivot, pivot + 3 * n, piv.begin());
+   } else {   // the population's own bulk velocity from its first moments
+      std::vector<double> mo(4 * n);
+      int rc = vacc_moments(c, mo.data());
+      if (rc) return rc;
+      for (size_t i = 0; i < n; ++i)
+         for (int d = 0; d < 3; ++d) piv[3 * i + d] = (mo[4 * i] != 0.0) ? mo[4 * i + 1 + d] / mo[4 * i] : 0.0;
+   }
+   DevTemps tmp;
+   double *d_piv = nullptr, *d_out = nullptr;
+   HIPCHK(tmp.alloc(&d_piv, 3 * n * sizeof(double)));
+   HIPCHK(tmp.alloc(&d_out, 6 * n * sizeof(double)));
+   HIPCHK(hipMemcpyAsync(d_piv, piv.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+   MomentArgs ma{};
+   ma.cell_off = c->off[c->lcur];
+   ma.gid = c->gid[c->lcur];
+   ma.src = c->src[c->lcur];
+   ma.pool = pool(c, c->fcur);
+   ma.ncells = c->ncells;
+   for (int d = 0; d < 3; ++d) {
+      ma.len[d] = c->mi.len[d];
+      ma.vmin[d] = c->mi.vmin[d];
+      ma.cell[d] = c->mi.cell[d];
+   }
+   launch_pressure(ma, d_piv, c->sp.mass, d_out, c->stream);
+   HIPCHK(hipGetLastError());
+   HIPCHK(hipMemcpyAsync(out, d_out, 6 * n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+   HIPCHK(hipStreamSynchronize(c->stream));
+   return 0;
+}
+
 int vacc_init_peaks(vacc_ctx* c, uint32_t ncells, uint32_t max_peaks, const vacc_peak* peaks, const int32_t* npeaks,
                     const int32_t* bbox) {
    if (!c || (ncells && (!peaks || !npeaks || !bbox)) || max_peaks == 0) return set_err(VACC_E_ARG, "bad argument");

@@ -213,6 +213,9 @@ void launch_cell_sums(const uint32_t* off, const uint32_t* src, const Pool& pool
 void launch_cell_scale(const uint32_t* off, const uint32_t* src, const Pool& pool, const double* pre,
                        const double* post, uint32_t ncells, hipStream_t s);
 void launch_fill_f64(double* dst, double v, uint32_t n, hipStream_t s);
+// Second moments (arch_moments.h:123-150): out[6c..] = mass * sum f (v - V)_a (v - V)_b DV3 for
+// (a,b) = xx, yy, zz, yz, xz, xy about the pivot pivot[3c..] (FP64 sums)
+void launch_pressure(const MomentArgs& a, const double* pivot, double mass, double* out, hipStream_t s);
 // dt_override: [ncells] or null; replaces fields[c].dt.
 void launch_transform(const MeshInfo& m, const TransformIn* fields, const double* dt_override, const double* moments, int use_moments,
                       double mass, double charge, double hall_min_rhoq, int ohm_gradpe, int map_order,

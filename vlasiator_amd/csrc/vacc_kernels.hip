@@ -1247,6 +1247,47 @@ __global__ __launch_bounds__(WG) void k_content(const uint32_t* src, Pool pl, ui
    }
 }
 
+// Second moments of one population about a pivot (arch_moments.h:123-150): one workgroup per
+// cell, one lane per velocity cell of a block, FP64 sums; DV3 and the mass applied once.
+__global__ __launch_bounds__(WG) void k_pressure(MomentArgs a, const double* pivot, double mass, double* out) {
+   __shared__ double red[6][WG / 64];
+   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+   const uint32_t c = blockIdx.x;
+   const uint32_t off = a.cell_off[c], n = a.cell_off[c + 1] - off;
+   const int i = lane & 3, j = (lane >> 2) & 3, k = lane >> 4;
+   const double px = pivot[3 * (size_t)c], py = pivot[3 * (size_t)c + 1], pz = pivot[3 * (size_t)c + 2];
+   const uint32_t nx = a.len[0], nxy = a.len[0] * a.len[1];
+   double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+   for (uint32_t b = w; b < n; b += WG / 64) {
+      const uint32_t sidx = a.src[off + b];
+      if (sidx == INVALID) continue;
+      const double f = pool_value(a.pool, sidx, lane);
+      const uint32_t g = a.gid[off + b];
+      const uint32_t z = g / nxy, r = g - z * nxy, y = r / nx, x = r - y * nx;
+      const double dx = (a.vmin[0] + (x * WID + i + 0.5) * a.cell[0]) - px;
+      const double dy = (a.vmin[1] + (y * WID + j + 0.5) * a.cell[1]) - py;
+      const double dz = (a.vmin[2] + (z * WID + k + 0.5) * a.cell[2]) - pz;
+      s[0] += f * dx * dx;
+      s[1] += f * dy * dy;
+      s[2] += f * dz * dz;
+      s[3] += f * dy * dz;
+      s[4] += f * dx * dz;
+      s[5] += f * dx * dy;
+   }
+#pragma unroll
+   for (int q = 0; q < 6; ++q) {
+      for (int o = 32; o > 0; o >>= 1) s[q] += __shfl_down(s[q], o, 64);
+      if (lane == 0) red[q][w] = s[q];
+   }
+   __syncthreads();
+   if (t < 6) {
+      const double DV3 = a.cell[0] * a.cell[1] * a.cell[2];
+      double sum = 0.0;
+      for (int q = 0; q < WG / 64; ++q) sum += red[t][q];
+      out[6 * (size_t)c + t] = mass * (sum * DV3);
+   }
+}
+
 // spatial_cell_cpu.cpp:699-727
 __global__ void k_sparse_min(SparseArgs p, const double* moments, const uint32_t* off, double* minval,
                              uint32_t ncells) {
@@ -1677,6 +1718,11 @@ __global__ void k_fill_f64(double* dst, double v, uint32_t n) {
 void launch_fill_f64(double* dst, double v, uint32_t n, hipStream_t s) {
    if (n == 0) return;
    hipLaunchKernelGGL(k_fill_f64, dim3((n + 255) / 256), dim3(256), 0, s, dst, v, n);
+}
+
+void launch_pressure(const MomentArgs& a, const double* pivot, double mass, double* out, hipStream_t s) {
+   if (a.ncells == 0) return;
+   hipLaunchKernelGGL(k_pressure, dim3(a.ncells), dim3(WG), 0, s, a, pivot, mass, out);
 }
 
 void launch_sparse_min(const SparseArgs& p, const double* moments, const uint32_t* off, double* minval,

@@ -27,7 +27,7 @@ EXPORTS = [
     "vacc_accelerate_cells", "vacc_adjust_blocks", "vacc_moments", "vacc_sync", "vacc_get_status",
     "vacc_init_peaks", "vacc_profile_begin", "vacc_profile_read", "vacc_set_fields", "vacc_step",
     "vacc_accelerate_populations", "vacc_adjust_blocks_spatial", "vacc_autotune",
-    "vacc_subcycle_count", "vacc_subcycle_step", "vacc_accelerate_subcycled",
+    "vacc_subcycle_count", "vacc_subcycle_step", "vacc_accelerate_subcycled", "vacc_pressure",
 ]
 
 
@@ -111,6 +111,7 @@ def lib():
         L.vacc_accelerate_populations.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_int32, C.c_int32, vp]
         L.vacc_adjust_blocks_spatial.argtypes = [vp, vp, vp, vp, vp, vp]
         L.vacc_autotune.argtypes = [vp, vp]
+        L.vacc_pressure.argtypes = [vp, vp, vp]
         L.vacc_subcycle_count.argtypes = [vp, C.c_double, C.POINTER(C.c_uint32)]
         L.vacc_subcycle_step.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_int32, C.c_int32, vp]
         L.vacc_accelerate_subcycled.argtypes = [vp, C.c_double, C.c_uint32, C.c_int32, C.POINTER(C.c_uint32), vp]
@@ -324,6 +325,14 @@ class Context:
         out = np.zeros(max(ncells, 1) * 4, np.float64)
         _check(lib().vacc_moments(self.h, _ptr(out)))
         return out[:ncells * 4].reshape(ncells, 4)
+
+    def pressure(self, ncells, pivot=None):
+        """Pressure tensor per cell (xx, yy, zz, yz, xz, xy) about pivot[ncells, 3] or, for None,
+        the population's own bulk velocity."""
+        out = np.zeros(max(ncells, 1) * 6, np.float64)
+        pv = None if pivot is None else np.ascontiguousarray(pivot, np.float64).reshape(-1)
+        _check(lib().vacc_pressure(self.h, None if pv is None else _ptr(pv), _ptr(out)))
+        return out[:ncells * 6].reshape(ncells, 6)
 
     def profile_begin(self):
         _check(lib().vacc_profile_begin(self.h))
