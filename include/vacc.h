@@ -185,6 +185,15 @@ int vacc_adjust_blocks_spatial(vacc_ctx* ctx, const uint32_t* nbr_off, const uin
  * write scratch buffers only, so the state is unchanged; results of later steps do not depend
  * on the choice.  mult_out (3 entries, may be NULL) receives the chosen multipliers. */
 int vacc_autotune(vacc_ctx* ctx, uint32_t* mult_out);
+/* Block lookup: index[i] = position of block gids[i] within cell cells[i]'s block list, in
+ * the ascending-GID order that vacc_download_cells returns (the local ID after a download),
+ * or VACC_INVALID_ID when the cell holds no such block or cells[i] >= ncells
+ * (VelocityMesh::getLocalID, spatial_cells/velocity_mesh_cpu.h:371-377, over the velocity
+ * mesh's GID -> LID map, open_bucket_hashtable.h:286).  The device keeps every cell's list in
+ * ascending GID order after an adjust (a remap leaves pass order; the call then reorders the
+ * lists, blocks unchanged), so the map is a bisection of the resident list.  Host arrays of
+ * nq entries; same state afterwards. */
+int vacc_find_blocks(vacc_ctx* ctx, uint64_t nq, const uint32_t* cells, const uint32_t* gids, uint32_t* index);
 /* FP64 moments per cell: out[4*c+0] = n, [1..3] = n*V (arch_moments.h:83-108).  They are
  * kept resident until f changes, so a following use_moments transform does not recompute
  * them. */

@@ -1,0 +1,98 @@
+"""GPU checks of the velocity-mesh lookup vacc_find_blocks (VelocityMesh::getLocalID,
+velocity_mesh_cpu.h:371-377): the position of a block within its cell's ascending-GID list,
+or VACC_INVALID_ID.
+
+Expected answers come from the downloaded lists (numpy searchsorted on each cell's GIDs).
+Queries cover every present block, absent GIDs next to present ones and at the mesh ends,
+cells outside the context, an empty cell, lists left in pass order by a remap (the call
+reorders them, blocks unchanged), and a second call whose queries span several cells.
+"""
+import numpy as np
+import pytest
+
+from tests import cases
+from vlasiator_amd import vacc as V
+from vlasiator_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _ctx(w):
+    return V.Context(V.make_mesh(w.grid_length, w.vmin, w.vmax), V.make_species(w.mass, w.charge, w.min_value,
+                                                                                w.add_width), V.make_options(w.method))
+
+
+def _expected(cells_dl, qc, qg):
+    out = np.full(qc.size, V.INVALID_ID, np.uint32)
+    for i, (c, g) in enumerate(zip(qc, qg)):
+        if c < len(cells_dl):
+            gl = cells_dl[c][0]
+            k = np.searchsorted(gl, g)
+            if k < gl.size and gl[k] == g:
+                out[i] = k
+    return out
+
+
+def _queries(cells_dl, nmesh, rng):
+    qc, qg = [], []
+    for c, (g, _) in enumerate(cells_dl):
+        qc += [c] * g.size
+        qg += list(g)
+        if g.size:
+            near = np.concatenate([g - 1, g + 1])
+            near = near[(near >= 0) & (near < nmesh)]
+            qc += [c] * near.size
+            qg += list(near)
+        qc += [c, c]
+        qg += [0, nmesh - 1]
+    n = len(qc)
+    qc += [len(cells_dl), len(cells_dl) + 7, 0xFFFFFFFF]
+    qg += [int(cells_dl[0][0][0]) if cells_dl[0][0].size else 0] * 3
+    qc = np.array(qc, np.uint32)
+    qg = np.array(qg, np.uint32)
+    p = rng.permutation(qc.size)
+    assert n > 0
+    return qc[p], qg[p]
+
+
+def test_find_blocks_matches_downloaded_lists():
+    w = W.config(0, ncells=6)
+    cells = cases.build_cells(w)
+    cells[3] = (np.zeros(0, np.uint32), np.zeros((0, 64), np.float32))   # an empty cell
+    nmesh = int(np.prod(w.grid_length))
+    rng = np.random.default_rng(7)
+    ctx = _ctx(w)
+    ctx.upload_cells(cells)
+    isx = cases.oracle_intersections(w, 0)
+    ctx.set_intersections(isx)
+    for step in range(3):
+        print(f"find_blocks step {step}", flush=True)
+        if step > 0:
+            ctx.accelerate(0)            # lists left in pass order
+        qc, qg = _queries(cells if step == 0 else ctx.download_cells(), nmesh, rng)
+        got = ctx.find_blocks(qc, qg)
+        dl = ctx.download_cells()        # the order the lookup refers to
+        assert np.array_equal(got, _expected(dl, qc, qg)), f"step {step}"
+        assert (got != V.INVALID_ID).sum() == sum(g.size for g, _ in dl)
+    ctx.close()
+
+
+def test_find_blocks_on_large_cells():
+    """Cells of the multi-peak workload (thousands of blocks each): every block found at its
+    position, and a sample of absent GIDs refused."""
+    w = W.config(2, ncells=4)
+    cells = cases.build_cells(w)
+    ctx = _ctx(w)
+    ctx.upload_cells(cells)
+    dl = ctx.download_cells()
+    qc = np.concatenate([np.full(g.size, c, np.uint32) for c, (g, _) in enumerate(dl)])
+    qg = np.concatenate([g for g, _ in dl]).astype(np.uint32)
+    want = np.concatenate([np.arange(g.size, dtype=np.uint32) for g, _ in dl])
+    assert np.array_equal(ctx.find_blocks(qc, qg), want)
+    nmesh = int(np.prod(w.grid_length))
+    rng = np.random.default_rng(3)
+    qa = rng.integers(0, nmesh, 20000).astype(np.uint32)
+    ca = rng.integers(0, 4, 20000).astype(np.uint32)
+    assert np.array_equal(ctx.find_blocks(ca, qa), _expected(dl, ca, qa))
+    assert ctx.find_blocks(np.zeros(0, np.uint32), np.zeros(0, np.uint32)).size == 0
+    ctx.close()

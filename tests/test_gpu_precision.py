@@ -13,7 +13,6 @@ bounded loosely so that a broken codec fails: L1 of f <= 2e-4 (FP32), 3e-3 (FP16
 import json
 import os
 
-import numpy as np
 import pytest
 
 import oracle as O
@@ -27,57 +26,12 @@ NCELLS, STEPS = 8, 20
 L1_BOUND = {"f32": 2e-4, "f16": 3e-3, "bf16": 3e-2}
 
 
-def _centres(m, gids):
-    """Cell-centre velocities (n, 64, 3) of blocks gids (e = i + 4j + 16k)."""
-    L = np.array(m.len, np.int64)
-    vmin = np.array(m.vmin)
-    dv = (np.array(m.vmax) - vmin) / L / 4
-    g = np.asarray(gids, np.int64)
-    b = np.stack([g % L[0], (g // L[0]) % L[1], g // (L[0] * L[1])], -1)
-    e = np.arange(64)
-    c = np.stack([e & 3, (e >> 2) & 3, e >> 4], -1)
-    return vmin + (b[:, None, :] * 4 + c[None, :, :] + 0.5) * dv
-
-
-def _moments(m, mass, gids, f):
-    v = _centres(m, gids)
-    dv3 = np.prod((np.array(m.vmax) - np.array(m.vmin)) / np.array(m.len) / 4)
-    f = np.asarray(f, np.float64)
-    n = f.sum() * dv3
-    V = (f[..., None] * v).sum((0, 1)) * dv3 / n
-    P = mass / 3.0 * (f * ((v - V) ** 2).sum(-1)).sum() * dv3
-    return n, V, P
-
-
-def _compare(m, mass, got, ref):
-    out = []
-    for (g1, f1), (g2, f2) in zip(got, ref):
-        allg = np.union1d(g1, g2)
-        A = np.zeros((allg.size, 64))
-        B = np.zeros((allg.size, 64))
-        A[np.searchsorted(allg, g1)] = f1
-        B[np.searchsorted(allg, g2)] = f2
-        n1, V1, P1 = _moments(m, mass, g1, f1)
-        n2, V2, P2 = _moments(m, mass, g2, f2)
-        out.append({"l1": np.abs(A - B).sum() / np.abs(B).sum(), "linf": np.abs(A - B).max() / np.abs(B).max(),
-                    "n": abs(n1 - n2) / n2, "V": float(np.linalg.norm(V1 - V2) / np.linalg.norm(V2)),
-                    "dV_mps": float(np.linalg.norm(V1 - V2)), "P": abs(P1 - P2) / P2})
-    return {k: float(max(o[k] for o in out)) for k in out[0]}
-
-
 def test_storage_forms_against_fp64_run():
     w = W.config(2, ncells=NCELLS)
     cells = cases.build_cells(w)
-    m = cases.oracle_mesh(w)
     isx = [cases.oracle_intersections(w, mo) for mo in range(3)]
     print("FP64 oracle run", flush=True)
-    ref = []
-    for i, (g, f) in enumerate(cells):
-        f = f.astype(np.float64)
-        for s in range(STEPS):
-            g, f, _ = O.accelerate_cell(m, g, f, isx[s % 3][i], s % 3, w.method, w.min_value, fp64=True)
-            g, f, _ = O.adjust(m, g, f, w.min_value, w.add_width, fp64=True)
-        ref.append((g, f))
+    ref = cases.oracle_run(w, cells, isx, STEPS, fp64=True)
     report = {"workload": w.name, "cells": NCELLS, "steps": STEPS}
     for name, store, fast in (("f32", V.STORE_F32, False), ("bf16", V.STORE_BF16, False), ("f16", V.STORE_F16, False),
                               ("f32-fast", V.STORE_F32, True), ("bf16-fast", V.STORE_BF16, True)):
@@ -92,7 +46,7 @@ def test_storage_forms_against_fp64_run():
             ctx.adjust(NCELLS)
         got = ctx.download_cells()
         ctx.close()
-        report[name] = _compare(m, w.mass, got, ref)
+        report[name] = cases.compare_runs(w, got, ref)
         print(name, report[name], flush=True)
     path = os.environ.get("VACC_PRECISION_OUT")
     if path:

@@ -1844,6 +1844,31 @@ int vacc_accelerate_populations(vacc_ctx* const* ctxs, uint32_t npop, uint32_t m
    return st;
 }
 
+int vacc_find_blocks(vacc_ctx* c, uint64_t nq, const uint32_t* cells, const uint32_t* gids, uint32_t* index) {
+   if (!c) return set_err(VACC_E_ARG, "null context");
+   if (!nq) return 0;
+   if (!cells || !gids || !index) return set_err(VACC_E_ARG, "null argument");
+   int rc = 0;
+   if (c->ncells && !c->sorted && (rc = do_relist(c, -1, false))) return rc;   // ascending GIDs, same blocks
+   // queries go through the device in chunks of FIND_CHUNK: cell, GID and the answer
+   constexpr uint64_t FIND_CHUNK = 1u << 24;
+   const uint64_t chunk = std::min<uint64_t>(nq, FIND_CHUNK);
+   DevTemps t;
+   uint32_t* d = nullptr;
+   HIPCHK(t.alloc(&d, 3 * chunk * sizeof(uint32_t)));
+   for (uint64_t first = 0; first < nq; first += chunk) {
+      const uint32_t m = (uint32_t)std::min<uint64_t>(chunk, nq - first);
+      HIPCHK(hipMemcpyAsync(d, cells + first, m * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+      HIPCHK(hipMemcpyAsync(d + chunk, gids + first, m * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+      if (c->ncells) launch_find(c->off[c->lcur], c->gid[c->lcur], c->ncells, d, d + chunk, d + 2 * chunk, m, c->stream);
+      else launch_fill_u32(d + 2 * chunk, VACC_INVALID_ID, m, c->stream);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipMemcpyAsync(index + first, d + 2 * chunk, m * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+   }
+   return 0;
+}
+
 int vacc_moments(vacc_ctx* c, double* out) {
    if (!c || (!out && c->ncells)) return set_err(VACC_E_ARG, "null argument");
    if (!c->ncells) return 0;

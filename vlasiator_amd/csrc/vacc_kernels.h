@@ -226,6 +226,8 @@ void launch_encode(const float* f, const Pool& out, uint32_t n, hipStream_t s);
 void launch_decode(const Pool& in, float* f, uint32_t n, hipStream_t s);
 void launch_iota(uint32_t* dst, uint32_t n, hipStream_t s);
 void launch_fill_u32(uint32_t* dst, uint32_t v, uint32_t n, hipStream_t s);
+void launch_find(const uint32_t* off, const uint32_t* gid, uint32_t ncells, const uint32_t* qcell, const uint32_t* qgid,
+                 uint32_t* out, uint32_t nq, hipStream_t s);
 
 int device_cus();   // compute units of the current device
 

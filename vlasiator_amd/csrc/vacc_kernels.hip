@@ -1754,4 +1754,33 @@ void launch_fill_u32(uint32_t* dst, uint32_t v, uint32_t n, hipStream_t s) {
    hipLaunchKernelGGL(k_fill_u32, dim3((n + 255) / 256), dim3(256), 0, s, dst, v, n);
 }
 
+// GID -> local index (VelocityMesh::getLocalID, velocity_mesh_cpu.h:371-377): lane i finds
+// qgid[i] in the ascending GID list of cell qcell[i] by bisection.  The index is the block's
+// position within its cell (the order vacc_download_cells returns); absent blocks and cells
+// outside the context give VACC_INVALID_ID.
+__global__ void k_find(const uint32_t* off, const uint32_t* gid, uint32_t ncells, const uint32_t* qcell,
+                       const uint32_t* qgid, uint32_t* out, uint32_t nq) {
+   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+   if (i >= nq) return;
+   const uint32_t c = qcell[i], g = qgid[i];
+   uint32_t r = 0xFFFFFFFFu;
+   if (c < ncells) {
+      const uint32_t b = off[c], e = off[c + 1];
+      uint32_t lo = b, hi = e;
+      while (lo < hi) {
+         const uint32_t mid = lo + ((hi - lo) >> 1);
+         if (gid[mid] < g) lo = mid + 1;
+         else hi = mid;
+      }
+      if (lo < e && gid[lo] == g) r = lo - b;
+   }
+   out[i] = r;
+}
+
+void launch_find(const uint32_t* off, const uint32_t* gid, uint32_t ncells, const uint32_t* qcell, const uint32_t* qgid,
+                 uint32_t* out, uint32_t nq, hipStream_t s) {
+   if (nq == 0) return;
+   hipLaunchKernelGGL(k_find, dim3((nq + 255) / 256), dim3(256), 0, s, off, gid, ncells, qcell, qgid, out, nq);
+}
+
 }  // namespace vacc

@@ -18,6 +18,7 @@ PLM, PPM, PQM = 0, 1, 2
 STORE_F32, STORE_BF16, STORE_F16 = 0, 1, 2
 STATUS_WALL, STATUS_DK, STATUS_CAPACITY = 1, 2, 4
 WID3 = 64
+INVALID_ID = 0xFFFFFFFF
 
 # symbols declared in include/vacc.h
 EXPORTS = [
@@ -28,6 +29,7 @@ EXPORTS = [
     "vacc_init_peaks", "vacc_profile_begin", "vacc_profile_read", "vacc_set_fields", "vacc_step",
     "vacc_accelerate_populations", "vacc_adjust_blocks_spatial", "vacc_autotune",
     "vacc_subcycle_count", "vacc_subcycle_step", "vacc_accelerate_subcycled", "vacc_pressure",
+    "vacc_find_blocks",
 ]
 
 
@@ -112,6 +114,7 @@ def lib():
         L.vacc_adjust_blocks_spatial.argtypes = [vp, vp, vp, vp, vp, vp]
         L.vacc_autotune.argtypes = [vp, vp]
         L.vacc_pressure.argtypes = [vp, vp, vp]
+        L.vacc_find_blocks.argtypes = [vp, C.c_uint64, vp, vp, vp]
         L.vacc_subcycle_count.argtypes = [vp, C.c_double, C.POINTER(C.c_uint32)]
         L.vacc_subcycle_step.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_int32, C.c_int32, vp]
         L.vacc_accelerate_subcycled.argtypes = [vp, C.c_double, C.c_uint32, C.c_int32, C.POINTER(C.c_uint32), vp]
@@ -325,6 +328,17 @@ class Context:
         out = np.zeros(max(ncells, 1) * 4, np.float64)
         _check(lib().vacc_moments(self.h, _ptr(out)))
         return out[:ncells * 4].reshape(ncells, 4)
+
+    def find_blocks(self, cells, gids):
+        """Position of each (cell, GID) within its cell's ascending-GID block list, or
+        INVALID_ID (VelocityMesh::getLocalID)."""
+        qc = np.ascontiguousarray(cells, np.uint32)
+        qg = np.ascontiguousarray(gids, np.uint32)
+        if qc.shape != qg.shape:
+            raise ValueError("cells and gids differ in length")
+        out = np.zeros(max(qc.size, 1), np.uint32)
+        _check(lib().vacc_find_blocks(self.h, qc.size, _ptr(qc), _ptr(qg), _ptr(out)))
+        return out[:qc.size]
 
     def pressure(self, ncells, pivot=None):
         """Pressure tensor per cell (xx, yy, zz, yz, xz, xy) about pivot[ncells, 3] or, for None,
