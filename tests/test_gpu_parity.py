@@ -329,9 +329,10 @@ def test_fast_arith_within_tolerance(cfg, method):
     the cell's max f (L-inf) and L1 norm (L1):
       against the FP64 run (oracle with Realf = double):   L-inf <= 5e-6, L1 <= 5e-6,
                                                           density <= 1e-7
-      against the FP32 oracle (the reference's FP32 path):  L-inf <= 5e-5, L1 <= 3e-5.
-    Three remaps per step; the FP32 path's own distance from the FP64 run is printed (its
-    FP32 Lagrangian coordinates carry about 1e-5 of a cell per pass)."""
+      against the FP32 oracle (the reference's FP32 path):  L-inf <= 2e-4, L1 <= 3e-5.
+    Three remaps per step.  The FP64 bound is the gate; the FP32 path is itself up to 9.2e-5
+    (L-inf) from the FP64 run on the PQM case (its FP32 Lagrangian coordinates carry about
+    1e-5 of a cell per pass), so the distance to it is bounded loosely and printed."""
     w = W.config(cfg, ncells=8)
     cells = cases.build_cells(w)
     m = cases.oracle_mesh(w)
@@ -367,7 +368,7 @@ def test_fast_arith_within_tolerance(cfg, method):
     print("fast arith worst: vs FP64 (Linf, L1, density)", worst["fp64"], "vs FP32", worst["fp32"],
           "FP32 oracle vs FP64", worst["ref"], flush=True)
     assert worst["fp64"][0] <= 5e-6 and worst["fp64"][1] <= 5e-6 and worst["fp64"][2] <= 1e-7
-    assert worst["fp32"][0] <= 5e-5 and worst["fp32"][1] <= 3e-5
+    assert worst["fp32"][0] <= 2e-4 and worst["fp32"][1] <= 3e-5
     ctx.close()
 
 
