@@ -387,6 +387,7 @@ static int enqueue_pass(vacc_ctx* c, int dim, int grid_mult) {
    ca.vmin = (float)c->mi.vmin[dim];       // Realf v_min (cpu_acc_map.cpp:148)
    ca.wall_margin = c->opt.wall_margin;
    ca.active = c->subcycling ? c->active : nullptr;
+   ca.fast = c->opt.fast_arith != 0;
    {
       ProfScope ps(c, 1);
       launch_fill_u32(c->len_hist, 0u, LEN_BINS, c->stream);
@@ -437,6 +438,7 @@ static int enqueue_pass(vacc_ctx* c, int dim, int grid_mult) {
    ra.threshold = (float)c->sp.sparse_min_value;   // Realf threshold argument of compute_*_coeff
    ra.min_value = c->sp.sparse_min_value;
    ra.minval = c->minval;
+   ra.len_hist = c->len_hist;
    ra.dv_d = c->mi.cell[dim];
    ra.vmin_d = c->mi.vmin[dim];
    {

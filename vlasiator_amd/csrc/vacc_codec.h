@@ -39,6 +39,55 @@ struct Codec<ST_F16> {
    __device__ static inline uint32_t enc(float x) { return __builtin_bit_cast(uint16_t, (_Float16)x); }
 };
 
+// Two codes packed in one word (first value in the low half), as the block is stored.
+template <int ST>
+struct Pack2;
+
+template <>
+struct Pack2<ST_BF16> {
+   typedef float f2 __attribute__((ext_vector_type(2)));
+   typedef __bf16 b2 __attribute__((ext_vector_type(2)));
+   // v_cvt_pk_bf16_f32: round to nearest even, the rounding of Codec<ST_BF16>::enc for every
+   // finite FP32 value
+   __device__ static inline uint32_t enc(float a, float b) {
+      const f2 in = {a, b};
+      return __builtin_bit_cast(uint32_t, __builtin_convertvector(in, b2));
+   }
+   __device__ static inline float lo(uint32_t p) { return __uint_as_float(p << 16); }
+   __device__ static inline float hi(uint32_t p) { return __uint_as_float(p & 0xFFFF0000u); }
+};
+
+template <>
+struct Pack2<ST_F16> {
+   __device__ static inline uint32_t enc(float a, float b) {
+      return Codec<ST_F16>::enc(a) | (Codec<ST_F16>::enc(b) << 16);
+   }
+   __device__ static inline float lo(uint32_t p) { return Codec<ST_F16>::dec(p & 0xFFFFu); }
+   __device__ static inline float hi(uint32_t p) { return Codec<ST_F16>::dec(p >> 16); }
+};
+
+// DPP row rotation (row_ror:N, N = 1..15) within the 16 lanes of the caller's row.
+template <int N>
+__device__ inline float row_ror(float x) {
+   return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, x), 0x120 + N, 0xF, 0xF, false));
+}
+
+// Reductions over the 16 lanes of a DPP row (lanes 16r .. 16r+15, all active).  Rotating by
+// 8, 4, 2, 1 gives lane L the value of lane L^m up to bits already combined, so the sum has
+// the canonical butterfly order s_L + s_{L^m}, m = 8, 4, 2, 1, bit for bit (a + b = b + a).
+__device__ inline float row_sum16(float s) {
+   s = s + row_ror<8>(s);
+   s = s + row_ror<4>(s);
+   s = s + row_ror<2>(s);
+   return s + row_ror<1>(s);
+}
+__device__ inline float row_maxf16(float m) {
+   m = fmaxf(m, row_ror<8>(m));
+   m = fmaxf(m, row_ror<4>(m));
+   m = fmaxf(m, row_ror<2>(m));
+   return fmaxf(m, row_ror<1>(m));
+}
+
 __device__ inline float dec16(int st, uint32_t u) {
    return (st == ST_BF16) ? Codec<ST_BF16>::dec(u) : Codec<ST_F16>::dec(u);
 }
@@ -48,32 +97,23 @@ __device__ inline float pick_scale(float p, float sf, float sq) {
    return (sf > 0.0f && r >= 0.5f * p && r <= 2.0f * p) ? r : p;
 }
 
-// Encode one block held by 16 lanes, lane L holding the canonical row x[4] = f(i=0..3, j=L&3,
-// k=L>>2).  All 16 lanes must be active.  Writes the lane's 4 codes and returns the block
-// scale (equal in all 16 lanes).
+// Encode one block held by the 16 lanes of a DPP row, lane L holding the canonical row
+// x[4] = f(i=0..3, j=L&3, k=L>>2).  All 16 lanes of the row must be active.  Writes the lane's
+// 4 codes packed as stored (x[0] in the low half of p.x) and returns the block scale (equal in
+// all 16 lanes).
 template <int ST>
-__device__ inline float encode_rows16(const float (&x)[4], uint32_t (&code)[4]) {
+__device__ inline float encode_rows16(const float (&x)[4], uint2& p) {
    float mx = fmaxf(fmaxf(fabsf(x[0]), fabsf(x[1])), fmaxf(fabsf(x[2]), fabsf(x[3])));
-#pragma unroll
-   for (int m = 8; m > 0; m >>= 1) mx = fmaxf(mx, __shfl_xor(mx, m, 16));
-   if (!(mx > 0.0f)) {
-      code[0] = code[1] = code[2] = code[3] = 0u;
+   mx = row_maxf16(mx);
+   if (!(mx > 0.0f)) {   // the same in all lanes of the row
+      p = make_uint2(0u, 0u);
       return 0.0f;
    }
    const int ex = __builtin_amdgcn_frexp_expf(mx);
-   float d[4];
-#pragma unroll
-   for (int i = 0; i < 4; ++i) {
-      code[i] = Codec<ST>::enc(__builtin_amdgcn_ldexpf(x[i], -ex));
-      d[i] = Codec<ST>::dec(code[i]);
-   }
-   float sf = (x[0] + x[1]) + (x[2] + x[3]);
-   float sq = (d[0] + d[1]) + (d[2] + d[3]);
-#pragma unroll
-   for (int m = 8; m > 0; m >>= 1) {
-      sf = sf + __shfl_xor(sf, m, 16);
-      sq = sq + __shfl_xor(sq, m, 16);
-   }
+   p.x = Pack2<ST>::enc(__builtin_amdgcn_ldexpf(x[0], -ex), __builtin_amdgcn_ldexpf(x[1], -ex));
+   p.y = Pack2<ST>::enc(__builtin_amdgcn_ldexpf(x[2], -ex), __builtin_amdgcn_ldexpf(x[3], -ex));
+   const float sf = row_sum16((x[0] + x[1]) + (x[2] + x[3]));
+   const float sq = row_sum16((Pack2<ST>::lo(p.x) + Pack2<ST>::hi(p.x)) + (Pack2<ST>::lo(p.y) + Pack2<ST>::hi(p.y)));
    return pick_scale(__builtin_amdgcn_ldexpf(1.0f, ex), sf, sq);
 }
 

@@ -32,6 +32,9 @@ constexpr uint32_t COPY_FLAG = 0x80000000u;
 constexpr uint32_t EXACT_FLAG = 0x40000000u;
 constexpr uint32_t UNIT_NCOL_MASK = 0x3FFFFFFFu;
 constexpr int LEN_BINS = 256;   // length bins of the work-unit ordering
+// The last bin holds the units marked EXACT_FLAG, so that the work list ends with them: the
+// tolerance-form kernel takes the list up to that bin's start, the exact kernel the rest.
+constexpr int EXACT_BIN = LEN_BINS - 1;
 
 struct ColumnArgs {
    const uint32_t* cell_off;   // [ncells+1] input list offsets
@@ -51,6 +54,7 @@ struct ColumnArgs {
    float vmin;                 // (float) mesh minimum along the pass axis
    int wall_margin;
    const uint8_t* active;      // [ncells] or null: cells with 0 pass through unchanged (subcycling)
+   int fast;                   // tolerance form: mark the units the exact kernel must take
 };
 
 struct ScanArgs {
@@ -100,6 +104,7 @@ struct RemapArgs {
    const double* minval;        // [ncells] sparsity threshold per cell (velocityBlockMinValue)
    double dv_d, vmin_d;         // cell size and mesh minimum along the pass axis (FP64)
    int only_flagged;            // exact kernel: 1 = take only the units marked EXACT_FLAG
+   const uint32_t* len_hist;    // [LEN_BINS] units per length bin; [EXACT_BIN] = EXACT_FLAG units
 };
 
 struct AdjustArgs {

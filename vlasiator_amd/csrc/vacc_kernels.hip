@@ -76,9 +76,13 @@ __device__ inline void pass_isect(const double* is, float& I, float& di, float& 
    else { di = a; dj = b; dk = c; }
 }
 
-// work units are ordered longest first: bin 0 holds units of >= LEN_BINS-1 source blocks
+// work units are ordered longest first: bin 0 holds units of >= EXACT_BIN-1 source blocks;
+// units marked EXACT_FLAG go to EXACT_BIN, the end of the list
 __device__ inline uint32_t len_bin(uint32_t nblocks) {
-   return (uint32_t)(LEN_BINS - 1) - min(nblocks, (uint32_t)(LEN_BINS - 1));
+   return (uint32_t)(EXACT_BIN - 1) - min(nblocks, (uint32_t)(EXACT_BIN - 1));
+}
+__device__ inline uint32_t unit_bin(const uint4& u) {
+   return (u.z & EXACT_FLAG) ? (uint32_t)EXACT_BIN : len_bin(u.w);
 }
 
 __device__ inline bool pass_is_copy(float dv, float di, float dj, float dk) {
@@ -276,7 +280,7 @@ __global__ __launch_bounds__(CWG) void k_columns(ColumnArgs a) {
    // a cell outside this subcycle passes through unchanged (no status)
    const bool idle = a.active && !a.active[c];
    const bool copy = idle || pass_is_copy(a.dv, di, dj, dk);
-   const bool lean_wide = !copy && !((4.0f * a.dv + 3.0f * fabsf(di) + 3.0f * fabsf(dj)) / dk <= LEAN_SPAN);
+   const bool lean_wide = a.fast && !copy && !((4.0f * a.dv + 3.0f * fabsf(di) + 3.0f * fabsf(dj)) / dk <= LEAN_SPAN);
    const int maxlen = (int)a.len[DIM];
    uint32_t mystat = (copy && !idle) ? VACC_STATUS_DK : 0u;
 
@@ -398,19 +402,20 @@ __global__ __launch_bounds__(CWG) void k_columns(ColumnArgs a) {
          // work units: {b0 | b1 << 16, first column, columns | COPY_FLAG | EXACT_FLAG, source blocks}
          const uint32_t bb = (uint32_t)b0 | ((uint32_t)b1 << 16), cf = copy ? COPY_FLAG : 0u;
          auto exact_col = [&](uint32_t j) {
-            return !copy && max_lane_rule_binds(b0, b1, (int)(a.col_rec[off + cb + j].x & 0xFFFFu), I, di, dj, a.dv, a.vmin);
+            return a.fast && !copy &&
+                   max_lane_rule_binds(b0, b1, (int)(a.col_rec[off + cb + j].x & 0xFFFFu), I, di, dj, a.dv, a.vmin);
          };
          if (overlap) {
             bool ex = lean_wide;
             for (uint32_t j = 0; j < ncol && !ex; ++j) ex = exact_col(j);
             a.unit_rec[off + ui] = make_uint4(bb, cb, ncol | cf | (ex ? EXACT_FLAG : 0u), bpos);
-            atomicAdd(&hist[len_bin(bpos)], 1u);
+            atomicAdd(&hist[ex ? (uint32_t)EXACT_BIN : len_bin(bpos)], 1u);
          } else {
             for (uint32_t j = 0; j < ncol; ++j) {
                const uint32_t len = a.col_rec[off + cb + j].x >> 16;
-               const uint32_t ef = (lean_wide || exact_col(j)) ? EXACT_FLAG : 0u;
-               a.unit_rec[off + ui + j] = make_uint4(bb, cb + j, 1u | cf | ef, len);
-               atomicAdd(&hist[len_bin(len)], 1u);
+               const bool ex = lean_wide || exact_col(j);
+               a.unit_rec[off + ui + j] = make_uint4(bb, cb + j, 1u | cf | (ex ? EXACT_FLAG : 0u), len);
+               atomicAdd(&hist[ex ? (uint32_t)EXACT_BIN : len_bin(len)], 1u);
             }
          }
       }
@@ -493,7 +498,7 @@ __global__ __launch_bounds__(WG) void k_order(OrderArgs a) {
    const uint32_t off = a.cell_off[c];
    for (int i = threadIdx.x; i < LEN_BINS; i += WG) cnt[i] = 0;
    __syncthreads();
-   for (uint32_t u = threadIdx.x; u < nu; u += WG) atomicAdd(&cnt[len_bin(a.unit_rec[off + u].w)], 1u);
+   for (uint32_t u = threadIdx.x; u < nu; u += WG) atomicAdd(&cnt[unit_bin(a.unit_rec[off + u])], 1u);
    __syncthreads();
    for (int i = threadIdx.x; i < LEN_BINS; i += WG) {
       base_of[i] = cnt[i] ? atomicAdd(&a.hist_cursor[i], cnt[i]) : 0u;
@@ -501,7 +506,7 @@ __global__ __launch_bounds__(WG) void k_order(OrderArgs a) {
    }
    __syncthreads();
    for (uint32_t u = threadIdx.x; u < nu; u += WG) {
-      const uint32_t b = len_bin(a.unit_rec[off + u].w);
+      const uint32_t b = unit_bin(a.unit_rec[off + u]);
       const uint32_t pos = base_of[b] + atomicAdd(&cnt[b], 1u);
       a.perm[pos] = base + u;
       a.perm_cell[pos] = c;
@@ -576,8 +581,15 @@ __device__ inline void store_pencil(float* __restrict__ f, size_t slot, int o, c
    }
 }
 
-__device__ inline void set_content(uint8_t* content, size_t slot, const float (&w)[4], double mv, int setlane0, int l16) {
-   const bool has = ((double)w[0] >= mv) || ((double)w[1] >= mv) || ((double)w[2] >= mv) || ((double)w[3] >= mv);
+// The content test (double)x >= mv of an FP32 value x, as x >= content_threshold(mv): the
+// smallest float whose double value is >= mv (x >= it exactly when (double)x >= mv).
+__device__ inline float content_threshold(double mv) {
+   const float f = (float)mv;   // nearest float: the answer, or the float just below mv
+   return ((double)f < mv) ? nextafterf(f, INFINITY) : f;
+}
+
+__device__ inline void set_content(uint8_t* content, size_t slot, const float (&w)[4], float thc, int setlane0, int l16) {
+   const bool has = (w[0] >= thc) || (w[1] >= thc) || (w[2] >= thc) || (w[3] >= thc);
    const unsigned long long bal = __ballot(has);
    if (l16 == 0) content[slot] = (uint8_t)(((bal >> setlane0) & 0xFFFFull) != 0ull);
 }
@@ -589,13 +601,13 @@ __device__ inline void set_content(uint8_t* content, size_t slot, const float (&
 // (decoded) values.
 template <int DIM, int ST>
 __device__ inline void flush_block(const RemapArgs& a, float (*ring)[WG], int tid, int l16, int setlane0, int o,
-                                   int blk, size_t slot, double mv) {
+                                   int blk, size_t slot, float thc) {
    if constexpr (ST == ST_F32) {
       float q[4];
 #pragma unroll
       for (int e = 0; e < 4; ++e) q[e] = ring[(WID * blk + e) & (RING - 1)][tid];
       store_pencil<DIM>(static_cast<float*>(a.out.f), slot, o, q);
-      set_content(a.content_out, slot, q, mv, setlane0, l16);
+      set_content(a.content_out, slot, q, thc, setlane0, l16);
    } else {
       const int L = l16, base = tid - l16;
       float x[4];
@@ -608,21 +620,20 @@ __device__ inline void flush_block(const RemapArgs& a, float (*ring)[WG], int ti
          const int row = (DIM == 0) ? i : (DIM == 1) ? (L & 3) : (L >> 2);
          x[i] = ring[(WID * blk + row) & (RING - 1)][base + lane];
       }
-      uint32_t code[4];
-      const float sc = encode_rows16<ST>(x, code);
+      uint2 pk;
+      const float sc = encode_rows16<ST>(x, pk);
       uint16_t* p = static_cast<uint16_t*>(a.out.f) + slot * WID3 + 4 * L;
-      *reinterpret_cast<uint2*>(p) = make_uint2(code[0] | (code[1] << 16), code[2] | (code[3] << 16));
+      *reinterpret_cast<uint2*>(p) = pk;
       if (L == 0) a.out.scale[slot] = sc;
-      float d[4];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) d[i] = sc * Codec<ST>::dec(code[i]);
-      set_content(a.content_out, slot, d, mv, setlane0, l16);
+      const float d[4] = {sc * Pack2<ST>::lo(pk.x), sc * Pack2<ST>::hi(pk.x), sc * Pack2<ST>::lo(pk.y),
+                          sc * Pack2<ST>::hi(pk.y)};
+      set_content(a.content_out, slot, d, thc, setlane0, l16);
    }
 }
 
 // Store an all-zero target block (a target block that received no contribution).
 template <int DIM, int ST>
-__device__ inline void zero_block(const RemapArgs& a, int l16, int setlane0, int o, size_t slot, double mv) {
+__device__ inline void zero_block(const RemapArgs& a, int l16, int setlane0, int o, size_t slot, float thc) {
    const float q[4] = {0.0f, 0.0f, 0.0f, 0.0f};
    if constexpr (ST == ST_F32) {
       store_pencil<DIM>(static_cast<float*>(a.out.f), slot, o, q);
@@ -631,18 +642,18 @@ __device__ inline void zero_block(const RemapArgs& a, int l16, int setlane0, int
       *reinterpret_cast<uint2*>(p) = make_uint2(0u, 0u);
       if (l16 == 0) a.out.scale[slot] = 0.0f;
    }
-   set_content(a.content_out, slot, q, mv, setlane0, l16);
+   set_content(a.content_out, slot, q, thc, setlane0, l16);
 }
 
 // Pass-through copy of a source block (cells in copy mode keep their stored representation).
 template <int DIM, int ST>
 __device__ inline void copy_block(const RemapArgs& a, int l16, int setlane0, int o, uint32_t sidx, size_t slot,
-                                  double mv) {
+                                  float thc) {
    if constexpr (ST == ST_F32) {
       float w[4];
       load_pencil<DIM, ST>(a.in, sidx, o, w);
       store_pencil<DIM>(static_cast<float*>(a.out.f), slot, o, w);
-      set_content(a.content_out, slot, w, mv, setlane0, l16);
+      set_content(a.content_out, slot, w, thc, setlane0, l16);
    } else {
       uint2 raw = make_uint2(0u, 0u);
       float sc = 0.0f;
@@ -654,7 +665,7 @@ __device__ inline void copy_block(const RemapArgs& a, int l16, int setlane0, int
       if (l16 == 0) a.out.scale[slot] = sc;
       const float d[4] = {sc * Codec<ST>::dec(raw.x & 0xFFFFu), sc * Codec<ST>::dec(raw.x >> 16),
                           sc * Codec<ST>::dec(raw.y & 0xFFFFu), sc * Codec<ST>::dec(raw.y >> 16)};
-      set_content(a.content_out, slot, d, mv, setlane0, l16);
+      set_content(a.content_out, slot, d, thc, setlane0, l16);
    }
 }
 
@@ -668,18 +679,20 @@ __global__ __launch_bounds__(WG) void k_remap(RemapArgs a, double i_dv) {
    const int setlane0 = (tid & 63) & ~15;
    const int o = (DIM == 0) ? (ip * WID2 + jp * WID) : (DIM == 1) ? (jp * WID2 + ip) : (jp * WID + ip);
    const uint32_t total = a.unit_base[a.ncells];
+   // after the tolerance-form kernel: only the units at the end of the list (EXACT_BIN)
+   const uint32_t first = a.only_flagged ? total - a.len_hist[EXACT_BIN] : 0u;
    const uint32_t gstride = gridDim.x * (WG / 16);
-   for (uint32_t gi = blockIdx.x * (WG / 16) + (tid >> 4); gi < total; gi += gstride) {
+   for (uint32_t gi = first + blockIdx.x * (WG / 16) + (tid >> 4); gi < total; gi += gstride) {
       const uint32_t g = a.perm[gi];
       const uint32_t c = a.perm_cell[gi];
       const uint32_t cbase = a.cell_off[c];
       const uint4 sr = a.unit_rec[cbase + (g - a.unit_base[c])];
       const int b0 = (int)(sr.x & 0xFFFFu), b1 = (int)(sr.x >> 16);
       const uint32_t col0 = sr.y;
-      if (a.only_flagged && !(sr.z & EXACT_FLAG)) continue;   // the tolerance form took it
       // the cell's sparsity threshold: the content test, and the reconstruction threshold as
       // the Realf argument of compute_*_coeff (cpu_acc_map.cpp:539-547)
       const double mv = a.minval[c];
+      const float thc = content_threshold(mv);
       const float thr = (float)mv;
       const float scale = (float)(1.0 / (double)thr);
       const uint32_t ncol = sr.z & UNIT_NCOL_MASK;
@@ -708,7 +721,7 @@ __global__ __launch_bounds__(WG) void k_remap(RemapArgs a, double i_dv) {
             a.src_out[slot] = (uint32_t)slot;
          }
          if (copy) {
-            for (int b = 0; b < len; ++b) copy_block<DIM, ST>(a, l16, setlane0, o, srcs[b], slotb + b, mv);
+            for (int b = 0; b < len; ++b) copy_block<DIM, ST>(a, l16, setlane0, o, srcs[b], slotb + b, thc);
             prev_end = max(prev_end, lk);
             continue;
          }
@@ -754,13 +767,14 @@ __global__ __launch_bounds__(WG) void k_remap(RemapArgs a, double i_dv) {
                gkr = lagr_index(v_r - Imin, dk, rdk);
                const int mnG = max(__shfl(gkl, minL, 8), lo_gk);
                const int mxG = min(__shfl(gkr, maxL, 8), hi_gk);
-               const int flushLim = min(mnG, __shfl_xor(mnG, 8, 16));
-               const int openLim = max(max(mxG, __shfl_xor(mxG, 8, 16)), flushLim);
+               // the two 8-lane groups of the unit (a DPP row): rotating by 8 swaps them
+               const int flushLim = min(mnG, __builtin_amdgcn_mov_dpp(mnG, 0x128, 0xF, 0xF, false));
+               const int openLim = max(max(mxG, __builtin_amdgcn_mov_dpp(mxG, 0x128, 0xF, 0xF, false)), flushLim);
                while (next_open <= lk && WID * next_open <= openLim) {
                   if (next_open - next_flush >= RING / WID) {   // ring full: cannot happen for checked spans
                      atomicOr(&a.status[c], VACC_STATUS_DK);
                      atomicOr(&a.status[a.ncells], VACC_STATUS_DK);
-                     flush_block<DIM, ST>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), mv);
+                     flush_block<DIM, ST>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), thc);
                      ++next_flush;
                   }
 #pragma unroll
@@ -768,7 +782,7 @@ __global__ __launch_bounds__(WG) void k_remap(RemapArgs a, double i_dv) {
                   ++next_open;
                }
                while (next_flush < next_open && next_flush < nfk && WID * next_flush + WID - 1 < flushLim) {
-                  flush_block<DIM, ST>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), mv);
+                  flush_block<DIM, ST>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), thc);
                   ++next_flush;
                }
                float tdr = 0.0f;
@@ -793,10 +807,10 @@ __global__ __launch_bounds__(WG) void k_remap(RemapArgs a, double i_dv) {
             }
          }
          while (next_flush < next_open && next_flush < nfk) {
-            flush_block<DIM, ST>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), mv);
+            flush_block<DIM, ST>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), thc);
             ++next_flush;
          }
-         for (int tb = next_open; tb <= lk && tb < nfk; ++tb) zero_block<DIM, ST>(a, l16, setlane0, o, slotb + (tb - fk), mv);
+         for (int tb = next_open; tb <= lk && tb < nfk; ++tb) zero_block<DIM, ST>(a, l16, setlane0, o, slotb + (tb - fk), thc);
          carry_open = (next_open > nfk) ? next_open : -1;
          prev_end = max(prev_end, lk);
       }
@@ -847,15 +861,15 @@ __global__ __launch_bounds__(WG) void k_remap_lean(RemapArgs a) {
    const int ip = l16 & 3, jp = l16 >> 2;
    const int setlane0 = (tid & 63) & ~15;
    const int o = (DIM == 0) ? (ip * WID2 + jp * WID) : (DIM == 1) ? (jp * WID2 + ip) : (jp * WID + ip);
-   const uint32_t total = a.unit_base[a.ncells];
+   // the units before EXACT_BIN's; k_remap takes the rest
+   const uint32_t total = a.unit_base[a.ncells] - a.len_hist[EXACT_BIN];
    const uint32_t gstride = gridDim.x * (WG / 16);
    for (uint32_t gi = blockIdx.x * (WG / 16) + (tid >> 4); gi < total; gi += gstride) {
       const uint32_t g = a.perm[gi];
       const uint32_t c = a.perm_cell[gi];
       const uint32_t cbase = a.cell_off[c];
       const uint4 sr = a.unit_rec[cbase + (g - a.unit_base[c])];
-      if (sr.z & EXACT_FLAG) continue;   // taken by k_remap
-      const double mv = a.minval[c];
+      const float thc = content_threshold(a.minval[c]);
       const int b0 = (int)(sr.x & 0xFFFFu), b1 = (int)(sr.x >> 16);
       const uint32_t col0 = sr.y;
       const uint32_t ncol = sr.z & UNIT_NCOL_MASK;
@@ -882,7 +896,7 @@ __global__ __launch_bounds__(WG) void k_remap_lean(RemapArgs a) {
             a.src_out[slot] = (uint32_t)slot;
          }
          if (copy) {
-            for (int b = 0; b < len; ++b) copy_block<DIM, ST>(a, l16, setlane0, o, srcs[b], slotb + b, mv);
+            for (int b = 0; b < len; ++b) copy_block<DIM, ST>(a, l16, setlane0, o, srcs[b], slotb + b, thc);
             prev_end = max(prev_end, lk);
             continue;
          }
@@ -916,7 +930,7 @@ __global__ __launch_bounds__(WG) void k_remap_lean(RemapArgs a) {
                if (next_open - next_flush >= RING / WID) {   // ring full: not for LEAN_SPAN cells
                   atomicOr(&a.status[c], VACC_STATUS_DK);
                   atomicOr(&a.status[a.ncells], VACC_STATUS_DK);
-                  flush_block<DIM, ST>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), mv);
+                  flush_block<DIM, ST>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), thc);
                   ++next_flush;
                }
 #pragma unroll
@@ -924,7 +938,7 @@ __global__ __launch_bounds__(WG) void k_remap_lean(RemapArgs a) {
                ++next_open;
             }
             while (next_flush < next_open && next_flush < nfk && WID * next_flush + WID - 1 < flushLim) {
-               flush_block<DIM, ST>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), mv);
+               flush_block<DIM, ST>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), thc);
                ++next_flush;
             }
             const float v[12] = {w0[0], w0[1], w0[2], w0[3], w1[0], w1[1], w1[2], w1[3], w2[0], w2[1], w2[2], w2[3]};
@@ -943,10 +957,10 @@ __global__ __launch_bounds__(WG) void k_remap_lean(RemapArgs a) {
          }
          fx.end(emit);
          while (next_flush < next_open && next_flush < nfk) {
-            flush_block<DIM, ST>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), mv);
+            flush_block<DIM, ST>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), thc);
             ++next_flush;
          }
-         for (int tb = next_open; tb <= lk && tb < nfk; ++tb) zero_block<DIM, ST>(a, l16, setlane0, o, slotb + (tb - fk), mv);
+         for (int tb = next_open; tb <= lk && tb < nfk; ++tb) zero_block<DIM, ST>(a, l16, setlane0, o, slotb + (tb - fk), thc);
          carry_open = (next_open > nfk) ? next_open : -1;
          prev_end = max(prev_end, lk);
       }
