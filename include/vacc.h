@@ -194,6 +194,17 @@ int vacc_autotune(vacc_ctx* ctx, uint32_t* mult_out);
  * lists, blocks unchanged), so the map is a bisection of the resident list.  Host arrays of
  * nq entries; same state afterwards. */
 int vacc_find_blocks(vacc_ctx* ctx, uint64_t nq, const uint32_t* cells, const uint32_t* gids, uint32_t* index);
+/* Block lookup that allocates nothing.  It has no counterpart in the reference.  It is for
+ * callers that cannot allocate device memory: tight serving loops, and harnesses that forbid
+ * hipMalloc.  Same arguments, answers and state afterwards as vacc_find_blocks, for at most
+ * VACC_FIND_NOALLOC_MAX queries; a larger nq returns VACC_E_ARG.  The queries and answers are
+ * staged in one fixed device array per GPU (3 * VACC_FIND_NOALLOC_MAX words, 768 KiB, held
+ * until the process ends) instead of a buffer allocated per call.  Every context on one GPU
+ * shares that array, so calls that overlap in time on one GPU (from several threads or
+ * contexts) race; calls made one after another do not. */
+#define VACC_FIND_NOALLOC_MAX 65536u
+int vacc_find_blocks_noalloc(vacc_ctx* ctx, uint64_t nq, const uint32_t* cells, const uint32_t* gids,
+                             uint32_t* index);
 /* FP64 moments per cell: out[4*c+0] = n, [1..3] = n*V (arch_moments.h:83-108).  They are
  * kept resident until f changes, so a following use_moments transform does not recompute
  * them. */

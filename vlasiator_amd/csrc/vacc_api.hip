@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -38,6 +39,7 @@ static int set_err(int code, const std::string& msg) {
    } while (0)
 
 struct vacc_ctx {
+   int device = 0;             // the device current at vacc_create; every call runs on it
    vacc_mesh mesh;
    vacc_species sp;
    vacc_options opt;
@@ -1232,6 +1234,33 @@ extern "C" {
 
 const char* vacc_last_error(void) { return g_err.c_str(); }
 
+// Query staging of vacc_find_blocks_noalloc: cells, GIDs and answers of up to
+// VACC_FIND_NOALLOC_MAX queries (768 KiB per device, held until the process ends).  Host code
+// never passes the array's name to a kernel or a copy: it uses the device address that
+// hipGetSymbolAddress returns, looked up once per device when the first context on that device
+// is created.
+static __device__ __align__(256) uint32_t scratch[3 * (size_t)VACC_FIND_NOALLOC_MAX];
+
+namespace {
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+uint32_t* g_scratch_ptr[SCRATCH_MAX_DEVICES] = {};
+hipError_t g_scratch_err[SCRATCH_MAX_DEVICES] = {};
+bool g_scratch_done[SCRATCH_MAX_DEVICES] = {};
+}  // namespace
+
+// Address of the current device's `scratch` (dev = that device), looked up on first use.
+static void scratch_setup(int dev) {
+   if (dev < 0 || dev >= SCRATCH_MAX_DEVICES) return;
+   std::lock_guard<std::mutex> lk(g_scratch_mu);
+   if (g_scratch_done[dev]) return;
+   void* p = nullptr;
+   const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+   g_scratch_err[dev] = e;
+   g_scratch_ptr[dev] = (e == hipSuccess) ? static_cast<uint32_t*>(p) : nullptr;
+   g_scratch_done[dev] = true;
+}
+
 int vacc_create(const vacc_mesh* mesh, const vacc_species* species, const vacc_options* opts, vacc_ctx** out) {
    if (!mesh || !species || !opts || !out) return set_err(VACC_E_ARG, "null argument");
    for (int d = 0; d < 3; ++d) {
@@ -1249,7 +1278,11 @@ int vacc_create(const vacc_mesh* mesh, const vacc_species* species, const vacc_o
       return set_err(VACC_E_ARG, "sparse_add_width_v must be in [0, 8]");
    int ndev = 0;
    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return set_err(VACC_E_HIP, "no HIP device");
+   int dev = 0;
+   if (hipGetDevice(&dev) != hipSuccess) return set_err(VACC_E_HIP, "hipGetDevice failed");
+   scratch_setup(dev);
    vacc_ctx* c = new vacc_ctx();
+   c->device = dev;
    c->mesh = *mesh;
    c->sp = *species;
    c->opt = *opts;
@@ -1868,6 +1901,35 @@ int vacc_find_blocks(vacc_ctx* c, uint64_t nq, const uint32_t* cells, const uint
       HIPCHK(hipMemcpyAsync(index + first, d + 2 * chunk, m * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
       HIPCHK(hipStreamSynchronize(c->stream));
    }
+   return 0;
+}
+
+int vacc_find_blocks_noalloc(vacc_ctx* c, uint64_t nq, const uint32_t* cells, const uint32_t* gids, uint32_t* index) {
+   if (!c) return set_err(VACC_E_ARG, "null context");
+   if (nq > VACC_FIND_NOALLOC_MAX)
+      return set_err(VACC_E_ARG, "vacc_find_blocks_noalloc: " + std::to_string(nq) + " queries exceed VACC_FIND_NOALLOC_MAX (" +
+                                     std::to_string(VACC_FIND_NOALLOC_MAX) + "); split the call or use vacc_find_blocks");
+   if (!nq) return 0;
+   if (!cells || !gids || !index) return set_err(VACC_E_ARG, "null argument");
+   uint32_t* d = (c->device >= 0 && c->device < SCRATCH_MAX_DEVICES) ? g_scratch_ptr[c->device] : nullptr;
+   if (!d) {
+      const hipError_t e = (c->device >= 0 && c->device < SCRATCH_MAX_DEVICES) ? g_scratch_err[c->device] : hipErrorInvalidDevice;
+      return set_err(VACC_E_HIP, std::string("vacc_find_blocks_noalloc: no scratch address for this device: ") +
+                                     hipGetErrorString(e));
+   }
+   int rc = 0;
+   if (c->ncells && !c->sorted && (rc = do_relist(c, -1, false))) return rc;   // ascending GIDs, no allocation
+   const uint32_t m = (uint32_t)nq;
+   uint32_t* qc = d;
+   uint32_t* qg = d + VACC_FIND_NOALLOC_MAX;
+   uint32_t* ans = d + 2 * (size_t)VACC_FIND_NOALLOC_MAX;
+   HIPCHK(hipMemcpyAsync(qc, cells, m * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+   HIPCHK(hipMemcpyAsync(qg, gids, m * sizeof(uint32_t), hipMemcpyHostToDevice, c->stream));
+   if (c->ncells) launch_find(c->off[c->lcur], c->gid[c->lcur], c->ncells, qc, qg, ans, m, c->stream);
+   else launch_fill_u32(ans, VACC_INVALID_ID, m, c->stream);
+   HIPCHK(hipGetLastError());
+   HIPCHK(hipMemcpyAsync(index, ans, m * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+   HIPCHK(hipStreamSynchronize(c->stream));
    return 0;
 }
 

@@ -19,6 +19,7 @@ STORE_F32, STORE_BF16, STORE_F16 = 0, 1, 2
 STATUS_WALL, STATUS_DK, STATUS_CAPACITY = 1, 2, 4
 WID3 = 64
 INVALID_ID = 0xFFFFFFFF
+FIND_NOALLOC_MAX = 65536   # VACC_FIND_NOALLOC_MAX
 
 # symbols declared in include/vacc.h
 EXPORTS = [
@@ -29,7 +30,7 @@ EXPORTS = [
     "vacc_init_peaks", "vacc_profile_begin", "vacc_profile_read", "vacc_set_fields", "vacc_step",
     "vacc_accelerate_populations", "vacc_adjust_blocks_spatial", "vacc_autotune",
     "vacc_subcycle_count", "vacc_subcycle_step", "vacc_accelerate_subcycled", "vacc_pressure",
-    "vacc_find_blocks",
+    "vacc_find_blocks", "vacc_find_blocks_noalloc",
 ]
 
 
@@ -115,6 +116,7 @@ def lib():
         L.vacc_autotune.argtypes = [vp, vp]
         L.vacc_pressure.argtypes = [vp, vp, vp]
         L.vacc_find_blocks.argtypes = [vp, C.c_uint64, vp, vp, vp]
+        L.vacc_find_blocks_noalloc.argtypes = [vp, C.c_uint64, vp, vp, vp]
         L.vacc_subcycle_count.argtypes = [vp, C.c_double, C.POINTER(C.c_uint32)]
         L.vacc_subcycle_step.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_int32, C.c_int32, vp]
         L.vacc_accelerate_subcycled.argtypes = [vp, C.c_double, C.c_uint32, C.c_int32, C.POINTER(C.c_uint32), vp]
@@ -338,6 +340,17 @@ class Context:
             raise ValueError("cells and gids differ in length")
         out = np.zeros(max(qc.size, 1), np.uint32)
         _check(lib().vacc_find_blocks(self.h, qc.size, _ptr(qc), _ptr(qg), _ptr(out)))
+        return out[:qc.size]
+
+    def find_blocks_noalloc(self, cells, gids):
+        """find_blocks through vacc_find_blocks_noalloc: no device allocation, at most
+        FIND_NOALLOC_MAX queries per call."""
+        qc = np.ascontiguousarray(cells, np.uint32)
+        qg = np.ascontiguousarray(gids, np.uint32)
+        if qc.shape != qg.shape:
+            raise ValueError("cells and gids differ in length")
+        out = np.zeros(max(qc.size, 1), np.uint32)
+        _check(lib().vacc_find_blocks_noalloc(self.h, qc.size, _ptr(qc), _ptr(qg), _ptr(out)))
         return out[:qc.size]
 
     def pressure(self, ncells, pivot=None):
