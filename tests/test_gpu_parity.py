@@ -372,6 +372,47 @@ def test_fast_arith_within_tolerance(cfg, method):
     ctx.close()
 
 
+@pytest.mark.parametrize("store", ["bf16", "f16"])
+def test_fast_arith_16bit_storage(store):
+    """fast_arith=1 with 16-bit storage (k_remap_lean decoding the source blocks and encoding
+    each target block), three PQM steps on the multi-peak workload, each step from the device's
+    stored state.  Against the oracle's FP32 path with the same storage (blocks re-quantized
+    after every pass): equal block lists and no status bit (no ring overflow), and, relative to
+    the cell's max f, L-inf <= two code steps (2^-7 BF16, 2^-10 FP16), since the tolerance form's
+    ~1e-6 differences can move a value across a rounding boundary of the code.  Against the
+    FP64 run: density <= 1e-6 (the per-block scale keeps each block's FP32 mass)."""
+    st, ost, linf = {"bf16": (V.STORE_BF16, O.ST_BF16, 2.0 ** -7), "f16": (V.STORE_F16, O.ST_F16, 2.0 ** -10)}[store]
+    w = W.config(2, ncells=8)
+    m = cases.oracle_mesh(w)
+    ctx = V.Context(V.make_mesh(w.grid_length, w.vmin, w.vmax),
+                    V.make_species(w.mass, w.charge, w.min_value, w.add_width),
+                    V.make_options(O.PQM, fast_arith=True, storage=st))
+    ctx.upload_cells(cases.build_cells(w))
+    ref = ctx.download_cells()   # the stored (quantized) initial state
+    worst = [0.0, 0.0, 0.0]
+    for step in range(3):
+        mo = step % 3
+        isx = cases.oracle_intersections(w, mo)
+        _say(f"fast arith {store} step {step}")
+        ctx.set_intersections(isx)
+        assert not (ctx.accelerate(mo) & V.STATUS_DK), "ring overflow or pass-through reported"
+        got = ctx.download_cells()
+        for i, ((g1, f1), (g, f)) in enumerate(zip(got, ref)):
+            ga, fa, _ = O.accelerate_cell(m, g, f, isx[i], mo, O.PQM, w.min_value, storage=ost)
+            gb, fb, _ = O.accelerate_cell(m, g, f.astype(np.float64), isx[i], mo, O.PQM, w.min_value, fp64=True)
+            ga, fa = cases.sort_cell(ga, fa)
+            gb, fb = cases.sort_cell(gb, fb)
+            assert np.array_equal(g1, ga) and np.array_equal(g1, gb), f"step {step} cell {i}"
+            x = f1.astype(np.float64)
+            mx = np.abs(fb).max()
+            worst = [max(worst[0], np.abs(x - fa).max() / mx), max(worst[1], np.abs(x - fa).sum() / np.abs(fa).sum()),
+                     max(worst[2], abs(x.sum() / fb.sum() - 1))]
+        ref = got
+    _say(f"fast arith {store}: vs same-storage oracle (Linf, L1) {worst[:2]}, density vs FP64 {worst[2]}")
+    assert worst[0] <= linf and worst[2] <= 1e-6
+    ctx.close()
+
+
 @pytest.mark.parametrize("per_cell,adjust_mode", [(0, 1), (1, 1), (0, 2), (0, 3)])
 def test_cpp_host_adapter_driver(tmp_path, per_cell, adjust_mode):
     """The C++ host adapter (vlasiator_amd/host/vacc_host.hpp) through its example driver:

@@ -385,6 +385,7 @@ static int enqueue_pass(vacc_ctx* c, int dim, int grid_mult) {
    ca.status = c->status;
    ca.ncells = c->ncells;
    for (int d = 0; d < 3; ++d) ca.len[d] = c->mi.len[d];
+   ca.gs = make_gid_split(c->mi.len);
    ca.dv = cell_f(c, dim);                 // Realf dv = cellSize (cpu_acc_map.cpp:147)
    ca.vmin = (float)c->mi.vmin[dim];       // Realf v_min (cpu_acc_map.cpp:148)
    ca.wall_margin = c->opt.wall_margin;
@@ -512,6 +513,7 @@ static int do_relist(vacc_ctx* c, int add_width, bool want_loss, const SpatialIn
    aa.rho_loss = want_loss ? c->rho_loss : nullptr;
    aa.ncells = c->ncells;
    for (int d = 0; d < 3; ++d) aa.len[d] = c->mi.len[d];
+   aa.gs = make_gid_split(c->mi.len);
    aa.min_value = c->sp.sparse_min_value;
    aa.dv3 = c->mi.cell[0] * c->mi.cell[1] * c->mi.cell[2];
    aa.add_width = add_width;
@@ -1465,6 +1467,7 @@ static void enqueue_moments(vacc_ctx* c) {
       ma.vmin[d] = c->mi.vmin[d];
       ma.cell[d] = c->mi.cell[d];
    }
+   ma.gs = make_gid_split(c->mi.len);
    launch_moments(ma, c->stream);
    c->moments_valid = true;
 }
@@ -1973,6 +1976,7 @@ int vacc_pressure(vacc_ctx* c, const double* pivot, double* out) {
       ma.vmin[d] = c->mi.vmin[d];
       ma.cell[d] = c->mi.cell[d];
    }
+   ma.gs = make_gid_split(c->mi.len);
    launch_pressure(ma, d_piv, c->sp.mass, d_out, c->stream);
    HIPCHK(hipGetLastError());
    HIPCHK(hipMemcpyAsync(out, d_out, 6 * n * sizeof(double), hipMemcpyDeviceToHost, c->stream));

@@ -66,10 +66,17 @@ struct Pack2<ST_F16> {
    __device__ static inline float hi(uint32_t p) { return Codec<ST_F16>::dec(p >> 16); }
 };
 
-// DPP row rotation (row_ror:N, N = 1..15) within the 16 lanes of the caller's row.
+// DPP row rotation (row_ror:N, N = 1..15) within the 16 lanes of the caller's row.  A rotation
+// reads no lane outside the row, and callers have all 16 lanes of the row active, so the
+// bound_ctrl setting changes no value; with it set the compiler folds the rotation into the
+// instruction that uses it (v_add_f32_dpp, v_max_u32_dpp, ...).
+template <int N>
+__device__ inline uint32_t row_ror_u(uint32_t x) {
+   return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x120 + N, 0xF, 0xF, true);
+}
 template <int N>
 __device__ inline float row_ror(float x) {
-   return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, x), 0x120 + N, 0xF, 0xF, false));
+   return __builtin_bit_cast(float, row_ror_u<N>(__builtin_bit_cast(uint32_t, x)));
 }
 
 // Reductions over the 16 lanes of a DPP row (lanes 16r .. 16r+15, all active).  Rotating by
@@ -81,11 +88,13 @@ __device__ inline float row_sum16(float s) {
    s = s + row_ror<2>(s);
    return s + row_ror<1>(s);
 }
+// maximum of non-negative finite values: their bit patterns order as unsigned integers
 __device__ inline float row_maxf16(float m) {
-   m = fmaxf(m, row_ror<8>(m));
-   m = fmaxf(m, row_ror<4>(m));
-   m = fmaxf(m, row_ror<2>(m));
-   return fmaxf(m, row_ror<1>(m));
+   uint32_t u = __builtin_bit_cast(uint32_t, m);
+   u = max(u, row_ror_u<8>(u));
+   u = max(u, row_ror_u<4>(u));
+   u = max(u, row_ror_u<2>(u));
+   return __builtin_bit_cast(float, max(u, row_ror_u<1>(u)));
 }
 
 __device__ inline float dec16(int st, uint32_t u) {

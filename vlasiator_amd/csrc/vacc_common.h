@@ -21,6 +21,41 @@ constexpr uint32_t INVALID = 0xFFFFFFFFu;
 constexpr int RING = 32;            // per-lane target ring (cells) of the remap kernel
 constexpr int MAX_SPAN_CELLS = 22;  // (dv + 3|di| + 3|dj|)/dk bound that keeps the ring in range
 
+// floor(g / d) for every 32-bit g without an integer division: (g + umulhi(g, m)) >> l, with
+// l = ceil(log2 d) and m = ceil(2^(32+l) / d) - 2^32.  Exact: with M = m + 2^32 the product
+// g M / 2^(32+l) exceeds g/d by less than g / 2^(32+l) < 2^-l <= 1/d, which cannot carry g/d
+// past the next integer (its fraction is at most (d-1)/d).
+struct FastDiv {
+   uint32_t d, m, l;
+};
+inline FastDiv make_fastdiv(uint32_t d) {   // host side, d >= 1
+   uint32_t l = 0;
+   while (l < 32 && (1ull << l) < (uint64_t)d) ++l;
+   const uint32_t k = 32 + l;   // ceil(2^k / d) = floor((2^k - 1) / d) + 1
+   const uint64_t M = ((k == 64) ? ~0ull : ((1ull << k) - 1ull)) / d + 1ull;
+   return FastDiv{d, (uint32_t)(M - (1ull << 32)), l};
+}
+VACC_HD uint32_t fdiv(uint32_t g, const FastDiv& f) {
+#if defined(__HIP_DEVICE_COMPILE__)
+   const uint32_t hi = __umulhi(g, f.m);
+#else
+   const uint32_t hi = (uint32_t)(((uint64_t)g * f.m) >> 32);
+#endif
+   return (uint32_t)(((uint64_t)g + hi) >> f.l);
+}
+
+// Block coordinates (x, y, z) of GID = x + y Nx + z Nx Ny.
+struct GidSplit {
+   FastDiv nx, nxy;
+};
+inline GidSplit make_gid_split(const uint32_t* len) { return GidSplit{make_fastdiv(len[0]), make_fastdiv(len[0] * len[1])}; }
+VACC_HD void gid_xyz(uint32_t g, const GidSplit& s, uint32_t& x, uint32_t& y, uint32_t& z) {
+   z = fdiv(g, s.nxy);
+   const uint32_t r = g - z * s.nxy.d;
+   y = fdiv(r, s.nx);
+   x = r - y * s.nx.d;
+}
+
 // physical constants (common.h physicalconstants)
 constexpr double MU_0 = 1.25663706e-6;
 constexpr double PI = 3.14159265358979323846;

@@ -50,6 +50,7 @@ struct ColumnArgs {
    uint32_t* status;           // [ncells + 1] OR-ed status bits; entry ncells ORs all cells
    uint32_t ncells;
    uint32_t len[3];
+   GidSplit gs;                // block coordinates of a GID (from len)
    float dv;                   // (float) cell size along the pass axis
    float vmin;                 // (float) mesh minimum along the pass axis
    int wall_margin;
@@ -121,6 +122,7 @@ struct AdjustArgs {
    const uint32_t* flags;      // global flags (capacity overflow -> the write phase is skipped)
    uint32_t ncells;
    uint32_t len[3];
+   GidSplit gs;                // block coordinates of a GID (from len)
    double min_value;
    double dv3;
    int add_width;              // < 0: keep exactly the existing blocks (re-sort only)
@@ -140,6 +142,7 @@ struct MomentArgs {
    double* out;                // [ncells*4]
    uint32_t ncells;
    uint32_t len[3];
+   GidSplit gs;                // block coordinates of a GID (from len)
    double vmin[3];
    double cell[3];
 };

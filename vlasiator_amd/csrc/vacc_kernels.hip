@@ -46,11 +46,16 @@ constexpr int ROW_MAX = 1536;
 // small helpers
 // ---------------------------------------------------------------------------
 
+// DPP row rotation of an int within the 16 lanes of the caller's row (all active)
+template <int N>
+__device__ inline int row_ror_i(int x) {
+   return (int)row_ror_u<N>((uint32_t)x);
+}
+
 template <int DIM>
-__device__ inline void pass_coords(uint32_t gid, const uint32_t* len, int& b0, int& b1, int& k) {
-   const uint32_t x = gid % len[0];
-   const uint32_t y = (gid / len[0]) % len[1];
-   const uint32_t z = gid / (len[0] * len[1]);
+__device__ inline void pass_coords(uint32_t gid, const GidSplit& gs, int& b0, int& b1, int& k) {
+   uint32_t x, y, z;
+   gid_xyz(gid, gs, x, y, z);
    // frame of map_1d after swapBlockIndices (cpu_acc_map.cpp:85-106)
    if (DIM == 0) { b0 = (int)z; b1 = (int)y; k = (int)x; }
    else if (DIM == 1) { b0 = (int)x; b1 = (int)z; k = (int)y; }
@@ -217,11 +222,15 @@ __device__ inline bool max_lane_rule_binds(int b0, int b1, int ks, float I, floa
    return none_pos[0] || none_pos[1];
 }
 
-// Deposits of one source block (4 cells) in the tolerance-form remap reach at most
-// 4 dv/dk + 2 cells past a lane's open target, and the 16 lanes of a unit are spread over
-// (3|di| + 3|dj|)/dk + 1 cells; with block alignment the ring of RING cells holds them when
-// (4 dv + 3|di| + 3|dj|)/dk <= LEAN_SPAN.
-constexpr float LEAN_SPAN = 20.0f;
+// Target ring of the tolerance-form remap, in cells per lane.  Before each source block the
+// kernel opens the target blocks up to the highest target any lane of the unit can reach with
+// the block's 4 cells (below 4 dv/dk + 2 past its open target) and has flushed those below the
+// lowest target any lane still accumulates; the lanes' boundaries lie within
+// (3|di| + 3|dj|)/dk + 1 cells of each other, and block alignment adds 7 cells.  So the open
+// span stays within (4 dv + 3|di| + 3|dj|)/dk + 10 cells, and a cell whose span bound
+// exceeds LEAN_SPAN = LEAN_RING - 10 is left to the exact kernel (32-cell ring).
+constexpr int LEAN_RING = 16;
+constexpr float LEAN_SPAN = (float)(LEAN_RING - 10);
 
 // ---------------------------------------------------------------------------
 // k_columns
@@ -259,7 +268,7 @@ __global__ __launch_bounds__(CWG) void k_columns(ColumnArgs a) {
       int lo0 = INT_MAX, lo1 = INT_MAX, lok = INT_MAX, hi0 = INT_MIN, hi1 = INT_MIN, hik = INT_MIN;
       for (uint32_t i = t; i < n; i += CWG) {
          int b0, b1, k;
-         pass_coords<DIM>(a.gid[off + i], a.len, b0, b1, k);
+         pass_coords<DIM>(a.gid[off + i], a.gs, b0, b1, k);
          lo0 = min(lo0, b0); hi0 = max(hi0, b0);
          lo1 = min(lo1, b1); hi1 = max(hi1, b1);
          lok = min(lok, k); hik = max(hik, k);
@@ -290,7 +299,7 @@ __global__ __launch_bounds__(CWG) void k_columns(ColumnArgs a) {
       __syncthreads();
       for (uint32_t i = t; i < n; i += CWG) {
          int b0, b1, k;
-         pass_coords<DIM>(a.gid[off + i], a.len, b0, b1, k);
+         pass_coords<DIM>(a.gid[off + i], a.gs, b0, b1, k);
          const int row = (b0 - mn0) * R1 + (b1 - mn1) - r0;
          if (row >= 0 && row < nr) {
             const int kk = k - mnk;
@@ -422,7 +431,7 @@ __global__ __launch_bounds__(CWG) void k_columns(ColumnArgs a) {
       // source index of every block in sorted (set, k) order
       for (uint32_t i = t; i < n; i += CWG) {
          int b0, b1, k;
-         pass_coords<DIM>(a.gid[off + i], a.len, b0, b1, k);
+         pass_coords<DIM>(a.gid[off + i], a.gs, b0, b1, k);
          const int row = (b0 - mn0) * R1 + (b1 - mn1) - r0;
          if (row >= 0 && row < nr) {
             const uint32_t pos = rowv[0][row] + rank_in_row(bm + row * W, k - mnk);
@@ -523,7 +532,26 @@ __device__ inline float pool_value(const Pool& pl, uint32_t b, int e) {
    return pl.scale[b] * dec16(pl.storage, static_cast<const uint16_t*>(pl.f)[(size_t)b * WID3 + e]);
 }
 
+// The same with the storage form fixed at compile time (ST_F32, ST_BF16, ST_F16).
+template <int ST>
+__device__ inline float pool_value_t(const Pool& pl, uint32_t b, int e) {
+   if constexpr (ST == ST_F32) return static_cast<const float*>(pl.f)[(size_t)b * WID3 + e];
+   else return pl.scale[b] * Codec<ST>::dec(static_cast<const uint16_t*>(pl.f)[(size_t)b * WID3 + e]);
+}
+
 // Row q (cells 4q..4q+3, i = 0..3 at j = q & 3, k = q >> 2) of block b, decoded to FP32.
+template <int ST>
+__device__ inline float4 pool_row_t(const Pool& pl, uint32_t b, int q) {
+   if (b == INVALID) return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+   if constexpr (ST == ST_F32) {
+      return reinterpret_cast<const float4*>(pl.f)[(size_t)b * (WID3 / 4) + q];
+   } else {
+      const uint2 r = reinterpret_cast<const uint2*>(pl.f)[(size_t)b * (WID3 / 4) + q];
+      const float sc = pl.scale[b];
+      return make_float4(sc * Pack2<ST>::lo(r.x), sc * Pack2<ST>::hi(r.x), sc * Pack2<ST>::lo(r.y), sc * Pack2<ST>::hi(r.y));
+   }
+}
+
 __device__ inline float4 pool_row(const Pool& pl, uint32_t b, int q) {
    if (b == INVALID) return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
    if (pl.storage == ST_F32) return reinterpret_cast<const float4*>(pl.f)[(size_t)b * (WID3 / 4) + q];
@@ -599,13 +627,13 @@ __device__ inline void set_content(uint8_t* content, size_t slot, const float (&
 // of the block from the ring columns of the unit's lanes, encodes the block and stores its 4
 // codes (8 contiguous bytes) and, lane 0, the scale.  The content flag is taken on the stored
 // (decoded) values.
-template <int DIM, int ST>
+template <int DIM, int ST, int RN>
 __device__ inline void flush_block(const RemapArgs& a, float (*ring)[WG], int tid, int l16, int setlane0, int o,
                                    int blk, size_t slot, float thc) {
    if constexpr (ST == ST_F32) {
       float q[4];
 #pragma unroll
-      for (int e = 0; e < 4; ++e) q[e] = ring[(WID * blk + e) & (RING - 1)][tid];
+      for (int e = 0; e < 4; ++e) q[e] = ring[(WID * blk + e) & (RN - 1)][tid];
       store_pencil<DIM>(static_cast<float*>(a.out.f), slot, o, q);
       set_content(a.content_out, slot, q, thc, setlane0, l16);
    } else {
@@ -618,7 +646,7 @@ __device__ inline void flush_block(const RemapArgs& a, float (*ring)[WG], int ti
          // f(ip, jp, kk) (dim 2) in ring row kk
          const int lane = (DIM == 0) ? ((L >> 2) + 4 * (L & 3)) : (DIM == 1) ? (i + 4 * (L >> 2)) : (i + 4 * (L & 3));
          const int row = (DIM == 0) ? i : (DIM == 1) ? (L & 3) : (L >> 2);
-         x[i] = ring[(WID * blk + row) & (RING - 1)][base + lane];
+         x[i] = ring[(WID * blk + row) & (RN - 1)][base + lane];
       }
       uint2 pk;
       const float sc = encode_rows16<ST>(x, pk);
@@ -741,9 +769,11 @@ __global__ __launch_bounds__(WG) void k_remap(RemapArgs a, double i_dv) {
             }
          }
          float w0[4] = {0.0f, 0.0f, 0.0f, 0.0f}, w1[4], w2[4];
+         // source indices are read one block ahead of the block data they locate, so that no
+         // block waits for an index load and then for the data load behind it
+         uint32_t s_next = (len > 2) ? srcs[2] : INVALID;   // index of block b + 2
          load_pencil<DIM, ST>(a.in, srcs[0], o, w1);
-         if (len > 1) load_pencil<DIM, ST>(a.in, srcs[1], o, w2);
-         else w2[0] = w2[1] = w2[2] = w2[3] = 0.0f;
+         load_pencil<DIM, ST>(a.in, (len > 1) ? srcs[1] : INVALID, o, w2);
          int next_flush = fk;
          int next_open = (carry_open >= 0) ? carry_open : fk;
          const int lo_gk = WID * fk, hi_gk = WID * lk + WID - 1;
@@ -753,9 +783,10 @@ __global__ __launch_bounds__(WG) void k_remap(RemapArgs a, double i_dv) {
             rc.start(v0 + 4, scale);
          }
          for (int b = 0; b < len; ++b) {
+            const uint32_t s_after = (b + 3 < len) ? srcs[b + 3] : INVALID;
             float w3[4];
-            if (b + 2 < len) load_pencil<DIM, ST>(a.in, srcs[b + 2], o, w3);
-            else w3[0] = w3[1] = w3[2] = w3[3] = 0.0f;
+            load_pencil<DIM, ST>(a.in, s_next, o, w3);   // zeros past the column's end
+            s_next = s_after;
             const float v[12] = {w0[0], w0[1], w0[2], w0[3], w1[0], w1[1], w1[2], w1[3], w2[0], w2[1], w2[2], w2[3]};
 #pragma unroll
             for (int kk = 0; kk < WID; ++kk) {
@@ -768,13 +799,13 @@ __global__ __launch_bounds__(WG) void k_remap(RemapArgs a, double i_dv) {
                const int mnG = max(__shfl(gkl, minL, 8), lo_gk);
                const int mxG = min(__shfl(gkr, maxL, 8), hi_gk);
                // the two 8-lane groups of the unit (a DPP row): rotating by 8 swaps them
-               const int flushLim = min(mnG, __builtin_amdgcn_mov_dpp(mnG, 0x128, 0xF, 0xF, false));
-               const int openLim = max(max(mxG, __builtin_amdgcn_mov_dpp(mxG, 0x128, 0xF, 0xF, false)), flushLim);
+               const int flushLim = min(mnG, row_ror_i<8>(mnG));
+               const int openLim = max(max(mxG, row_ror_i<8>(mxG)), flushLim);
                while (next_open <= lk && WID * next_open <= openLim) {
                   if (next_open - next_flush >= RING / WID) {   // ring full: cannot happen for checked spans
                      atomicOr(&a.status[c], VACC_STATUS_DK);
                      atomicOr(&a.status[a.ncells], VACC_STATUS_DK);
-                     flush_block<DIM, ST>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), thc);
+                     flush_block<DIM, ST, RING>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), thc);
                      ++next_flush;
                   }
 #pragma unroll
@@ -782,7 +813,7 @@ __global__ __launch_bounds__(WG) void k_remap(RemapArgs a, double i_dv) {
                   ++next_open;
                }
                while (next_flush < next_open && next_flush < nfk && WID * next_flush + WID - 1 < flushLim) {
-                  flush_block<DIM, ST>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), thc);
+                  flush_block<DIM, ST, RING>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), thc);
                   ++next_flush;
                }
                float tdr = 0.0f;
@@ -807,7 +838,7 @@ __global__ __launch_bounds__(WG) void k_remap(RemapArgs a, double i_dv) {
             }
          }
          while (next_flush < next_open && next_flush < nfk) {
-            flush_block<DIM, ST>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), thc);
+            flush_block<DIM, ST, RING>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), thc);
             ++next_flush;
          }
          for (int tb = next_open; tb <= lk && tb < nfk; ++tb) zero_block<DIM, ST>(a, l16, setlane0, o, slotb + (tb - fk), thc);
@@ -836,25 +867,26 @@ __device__ inline void pass_isect_d(const double* is, double& I, double& di, dou
    else { di = a; dj = b; dk = c; }
 }
 
-// maximum / minimum over the 16 lanes of the caller's DPP row, in every lane of the row
+// maximum / minimum over the 16 lanes of the caller's DPP row (all active), in every lane of
+// the row (rotations folded into v_max_i32_dpp / v_min_i32_dpp)
 __device__ inline int row_max16(int x) {
-   x = max(x, __builtin_amdgcn_mov_dpp(x, 0x128, 0xF, 0xF, false));   // row_ror:8
-   x = max(x, __builtin_amdgcn_mov_dpp(x, 0x124, 0xF, 0xF, false));   // row_ror:4
-   x = max(x, __builtin_amdgcn_mov_dpp(x, 0x122, 0xF, 0xF, false));   // row_ror:2
-   return max(x, __builtin_amdgcn_mov_dpp(x, 0x121, 0xF, 0xF, false));   // row_ror:1
+   x = max(x, row_ror_i<8>(x));
+   x = max(x, row_ror_i<4>(x));
+   x = max(x, row_ror_i<2>(x));
+   return max(x, row_ror_i<1>(x));
 }
 __device__ inline int row_min16(int x) {
-   x = min(x, __builtin_amdgcn_mov_dpp(x, 0x128, 0xF, 0xF, false));
-   x = min(x, __builtin_amdgcn_mov_dpp(x, 0x124, 0xF, 0xF, false));
-   x = min(x, __builtin_amdgcn_mov_dpp(x, 0x122, 0xF, 0xF, false));
-   return min(x, __builtin_amdgcn_mov_dpp(x, 0x121, 0xF, 0xF, false));
+   x = min(x, row_ror_i<8>(x));
+   x = min(x, row_ror_i<4>(x));
+   x = min(x, row_ror_i<2>(x));
+   return min(x, row_ror_i<1>(x));
 }
 
 template <int DIM, int METHOD, int ST>
 __global__ __launch_bounds__(WG) void k_remap_lean(RemapArgs a) {
    using R = ReconLean<METHOD>;
    constexpr int NC = R::NC;
-   __shared__ float ring[RING][WG];
+   __shared__ float ring[LEAN_RING][WG];
    if (*a.flags & VACC_STATUS_CAPACITY) return;
    const int tid = threadIdx.x;
    const int l16 = tid & 15;
@@ -902,14 +934,16 @@ __global__ __launch_bounds__(WG) void k_remap_lean(RemapArgs a) {
          }
          const int lo = WID * fk, hi = WID * lk + WID - 1;
          auto emit = [&](int gt, float m) {
-            if (gt >= lo && gt <= hi) atomicAdd(&ring[gt & (RING - 1)][tid], m);
+            if (gt >= lo && gt <= hi) atomicAdd(&ring[gt & (LEAN_RING - 1)][tid], m);
          };
          Flux<NC> fx;
          fx.begin((a.vmin_d + (double)(WID * ks) * a.dv_d - imin) * rdk, r);
          float w0[4] = {0.0f, 0.0f, 0.0f, 0.0f}, w1[4], w2[4];
+         // source indices are read one block ahead of the block data they locate, so that no
+         // block waits for an index load and then for the data load behind it
+         uint32_t s_next = (len > 2) ? srcs[2] : INVALID;   // index of block b + 2
          load_pencil<DIM, ST>(a.in, srcs[0], o, w1);
-         if (len > 1) load_pencil<DIM, ST>(a.in, srcs[1], o, w2);
-         else w2[0] = w2[1] = w2[2] = w2[3] = 0.0f;
+         load_pencil<DIM, ST>(a.in, (len > 1) ? srcs[1] : INVALID, o, w2);
          int next_flush = fk;
          int next_open = (carry_open >= 0) ? carry_open : fk;
          R rc;
@@ -918,27 +952,28 @@ __global__ __launch_bounds__(WG) void k_remap_lean(RemapArgs a) {
             rc.start(v0 + 4);
          }
          for (int b = 0; b < len; ++b) {
+            const uint32_t s_after = (b + 3 < len) ? srcs[b + 3] : INVALID;
             float w3[4];
-            if (b + 2 < len) load_pencil<DIM, ST>(a.in, srcs[b + 2], o, w3);
-            else w3[0] = w3[1] = w3[2] = w3[3] = 0.0f;
+            load_pencil<DIM, ST>(a.in, s_next, o, w3);   // zeros past the column's end
+            s_next = s_after;
             // targets the block's 4 cells can reach (one cell of margin for rounding), and the
             // lowest target any lane still accumulates
             const int reach = fx.g + ((fx.t < 4.0f) ? (int)((4.0f - fx.t) * inv_r) + 1 : 0);
             const int openLim = row_max16(min(reach, hi));
             const int flushLim = row_min16(fx.g - 1);
             while (next_open <= lk && WID * next_open <= openLim) {
-               if (next_open - next_flush >= RING / WID) {   // ring full: not for LEAN_SPAN cells
+               if (next_open - next_flush >= LEAN_RING / WID) {   // ring full: not for LEAN_SPAN cells
                   atomicOr(&a.status[c], VACC_STATUS_DK);
                   atomicOr(&a.status[a.ncells], VACC_STATUS_DK);
-                  flush_block<DIM, ST>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), thc);
+                  flush_block<DIM, ST, LEAN_RING>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), thc);
                   ++next_flush;
                }
 #pragma unroll
-               for (int e = 0; e < 4; ++e) ring[(WID * next_open + e) & (RING - 1)][tid] = 0.0f;
+               for (int e = 0; e < 4; ++e) ring[(WID * next_open + e) & (LEAN_RING - 1)][tid] = 0.0f;
                ++next_open;
             }
             while (next_flush < next_open && next_flush < nfk && WID * next_flush + WID - 1 < flushLim) {
-               flush_block<DIM, ST>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), thc);
+               flush_block<DIM, ST, LEAN_RING>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), thc);
                ++next_flush;
             }
             const float v[12] = {w0[0], w0[1], w0[2], w0[3], w1[0], w1[1], w1[2], w1[3], w2[0], w2[1], w2[2], w2[3]};
@@ -957,7 +992,7 @@ __global__ __launch_bounds__(WG) void k_remap_lean(RemapArgs a) {
          }
          fx.end(emit);
          while (next_flush < next_open && next_flush < nfk) {
-            flush_block<DIM, ST>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), thc);
+            flush_block<DIM, ST, LEAN_RING>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), thc);
             ++next_flush;
          }
          for (int tb = next_open; tb <= lk && tb < nfk; ++tb) zero_block<DIM, ST>(a, l16, setlane0, o, slotb + (tb - fk), thc);
@@ -1020,7 +1055,7 @@ __device__ inline void dilate_box1(uint32_t* bm, int W, int Rx, int Ry, int Rz) 
    __syncthreads();
 }
 
-template <bool WRITE>
+template <bool WRITE, int ST>
 __global__ __launch_bounds__(AWG) void k_adjust(AdjustArgs a) {
    __shared__ uint32_t bm[BM_WORDS];
    __shared__ uint32_t wpre[WRITE ? BM_WORDS : 1];
@@ -1077,7 +1112,9 @@ __global__ __launch_bounds__(AWG) void k_adjust(AdjustArgs a) {
    {
       int lx = INT_MAX, ly = INT_MAX, lz = INT_MAX, hx = INT_MIN, hy = INT_MIN, hz = INT_MIN;
       auto grow = [&](uint32_t g) {
-         const int x = (int)(g % nx), y = (int)((g / nx) % ny), z = (int)(g / (nx * ny));
+         uint32_t ux, uy, uz;
+         gid_xyz(g, a.gs, ux, uy, uz);
+         const int x = (int)ux, y = (int)uy, z = (int)uz;
          lx = min(lx, x); hx = max(hx, x);
          ly = min(ly, y); hy = max(hy, y);
          lz = min(lz, z); hz = max(hz, z);
@@ -1103,7 +1140,7 @@ __global__ __launch_bounds__(AWG) void k_adjust(AdjustArgs a) {
    const int rpc = BM_WORDS / W;
    // one chunk holds the whole box: mark content blocks, then dilate separably in LDS
    const bool separable = (w >= 0) && (rows <= rpc);
-   double myloss = 0.0;
+   double lsum = 0.0;      // removed f, summed per lane (count phase)
    for (int r0 = 0; r0 < rows; r0 += rpc) {
       const int nr = min(rpc, rows - r0);
       for (int i = t; i < nr * W; i += AWG) bm[i] = 0;
@@ -1112,21 +1149,27 @@ __global__ __launch_bounds__(AWG) void k_adjust(AdjustArgs a) {
          for (uint32_t i = t; i < n; i += AWG) {
             if (!a.content[off + i]) continue;
             const uint32_t g = a.gid[off + i];
-            const int x = (int)(g % nx), y = (int)((g / nx) % ny), z = (int)(g / (nx * ny));
+            uint32_t ux, uy, uz;
+            gid_xyz(g, a.gs, ux, uy, uz);
+            const int x = (int)ux, y = (int)uy, z = (int)uz;
             const int kx = x - mnx;
             atomicOr(&bm[((z - mnz) * Ry + (y - mny)) * W + (kx >> 5)], 1u << (kx & 31));
          }
          __syncthreads();
          for (int k = 0; k < w; ++k) dilate_box1(bm, W, Rx, Ry, Rz);
          for_spatial([&](uint32_t g) {
-            const int x = (int)(g % nx), y = (int)((g / nx) % ny), z = (int)(g / (nx * ny));
+            uint32_t ux, uy, uz;
+            gid_xyz(g, a.gs, ux, uy, uz);
+            const int x = (int)ux, y = (int)uy, z = (int)uz;
             const int kx = x - mnx;
             atomicOr(&bm[((z - mnz) * Ry + (y - mny)) * W + (kx >> 5)], 1u << (kx & 31));
          });
          __syncthreads();
       } else {
       for_spatial([&](uint32_t g) {
-         const int x = (int)(g % nx), y = (int)((g / nx) % ny), z = (int)(g / (nx * ny));
+         uint32_t ux, uy, uz;
+         gid_xyz(g, a.gs, ux, uy, uz);
+         const int x = (int)ux, y = (int)uy, z = (int)uz;
          const int row = (z - mnz) * Ry + (y - mny) - r0;
          if (row >= 0 && row < nr) {
             const int kx = x - mnx;
@@ -1135,7 +1178,9 @@ __global__ __launch_bounds__(AWG) void k_adjust(AdjustArgs a) {
       });
       for (uint32_t i = t; i < n; i += AWG) {
          const uint32_t g = a.gid[off + i];
-         const int x = (int)(g % nx), y = (int)((g / nx) % ny), z = (int)(g / (nx * ny));
+         uint32_t ux, uy, uz;
+         gid_xyz(g, a.gs, ux, uy, uz);
+         const int x = (int)ux, y = (int)uy, z = (int)uz;
          if (w < 0) {
             const int row = (z - mnz) * Ry + (y - mny) - r0;
             if (row >= 0 && row < nr) {
@@ -1194,7 +1239,9 @@ __global__ __launch_bounds__(AWG) void k_adjust(AdjustArgs a) {
          __syncthreads();
          for (uint32_t i = t; i < n; i += AWG) {
             const uint32_t g = a.gid[off + i];
-            const int x = (int)(g % nx), y = (int)((g / nx) % ny), z = (int)(g / (nx * ny));
+            uint32_t ux, uy, uz;
+            gid_xyz(g, a.gs, ux, uy, uz);
+            const int x = (int)ux, y = (int)uy, z = (int)uz;
             const int row = (z - mnz) * Ry + (y - mny) - r0;
             if (row < 0 || row >= nr) continue;
             const int kx = x - mnx;
@@ -1207,24 +1254,45 @@ __global__ __launch_bounds__(AWG) void k_adjust(AdjustArgs a) {
          }
       } else {
          // removed blocks: no content and not kept; their mass goes to RHOLOSSADJUST
-         // (spatial_cell_cpu.cpp:227-267), one thread per block, 16 row loads in flight
-         for (uint32_t i = t; i < n; i += AWG) {
-            if (w < 0 || a.content[off + i]) continue;
-            const uint32_t g = a.gid[off + i];
-            const int x = (int)(g % nx), y = (int)((g / nx) % ny), z = (int)(g / (nx * ny));
-            const int row = (z - mnz) * Ry + (y - mny) - r0;
-            if (row < 0 || row >= nr) continue;
-            const int kx = x - mnx;
-            if (bm[row * W + (kx >> 5)] & (1u << (kx & 31))) continue;
-            const uint32_t sidx = a.src[off + i];
-            if (sidx == INVALID) continue;
-            float4 r[WID3 / 4];
+         // (spatial_cell_cpu.cpp:227-267).  Each wave takes 64 consecutive list entries, one
+         // per lane, ballots the removed ones and then sums them block by block in lane order,
+         // all 64 lanes reading one block (coalesced) into per-lane FP64 sums: the same blocks
+         // reach the same lanes in the same order on every run.
+         if (w >= 0) {
+            const int lane = t & 63;
+            for (uint32_t i0 = (uint32_t)(t - lane); i0 < n; i0 += AWG) {
+               const uint32_t i = i0 + (uint32_t)lane;
+               uint32_t sidx = INVALID;
+               if (i < n && !a.content[off + i]) {
+                  uint32_t ux, uy, uz;
+                  gid_xyz(a.gid[off + i], a.gs, ux, uy, uz);
+                  const int row = ((int)uz - mnz) * Ry + ((int)uy - mny) - r0;
+                  const int kx = (int)ux - mnx;
+                  if (row >= 0 && row < nr && !(bm[row * W + (kx >> 5)] & (1u << (kx & 31)))) sidx = a.src[off + i];
+               }
+               unsigned long long bal = __ballot(sidx != INVALID);
+               while (bal) {   // up to 8 removed blocks per batch, their loads in flight together
+                  constexpr int B = 8;
+                  uint32_t sb[B];
 #pragma unroll
-            for (int q = 0; q < WID3 / 4; ++q) r[q] = pool_row(a.pool, sidx, q);
-            double sum = 0.0;
+                  for (int u = 0; u < B; ++u) {
+                     sb[u] = INVALID;
+                     if (bal) {
+                        sb[u] = (uint32_t)__builtin_amdgcn_readlane((int)sidx, __ffsll((long long)bal) - 1);
+                        bal &= bal - 1ull;
+                     }
+                  }
+                  // unconditional loads (an empty slot reads pool entry 0, which exists when a
+                  // removed block does, and is discarded) so that all of them are in flight
+                  float v[B];
 #pragma unroll
-            for (int q = 0; q < WID3 / 4; ++q) sum += ((double)r[q].x + (double)r[q].y) + ((double)r[q].z + (double)r[q].w);
-            myloss += a.dv3 * sum;
+                  for (int u = 0; u < B; ++u) v[u] = pool_value_t<ST>(a.pool, (sb[u] == INVALID) ? 0u : sb[u], lane);
+#pragma unroll
+                  for (int u = 0; u < B; ++u) v[u] = (sb[u] == INVALID) ? 0.0f : v[u];
+#pragma unroll
+                  for (int u = 0; u < B; ++u) lsum += (double)v[u];
+               }
+            }
          }
       }
       __syncthreads();
@@ -1233,7 +1301,7 @@ __global__ __launch_bounds__(AWG) void k_adjust(AdjustArgs a) {
    }
    if (!WRITE) {
       // fixed-order reduction (wave tree, then waves in order): the loss is reproducible
-      double v = myloss;
+      double v = a.dv3 * lsum;
       for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
       if ((t & 63) == 0) wloss[t >> 6] = v;
       __syncthreads();
@@ -1277,7 +1345,8 @@ __global__ __launch_bounds__(WG) void k_pressure(MomentArgs a, const double* piv
       if (sidx == INVALID) continue;
       const double f = pool_value(a.pool, sidx, lane);
       const uint32_t g = a.gid[off + b];
-      const uint32_t z = g / nxy, r = g - z * nxy, y = r / nx, x = r - y * nx;
+      uint32_t x, y, z;
+      gid_xyz(g, a.gs, x, y, z);
       const double dx = (a.vmin[0] + (x * WID + i + 0.5) * a.cell[0]) - px;
       const double dy = (a.vmin[1] + (y * WID + j + 0.5) * a.cell[1]) - py;
       const double dz = (a.vmin[2] + (z * WID + k + 0.5) * a.cell[2]) - pz;
@@ -1375,10 +1444,14 @@ __global__ __launch_bounds__(WG) void k_cell_scale(const uint32_t* off, const ui
 // moments, transform, gather
 // ---------------------------------------------------------------------------
 
+template <int ST>
 __global__ __launch_bounds__(WG) void k_moments(MomentArgs a) {
    // One workgroup per cell; 16 lanes per block, each lane one float4 (i = 0..3 at fixed j, k),
    // UNR blocks per 16-lane group in flight.  Per value the reference adds f*DV3, f*VX*DV3, ...
-   // (arch_moments.h:98-108); here DV3 is applied once to the cell's FP64 sums.
+   // (arch_moments.h:98-108); here DV3 is applied once to the cell's FP64 sums.  A lane keeps
+   // per row the sum S and the index-weighted sum W = f1 + 2 f2 + 3 f3, and accumulates S, W and
+   // S times the block's corner velocity; its fixed offsets within the block (x: 0.5 dv and W dv,
+   // y, z: (j, k) + 0.5 cells) are applied once at the end.
    __shared__ double red[4][WG / 64];
    const int t = threadIdx.x;
    const uint32_t c = blockIdx.x;
@@ -1386,11 +1459,8 @@ __global__ __launch_bounds__(WG) void k_moments(MomentArgs a) {
    const uint32_t n = a.cell_off[c + 1] - off;
    const int q = t & 15;
    const int j = q & 3, k = q >> 2;
-   const double cx0 = 0.5 * a.cell[0], cx1 = 1.5 * a.cell[0], cx2 = 2.5 * a.cell[0], cx3 = 3.5 * a.cell[0];
-   const double cj = (j + 0.5) * a.cell[1], ck = (k + 0.5) * a.cell[2];
    const double bx = a.cell[0] * WID, by = a.cell[1] * WID, bz = a.cell[2] * WID;
-   const uint32_t nx = a.len[0], nxy = a.len[0] * a.len[1];
-   double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+   double s0 = 0.0, sw = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
    constexpr uint32_t GROUPS = WG / 16;
    constexpr int UNR = 4;
    for (uint32_t b = t >> 4; b < n; b += UNR * GROUPS) {
@@ -1403,22 +1473,24 @@ __global__ __launch_bounds__(WG) void k_moments(MomentArgs a) {
          gv[h] = (bb < n) ? a.gid[off + bb] : 0u;
       }
 #pragma unroll
-      for (int h = 0; h < UNR; ++h) v[h] = pool_row(a.pool, sv[h], q);
+      for (int h = 0; h < UNR; ++h) v[h] = pool_row_t<ST>(a.pool, sv[h], q);
 #pragma unroll
       for (int h = 0; h < UNR; ++h) {
-         const uint32_t g = gv[h];
-         const uint32_t z = g / nxy, r = g - z * nxy, y = r / nx, x = r - y * nx;
-         const double X0 = a.vmin[0] + x * bx;
-         const double VY = (a.vmin[1] + y * by) + cj;
-         const double VZ = (a.vmin[2] + z * bz) + ck;
+         uint32_t x, y, z;
+         gid_xyz(gv[h], a.gs, x, y, z);
          const double v0 = v[h].x, v1 = v[h].y, v2 = v[h].z, v3 = v[h].w;
-         const double sum = (v0 + v1) + (v2 + v3);
-         s0 += sum;
-         s1 += ((v0 * (X0 + cx0) + v1 * (X0 + cx1)) + (v2 * (X0 + cx2) + v3 * (X0 + cx3)));
-         s2 += sum * VY;
-         s3 += sum * VZ;
+         const double S = (v0 + v1) + (v2 + v3);
+         const double Wt = fma(3.0, v3, fma(2.0, v2, v1));
+         s0 += S;
+         sw += Wt;
+         sx = fma(S, fma((double)x, bx, a.vmin[0]), sx);
+         sy = fma(S, fma((double)y, by, a.vmin[1]), sy);
+         sz = fma(S, fma((double)z, bz, a.vmin[2]), sz);
       }
    }
+   double s1 = sx + (0.5 * a.cell[0]) * s0 + a.cell[0] * sw;
+   double s2 = sy + ((j + 0.5) * a.cell[1]) * s0;
+   double s3 = sz + ((k + 0.5) * a.cell[2]) * s0;
    for (int o = 32; o > 0; o >>= 1) {
       s0 += __shfl_down(s0, o, 64);
       s1 += __shfl_down(s1, o, 64);
@@ -1620,8 +1692,17 @@ void launch_remap(int dim, int method, bool fast, RemapArgs a, int grid_mult, hi
 
 void launch_adjust(const AdjustArgs& a, bool write_phase, hipStream_t s) {
    if (a.ncells == 0) return;
-   if (write_phase) hipLaunchKernelGGL(k_adjust<true>, dim3(a.ncells), dim3(AWG), 0, s, a);
-   else hipLaunchKernelGGL(k_adjust<false>, dim3(a.ncells), dim3(AWG), 0, s, a);
+   const int st = a.pool.storage;
+   if (write_phase) {
+      // the write phase reads no f
+      hipLaunchKernelGGL((k_adjust<true, ST_F32>), dim3(a.ncells), dim3(AWG), 0, s, a);
+   } else if (st == ST_F32) {
+      hipLaunchKernelGGL((k_adjust<false, ST_F32>), dim3(a.ncells), dim3(AWG), 0, s, a);
+   } else if (st == ST_BF16) {
+      hipLaunchKernelGGL((k_adjust<false, ST_BF16>), dim3(a.ncells), dim3(AWG), 0, s, a);
+   } else {
+      hipLaunchKernelGGL((k_adjust<false, ST_F16>), dim3(a.ncells), dim3(AWG), 0, s, a);
+   }
 }
 
 // Workgroups for one wave per block, capped so that a dispatch stays far below 2^32 work-items.
@@ -1638,7 +1719,9 @@ void launch_content(const uint32_t* src, const Pool& pool, uint8_t* content, uin
 
 void launch_moments(const MomentArgs& a, hipStream_t s) {
    if (a.ncells == 0) return;
-   hipLaunchKernelGGL(k_moments, dim3(a.ncells), dim3(WG), 0, s, a);
+   if (a.pool.storage == ST_F32) hipLaunchKernelGGL(k_moments<ST_F32>, dim3(a.ncells), dim3(WG), 0, s, a);
+   else if (a.pool.storage == ST_BF16) hipLaunchKernelGGL(k_moments<ST_BF16>, dim3(a.ncells), dim3(WG), 0, s, a);
+   else hipLaunchKernelGGL(k_moments<ST_F16>, dim3(a.ncells), dim3(WG), 0, s, a);
 }
 
 void launch_transform(const MeshInfo& m, const TransformIn* fields, const double* dt_override, const double* moments,
