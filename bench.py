@@ -321,7 +321,8 @@ def measure(pops, storage: str, steps: int, warmup: int, graph: bool = False, fa
             f"in {time.perf_counter() - t0:.1f} s")
         ctx.set_fields([V.make_fields(c.B, c.V, c.dt) for c in w.cells])
         if AUTOTUNE:
-            tuned.append(ctx.autotune())   # persistent remap grid per pass dimension (untimed)
+            ctx.autotune()   # remap launch shape per pass dimension (untimed)
+        tuned.append(ctx.remap_shape())
         ctxs.append(ctx)
     adjust = pops[0].adjust_every_step
 
@@ -457,7 +458,8 @@ def main():
                       "arith": "tolerance" if FAST else "exact",
                       "adjust_every_step": w.adjust_every_step, "blocks_at_start": r["blocks_at_start"],
                       "rotation_deg_per_step": w.rotation_deg,
-                      "remap_wg_per_cu": r["remap_wg_per_cu"]},
+                      "remap_wg_per_cu": [t[0] for t in r["remap_wg_per_cu"]],
+                      "remap_lanes_per_wg": [t[1] for t in r["remap_wg_per_cu"]]},
            "roofline": r["roofline"], "breakdown_ms_per_step": r["breakdown"],
            "step_ms_percentiles": r["step_ms_percentiles"]}
     # the achievable copy bandwidth of this GPU, the second roofline denominator

@@ -104,9 +104,14 @@ typedef struct vacc_options {
    double hall_minimum_rhoq;  /* P::hallMinimumRhoq */
    double max_rotation_deg;   /* vlasovsolver.maxSlAccelerationRotation */
    int32_t fast_arith;        /* 0: remap arithmetic rounds exactly as the reference CPU path
-                                 (bit-identical f); 1: fused multiply-adds and hardware
-                                 reciprocal/sqrt in the reconstruction and integration
-                                 (f within a few ulp; block lists unchanged) */
+                                 (bit-identical f); 1: the tolerance form: reconstructions in
+                                 deviations from the cell average with fused multiply-adds and
+                                 hardware reciprocal/sqrt, and a flux-form deposit from the FP64
+                                 intersections (f within 5e-6 of an FP64 run per cell, L-inf
+                                 relative to the cell's max and L1; block lists unchanged).
+                                 Cells whose deposits outrun the tolerance kernel's target ring,
+                                 and columns where the reference's 8-lane maximum rule binds,
+                                 take the exact arithmetic. */
    int32_t use_graph;         /* 1: vacc_step captures its launches in a hipGraph per state
                                  variant (map order, list/pool parity, options) on first use
                                  and replays it afterwards; results are identical.  Not used
@@ -180,11 +185,16 @@ int vacc_adjust_blocks(vacc_ctx* ctx, double* rho_loss);
 int vacc_adjust_blocks_spatial(vacc_ctx* ctx, const uint32_t* nbr_off, const uint32_t* nbr, const uint64_t* extra_off,
                                const uint32_t* extra_gids, double* rho_loss);
 /* Launch-shape autotuner (no counterpart in the reference): times the remap of each pass
- * dimension on the current state with 8, 32, 64, 128, 256 and 512 workgroups per compute
- * unit (twice each) and keeps the fastest per dimension for later steps.  The trial passes
- * write scratch buffers only, so the state is unchanged; results of later steps do not depend
- * on the choice.  mult_out (3 entries, may be NULL) receives the chosen multipliers. */
+ * dimension on the current state with 8, 32, 128, 512, 1024, 2048 and 4096 workgroups per compute
+ * unit and, in the tolerance form (fast_arith = 1), workgroups of 64, 128 or 256 lanes (4, 8
+ * or 16 work units of 16 lanes, i.e. columns or column sets, per workgroup), twice each, and
+ * keeps the fastest shape per dimension for later steps.  The trial passes write scratch
+ * buffers only, so the state is unchanged; results of later steps do not depend on the
+ * choice.  mult_out (3 entries, may be NULL) receives the chosen workgroups per CU. */
 int vacc_autotune(vacc_ctx* ctx, uint32_t* mult_out);
+/* The remap launch shape per pass dimension (no counterpart in the reference): workgroups per
+ * compute unit and lanes per workgroup (3 entries each, either may be NULL). */
+int vacc_get_remap_shape(vacc_ctx* ctx, uint32_t* wg_per_cu, uint32_t* lanes_per_wg);
 /* Block lookup: index[i] = position of block gids[i] within cell cells[i]'s block list, in
  * the ascending-GID order that vacc_download_cells returns (the local ID after a download),
  * or VACC_INVALID_ID when the cell holds no such block or cells[i] >= ncells

@@ -591,19 +591,30 @@ def test_step_graph_replays_match_eager_steps():
     graph.close()
 
 
-def test_autotune_keeps_state_and_results():
+@pytest.mark.parametrize("fast", [False, True])
+def test_autotune_keeps_state_and_results(fast):
     """vacc_autotune times trial remaps into scratch buffers: the state is unchanged, and
-    steps after tuning equal steps of an untuned context bit for bit."""
+    steps after tuning equal steps of an untuned context bit for bit, in the exact form and in
+    the tolerance form (whose workgroup size, 4, 8 or 16 work units, is tuned as well)."""
     w = W.config(2, ncells=6)
     cells = cases.build_cells(w)
     fields = [V.make_fields(c.B, c.V, c.dt) for c in w.cells]
-    a, b = _ctx(w), _ctx(w)
+
+    def ctx():
+        return V.Context(V.make_mesh(w.grid_length, w.vmin, w.vmax),
+                         V.make_species(w.mass, w.charge, w.min_value, w.add_width),
+                         V.make_options(w.method, fast_arith=fast))
+
+    a, b = ctx(), ctx()
     for c in (a, b):
         c.upload_cells(cells)
         c.set_fields(fields)
-    _say("autotune")
+    _say(f"autotune fast={fast}")
     mult = a.autotune()
-    assert all(m in (8, 32, 64, 128, 256, 512) for m in mult)
+    assert all(m in (8, 32, 128, 512, 1024, 2048, 4096) for m in mult)
+    m2, lanes = a.remap_shape()
+    assert m2 == mult and all(x in ((64, 128, 256) if fast else (256,)) for x in lanes)
+    assert b.remap_shape() == ([128] * 3, [256] * 3)
     cases.assert_cells_equal(a.download_cells(), cells, "state after autotune")
     for step in range(3):
         la, lb = np.zeros(len(cells)), np.zeros(len(cells))

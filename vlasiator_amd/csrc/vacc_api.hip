@@ -113,6 +113,7 @@ struct vacc_ctx {
    std::vector<StepGraph> graphs;
    uint64_t gen = 0;
    int remap_mult[3] = {128, 128, 128};   // remap workgroups per CU, per pass dimension (vacc_autotune)
+   int remap_wgs[3] = {256, 256, 256};    // lanes per workgroup of the tolerance-form remap (vacc_autotune)
    uint32_t* h_sum = nullptr;        // pinned host words: capacity flag, block total, status OR
    // streamed block transfer (vacc_blocks_io.h): two pinned chunks, their copy events and,
    // for the 16-bit forms, a device FP32 stage of two chunks; allocated on first use
@@ -369,7 +370,8 @@ static float cell_f(const vacc_ctx* c, int d) { return (float)c->mi.cell[d]; }
 // Column build and remap of one pass from (lcur, fcur) into the other list and pool; the
 // caller flips the parity.  grid_mult: persistent remap workgroups per CU (0: the tuned or
 // default value).
-static int enqueue_pass(vacc_ctx* c, int dim, int grid_mult) {
+// grid_mult / wgs: the remap launch shape, or 0 for the context's tuned shape
+static int enqueue_pass(vacc_ctx* c, int dim, int grid_mult, int wgs = 0) {
    const int li = c->lcur, lo = 1 - c->lcur, fi = c->fcur, fo = 1 - c->fcur;
    ColumnArgs ca{};
    ca.cell_off = c->off[li];
@@ -447,6 +449,7 @@ static int enqueue_pass(vacc_ctx* c, int dim, int grid_mult) {
    {
       ProfScope ps(c, 0);
       launch_remap(dim, c->opt.method, c->opt.fast_arith != 0, ra, grid_mult ? grid_mult : c->remap_mult[dim],
+                   wgs ? wgs : c->remap_wgs[dim],
                    c->stream);
    }
    HIPCHK(hipGetLastError());
@@ -1641,28 +1644,37 @@ int vacc_autotune(vacc_ctx* c, uint32_t* mult_out) {
                          c->stream));
    const bool prof = c->prof;
    c->prof = false;
-   static const int cand[] = {8, 32, 64, 128, 256, 512};
+   // workgroups per CU, and (tolerance form) lanes per workgroup: 4, 8 or 16 work units each
+   static const int cand[] = {8, 32, 128, 512, 1024, 2048, 4096};
+   static const int wg_fast[] = {64, 128, 256};
+   static const int wg_exact[] = {256};
+   const bool fast = c->opt.fast_arith != 0;
+   const int* wgc = fast ? wg_fast : wg_exact;
+   const int nwg = fast ? 3 : 1;
    hipEvent_t e0, e1;
    HIPCHK(hipEventCreate(&e0));
    HIPCHK(hipEventCreate(&e1));
    for (int dim = 0; dim < 3 && rc == 0; ++dim) {
       float best = 1e30f;
-      int arg = c->remap_mult[dim];
+      int arg = c->remap_mult[dim], argw = c->remap_wgs[dim];
       for (int rep = 0; rep < 2 && rc == 0; ++rep)
-         for (int m : cand) {
-            (void)hipEventRecord(e0, c->stream);
-            rc = enqueue_pass(c, dim, m);
-            (void)hipEventRecord(e1, c->stream);
-            if (rc) break;
-            HIPCHK(hipEventSynchronize(e1));
-            float ms = 0.0f;
-            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-            if (ms < best) {
-               best = ms;
-               arg = m;
+         for (int wi = 0; wi < nwg && rc == 0; ++wi)
+            for (int m : cand) {
+               (void)hipEventRecord(e0, c->stream);
+               rc = enqueue_pass(c, dim, m, wgc[wi]);
+               (void)hipEventRecord(e1, c->stream);
+               if (rc) break;
+               HIPCHK(hipEventSynchronize(e1));
+               float ms = 0.0f;
+               HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+               if (ms < best) {
+                  best = ms;
+                  arg = m;
+                  argw = wgc[wi];
+               }
             }
-         }
       c->remap_mult[dim] = arg;
+      c->remap_wgs[dim] = argw;
    }
    (void)hipEventDestroy(e0);
    (void)hipEventDestroy(e1);
@@ -1678,6 +1690,15 @@ int vacc_autotune(vacc_ctx* c, uint32_t* mult_out) {
    if (fl < 0) return fl;
    if (mult_out)
       for (int d = 0; d < 3; ++d) mult_out[d] = (uint32_t)c->remap_mult[d];
+   return 0;
+}
+
+int vacc_get_remap_shape(vacc_ctx* c, uint32_t* wg_per_cu, uint32_t* lanes_per_wg) {
+   if (!c) return set_err(VACC_E_ARG, "null context");
+   for (int d = 0; d < 3; ++d) {
+      if (wg_per_cu) wg_per_cu[d] = (uint32_t)c->remap_mult[d];
+      if (lanes_per_wg) lanes_per_wg[d] = (uint32_t)(c->opt.fast_arith ? c->remap_wgs[d] : 256);   // the exact kernel: 256
+   }
    return 0;
 }
 

@@ -177,7 +177,9 @@ void launch_columns(int dim, const ColumnArgs& a, hipStream_t s);
 void launch_scan(const ScanArgs& a, hipStream_t s);
 void launch_order(const OrderArgs& a, hipStream_t s);
 // grid: grid_mult persistent workgroups per CU
-void launch_remap(int dim, int method, bool fast, RemapArgs a, int grid_mult, hipStream_t s);
+// grid_mult workgroups per compute unit; wgs: lanes per workgroup of the tolerance-form kernel
+// (64, 128 or 256: 4, 8 or 16 work units); the exact kernel always runs 256-lane workgroups
+void launch_remap(int dim, int method, bool fast, RemapArgs a, int grid_mult, int wgs, hipStream_t s);
 void launch_adjust(const AdjustArgs& a, bool write_phase, hipStream_t s);
 void launch_moments(const MomentArgs& a, hipStream_t s);
 // plasma pivot from the moments of up to MAX_POP populations holding the same cells

@@ -627,8 +627,8 @@ __device__ inline void set_content(uint8_t* content, size_t slot, const float (&
 // of the block from the ring columns of the unit's lanes, encodes the block and stores its 4
 // codes (8 contiguous bytes) and, lane 0, the scale.  The content flag is taken on the stored
 // (decoded) values.
-template <int DIM, int ST, int RN>
-__device__ inline void flush_block(const RemapArgs& a, float (*ring)[WG], int tid, int l16, int setlane0, int o,
+template <int DIM, int ST, int RN, int WGS>
+__device__ inline void flush_block(const RemapArgs& a, float (*ring)[WGS], int tid, int l16, int setlane0, int o,
                                    int blk, size_t slot, float thc) {
    if constexpr (ST == ST_F32) {
       float q[4];
@@ -805,7 +805,7 @@ __global__ __launch_bounds__(WG) void k_remap(RemapArgs a, double i_dv) {
                   if (next_open - next_flush >= RING / WID) {   // ring full: cannot happen for checked spans
                      atomicOr(&a.status[c], VACC_STATUS_DK);
                      atomicOr(&a.status[a.ncells], VACC_STATUS_DK);
-                     flush_block<DIM, ST, RING>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), thc);
+                     flush_block<DIM, ST, RING, WG>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), thc);
                      ++next_flush;
                   }
 #pragma unroll
@@ -813,7 +813,7 @@ __global__ __launch_bounds__(WG) void k_remap(RemapArgs a, double i_dv) {
                   ++next_open;
                }
                while (next_flush < next_open && next_flush < nfk && WID * next_flush + WID - 1 < flushLim) {
-                  flush_block<DIM, ST, RING>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), thc);
+                  flush_block<DIM, ST, RING, WG>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), thc);
                   ++next_flush;
                }
                float tdr = 0.0f;
@@ -838,7 +838,7 @@ __global__ __launch_bounds__(WG) void k_remap(RemapArgs a, double i_dv) {
             }
          }
          while (next_flush < next_open && next_flush < nfk) {
-            flush_block<DIM, ST, RING>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), thc);
+            flush_block<DIM, ST, RING, WG>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), thc);
             ++next_flush;
          }
          for (int tb = next_open; tb <= lk && tb < nfk; ++tb) zero_block<DIM, ST>(a, l16, setlane0, o, slotb + (tb - fk), thc);
@@ -882,11 +882,11 @@ __device__ inline int row_min16(int x) {
    return min(x, row_ror_i<1>(x));
 }
 
-template <int DIM, int METHOD, int ST>
-__global__ __launch_bounds__(WG) void k_remap_lean(RemapArgs a) {
+template <int DIM, int METHOD, int ST, int WGS>
+__global__ __launch_bounds__(WGS) void k_remap_lean(RemapArgs a) {
    using R = ReconLean<METHOD>;
    constexpr int NC = R::NC;
-   __shared__ float ring[LEAN_RING][WG];
+   __shared__ float ring[LEAN_RING][WGS];
    if (*a.flags & VACC_STATUS_CAPACITY) return;
    const int tid = threadIdx.x;
    const int l16 = tid & 15;
@@ -895,8 +895,8 @@ __global__ __launch_bounds__(WG) void k_remap_lean(RemapArgs a) {
    const int o = (DIM == 0) ? (ip * WID2 + jp * WID) : (DIM == 1) ? (jp * WID2 + ip) : (jp * WID + ip);
    // the units before EXACT_BIN's; k_remap takes the rest
    const uint32_t total = a.unit_base[a.ncells] - a.len_hist[EXACT_BIN];
-   const uint32_t gstride = gridDim.x * (WG / 16);
-   for (uint32_t gi = blockIdx.x * (WG / 16) + (tid >> 4); gi < total; gi += gstride) {
+   const uint32_t gstride = gridDim.x * (WGS / 16);
+   for (uint32_t gi = blockIdx.x * (WGS / 16) + (tid >> 4); gi < total; gi += gstride) {
       const uint32_t g = a.perm[gi];
       const uint32_t c = a.perm_cell[gi];
       const uint32_t cbase = a.cell_off[c];
@@ -965,7 +965,7 @@ __global__ __launch_bounds__(WG) void k_remap_lean(RemapArgs a) {
                if (next_open - next_flush >= LEAN_RING / WID) {   // ring full: not for LEAN_SPAN cells
                   atomicOr(&a.status[c], VACC_STATUS_DK);
                   atomicOr(&a.status[a.ncells], VACC_STATUS_DK);
-                  flush_block<DIM, ST, LEAN_RING>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), thc);
+                  flush_block<DIM, ST, LEAN_RING, WGS>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), thc);
                   ++next_flush;
                }
 #pragma unroll
@@ -973,7 +973,7 @@ __global__ __launch_bounds__(WG) void k_remap_lean(RemapArgs a) {
                ++next_open;
             }
             while (next_flush < next_open && next_flush < nfk && WID * next_flush + WID - 1 < flushLim) {
-               flush_block<DIM, ST, LEAN_RING>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), thc);
+               flush_block<DIM, ST, LEAN_RING, WGS>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), thc);
                ++next_flush;
             }
             const float v[12] = {w0[0], w0[1], w0[2], w0[3], w1[0], w1[1], w1[2], w1[3], w2[0], w2[1], w2[2], w2[3]};
@@ -992,7 +992,7 @@ __global__ __launch_bounds__(WG) void k_remap_lean(RemapArgs a) {
          }
          fx.end(emit);
          while (next_flush < next_open && next_flush < nfk) {
-            flush_block<DIM, ST, LEAN_RING>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), thc);
+            flush_block<DIM, ST, LEAN_RING, WGS>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), thc);
             ++next_flush;
          }
          for (int tb = next_open; tb <= lk && tb < nfk; ++tb) zero_block<DIM, ST>(a, l16, setlane0, o, slotb + (tb - fk), thc);
@@ -1653,41 +1653,44 @@ int device_cus() {
 }
 
 template <int DIM, int METHOD, int ST>
-static void remap_dms(RemapArgs a, uint32_t grid, double i_dv, bool fast, hipStream_t s) {
+static void remap_dms(RemapArgs a, uint32_t grid, double i_dv, bool fast, int wgs, hipStream_t s) {
    if (!fast) {
       a.only_flagged = 0;
       hipLaunchKernelGGL((k_remap<DIM, METHOD, ST>), dim3(grid), dim3(WG), 0, s, a, i_dv);
       return;
    }
-   // tolerance form: the lean kernel, then the exact kernel over the few units it leaves
-   hipLaunchKernelGGL((k_remap_lean<DIM, METHOD, ST>), dim3(grid), dim3(WG), 0, s, a);
+   // tolerance form: the lean kernel (workgroups of wgs lanes, wgs/16 work units each), then the
+   // exact kernel over the few units it leaves
+   if (wgs == 64) hipLaunchKernelGGL((k_remap_lean<DIM, METHOD, ST, 64>), dim3(grid), dim3(64), 0, s, a);
+   else if (wgs == 128) hipLaunchKernelGGL((k_remap_lean<DIM, METHOD, ST, 128>), dim3(grid), dim3(128), 0, s, a);
+   else hipLaunchKernelGGL((k_remap_lean<DIM, METHOD, ST, WG>), dim3(grid), dim3(WG), 0, s, a);
    a.only_flagged = 1;
    hipLaunchKernelGGL((k_remap<DIM, METHOD, ST>), dim3((uint32_t)device_cus()), dim3(WG), 0, s, a, i_dv);
 }
 
 template <int DIM, int METHOD>
-static void remap_dm(const RemapArgs& a, uint32_t grid, double i_dv, bool fast, hipStream_t s) {
+static void remap_dm(const RemapArgs& a, uint32_t grid, double i_dv, bool fast, int wgs, hipStream_t s) {
    // the source and target pools share one storage form
    const int st = a.in.storage;
-   if (st == ST_F32) remap_dms<DIM, METHOD, ST_F32>(a, grid, i_dv, fast, s);
-   else if (st == ST_BF16) remap_dms<DIM, METHOD, ST_BF16>(a, grid, i_dv, fast, s);
-   else remap_dms<DIM, METHOD, ST_F16>(a, grid, i_dv, fast, s);
+   if (st == ST_F32) remap_dms<DIM, METHOD, ST_F32>(a, grid, i_dv, fast, wgs, s);
+   else if (st == ST_BF16) remap_dms<DIM, METHOD, ST_BF16>(a, grid, i_dv, fast, wgs, s);
+   else remap_dms<DIM, METHOD, ST_F16>(a, grid, i_dv, fast, wgs, s);
 }
 
 template <int DIM>
-static void remap_d(int method, const RemapArgs& a, uint32_t grid, double i_dv, bool fast, hipStream_t s) {
-   if (method == VACC_PLM) remap_dm<DIM, VACC_PLM>(a, grid, i_dv, fast, s);
-   else if (method == VACC_PPM) remap_dm<DIM, VACC_PPM>(a, grid, i_dv, fast, s);
-   else remap_dm<DIM, VACC_PQM>(a, grid, i_dv, fast, s);
+static void remap_d(int method, const RemapArgs& a, uint32_t grid, double i_dv, bool fast, int wgs, hipStream_t s) {
+   if (method == VACC_PLM) remap_dm<DIM, VACC_PLM>(a, grid, i_dv, fast, wgs, s);
+   else if (method == VACC_PPM) remap_dm<DIM, VACC_PPM>(a, grid, i_dv, fast, wgs, s);
+   else remap_dm<DIM, VACC_PQM>(a, grid, i_dv, fast, wgs, s);
 }
 
-void launch_remap(int dim, int method, bool fast, RemapArgs a, int grid_mult, hipStream_t s) {
+void launch_remap(int dim, int method, bool fast, RemapArgs a, int grid_mult, int wgs, hipStream_t s) {
    if (a.ncells == 0) return;
    const uint32_t grid = (uint32_t)device_cus() * (uint32_t)std::max(grid_mult, 1);
    const double i_dv = 1.0 / (double)a.dv;   // cpu_acc_map.cpp:201, Real i_dv = 1.0/dv
-   if (dim == 0) remap_d<0>(method, a, grid, i_dv, fast, s);
-   else if (dim == 1) remap_d<1>(method, a, grid, i_dv, fast, s);
-   else remap_d<2>(method, a, grid, i_dv, fast, s);
+   if (dim == 0) remap_d<0>(method, a, grid, i_dv, fast, wgs, s);
+   else if (dim == 1) remap_d<1>(method, a, grid, i_dv, fast, wgs, s);
+   else remap_d<2>(method, a, grid, i_dv, fast, wgs, s);
 }
 
 void launch_adjust(const AdjustArgs& a, bool write_phase, hipStream_t s) {

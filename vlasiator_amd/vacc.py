@@ -30,7 +30,7 @@ EXPORTS = [
     "vacc_init_peaks", "vacc_profile_begin", "vacc_profile_read", "vacc_set_fields", "vacc_step",
     "vacc_accelerate_populations", "vacc_adjust_blocks_spatial", "vacc_autotune",
     "vacc_subcycle_count", "vacc_subcycle_step", "vacc_accelerate_subcycled", "vacc_pressure",
-    "vacc_find_blocks", "vacc_find_blocks_noalloc",
+    "vacc_find_blocks", "vacc_find_blocks_noalloc", "vacc_get_remap_shape",
 ]
 
 
@@ -114,6 +114,7 @@ def lib():
         L.vacc_accelerate_populations.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_int32, C.c_int32, vp]
         L.vacc_adjust_blocks_spatial.argtypes = [vp, vp, vp, vp, vp, vp]
         L.vacc_autotune.argtypes = [vp, vp]
+        L.vacc_get_remap_shape.argtypes = [vp, vp, vp]
         L.vacc_pressure.argtypes = [vp, vp, vp]
         L.vacc_find_blocks.argtypes = [vp, C.c_uint64, vp, vp, vp]
         L.vacc_find_blocks_noalloc.argtypes = [vp, C.c_uint64, vp, vp, vp]
@@ -321,10 +322,18 @@ class Context:
         return loss[:ncells]
 
     def autotune(self):
-        """Pick the persistent remap grid per pass dimension on the current state."""
+        """Pick the remap launch shape per pass dimension on the current state; returns the
+        workgroups per compute unit (remap_shape() gives the lanes per workgroup as well)."""
         out = np.zeros(3, np.uint32)
         _check(lib().vacc_autotune(self.h, _ptr(out)))
         return out.tolist()
+
+    def remap_shape(self):
+        """(workgroups per compute unit, lanes per workgroup) of the remap, per pass dimension."""
+        m = np.zeros(3, np.uint32)
+        w = np.zeros(3, np.uint32)
+        _check(lib().vacc_get_remap_shape(self.h, _ptr(m), _ptr(w)))
+        return m.tolist(), w.tolist()
 
     def moments(self, ncells):
         out = np.zeros(max(ncells, 1) * 4, np.float64)
