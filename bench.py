@@ -383,9 +383,11 @@ def measure(pops, storage: str, steps: int, warmup: int, graph: bool = False, fa
         roofline["traffic_source"] = f"{src}: rocprofv3 FETCH_SIZE + WRITE_SIZE per launch, x{ratio:.3f} algorithmic"
     busy, src = profiled_traffic(wname, storage, key=f"valu-busy-{kernel}")
     if busy is not None:
-        # AMD's VALUBusy: SQ_ACTIVE_INST_VALU / (CU_NUM * GRBM_GUI_ACTIVE per XCD), 0..1
+        # share of SIMD quad-cycles in which at least one VALU instruction issued (gfx950 can
+        # issue two in one quad-cycle), at most 1 by construction
         roofline["valu_busy"] = busy
-        roofline["valu_busy_source"] = f"{src}: rocprofv3 SQ_ACTIVE_INST_VALU / (CUs x GRBM_GUI_ACTIVE)"
+        roofline["valu_busy_source"] = (f"{src}: rocprofv3 (SQ_ACTIVE_INST_VALU - SQ_ACTIVE_INST_VALU2) / "
+                                        f"(CUs x GRBM_GUI_ACTIVE), quad-cycles with a VALU issue")
     pct = {f"p{q}": round(float(np.percentile(step_ms, q)), 3) for q in (50, 90, 99)}
     return {"value": value, "ms_per_step": ms_per_step, "blocks_at_start": n_init, "roofline": roofline,
             "breakdown": breakdown, "step_ms_percentiles": pct, "remap_wg_per_cu": tuned}
