@@ -933,8 +933,10 @@ __global__ __launch_bounds__(WGS) void k_remap_lean(RemapArgs a) {
             continue;
          }
          const int lo = WID * fk, hi = WID * lk + WID - 1;
+         // a target outside the column's range adds 0 to its lowest target instead (no branch)
          auto emit = [&](int gt, float m) {
-            if (gt >= lo && gt <= hi) atomicAdd(&ring[gt & (LEAN_RING - 1)][tid], m);
+            const bool in = (uint32_t)(gt - lo) <= (uint32_t)(hi - lo);
+            atomicAdd(&ring[(in ? gt : lo) & (LEAN_RING - 1)][tid], in ? m : 0.0f);
          };
          Flux<NC> fx;
          fx.begin((a.vmin_d + (double)(WID * ks) * a.dv_d - imin) * rdk, r);
@@ -952,10 +954,6 @@ __global__ __launch_bounds__(WGS) void k_remap_lean(RemapArgs a) {
             rc.start(v0 + 4);
          }
          for (int b = 0; b < len; ++b) {
-            const uint32_t s_after = (b + 3 < len) ? srcs[b + 3] : INVALID;
-            float w3[4];
-            load_pencil<DIM, ST>(a.in, s_next, o, w3);   // zeros past the column's end
-            s_next = s_after;
             // targets the block's 4 cells can reach (one cell of margin for rounding), and the
             // lowest target any lane still accumulates
             const int reach = fx.g + ((fx.t < 4.0f) ? (int)((4.0f - fx.t) * inv_r) + 1 : 0);
@@ -972,13 +970,29 @@ __global__ __launch_bounds__(WGS) void k_remap_lean(RemapArgs a) {
                for (int e = 0; e < 4; ++e) ring[(WID * next_open + e) & (LEAN_RING - 1)][tid] = 0.0f;
                ++next_open;
             }
+            // Memory order of the iteration: the first cell uses block b + 1, whose load (and the
+            // previous iteration's stores) the compiler waits for in full, since the stores vary
+            // in number; only then are block b + 2 read and the finished target blocks (all below
+            // flushLim, which no deposit of this block reaches) stored, so that cells 1..3 cover
+            // their latency.  The scheduling barriers keep that order.
+            const float v[12] = {w0[0], w0[1], w0[2], w0[3], w1[0], w1[1], w1[2], w1[3], w2[0], w2[1], w2[2], w2[3]};
+            {
+               float cf[NC];
+               rc.coeffs(v + 4, cf);
+               fx.cell(cf, v[4], emit);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+            const uint32_t s_after = (b + 3 < len) ? srcs[b + 3] : INVALID;
+            float w3[4];
+            load_pencil<DIM, ST>(a.in, s_next, o, w3);   // zeros past the column's end
+            s_next = s_after;
             while (next_flush < next_open && next_flush < nfk && WID * next_flush + WID - 1 < flushLim) {
                flush_block<DIM, ST, LEAN_RING, WGS>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), thc);
                ++next_flush;
             }
-            const float v[12] = {w0[0], w0[1], w0[2], w0[3], w1[0], w1[1], w1[2], w1[3], w2[0], w2[1], w2[2], w2[3]};
+            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-            for (int kk = 0; kk < WID; ++kk) {
+            for (int kk = 1; kk < WID; ++kk) {
                float cf[NC];
                rc.coeffs(v + 4 + kk, cf);
                fx.cell(cf, v[4 + kk], emit);

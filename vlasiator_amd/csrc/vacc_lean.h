@@ -230,22 +230,22 @@ struct Flux {
    }
    // One source cell with integral coefficients c and average v; emit(target, mass) receives
    // each target this cell closes.
+   // The first boundary is handled without a branch: a cell that holds none emits 0 to its
+   // open target, which leaves every sum as it is.
    template <class Emit>
    VACC_HD void cell(const float* c, float v, Emit&& emit) {
       const bool hit = t < 1.0f;
       float Fp = lean_integral<NC>(hit ? t : 0.0f, c);   // F(0) = 0
-      if (hit) {
-         emit(g - 1, acc + Fp);
-         acc = 0.0f;
+      emit(g - 1, hit ? acc + Fp : 0.0f);
+      acc = hit ? 0.0f : acc;
+      g += hit ? 1 : 0;
+      t += hit ? r : 0.0f;
+      while (t < 1.0f) {   // more boundaries in this cell (dk < dv)
+         const float Ft = lean_integral<NC>(t, c);
+         emit(g - 1, Ft - Fp);
+         Fp = Ft;
          ++g;
          t += r;
-         while (t < 1.0f) {   // more boundaries in this cell (dk < dv)
-            const float Ft = lean_integral<NC>(t, c);
-            emit(g - 1, Ft - Fp);
-            Fp = Ft;
-            ++g;
-            t += r;
-         }
       }
       acc += v - Fp;
       t -= 1.0f;
