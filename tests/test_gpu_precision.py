@@ -6,9 +6,14 @@ Per cell: L1 and L-inf of f relative to the FP64 run (blocks missing on one side
 zero), and the relative error of density n, bulk velocity |V - V64| / |V64| and scalar
 pressure P = (m/3) sum f |v - V|^2 dV^3 (arch_moments.h:123-150).
 
-Bounds asserted (the precision contract of the storage forms): density within 1e-6 relative for every
-form.  The other figures are reported (set VACC_PRECISION_OUT to write them as JSON) and
-bounded loosely so that a broken codec fails: L1 of f <= 2e-4 (FP32), 3e-3 (FP16), 3e-2 (BF16).
+Bounds asserted: density within 1e-6 relative for every form (the precision contract of the
+storage forms).  Bulk velocity within 1e-6 relative is not met, by the reference's own FP32
+path either (tests/test_oracle_precision.py measures the FP32 oracle against the FP64 one), so
+the measured figures are bounded with a margin of about 2-3x: |V - V64| / |V64| <= 5e-5 (FP32),
+1e-4 (FP16), 1e-3 (BF16), which is below 0.01 (FP32), 0.02 (FP16) and 0.1 (BF16) of a velocity
+cell and below 1e-3 of the thermal speed; pressure <= 1e-5 (FP32), 3e-5 (FP16), 3e-4 (BF16);
+L1 of f <= 2e-4 (FP32), 3e-3 (FP16), 3e-2 (BF16).  The figures are written as JSON when
+VACC_PRECISION_OUT names a file.
 """
 import json
 import os
@@ -24,6 +29,9 @@ pytestmark = pytest.mark.gpu
 
 NCELLS, STEPS = 8, 20
 L1_BOUND = {"f32": 2e-4, "f16": 3e-3, "bf16": 3e-2}
+V_BOUND = {"f32": 5e-5, "f16": 1e-4, "bf16": 1e-3}
+DV_CELLS_BOUND = {"f32": 0.01, "f16": 0.02, "bf16": 0.1}
+P_BOUND = {"f32": 1e-5, "f16": 3e-5, "bf16": 3e-4}
 
 
 def test_storage_forms_against_fp64_run():
@@ -53,5 +61,9 @@ def test_storage_forms_against_fp64_run():
         with open(path, "w") as fh:
             json.dump(report, fh, indent=1)
     for name in ("f32", "bf16", "f16", "f32-fast", "bf16-fast"):
-        assert report[name]["n"] <= 1e-6, (name, report[name])
-        assert report[name]["l1"] <= L1_BOUND[name.split("-")[0]], (name, report[name])
+        st = name.split("-")[0]
+        r = report[name]
+        assert r["n"] <= 1e-6, (name, r)
+        assert r["l1"] <= L1_BOUND[st], (name, r)
+        assert r["V"] <= V_BOUND[st] and r["dV_dv"] <= DV_CELLS_BOUND[st] and r["dV_vth"] <= 1e-3, (name, r)
+        assert r["P"] <= P_BOUND[st], (name, r)
