@@ -933,10 +933,12 @@ __global__ __launch_bounds__(WGS) void k_remap_lean(RemapArgs a) {
             continue;
          }
          const int lo = WID * fk, hi = WID * lk + WID - 1;
-         // a target outside the column's range adds 0 to its lowest target instead (no branch)
+         // a target outside the column's range adds 0 to its lowest target instead (no branch);
+         // each lane owns its ring column, so a plain read-add-write suffices
          auto emit = [&](int gt, float m) {
             const bool in = (uint32_t)(gt - lo) <= (uint32_t)(hi - lo);
-            atomicAdd(&ring[(in ? gt : lo) & (LEAN_RING - 1)][tid], in ? m : 0.0f);
+            float& slot = ring[(in ? gt : lo) & (LEAN_RING - 1)][tid];
+            slot = slot + (in ? m : 0.0f);
          };
          Flux<NC> fx;
          fx.begin((a.vmin_d + (double)(WID * ks) * a.dv_d - imin) * rdk, r);
