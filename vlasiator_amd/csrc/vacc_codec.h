@@ -101,8 +101,12 @@ __device__ inline float dec16(int st, uint32_t u) {
    return (st == ST_BF16) ? Codec<ST_BF16>::dec(u) : Codec<ST_F16>::dec(u);
 }
 
+// FAST: the quotient Sf/Sq from the hardware reciprocal (within 2 ulp of the IEEE quotient),
+// for the tolerance form of the remap; otherwise the IEEE division.
+template <bool FAST = false>
 __device__ inline float pick_scale(float p, float sf, float sq) {
-   const float r = (sq > 0.0f) ? sf / sq : 0.0f;
+   const float q = FAST ? sf * __builtin_amdgcn_rcpf(sq) : sf / sq;
+   const float r = (sq > 0.0f) ? q : 0.0f;
    return (sf > 0.0f && r >= 0.5f * p && r <= 2.0f * p) ? r : p;
 }
 
@@ -110,20 +114,32 @@ __device__ inline float pick_scale(float p, float sf, float sq) {
 // x[4] = f(i=0..3, j=L&3, k=L>>2).  All 16 lanes of the row must be active.  Writes the lane's
 // 4 codes packed as stored (x[0] in the low half of p.x) and returns the block scale (equal in
 // all 16 lanes).
-template <int ST>
-__device__ inline float encode_rows16(const float (&x)[4], uint2& p) {
+template <int ST, bool FAST = false>
+// qmax: the largest of the lane's 4 decoded codes (0 for an all-zero block).  With the block
+// scale sc >= 0, the largest stored value max_e RN(sc * d_e) is RN(sc * qmax), rounding being
+// monotone, so one product gives the block's content test.
+__device__ inline float encode_rows16(const float (&x)[4], uint2& p, float& qmax) {
    float mx = fmaxf(fmaxf(fabsf(x[0]), fabsf(x[1])), fmaxf(fabsf(x[2]), fabsf(x[3])));
    mx = row_maxf16(mx);
    if (!(mx > 0.0f)) {   // the same in all lanes of the row
       p = make_uint2(0u, 0u);
+      qmax = 0.0f;
       return 0.0f;
    }
    const int ex = __builtin_amdgcn_frexp_expf(mx);
    p.x = Pack2<ST>::enc(__builtin_amdgcn_ldexpf(x[0], -ex), __builtin_amdgcn_ldexpf(x[1], -ex));
    p.y = Pack2<ST>::enc(__builtin_amdgcn_ldexpf(x[2], -ex), __builtin_amdgcn_ldexpf(x[3], -ex));
+   const float q0 = Pack2<ST>::lo(p.x), q1 = Pack2<ST>::hi(p.x), q2 = Pack2<ST>::lo(p.y), q3 = Pack2<ST>::hi(p.y);
+   qmax = fmaxf(fmaxf(q0, q1), fmaxf(q2, q3));
    const float sf = row_sum16((x[0] + x[1]) + (x[2] + x[3]));
-   const float sq = row_sum16((Pack2<ST>::lo(p.x) + Pack2<ST>::hi(p.x)) + (Pack2<ST>::lo(p.y) + Pack2<ST>::hi(p.y)));
-   return pick_scale(__builtin_amdgcn_ldexpf(1.0f, ex), sf, sq);
+   const float sq = row_sum16((q0 + q1) + (q2 + q3));
+   return pick_scale<FAST>(__builtin_amdgcn_ldexpf(1.0f, ex), sf, sq);
+}
+
+// Largest decoded code of a lane's 4 packed codes (the content test of a stored block, as above).
+template <int ST>
+__device__ inline float max_code(const uint2& p) {
+   return fmaxf(fmaxf(Pack2<ST>::lo(p.x), Pack2<ST>::hi(p.x)), fmaxf(Pack2<ST>::lo(p.y), Pack2<ST>::hi(p.y)));
 }
 
 // Encode one block held by a whole wave, lane e holding x = f_e (e = i + 4j + 16k).

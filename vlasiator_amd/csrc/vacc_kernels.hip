@@ -616,18 +616,21 @@ __device__ inline float content_threshold(double mv) {
    return ((double)f < mv) ? nextafterf(f, INFINITY) : f;
 }
 
-__device__ inline void set_content(uint8_t* content, size_t slot, const float (&w)[4], float thc, int setlane0, int l16) {
-   const bool has = (w[0] >= thc) || (w[1] >= thc) || (w[2] >= thc) || (w[3] >= thc);
+__device__ inline void set_content_flag(uint8_t* content, size_t slot, bool has, int setlane0, int l16) {
    const unsigned long long bal = __ballot(has);
    if (l16 == 0) content[slot] = (uint8_t)(((bal >> setlane0) & 0xFFFFull) != 0ull);
+}
+__device__ inline void set_content(uint8_t* content, size_t slot, const float (&w)[4], float thc, int setlane0, int l16) {
+   set_content_flag(content, slot, (w[0] >= thc) || (w[1] >= thc) || (w[2] >= thc) || (w[3] >= thc), setlane0, l16);
 }
 
 // Store one finished target block from the ring (all 16 lanes of the unit active).  FP32: each
 // lane stores its own pencil.  16-bit forms: lane L reads the canonical row (j,k) = (L&3, L>>2)
 // of the block from the ring columns of the unit's lanes, encodes the block and stores its 4
 // codes (8 contiguous bytes) and, lane 0, the scale.  The content flag is taken on the stored
-// (decoded) values.
-template <int DIM, int ST, int RN, int WGS>
+// (decoded) values.  FAST: the block scale's quotient from the hardware reciprocal (the
+// tolerance form).
+template <int DIM, int ST, int RN, int WGS, bool FAST = false>
 __device__ inline void flush_block(const RemapArgs& a, float (*ring)[WGS], int tid, int l16, int setlane0, int o,
                                    int blk, size_t slot, float thc) {
    if constexpr (ST == ST_F32) {
@@ -649,13 +652,12 @@ __device__ inline void flush_block(const RemapArgs& a, float (*ring)[WGS], int t
          x[i] = ring[(WID * blk + row) & (RN - 1)][base + lane];
       }
       uint2 pk;
-      const float sc = encode_rows16<ST>(x, pk);
+      float qmax;
+      const float sc = encode_rows16<ST, FAST>(x, pk, qmax);
       uint16_t* p = static_cast<uint16_t*>(a.out.f) + slot * WID3 + 4 * L;
       *reinterpret_cast<uint2*>(p) = pk;
       if (L == 0) a.out.scale[slot] = sc;
-      const float d[4] = {sc * Pack2<ST>::lo(pk.x), sc * Pack2<ST>::hi(pk.x), sc * Pack2<ST>::lo(pk.y),
-                          sc * Pack2<ST>::hi(pk.y)};
-      set_content(a.content_out, slot, d, thc, setlane0, l16);
+      set_content_flag(a.content_out, slot, sc * qmax >= thc, setlane0, l16);
    }
 }
 
@@ -691,9 +693,7 @@ __device__ inline void copy_block(const RemapArgs& a, int l16, int setlane0, int
       }
       *reinterpret_cast<uint2*>(static_cast<uint16_t*>(a.out.f) + slot * WID3 + 4 * l16) = raw;
       if (l16 == 0) a.out.scale[slot] = sc;
-      const float d[4] = {sc * Codec<ST>::dec(raw.x & 0xFFFFu), sc * Codec<ST>::dec(raw.x >> 16),
-                          sc * Codec<ST>::dec(raw.y & 0xFFFFu), sc * Codec<ST>::dec(raw.y >> 16)};
-      set_content(a.content_out, slot, d, thc, setlane0, l16);
+      set_content_flag(a.content_out, slot, sc * max_code<ST>(raw) >= thc, setlane0, l16);
    }
 }
 
@@ -882,8 +882,11 @@ __device__ inline int row_min16(int x) {
    return min(x, row_ror_i<1>(x));
 }
 
+// At least five waves per SIMD (at most 96 VGPRs): the few values that no longer fit are kept
+// in scratch across the per-unit setup and the column ends, not in the block loop.  Five waves
+// issue about 7 % more per pass than four on the PQM workloads; six and eight are slower.
 template <int DIM, int METHOD, int ST, int WGS>
-__global__ __launch_bounds__(WGS) void k_remap_lean(RemapArgs a) {
+__global__ __launch_bounds__(WGS, 5) void k_remap_lean(RemapArgs a) {
    using R = ReconLean<METHOD>;
    constexpr int NC = R::NC;
    __shared__ float ring[LEAN_RING][WGS];
@@ -965,7 +968,7 @@ __global__ __launch_bounds__(WGS) void k_remap_lean(RemapArgs a) {
                if (next_open - next_flush >= LEAN_RING / WID) {   // ring full: not for LEAN_SPAN cells
                   atomicOr(&a.status[c], VACC_STATUS_DK);
                   atomicOr(&a.status[a.ncells], VACC_STATUS_DK);
-                  flush_block<DIM, ST, LEAN_RING, WGS>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), thc);
+                  flush_block<DIM, ST, LEAN_RING, WGS, true>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), thc);
                   ++next_flush;
                }
 #pragma unroll
@@ -989,7 +992,7 @@ __global__ __launch_bounds__(WGS) void k_remap_lean(RemapArgs a) {
             load_pencil<DIM, ST>(a.in, s_next, o, w3);   // zeros past the column's end
             s_next = s_after;
             while (next_flush < next_open && next_flush < nfk && WID * next_flush + WID - 1 < flushLim) {
-               flush_block<DIM, ST, LEAN_RING, WGS>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), thc);
+               flush_block<DIM, ST, LEAN_RING, WGS, true>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), thc);
                ++next_flush;
             }
             __builtin_amdgcn_sched_barrier(0);
@@ -1008,7 +1011,7 @@ __global__ __launch_bounds__(WGS) void k_remap_lean(RemapArgs a) {
          }
          fx.end(emit);
          while (next_flush < next_open && next_flush < nfk) {
-            flush_block<DIM, ST, LEAN_RING, WGS>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), thc);
+            flush_block<DIM, ST, LEAN_RING, WGS, true>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), thc);
             ++next_flush;
          }
          for (int tb = next_open; tb <= lk && tb < nfk; ++tb) zero_block<DIM, ST>(a, l16, setlane0, o, slotb + (tb - fk), thc);

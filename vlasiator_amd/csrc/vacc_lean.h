@@ -13,10 +13,12 @@
 //   c3 = 8 u_l + 7 u_r + (3 d_l - 2 d_r) / 2, c4 = -3 (u_l + u_r) + (d_r - d_l) / 2,
 // whose monotonicity coefficients are b0 = 6 c2, b1 = 24 c3, b2 = 60 c4 (White & Adcroft
 // 2008).  The critical points of F' are then (-2 c3 +- sqrt(4 c3^2 - 10 c2 c4)) / (10 c4) and
-// F' = d_l + x (6 c2 + x (12 c3 + x 20 c4)).  The reference's fallback for a negative
-// discriminant, sqrt -> b1 + 200 b2, becomes 2 c3 + 1000 c4 in these units.  Its repair of a
-// non-monotone cell is taken without branches.  Multiplies are fused, 1/x and sqrt are the
-// hardware's; results agree with the reference's FP32 rounding to a few ulp of the faces.
+// F' = d_l + x (6 c2 + x (12 c3 + x 20 c4)).  The reference tests F' at both critical points
+// (with a fallback root for a negative discriminant, sqrt -> b1 + 200 b2); here a Bernstein
+// bound passes most cells without either, and the others test the one critical point where
+// F' can first take the wrong sign (the same decision: see ReconLean<PQM>::coeffs).
+// Multiplies are fused, 1/x and sqrt are the hardware's; results agree with the reference's
+// FP32 rounding to a few ulp of the faces.
 //
 // Deposit (cpu_acc_map.cpp:533-628) in flux form: Lagrangian boundary g of the pencil lies at
 // Imin + g dk, t is the next boundary's offset from the current source cell's left edge in
@@ -162,19 +164,29 @@ struct ReconLean<VACC_PQM> {
       float c3 = fmaf(8.0f, ul, fmaf(7.0f, ur, fmaf(1.5f, dl, -dr)));
       float c4 = fmaf(-3.0f, ul + ur, 0.5f * (dr - dl));
       // monotonicity: the quartic's derivative at its critical points inside [0, 1] must
-      // have the sign of psl + psr = 2 (a + b)
+      // have the sign of psl + psr = 2 (a + b).  F' = dl + x (6 c2 + x (12 c3 + x 20 c4)) has
+      // the Bernstein coefficients dl, dl + 2 c2, dl + 4 (c2 + c3), dr on [0, 1]; the end ones
+      // carry the sign of ssg (or are 0) after the face checks above.  When the inner two do
+      // as well, so does F' on all of [0, 1] (it lies in their convex hull) and the cell
+      // passes; otherwise the critical points are tested.
       const float ssg = a + bo;
-      const float t1 = c2 * c4;
-      const float disc = fmaf(4.0f * c3, c3, -10.0f * t1);
-      const float sq = (disc < 0.0f) ? fmaf(1000.0f, c4, 2.0f * c3) : lean_sqrt(disc);
-      const float rc = lean_rcp(10.0f * c4);
-      const float r1 = (sq - 2.0f * c3) * rc, r2 = -(2.0f * c3 + sq) * rc;
-      const float k1 = 6.0f * c2, k2 = 12.0f * c3, k3 = 20.0f * c4;
-      const float d1 = fmaf(r1, fmaf(r1, fmaf(r1, k3, k2), k1), dl);
-      const float d2 = fmaf(r2, fmaf(r2, fmaf(r2, k3, k2), k1), dl);
-      const bool bad1 = (r1 >= 0.0f && r1 <= 1.0f && d1 * ssg < 0.0f);
-      const bool bad2 = (r2 >= 0.0f && r2 <= 1.0f && d2 * ssg < 0.0f);
-      if (bad1 || bad2) {
+      const float bn1 = fmaf(2.0f, c2, dl) * ssg, bn2 = fmaf(4.0f, c2 + c3, dl) * ssg;
+      bool bad = false;
+      if (!(fminf(bn1, bn2) >= 0.0f)) {
+         // The critical points are x = (-2 c3 +- sq) / (10 c4), sq = sqrt(4 c3^2 - 10 c2 c4).
+         // The third derivative there is +-12 sq, so the + root is the local minimum of F' and
+         // the - root its local maximum.  With both ends of F' on the side of ssg, F' crosses
+         // to the other side at a critical point only if it does at the local minimum of
+         // ssg F': the + root for ssg > 0, the - root for ssg < 0 (a wrong-signed local maximum
+         // of ssg F' inside [0, 1] has a lower local minimum between it and an end).  A
+         // negative discriminant leaves F' monotone between its two well-signed ends, and the
+         // reference's fallback root then finds nothing either.
+         const float disc = fmaf(4.0f * c3, c3, -10.0f * (c2 * c4));
+         const float x = (copysignf(lean_sqrt(fmaxf(disc, 0.0f)), ssg) - 2.0f * c3) * lean_rcp(10.0f * c4);
+         const float d = fmaf(x, fmaf(x, fmaf(x, 20.0f * c4, 12.0f * c3), 6.0f * c2), dl);
+         bad = disc >= 0.0f && x >= 0.0f && x <= 1.0f && d * ssg < 0.0f;
+      }
+      if (bad) {
          // repair (cpu_1d_pqm.hpp:79-145) on the side of the smaller one-sided slope
          const bool lft = fabsf(bo) <= fabsf(a);
          const float dlx = lft ? (-2.0f / 3.0f) * fmaf(4.0f, ul, ur) : fmaf(-6.0f, ul, -4.0f * ur);
