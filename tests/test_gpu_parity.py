@@ -372,6 +372,47 @@ def test_fast_arith_within_tolerance(cfg, method):
     ctx.close()
 
 
+def test_fast_arith_wide_targets_in_column_sets():
+    """fast_arith=1 with target cells 3.5 source cells wide on the x pass (dk scaled by 3.5):
+    two columns of one pencil, one block apart, can then deposit into one target cell, which
+    k_remap_lean (one store per target) must not take; the column build hands such column sets
+    to the exact kernel and single columns stay in k_remap_lean.  Against the oracle with the
+    same intersections: equal block lists, density within 1e-6 of the FP64 run (a lost deposit
+    would show here), and L-inf within 2e-4 (the exact kernel's FP32 Lagrangian coordinates)."""
+    w = W.config(2, ncells=8)
+    cells = cases.build_cells(w)
+    m = cases.oracle_mesh(w)
+    nx, ny = w.grid_length[0], w.grid_length[1]
+    gaps = 0
+    for g, _ in cells:   # pencils along x with a one-block gap: column sets of the x pass
+        key = (g // nx).astype(np.int64)
+        for k in np.unique(key):
+            gaps += int((np.diff(np.sort(g[key == k] % nx)) == 2).sum())
+    assert gaps > 0
+    ctx = V.Context(V.make_mesh(w.grid_length, w.vmin, w.vmax),
+                    V.make_species(w.mass, w.charge, w.min_value, w.add_width),
+                    V.make_options(O.PQM, fast_arith=True))
+    ctx.upload_cells(cells)
+    isx = cases.oracle_intersections(w, 0)
+    isx[:, 1] *= 3.5   # the x pass's dk (map order 0 runs x first)
+    ctx.set_intersections(isx)
+    _say("fast arith, dk = 3.5 dv on the x pass")
+    ctx.accelerate(0)
+    got = ctx.download_cells()
+    worst = [0.0, 0.0]
+    for i, ((g1, f1), (g, f)) in enumerate(zip(got, cells)):
+        ga, fa, _ = O.accelerate_cell(m, g, f, isx[i], 0, O.PQM, w.min_value)
+        gb, fb, _ = O.accelerate_cell(m, g, f.astype(np.float64), isx[i], 0, O.PQM, w.min_value, fp64=True)
+        ga, fa = cases.sort_cell(ga, fa)
+        gb, fb = cases.sort_cell(gb, fb)
+        assert np.array_equal(g1, ga) and np.array_equal(g1, gb), f"cell {i}"
+        x = f1.astype(np.float64)
+        worst = [max(worst[0], np.abs(x - fb).max() / np.abs(fb).max()), max(worst[1], abs(x.sum() / fb.sum() - 1))]
+    _say(f"wide targets: vs FP64 L-inf {worst[0]:.3g}, density {worst[1]:.3g}")
+    assert worst[0] <= 2e-4 and worst[1] <= 1e-6
+    ctx.close()
+
+
 @pytest.mark.parametrize("store", ["bf16", "f16"])
 def test_fast_arith_16bit_storage(store):
     """fast_arith=1 with 16-bit storage (k_remap_lean decoding the source blocks and encoding

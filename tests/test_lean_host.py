@@ -44,7 +44,8 @@ def lib():
                        check=True, capture_output=True)
     L = C.CDLL(OUT)
     L.lean_coeff_batch.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_void_p]
-    L.lean_column.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_double, C.c_float, C.c_int, C.c_int, C.c_void_p]
+    L.lean_column.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_double, C.c_float, C.c_int, C.c_int, C.c_void_p,
+                              C.c_void_p]
     return L
 
 
@@ -66,7 +67,9 @@ def _dense(g, f, nx, ny, pad):
 
 def lean_map_1d(lib, mesh_len, vmin, vmax, g, f, isx4, dim, method):
     """The lean pass over a whole cell: each column (run of blocks along dim) of each pencil
-    through lean_column, from the FP64 intersections."""
+    through lean_column, from the FP64 intersections.  Like k_remap_lean, each column stores
+    its targets; the columns of a pencil must write disjoint target ranges (k_remap_lean keeps
+    the blocks they share open across them), which is asserted."""
     nx, ny, nz = mesh_len
     dv = (vmax[dim] - vmin[dim]) / (4 * mesh_len[dim])
     pad = 3
@@ -89,6 +92,7 @@ def lean_map_1d(lib, mesh_len, vmin, vmax, g, f, isx4, dim, method):
             imin = isx4[0] + gt[tq[0]] * isx4[1 + tq[0]] + gt[tq[1]] * isx4[1 + tq[1]]
             blocks = np.nonzero(line_occ)[0]
             runs = np.split(blocks, np.nonzero(np.diff(blocks) != 1)[0] + 1)
+            prev_hi = None
             for r in runs:
                 k0, k1 = 4 * int(r[0]), 4 * int(r[-1]) + 4
                 col = np.zeros(k1 - k0 + 8, np.float32)
@@ -96,8 +100,12 @@ def lean_map_1d(lib, mesh_len, vmin, vmax, g, f, isx4, dim, method):
                 x0 = (vmin[dim] + (4 * lo[dim] + k0) * dv - imin) / dk
                 o = np.zeros(dm.shape[-1], np.float32)
                 base = 4 * int(lo[dim])
+                span = np.zeros(2, np.int32)
                 lib.lean_column(method, col.ctypes.data, k1 - k0, x0, np.float32(dk / dv), base,
-                                base + dm.shape[-1] - 1, o.ctypes.data)
+                                base + dm.shape[-1] - 1, o.ctypes.data, span.ctypes.data)
+                if span[0] <= span[1]:
+                    assert prev_hi is None or span[0] > prev_hi, "columns of a pencil write a shared target"
+                    prev_hi = span[1]
                 out[i0, i1] += o
     return np.moveaxis(out, -1, ax), lo
 

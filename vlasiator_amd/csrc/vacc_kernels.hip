@@ -290,6 +290,10 @@ __global__ __launch_bounds__(CWG) void k_columns(ColumnArgs a) {
    const bool idle = a.active && !a.active[c];
    const bool copy = idle || pass_is_copy(a.dv, di, dj, dk);
    const bool lean_wide = a.fast && !copy && !((4.0f * a.dv + 3.0f * fabsf(di) + 3.0f * fabsf(dj)) / dk <= LEAN_SPAN);
+   // the tolerance form writes each target cell once; consecutive columns of a column set
+   // (sources at least one block apart) then reach distinct target cells when dk <= 4 dv, here
+   // with a margin for the running boundary offsets: wider target cells go to the exact kernel
+   const bool lean_shared = a.fast && !copy && !(fabsf(dk) <= 3.0f * a.dv);
    const int maxlen = (int)a.len[DIM];
    uint32_t mystat = (copy && !idle) ? VACC_STATUS_DK : 0u;
 
@@ -415,7 +419,7 @@ __global__ __launch_bounds__(CWG) void k_columns(ColumnArgs a) {
                    max_lane_rule_binds(b0, b1, (int)(a.col_rec[off + cb + j].x & 0xFFFFu), I, di, dj, a.dv, a.vmin);
          };
          if (overlap) {
-            bool ex = lean_wide;
+            bool ex = lean_wide || lean_shared;
             for (uint32_t j = 0; j < ncol && !ex; ++j) ex = exact_col(j);
             a.unit_rec[off + ui] = make_uint4(bb, cb, ncol | cf | (ex ? EXACT_FLAG : 0u), bpos);
             atomicAdd(&hist[ex ? (uint32_t)EXACT_BIN : len_bin(bpos)], 1u);
@@ -889,7 +893,9 @@ template <int DIM, int METHOD, int ST, int WGS>
 __global__ __launch_bounds__(WGS, 5) void k_remap_lean(RemapArgs a) {
    using R = ReconLean<METHOD>;
    constexpr int NC = R::NC;
-   __shared__ float ring[LEAN_RING][WGS];
+   // rows 0 .. LEAN_RING - 1: the target ring; row LEAN_RING: sink of deposits outside the
+   // column's target range
+   __shared__ float ring[LEAN_RING + 1][WGS];
    if (*a.flags & VACC_STATUS_CAPACITY) return;
    const int tid = threadIdx.x;
    const int l16 = tid & 15;
@@ -936,12 +942,15 @@ __global__ __launch_bounds__(WGS, 5) void k_remap_lean(RemapArgs a) {
             continue;
          }
          const int lo = WID * fk, hi = WID * lk + WID - 1;
-         // a target outside the column's range adds 0 to its lowest target instead (no branch);
-         // each lane owns its ring column, so a plain read-add-write suffices
+         // Each target cell is written once, with its whole mass: the deposit closes a target
+         // once (the shares of earlier source cells ride in the running sum), a zero written to
+         // the open target by a cell without a boundary is overwritten when it closes, and the
+         // columns of a unit reach distinct target cells (dk <= 3 dv, checked in the column
+         // build).  Ring cells that no deposit reaches keep the zero of their opening.  A target
+         // outside the column's range goes to the sink row (no branch).
          auto emit = [&](int gt, float m) {
             const bool in = (uint32_t)(gt - lo) <= (uint32_t)(hi - lo);
-            float& slot = ring[(in ? gt : lo) & (LEAN_RING - 1)][tid];
-            slot = slot + (in ? m : 0.0f);
+            ring[in ? (gt & (LEAN_RING - 1)) : LEAN_RING][tid] = m;
          };
          Flux<NC> fx;
          fx.begin((a.vmin_d + (double)(WID * ks) * a.dv_d - imin) * rdk, r);

@@ -243,7 +243,8 @@ struct Flux {
    // One source cell with integral coefficients c and average v; emit(target, mass) receives
    // each target this cell closes.
    // The first boundary is handled without a branch: a cell that holds none emits 0 to its
-   // open target, which leaves every sum as it is.
+   // open target.  Every other emit closes a target with its whole mass from this pencil, so
+   // a receiver may store emits (the closing emit supersedes such a 0) or add them.
    template <class Emit>
    VACC_HD void cell(const float* c, float v, Emit&& emit) {
       const bool hit = t < 1.0f;
