@@ -66,6 +66,56 @@ def test_streamed_device_reconstruction_matches_oracle(lib, method):
         assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
 
 
+def _pqm_stress_columns(rng, count):
+    """Columns where the PQM monotonicity test is close to its boundary: smooth profiles with
+    relative noise from 1e-7 to 1e-1, plateaus with tiny steps, near-linear ramps, values near
+    the FP32 subnormal range, and single spikes."""
+    out = []
+    for _ in range(count):
+        n = int(rng.integers(8, 64))
+        x = np.linspace(-1, 1, n)
+        kind = int(rng.integers(0, 6))
+        if kind == 0:
+            base = np.exp(-((x - rng.uniform(-0.5, 0.5)) / rng.uniform(0.2, 1.0)) ** 2)
+            v = base * (1 + 10 ** rng.uniform(-7, -1) * rng.standard_normal(n))
+        elif kind == 1:
+            v = 1.0 + 10 ** rng.uniform(-7, -3) * np.round(rng.standard_normal(n))
+        elif kind == 2:
+            v = 1.0 + rng.uniform(-1, 1) * x + 10 ** rng.uniform(-8, -4) * rng.standard_normal(n)
+        elif kind == 3:
+            v = np.exp(-((x / rng.uniform(0.1, 0.5)) ** 2)) * 10 ** rng.uniform(-36, -28)
+        elif kind == 4:
+            v = np.full(n, 1e-3)
+            v[rng.integers(0, n)] = 1.0
+        else:
+            v = np.abs(np.sin(rng.uniform(1, 20) * x)) + 10 ** rng.uniform(-6, -2)
+        col = np.zeros(n + 8, np.float32)
+        col[4:n + 4] = np.abs(v) * 10 ** rng.uniform(-14, -9) if kind != 3 else np.abs(v)
+        out.append(col)
+    return out
+
+
+def test_pqm_near_the_monotonicity_boundary_keeps_bit_parity(lib):
+    """On columns built to sit near the PQM monotonicity test's boundary (where a rounding
+    difference would flip the repair), every device coefficient equals the oracle's bit for
+    bit, and the reference's own headers' where oracle/_ref is built."""
+    rng = np.random.default_rng(2024)
+    cols = _pqm_stress_columns(rng, 3000)
+    cells = 0
+    for col in cols:
+        n = col.size - 8
+        got = np.zeros(n * 5, np.float32)
+        lib.recon_stream(O.PQM, col.ctypes.data, n, np.float32(1e-15), got.ctypes.data)
+        st = np.stack([col[k:k + 9] for k in range(n)])
+        want = O.coeff_batch(O.PQM, st, 1e-15).reshape(-1)
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+        if O.ref_lib() is not None:
+            ref = O.ref_coeff_batch(O.PQM, st, 1e-15).reshape(-1)
+            assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+        cells += n
+    assert cells > 100000
+
+
 def test_div3_equals_ieee_division_for_every_float(lib):
     """vacc::div3 (FMA-corrected product) against x / 3 over all 2^32 bit patterns."""
     import concurrent.futures as cf
