@@ -193,12 +193,14 @@ struct ReconLean<VACC_PQM> {
          const float drx = lft ? fmaf(4.0f, ul, 6.0f * ur) : (2.0f / 3.0f) * fmaf(4.0f, ur, ul);
          if (ssg * dlx < 0.0f) {          // left derivative flattened
             dl = 0.0f;
-            ur = (lft ? -4.0f : -1.5f) * ul;
-            dr = (lft ? -20.0f : -10.0f / 3.0f) * ul;
+            // products first, then the choice: a choice between two constants would hold
+            // both in registers for the whole kernel
+            ur = lft ? -4.0f * ul : -1.5f * ul;
+            dr = lft ? -20.0f * ul : (-10.0f / 3.0f) * ul;
          } else if (ssg * drx < 0.0f) {   // right derivative flattened
             dr = 0.0f;
-            ul = (lft ? -1.5f : -4.0f) * ur;
-            dl = (lft ? 10.0f / 3.0f : 20.0f) * ur;
+            ul = lft ? -1.5f * ur : -4.0f * ur;
+            dl = lft ? (10.0f / 3.0f) * ur : 20.0f * ur;
          } else {
             dl = dlx;
             dr = drx;
