@@ -58,7 +58,8 @@ enum { VACC_STORE_F32 = 0, VACC_STORE_BF16 = 1, VACC_STORE_F16 = 2 };
 
 /* per-cell status bits (OR-ed over cells in return values) */
 #define VACC_STATUS_WALL 1u        /* target blocks within wall margin (cpu_acc_map.cpp:325-338) */
-#define VACC_STATUS_DK 2u          /* non-positive or too small intersection_dk: cell not remapped */
+#define VACC_STATUS_DK 2u          /* non-positive intersection_dk, or a deposit span (dv + 3|di| + 3|dj|)/dk
+                                      above 118 cells: cell passed through, not remapped */
 #define VACC_STATUS_CAPACITY 4u    /* block pool capacity exceeded: step not performed */
 
 /* errors */

@@ -19,10 +19,10 @@ pytestmark = pytest.mark.gpu
 HALL_MIN = 1.60217653e-19
 
 
-def _ctx(w: W.Workload, method=None, wall_margin=0):
+def _ctx(w: W.Workload, method=None, wall_margin=0, fast_arith=False):
     mesh = V.make_mesh(w.grid_length, w.vmin, w.vmax)
     sp = V.make_species(w.mass, w.charge, w.min_value, w.add_width)
-    op = V.make_options(w.method if method is None else method, wall_margin=wall_margin)
+    op = V.make_options(w.method if method is None else method, wall_margin=wall_margin, fast_arith=fast_arith)
     return V.Context(mesh, sp, op)
 
 
@@ -165,6 +165,48 @@ def test_edge_cells_empty_single_and_wall():
     assert [(int(s) & V.STATUS_WALL) != 0 for s in status] == [bool(x) for x in wflags]
     assert wflags[2] == 1 and st & V.STATUS_WALL
     ctx.close()
+
+
+def test_wide_deposit_spans_are_remapped():
+    """Cells whose per-cell deposit span (dv + 3|di| + 3|dj|)/dk exceeds the 32-cell ring's
+    bound (22 cells) go to the exact kernel's 128-cell-ring instance and match the oracle bit for
+    bit, in both arithmetic forms; a span above 118 cells is still passed through and flagged.
+    The cross shears di, dj of the x pass are scaled up, with the Lagrangian base moved so that
+    the pencil through each cell's block centroid keeps its mapping."""
+    w = W.config(0, ncells=4)
+    cells = cases.build_cells(w)
+    m = cases.oracle_mesh(w)
+    nx, ny = w.grid_length[0], w.grid_length[1]
+    dv = (w.vmax[0] - w.vmin[0]) / (4 * nx)
+    isx = cases.oracle_intersections(w, 0)
+    spans = []
+    for i, factor in enumerate([6.0, 14.0, 1.0, 30.0]):
+        g = cells[i][0]
+        # x pass frame: di multiplies the z index of a pencil, dj its y index
+        ic, jc = 4 * (g // (nx * ny)).mean() + 2, 4 * (g // nx % ny).mean() + 2   # z, y centroid cells
+        I, dk, dj, di = isx[i, 0:4]
+        di2, dj2 = di * factor + 2.0 * dv * factor, dj * factor
+        isx[i, 0] = I + ic * di + jc * dj - (ic * di2 + jc * dj2)
+        isx[i, 2], isx[i, 3] = dj2, di2
+        spans.append((dv + 3 * abs(di2) + 3 * abs(dj2)) / dk)
+    assert 22 < spans[0] < 118 and 22 < spans[1] < 118 and spans[2] < 22 and spans[3] > 118, spans
+    want = []
+    for i, (g, f) in enumerate(cells):
+        g2, f2, _ = O.map_1d(m, g, f, isx[i, 0:4], 0, O.PPM, w.min_value)
+        want.append((g2, f2))
+    assert sum(g.size for g, _ in want[:2]) > 100
+    for fast in (False, True):
+        _say(f"wide spans {[round(x, 1) for x in spans]}, fast_arith={fast}")
+        ctx = _ctx(w, O.PPM, fast_arith=fast)
+        ctx.upload_cells(cells)
+        ctx.set_intersections(isx)
+        ctx.map_1d(0)
+        st = ctx.status(4)
+        got = ctx.download_cells()
+        assert [bool(x & V.STATUS_DK) for x in st] == [False, False, False, True]
+        cases.assert_cells_equal(got[:2], want[:2], f"wide spans fast={fast}")
+        cases.assert_cells_equal([got[3]], [cells[3]], "pass-through beyond the wide ring")
+        ctx.close()
 
 
 def test_nonpositive_dk_passes_cell_through():

@@ -20,6 +20,11 @@ constexpr int WID3 = 64;
 constexpr uint32_t INVALID = 0xFFFFFFFFu;
 constexpr int RING = 32;            // per-lane target ring (cells) of the remap kernel
 constexpr int MAX_SPAN_CELLS = 22;  // (dv + 3|di| + 3|dj|)/dk bound that keeps the ring in range
+// cells with a wider span go to the wide instance of the exact kernel: a 128-cell ring in
+// workgroups of 64 lanes (32 KiB of LDS, as the 256-lane instance)
+constexpr int RING_WIDE = 128;
+constexpr int WG_WIDE = 64;
+constexpr int MAX_SPAN_WIDE = RING_WIDE - 10;
 
 // floor(g / d) for every 32-bit g without an integer division: (g + umulhi(g, m)) >> l, with
 // l = ceil(log2 d) and m = ceil(2^(32+l) / d) - 2^32.  Exact: with M = m + 2^32 the product

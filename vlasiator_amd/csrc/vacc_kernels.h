@@ -17,7 +17,7 @@ struct Pool {
 // share a block), 16 bytes.
 //   x: b0 | b1 << 16   transverse block indices in the pass frame
 //   y: first column (cell-local)
-//   z: number of columns | COPY_FLAG | EXACT_FLAG
+//   z: number of columns | COPY_FLAG | EXACT_FLAG | WIDE_FLAG
 //   w: source blocks of the unit (its work)
 // Column record, 16 bytes.
 //   x: first block index along the pass axis | length << 16
@@ -30,11 +30,19 @@ constexpr uint32_t COPY_FLAG = 0x80000000u;
 // reference's maximum-lane rule, cpu_acc_map.cpp:515-529, then bounds the deposits by the
 // group's first lane), or when its cell's deposits over a source block could outrun the ring.
 constexpr uint32_t EXACT_FLAG = 0x40000000u;
-constexpr uint32_t UNIT_NCOL_MASK = 0x3FFFFFFFu;
+// Units of cells whose deposit span exceeds MAX_SPAN_CELLS (up to MAX_SPAN_WIDE) go to the
+// wide instance of the exact kernel, in either arithmetic form.
+constexpr uint32_t WIDE_FLAG = 0x20000000u;
+constexpr uint32_t UNIT_NCOL_MASK = 0x1FFFFFFFu;
 constexpr int LEN_BINS = 256;   // length bins of the work-unit ordering
-// The last bin holds the units marked EXACT_FLAG, so that the work list ends with them: the
-// tolerance-form kernel takes the list up to that bin's start, the exact kernel the rest.
-constexpr int EXACT_BIN = LEN_BINS - 1;
+// The last two bins hold the units marked EXACT_FLAG and then those marked WIDE_FLAG, so that
+// the work list ends with them: the tolerance-form kernel takes the list up to EXACT_BIN's
+// start, the exact kernel the EXACT_BIN units (or, in the exact form, everything before
+// WIDE_BIN), and the wide instance the WIDE_BIN units.
+constexpr int EXACT_BIN = LEN_BINS - 2;
+constexpr int WIDE_BIN = LEN_BINS - 1;
+// RemapArgs::range of an exact-kernel launch
+enum { RANGE_ALL = 0, RANGE_EXACT = 1, RANGE_WIDE = 2 };
 
 struct ColumnArgs {
    const uint32_t* cell_off;   // [ncells+1] input list offsets
@@ -104,8 +112,10 @@ struct RemapArgs {
    double min_value;            // content test threshold
    const double* minval;        // [ncells] sparsity threshold per cell (velocityBlockMinValue)
    double dv_d, vmin_d;         // cell size and mesh minimum along the pass axis (FP64)
-   int only_flagged;            // exact kernel: 1 = take only the units marked EXACT_FLAG
-   const uint32_t* len_hist;    // [LEN_BINS] units per length bin; [EXACT_BIN] = EXACT_FLAG units
+   int range;                   // exact kernel: RANGE_ALL (every unit before WIDE_BIN's),
+                                // RANGE_EXACT (EXACT_BIN's units), RANGE_WIDE (WIDE_BIN's units)
+   const uint32_t* len_hist;    // [LEN_BINS] units per length bin; [EXACT_BIN] = EXACT_FLAG units,
+                                // [WIDE_BIN] = WIDE_FLAG units
 };
 
 struct AdjustArgs {

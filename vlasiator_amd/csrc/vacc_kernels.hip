@@ -82,20 +82,28 @@ __device__ inline void pass_isect(const double* is, float& I, float& di, float& 
 }
 
 // work units are ordered longest first: bin 0 holds units of >= EXACT_BIN-1 source blocks;
-// units marked EXACT_FLAG go to EXACT_BIN, the end of the list
+// units marked EXACT_FLAG go to EXACT_BIN and units marked WIDE_FLAG to WIDE_BIN, the end of
+// the list
 __device__ inline uint32_t len_bin(uint32_t nblocks) {
    return (uint32_t)(EXACT_BIN - 1) - min(nblocks, (uint32_t)(EXACT_BIN - 1));
 }
 __device__ inline uint32_t unit_bin(const uint4& u) {
-   return (u.z & EXACT_FLAG) ? (uint32_t)EXACT_BIN : len_bin(u.w);
+   return (u.z & WIDE_FLAG) ? (uint32_t)WIDE_BIN : (u.z & EXACT_FLAG) ? (uint32_t)EXACT_BIN : len_bin(u.w);
 }
 
+// Per-cell deposit span (dv + 3|di| + 3|dj|)/dk in target cells.  Cells whose Lagrangian
+// spacing is not positive, or whose span would not fit even the wide ring, are passed through
+// unchanged and flagged VACC_STATUS_DK; cells whose span exceeds the 32-cell ring's bound are
+// remapped by the wide instance.
+__device__ inline float pass_span(float dv, float di, float dj, float dk) {
+   return (dv + 3.0f * fabsf(di) + 3.0f * fabsf(dj)) / dk;
+}
 __device__ inline bool pass_is_copy(float dv, float di, float dj, float dk) {
-   // cells whose Lagrangian spacing is not positive, or whose per-cell deposit span would not
-   // fit the target ring, are passed through unchanged and flagged VACC_STATUS_DK
    if (!(dk > 0.0f)) return true;
-   const float span = (dv + 3.0f * fabsf(di) + 3.0f * fabsf(dj)) / dk;
-   return !(span <= (float)MAX_SPAN_CELLS);
+   return !(pass_span(dv, di, dj, dk) <= (float)MAX_SPAN_WIDE);
+}
+__device__ inline bool pass_is_wide(float dv, float di, float dj, float dk) {
+   return !(pass_span(dv, di, dj, dk) <= (float)MAX_SPAN_CELLS);
 }
 
 __device__ inline uint32_t wave_incl_scan(uint32_t v) {
@@ -289,6 +297,8 @@ __global__ __launch_bounds__(CWG) void k_columns(ColumnArgs a) {
    // a cell outside this subcycle passes through unchanged (no status)
    const bool idle = a.active && !a.active[c];
    const bool copy = idle || pass_is_copy(a.dv, di, dj, dk);
+   const bool wide = !copy && pass_is_wide(a.dv, di, dj, dk);
+   const uint32_t wf = wide ? WIDE_FLAG : 0u;
    const bool lean_wide = a.fast && !copy && !((4.0f * a.dv + 3.0f * fabsf(di) + 3.0f * fabsf(dj)) / dk <= LEAN_SPAN);
    // the tolerance form writes each target cell once; consecutive columns of a column set
    // (sources at least one block apart) then reach distinct target cells when dk <= 4 dv, here
@@ -421,14 +431,14 @@ __global__ __launch_bounds__(CWG) void k_columns(ColumnArgs a) {
          if (overlap) {
             bool ex = lean_wide || lean_shared;
             for (uint32_t j = 0; j < ncol && !ex; ++j) ex = exact_col(j);
-            a.unit_rec[off + ui] = make_uint4(bb, cb, ncol | cf | (ex ? EXACT_FLAG : 0u), bpos);
-            atomicAdd(&hist[ex ? (uint32_t)EXACT_BIN : len_bin(bpos)], 1u);
+            a.unit_rec[off + ui] = make_uint4(bb, cb, ncol | cf | wf | (ex ? EXACT_FLAG : 0u), bpos);
+            atomicAdd(&hist[wide ? (uint32_t)WIDE_BIN : ex ? (uint32_t)EXACT_BIN : len_bin(bpos)], 1u);
          } else {
             for (uint32_t j = 0; j < ncol; ++j) {
                const uint32_t len = a.col_rec[off + cb + j].x >> 16;
                const bool ex = lean_wide || exact_col(j);
-               a.unit_rec[off + ui + j] = make_uint4(bb, cb + j, 1u | cf | (ex ? EXACT_FLAG : 0u), len);
-               atomicAdd(&hist[ex ? (uint32_t)EXACT_BIN : len_bin(len)], 1u);
+               a.unit_rec[off + ui + j] = make_uint4(bb, cb + j, 1u | cf | wf | (ex ? EXACT_FLAG : 0u), len);
+               atomicAdd(&hist[wide ? (uint32_t)WIDE_BIN : ex ? (uint32_t)EXACT_BIN : len_bin(len)], 1u);
             }
          }
       }
@@ -701,20 +711,25 @@ __device__ inline void copy_block(const RemapArgs& a, int l16, int setlane0, int
    }
 }
 
-template <int DIM, int METHOD, int ST>
-__global__ __launch_bounds__(WG) void k_remap(RemapArgs a, double i_dv) {
-   __shared__ float ring[RING][WG];
+// RN-cell ring per lane in workgroups of WGS lanes: RING / WG for the work list, RING_WIDE /
+// WG_WIDE for the units of wide-span cells (range RANGE_WIDE)
+template <int DIM, int METHOD, int ST, int RN, int WGS>
+__global__ __launch_bounds__(WGS) void k_remap(RemapArgs a, double i_dv) {
+   __shared__ float ring[RN][WGS];
    if (*a.flags & VACC_STATUS_CAPACITY) return;
    const int tid = threadIdx.x;
    const int l16 = tid & 15;
    const int ip = l16 & 3, jp = l16 >> 2;
    const int setlane0 = (tid & 63) & ~15;
    const int o = (DIM == 0) ? (ip * WID2 + jp * WID) : (DIM == 1) ? (jp * WID2 + ip) : (jp * WID + ip);
-   const uint32_t total = a.unit_base[a.ncells];
-   // after the tolerance-form kernel: only the units at the end of the list (EXACT_BIN)
-   const uint32_t first = a.only_flagged ? total - a.len_hist[EXACT_BIN] : 0u;
-   const uint32_t gstride = gridDim.x * (WG / 16);
-   for (uint32_t gi = first + blockIdx.x * (WG / 16) + (tid >> 4); gi < total; gi += gstride) {
+   // the list ends with the EXACT_BIN units and then the WIDE_BIN units
+   const uint32_t all = a.unit_base[a.ncells], nwide = a.len_hist[WIDE_BIN];
+   const uint32_t first = (a.range == RANGE_WIDE)    ? all - nwide
+                          : (a.range == RANGE_EXACT) ? all - nwide - a.len_hist[EXACT_BIN]
+                                                     : 0u;
+   const uint32_t total = (a.range == RANGE_WIDE) ? all : all - nwide;
+   const uint32_t gstride = gridDim.x * (WGS / 16);
+   for (uint32_t gi = first + blockIdx.x * (WGS / 16) + (tid >> 4); gi < total; gi += gstride) {
       const uint32_t g = a.perm[gi];
       const uint32_t c = a.perm_cell[gi];
       const uint32_t cbase = a.cell_off[c];
@@ -806,18 +821,18 @@ __global__ __launch_bounds__(WG) void k_remap(RemapArgs a, double i_dv) {
                const int flushLim = min(mnG, row_ror_i<8>(mnG));
                const int openLim = max(max(mxG, row_ror_i<8>(mxG)), flushLim);
                while (next_open <= lk && WID * next_open <= openLim) {
-                  if (next_open - next_flush >= RING / WID) {   // ring full: cannot happen for checked spans
+                  if (next_open - next_flush >= RN / WID) {   // ring full: cannot happen for checked spans
                      atomicOr(&a.status[c], VACC_STATUS_DK);
                      atomicOr(&a.status[a.ncells], VACC_STATUS_DK);
-                     flush_block<DIM, ST, RING, WG>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), thc);
+                     flush_block<DIM, ST, RN, WGS>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), thc);
                      ++next_flush;
                   }
 #pragma unroll
-                  for (int e = 0; e < 4; ++e) ring[(WID * next_open + e) & (RING - 1)][tid] = 0.0f;
+                  for (int e = 0; e < 4; ++e) ring[(WID * next_open + e) & (RN - 1)][tid] = 0.0f;
                   ++next_open;
                }
                while (next_flush < next_open && next_flush < nfk && WID * next_flush + WID - 1 < flushLim) {
-                  flush_block<DIM, ST, RING, WG>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), thc);
+                  flush_block<DIM, ST, RN, WGS>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), thc);
                   ++next_flush;
                }
                float tdr = 0.0f;
@@ -830,7 +845,7 @@ __global__ __launch_bounds__(WG) void k_remap(RemapArgs a, double i_dv) {
                   const float tdl = tdr;
                   const float vn = (float)((double)(mn - v_l) * i_dv);
                   tdr = integral<METHOD>(vn, cf);
-                  float& r = ring[gk & (RING - 1)][tid];
+                  float& r = ring[gk & (RN - 1)][tid];
                   r = r + (tdr - tdl);
                }
             }
@@ -842,7 +857,7 @@ __global__ __launch_bounds__(WG) void k_remap(RemapArgs a, double i_dv) {
             }
          }
          while (next_flush < next_open && next_flush < nfk) {
-            flush_block<DIM, ST, RING, WG>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), thc);
+            flush_block<DIM, ST, RN, WGS>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), thc);
             ++next_flush;
          }
          for (int tb = next_open; tb <= lk && tb < nfk; ++tb) zero_block<DIM, ST>(a, l16, setlane0, o, slotb + (tb - fk), thc);
@@ -860,7 +875,8 @@ __global__ __launch_bounds__(WG) void k_remap(RemapArgs a, double i_dv) {
 // average and the flux-form deposit (one integral per Lagrangian boundary, the cell average
 // for the rest of the cell), from the FP64 intersections.  Ring opening and block flushing
 // run once per source block on unit-wide bounds (DPP row reductions: a unit is one 16-lane
-// row).  Units marked EXACT_FLAG are left to k_remap (only_flagged = 1).
+// row).  Units marked EXACT_FLAG are left to k_remap (range RANGE_EXACT), and those marked
+// WIDE_FLAG to its wide instance.
 
 template <int DIM>
 __device__ inline void pass_isect_d(const double* is, double& I, double& di, double& dj, double& dk) {
@@ -905,8 +921,8 @@ __global__ __launch_bounds__(WGS, (ST == ST_F32) ? 5 : 4) void k_remap_lean(Rema
    const int ip = l16 & 3, jp = l16 >> 2;
    const int setlane0 = (tid & 63) & ~15;
    const int o = (DIM == 0) ? (ip * WID2 + jp * WID) : (DIM == 1) ? (jp * WID2 + ip) : (jp * WID + ip);
-   // the units before EXACT_BIN's; k_remap takes the rest
-   const uint32_t total = a.unit_base[a.ncells] - a.len_hist[EXACT_BIN];
+   // the units before EXACT_BIN's; k_remap takes the EXACT_BIN and WIDE_BIN units
+   const uint32_t total = a.unit_base[a.ncells] - a.len_hist[EXACT_BIN] - a.len_hist[WIDE_BIN];
    const uint32_t gstride = gridDim.x * (WGS / 16);
    for (uint32_t gi = blockIdx.x * (WGS / 16) + (tid >> 4); gi < total; gi += gstride) {
       const uint32_t g = a.perm[gi];
@@ -1686,17 +1702,21 @@ int device_cus() {
 template <int DIM, int METHOD, int ST>
 static void remap_dms(RemapArgs a, uint32_t grid, double i_dv, bool fast, int wgs, hipStream_t s) {
    if (!fast) {
-      a.only_flagged = 0;
-      hipLaunchKernelGGL((k_remap<DIM, METHOD, ST>), dim3(grid), dim3(WG), 0, s, a, i_dv);
-      return;
+      a.range = RANGE_ALL;
+      hipLaunchKernelGGL((k_remap<DIM, METHOD, ST, RING, WG>), dim3(grid), dim3(WG), 0, s, a, i_dv);
+   } else {
+      // tolerance form: the lean kernel (workgroups of wgs lanes, wgs/16 work units each), then
+      // the exact kernel over the few units it leaves
+      if (wgs == 64) hipLaunchKernelGGL((k_remap_lean<DIM, METHOD, ST, 64>), dim3(grid), dim3(64), 0, s, a);
+      else if (wgs == 128) hipLaunchKernelGGL((k_remap_lean<DIM, METHOD, ST, 128>), dim3(grid), dim3(128), 0, s, a);
+      else hipLaunchKernelGGL((k_remap_lean<DIM, METHOD, ST, WG>), dim3(grid), dim3(WG), 0, s, a);
+      a.range = RANGE_EXACT;
+      hipLaunchKernelGGL((k_remap<DIM, METHOD, ST, RING, WG>), dim3((uint32_t)device_cus()), dim3(WG), 0, s, a, i_dv);
    }
-   // tolerance form: the lean kernel (workgroups of wgs lanes, wgs/16 work units each), then the
-   // exact kernel over the few units it leaves
-   if (wgs == 64) hipLaunchKernelGGL((k_remap_lean<DIM, METHOD, ST, 64>), dim3(grid), dim3(64), 0, s, a);
-   else if (wgs == 128) hipLaunchKernelGGL((k_remap_lean<DIM, METHOD, ST, 128>), dim3(grid), dim3(128), 0, s, a);
-   else hipLaunchKernelGGL((k_remap_lean<DIM, METHOD, ST, WG>), dim3(grid), dim3(WG), 0, s, a);
-   a.only_flagged = 1;
-   hipLaunchKernelGGL((k_remap<DIM, METHOD, ST>), dim3((uint32_t)device_cus()), dim3(WG), 0, s, a, i_dv);
+   // the units of wide-span cells (none in most passes: the launch then exits at once)
+   a.range = RANGE_WIDE;
+   hipLaunchKernelGGL((k_remap<DIM, METHOD, ST, RING_WIDE, WG_WIDE>), dim3((uint32_t)device_cus()), dim3(WG_WIDE), 0, s,
+                      a, i_dv);
 }
 
 template <int DIM, int METHOD>
