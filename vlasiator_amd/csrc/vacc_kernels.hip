@@ -244,6 +244,25 @@ constexpr float LEAN_SPAN = (float)(LEAN_RING - 10);
 // k_columns
 // ---------------------------------------------------------------------------
 
+// f(gid[i], i) for i = t, t + STRIDE, ... < n (a workgroup of STRIDE threads walking one cell's
+// blocks), with four GID loads issued before the first use
+template <int STRIDE, class F>
+__device__ inline void for_blocks4(const uint32_t* gid, uint32_t n, int t, F&& f) {
+   for (uint32_t i0 = (uint32_t)t; i0 < n; i0 += 4u * STRIDE) {
+      uint32_t g[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+         const uint32_t i = i0 + (uint32_t)u * STRIDE;
+         g[u] = (i < n) ? gid[i] : 0u;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+         const uint32_t i = i0 + (uint32_t)u * STRIDE;
+         if (i < n) f(g[u], i);
+      }
+   }
+}
+
 template <int DIM>
 __global__ __launch_bounds__(CWG) void k_columns(ColumnArgs a) {
    __shared__ uint32_t bm[BM_WORDS];
@@ -274,13 +293,13 @@ __global__ __launch_bounds__(CWG) void k_columns(ColumnArgs a) {
    __syncthreads();
    {
       int lo0 = INT_MAX, lo1 = INT_MAX, lok = INT_MAX, hi0 = INT_MIN, hi1 = INT_MIN, hik = INT_MIN;
-      for (uint32_t i = t; i < n; i += CWG) {
+      for_blocks4<CWG>(a.gid + off, n, t, [&](uint32_t gid, uint32_t) {
          int b0, b1, k;
-         pass_coords<DIM>(a.gid[off + i], a.gs, b0, b1, k);
+         pass_coords<DIM>(gid, a.gs, b0, b1, k);
          lo0 = min(lo0, b0); hi0 = max(hi0, b0);
          lo1 = min(lo1, b1); hi1 = max(hi1, b1);
          lok = min(lok, k); hik = max(hik, k);
-      }
+      });
       atomicMin(&sbox[0], lo0); atomicMin(&sbox[1], lo1); atomicMin(&sbox[2], lok);
       atomicMax(&sbox[3], hi0); atomicMax(&sbox[4], hi1); atomicMax(&sbox[5], hik);
    }
@@ -311,15 +330,15 @@ __global__ __launch_bounds__(CWG) void k_columns(ColumnArgs a) {
       const int nr = min(rpc, rows - r0);
       for (int i = t; i < nr * W; i += CWG) bm[i] = 0;
       __syncthreads();
-      for (uint32_t i = t; i < n; i += CWG) {
+      for_blocks4<CWG>(a.gid + off, n, t, [&](uint32_t gid, uint32_t) {
          int b0, b1, k;
-         pass_coords<DIM>(a.gid[off + i], a.gs, b0, b1, k);
+         pass_coords<DIM>(gid, a.gs, b0, b1, k);
          const int row = (b0 - mn0) * R1 + (b1 - mn1) - r0;
          if (row >= 0 && row < nr) {
             const int kk = k - mnk;
             atomicOr(&bm[row * W + (kk >> 5)], 1u << (kk & 31));
          }
-      }
+      });
       __syncthreads();
       // per-row counts: blocks, columns, work units, target blocks.  The columns of a set are
       // separate work units unless the target ranges of two of them share a block, in which
@@ -443,15 +462,15 @@ __global__ __launch_bounds__(CWG) void k_columns(ColumnArgs a) {
          }
       }
       // source index of every block in sorted (set, k) order
-      for (uint32_t i = t; i < n; i += CWG) {
+      for_blocks4<CWG>(a.gid + off, n, t, [&](uint32_t gid, uint32_t i) {
          int b0, b1, k;
-         pass_coords<DIM>(a.gid[off + i], a.gs, b0, b1, k);
+         pass_coords<DIM>(gid, a.gs, b0, b1, k);
          const int row = (b0 - mn0) * R1 + (b1 - mn1) - r0;
          if (row >= 0 && row < nr) {
             const uint32_t pos = rowv[0][row] + rank_in_row(bm + row * W, k - mnk);
             a.sorted_src[off + pos] = a.src[off + i];
          }
-      }
+      });
       __syncthreads();
       if (t < 4) carry[t] += tot[t];
       __syncthreads();
@@ -1166,7 +1185,7 @@ __global__ __launch_bounds__(AWG) void k_adjust(AdjustArgs a) {
          ly = min(ly, y); hy = max(hy, y);
          lz = min(lz, z); hz = max(hz, z);
       };
-      for (uint32_t i = t; i < n; i += AWG) grow(a.gid[off + i]);
+      for_blocks4<AWG>(a.gid + off, n, t, [&](uint32_t g, uint32_t) { grow(g); });
       for_spatial(grow);
       atomicMin(&sbox[0], lx); atomicMin(&sbox[1], ly); atomicMin(&sbox[2], lz);
       atomicMax(&sbox[3], hx); atomicMax(&sbox[4], hy); atomicMax(&sbox[5], hz);
@@ -1193,15 +1212,14 @@ __global__ __launch_bounds__(AWG) void k_adjust(AdjustArgs a) {
       for (int i = t; i < nr * W; i += AWG) bm[i] = 0;
       __syncthreads();
       if (separable) {
-         for (uint32_t i = t; i < n; i += AWG) {
-            if (!a.content[off + i]) continue;
-            const uint32_t g = a.gid[off + i];
+         for_blocks4<AWG>(a.gid + off, n, t, [&](uint32_t g, uint32_t i) {
+            if (!a.content[off + i]) return;
             uint32_t ux, uy, uz;
             gid_xyz(g, a.gs, ux, uy, uz);
             const int x = (int)ux, y = (int)uy, z = (int)uz;
             const int kx = x - mnx;
             atomicOr(&bm[((z - mnz) * Ry + (y - mny)) * W + (kx >> 5)], 1u << (kx & 31));
-         }
+         });
          __syncthreads();
          for (int k = 0; k < w; ++k) dilate_box1(bm, W, Rx, Ry, Rz);
          for_spatial([&](uint32_t g) {
@@ -1284,13 +1302,12 @@ __global__ __launch_bounds__(AWG) void k_adjust(AdjustArgs a) {
             }
          }
          __syncthreads();
-         for (uint32_t i = t; i < n; i += AWG) {
-            const uint32_t g = a.gid[off + i];
+         for_blocks4<AWG>(a.gid + off, n, t, [&](uint32_t g, uint32_t i) {
             uint32_t ux, uy, uz;
             gid_xyz(g, a.gs, ux, uy, uz);
             const int x = (int)ux, y = (int)uy, z = (int)uz;
             const int row = (z - mnz) * Ry + (y - mny) - r0;
-            if (row < 0 || row >= nr) continue;
+            if (row < 0 || row >= nr) return;
             const int kx = x - mnx;
             const int q = row * W + (kx >> 5);
             const uint32_t bit = 1u << (kx & 31);
@@ -1298,7 +1315,7 @@ __global__ __launch_bounds__(AWG) void k_adjust(AdjustArgs a) {
                const uint32_t pos = wpre[q] + __popc(bm[q] & (bit - 1u));
                a.src_out[obase + pos] = a.src[off + i];
             }
-         }
+         });
       } else {
          // removed blocks: no content and not kept; their mass goes to RHOLOSSADJUST
          // (spatial_cell_cpu.cpp:227-267).  Each wave takes 64 consecutive list entries, one
