@@ -389,8 +389,12 @@ def measure(pops, storage: str, steps: int, warmup: int, graph: bool = False, fa
         roofline["valu_busy_source"] = (f"{src}: rocprofv3 (SQ_ACTIVE_INST_VALU - SQ_ACTIVE_INST_VALU2) / "
                                         f"(CUs x GRBM_GUI_ACTIVE), quad-cycles with a VALU issue")
     pct = {f"p{q}": round(float(np.percentile(step_ms, q)), 3) for q in (50, 90, 99)}
+    # the same updates over the remap passes' time alone (column build, adjust, moments and
+    # transform excluded)
+    remap_only = {"value": updates / remap_s if remap_s > 0 else 0.0, "unit": "cell-updates/s",
+                  "ms_per_step": round(prof["remap_ms"] / steps, 4)}
     return {"value": value, "ms_per_step": ms_per_step, "blocks_at_start": n_init, "roofline": roofline,
-            "breakdown": breakdown, "step_ms_percentiles": pct, "remap_wg_per_cu": tuned}
+            "breakdown": breakdown, "step_ms_percentiles": pct, "remap_wg_per_cu": tuned, "remap_only": remap_only}
 
 
 def main():
@@ -463,7 +467,7 @@ def main():
                       "remap_wg_per_cu": [t[0] for t in r["remap_wg_per_cu"]],
                       "remap_lanes_per_wg": [t[1] for t in r["remap_wg_per_cu"]]},
            "roofline": r["roofline"], "breakdown_ms_per_step": r["breakdown"],
-           "step_ms_percentiles": r["step_ms_percentiles"]}
+           "step_ms_percentiles": r["step_ms_percentiles"], "remap_only": r["remap_only"]}
     # the achievable copy bandwidth of this GPU, the second roofline denominator
     copy_gbps = hbm_probe.measure(2.0, 5)["copy_GBps"]
     out["roofline"]["copy_GBps"] = round(copy_gbps, 1)

@@ -16,12 +16,12 @@ struct Pool {
 // Work unit of the remap (one column, or a whole column set whose columns' target ranges
 // share a block), 16 bytes.
 //   x: b0 | b1 << 16   transverse block indices in the pass frame
-//   y: first column (cell-local)
+//   y: record index of the first column (col_rec)
 //   z: number of columns | COPY_FLAG | EXACT_FLAG | WIDE_FLAG
 //   w: source blocks of the unit (its work)
 // Column record, 16 bytes.
 //   x: first block index along the pass axis | length << 16
-//   y: position of the first block in the sorted source list (cell-local)
+//   y: position of the first block in the sorted source list (sorted_src)
 //   z: first target block | last target block << 16
 //   w: output slot of the first target block (cell-local)
 constexpr uint32_t COPY_FLAG = 0x80000000u;
@@ -83,7 +83,7 @@ struct OrderArgs {
    const uint32_t* unit_base;   // [ncells+1]
    const uint4* unit_rec;
    uint32_t* hist_cursor;       // [LEN_BINS] insertion cursors
-   uint32_t* perm;              // [total units] global unit indices, longest first
+   uint32_t* perm;              // [total units] unit record indices (unit_rec), longest first
    uint32_t* perm_cell;         // [total units] the cell of each perm entry
    const uint32_t* flags;
    uint32_t ncells;
@@ -93,7 +93,7 @@ struct RemapArgs {
    const uint32_t* cell_off;    // input list offsets (record base of each cell)
    const uint32_t* unit_base;   // [ncells+1] global work-unit index base
    const uint32_t* out_off;     // [ncells+1] output block offsets
-   const uint32_t* perm;        // [total units] processing order
+   const uint32_t* perm;        // [total units] unit record indices in processing order
    const uint32_t* perm_cell;   // [total units] cell of each entry of perm
    const uint4* unit_rec;
    const uint4* col_rec;

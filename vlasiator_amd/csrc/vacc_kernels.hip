@@ -435,7 +435,7 @@ __global__ __launch_bounds__(CWG) void k_columns(ColumnArgs a) {
                end = lk;
             }
             const uint32_t len = (uint32_t)(e - s + 1);
-            a.col_rec[off + cb + ncol] = make_uint4((uint32_t)(mnk + s) | (len << 16), spos0 + bpos,
+            a.col_rec[off + cb + ncol] = make_uint4((uint32_t)(mnk + s) | (len << 16), off + spos0 + bpos,
                                                     (uint32_t)fk | ((uint32_t)lk << 16), slot0 + first);
             bpos += len;
             ++ncol;
@@ -450,13 +450,13 @@ __global__ __launch_bounds__(CWG) void k_columns(ColumnArgs a) {
          if (overlap) {
             bool ex = lean_wide || lean_shared;
             for (uint32_t j = 0; j < ncol && !ex; ++j) ex = exact_col(j);
-            a.unit_rec[off + ui] = make_uint4(bb, cb, ncol | cf | wf | (ex ? EXACT_FLAG : 0u), bpos);
+            a.unit_rec[off + ui] = make_uint4(bb, off + cb, ncol | cf | wf | (ex ? EXACT_FLAG : 0u), bpos);
             atomicAdd(&hist[wide ? (uint32_t)WIDE_BIN : ex ? (uint32_t)EXACT_BIN : len_bin(bpos)], 1u);
          } else {
             for (uint32_t j = 0; j < ncol; ++j) {
                const uint32_t len = a.col_rec[off + cb + j].x >> 16;
                const bool ex = lean_wide || exact_col(j);
-               a.unit_rec[off + ui + j] = make_uint4(bb, cb + j, 1u | cf | wf | (ex ? EXACT_FLAG : 0u), len);
+               a.unit_rec[off + ui + j] = make_uint4(bb, off + cb + j, 1u | cf | wf | (ex ? EXACT_FLAG : 0u), len);
                atomicAdd(&hist[wide ? (uint32_t)WIDE_BIN : ex ? (uint32_t)EXACT_BIN : len_bin(len)], 1u);
             }
          }
@@ -550,7 +550,7 @@ __global__ __launch_bounds__(WG) void k_order(OrderArgs a) {
    for (uint32_t u = threadIdx.x; u < nu; u += WG) {
       const uint32_t b = unit_bin(a.unit_rec[off + u]);
       const uint32_t pos = base_of[b] + atomicAdd(&cnt[b], 1u);
-      a.perm[pos] = base + u;
+      a.perm[pos] = off + u;   // the unit's record
       a.perm_cell[pos] = c;
    }
 }
@@ -749,10 +749,8 @@ __global__ __launch_bounds__(WGS) void k_remap(RemapArgs a, double i_dv) {
    const uint32_t total = (a.range == RANGE_WIDE) ? all : all - nwide;
    const uint32_t gstride = gridDim.x * (WGS / 16);
    for (uint32_t gi = first + blockIdx.x * (WGS / 16) + (tid >> 4); gi < total; gi += gstride) {
-      const uint32_t g = a.perm[gi];
       const uint32_t c = a.perm_cell[gi];
-      const uint32_t cbase = a.cell_off[c];
-      const uint4 sr = a.unit_rec[cbase + (g - a.unit_base[c])];
+      const uint4 sr = a.unit_rec[a.perm[gi]];
       const int b0 = (int)(sr.x & 0xFFFFu), b1 = (int)(sr.x >> 16);
       const uint32_t col0 = sr.y;
       // the cell's sparsity threshold: the content test, and the reconstruction threshold as
@@ -775,11 +773,11 @@ __global__ __launch_bounds__(WGS) void k_remap(RemapArgs a, double i_dv) {
       int prev_end = -1;
       int carry_open = -1;
       for (uint32_t ci = 0; ci < ncol; ++ci) {
-         const uint4 cr = a.col_rec[cbase + col0 + ci];
+         const uint4 cr = a.col_rec[col0 + ci];
          const int ks = (int)(cr.x & 0xFFFFu), len = (int)(cr.x >> 16);
          const int fk = (int)(cr.z & 0xFFFFu), lk = (int)(cr.z >> 16);
-         const int nfk = (ci + 1 < ncol) ? (int)(a.col_rec[cbase + col0 + ci + 1].z & 0xFFFFu) : INT_MAX;
-         const uint32_t* srcs = a.sorted_src + cbase + cr.y;
+         const int nfk = (ci + 1 < ncol) ? (int)(a.col_rec[col0 + ci + 1].z & 0xFFFFu) : INT_MAX;
+         const uint32_t* srcs = a.sorted_src + cr.y;
          const size_t slotb = (size_t)obase + cr.w;   // output slot of block fk
          for (int tb = max(fk, prev_end + 1) + l16; tb <= lk; tb += 16) {
             const size_t slot = slotb + (size_t)(tb - fk);
@@ -944,10 +942,8 @@ __global__ __launch_bounds__(WGS, (ST == ST_F32) ? 5 : 4) void k_remap_lean(Rema
    const uint32_t total = a.unit_base[a.ncells] - a.len_hist[EXACT_BIN] - a.len_hist[WIDE_BIN];
    const uint32_t gstride = gridDim.x * (WGS / 16);
    for (uint32_t gi = blockIdx.x * (WGS / 16) + (tid >> 4); gi < total; gi += gstride) {
-      const uint32_t g = a.perm[gi];
       const uint32_t c = a.perm_cell[gi];
-      const uint32_t cbase = a.cell_off[c];
-      const uint4 sr = a.unit_rec[cbase + (g - a.unit_base[c])];
+      const uint4 sr = a.unit_rec[a.perm[gi]];
       const float thc = content_threshold(a.minval[c]);
       const int b0 = (int)(sr.x & 0xFFFFu), b1 = (int)(sr.x >> 16);
       const uint32_t col0 = sr.y;
@@ -963,11 +959,11 @@ __global__ __launch_bounds__(WGS, (ST == ST_F32) ? 5 : 4) void k_remap_lean(Rema
       int prev_end = -1;
       int carry_open = -1;
       for (uint32_t ci = 0; ci < ncol; ++ci) {
-         const uint4 cr = a.col_rec[cbase + col0 + ci];
+         const uint4 cr = a.col_rec[col0 + ci];
          const int ks = (int)(cr.x & 0xFFFFu), len = (int)(cr.x >> 16);
          const int fk = (int)(cr.z & 0xFFFFu), lk = (int)(cr.z >> 16);
-         const int nfk = (ci + 1 < ncol) ? (int)(a.col_rec[cbase + col0 + ci + 1].z & 0xFFFFu) : INT_MAX;
-         const uint32_t* srcs = a.sorted_src + cbase + cr.y;
+         const int nfk = (ci + 1 < ncol) ? (int)(a.col_rec[col0 + ci + 1].z & 0xFFFFu) : INT_MAX;
+         const uint32_t* srcs = a.sorted_src + cr.y;
          const size_t slotb = (size_t)obase + cr.w;   // output slot of block fk
          for (int tb = max(fk, prev_end + 1) + l16; tb <= lk; tb += 16) {
             const size_t slot = slotb + (size_t)(tb - fk);
