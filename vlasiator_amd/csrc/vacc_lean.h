@@ -101,10 +101,12 @@ template <>
 struct ReconLean<VACC_PPM> {
    static constexpr int NC = 3;
    float fl, b, b2;   // h4 left face of the current cell; v[0] - v[-1]; 2|v[0] - v[-1]|
+   bool out;          // the left face lies outside [v[-1], v[0]] (the previous cell's right test)
    VACC_HD void start(const float* v) {
       fl = lean_h4(v);
       b = v[0] - v[-1];
       b2 = 2.0f * fabsf(b);
+      out = (fl - v[-1]) * (fl - v[0]) > 0.0f;
    }
    VACC_HD void coeffs(const float* v, float* c) {
       const float vk = v[0];
@@ -112,10 +114,14 @@ struct ReconLean<VACC_PPM> {
       const float a = v[1] - vk, a2 = 2.0f * fabsf(a);
       const float hs = 0.5f * lean_superbee(a, b, a2, b2);
       float ul = fl - vk, ur = fr - vk;
-      // flat cell (zero slope) or a face outside its neighbours: the face moves to v -+ slope/2
+      // flat cell (zero slope) or a face outside its neighbours: the face moves to v -+ slope/2.
+      // The test of a face is the same product for both cells that share it, so the right
+      // face's result serves the next cell's left face.
       const bool flat = (hs == 0.0f);
-      if (flat || (b + ul) * ul > 0.0f) ul = -hs;
-      if (flat || (a - ur) * ur < 0.0f) ur = hs;
+      const bool lout = out;
+      out = (a - ur) * ur < 0.0f;
+      if (flat || lout) ul = -hs;
+      if (flat || out) ur = hs;
       // Colella-Woodward: (p - m)(v - (m + p)/2) > (p - m)^2 / 6  <=>  (u_r - u_l)(u_l + 2 u_r) < 0
       if ((ur - ul) * (ul + 2.0f * ur) < 0.0f) ul = -2.0f * ur;
       if ((ur - ul) * (2.0f * ul + ur) > 0.0f) ur = -2.0f * ul;
@@ -132,11 +138,13 @@ template <>
 struct ReconLean<VACC_PQM> {
    static constexpr int NC = 5;
    float fv, fd, b, b2;   // h8 value and h7 derivative of the left face; v[0] - v[-1]; 2|.|
+   bool out;              // the left face lies outside [v[-1], v[0]] (the previous cell's right test)
    VACC_HD void start(const float* v) {
       fv = lean_h8(v);
       fd = lean_h7(v);
       b = v[0] - v[-1];
       b2 = 2.0f * fabsf(b);
+      out = (fv - v[-1]) * (fv - v[0]) > 0.0f;
    }
    VACC_HD void coeffs(const float* v, float* c) {
       const float vk = v[0];
@@ -150,13 +158,17 @@ struct ReconLean<VACC_PQM> {
       fd = fdr;
       b = a;
       b2 = a2;
-      // flat cell, a face outside its neighbours or a face derivative against the slope: PLM face
+      // flat cell, a face outside its neighbours or a face derivative against the slope: PLM
+      // face.  The outside test of a face is the same product for both cells that share it, so
+      // the right face's result serves the next cell's left face.
       const bool flat = (ds == 0.0f);
-      if (flat || (bo + ul) * ul > 0.0f || ds * dl < 0.0f) {
+      const bool lout = out;
+      out = (a - ur) * ur < 0.0f;
+      if (flat || lout || ds * dl < 0.0f) {
          ul = -hs;
          dl = ds;
       }
-      if (flat || (a - ur) * ur < 0.0f || ds * dr < 0.0f) {
+      if (flat || out || ds * dr < 0.0f) {
          ur = hs;
          dr = ds;
       }
