@@ -976,11 +976,11 @@ __global__ __launch_bounds__(WGS, (ST == ST_F32) ? 5 : 4) void k_remap_lean(Rema
             continue;
          }
          const int lo = WID * fk, hi = WID * lk + WID - 1;
-         // Each target cell is written once, with its whole mass: the deposit closes a target
-         // once (the shares of earlier source cells ride in the running sum), a zero written to
-         // the open target by a cell without a boundary is overwritten when it closes, and the
-         // columns of a unit reach distinct target cells (dk <= 3 dv, checked in the column
-         // build).  Ring cells that no deposit reaches keep the zero of their opening.  A target
+         // Each target cell ends with its whole mass in one store: the deposit closes a target
+         // once (the shares of earlier source cells ride in the running sum), the partial sum a
+         // cell without a boundary stores into the open target is overwritten when it closes,
+         // and the columns of a unit reach distinct target cells (dk <= 3 dv, checked in the
+         // column build).  Ring cells that no deposit reaches keep the zero of their opening.  A target
          // outside the column's range goes to the sink row (no branch).
          auto emit = [&](int gt, float m) {
             const bool in = (uint32_t)(gt - lo) <= (uint32_t)(hi - lo);
@@ -1004,9 +1004,9 @@ __global__ __launch_bounds__(WGS, (ST == ST_F32) ? 5 : 4) void k_remap_lean(Rema
          for (int b = 0; b < len; ++b) {
             // targets the block's 4 cells can reach (one cell of margin for rounding), and the
             // lowest target any lane still accumulates
-            const int reach = fx.g + ((fx.t < 4.0f) ? (int)((4.0f - fx.t) * inv_r) + 1 : 0);
+            const int reach = fx.o + 1 + ((fx.t < 4.0f) ? (int)((4.0f - fx.t) * inv_r) + 1 : 0);
             const int openLim = row_max16(min(reach, hi));
-            const int flushLim = row_min16(fx.g - 1);
+            const int flushLim = row_min16(fx.o);
             while (next_open <= lk && WID * next_open <= openLim) {
                if (next_open - next_flush >= LEAN_RING / WID) {   // ring full: not for LEAN_SPAN cells
                   atomicOr(&a.status[c], VACC_STATUS_DK);

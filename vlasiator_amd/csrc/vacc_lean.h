@@ -243,35 +243,34 @@ template <int NC>
 struct Flux {
    float t;     // next boundary, from the current source cell's left edge, in units of dv
    float r;     // dk / dv
-   float acc;   // mass of the open target (g - 1) from earlier source cells
-   int g;       // index of the next boundary (the right edge of the open target g - 1)
+   float acc;   // mass of the open target o from earlier source cells
+   int o;       // the open target: its right edge is the next boundary
 
    // x0 = (v0 - Imin) / dk: the Lagrangian coordinate of the first source cell's left edge
    VACC_HD void begin(double x0, float r_) {
       const double fl = floor(x0);
-      g = (int)fl + 1;
+      o = (int)fl;
       t = (float)((fl + 1.0 - x0) * (double)r_);
       r = r_;
       acc = 0.0f;
    }
    // One source cell with integral coefficients c and average v; emit(target, mass) receives
-   // each target this cell closes.
-   // The first boundary is handled without a branch: a cell that holds none emits 0 to its
-   // open target.  Every other emit closes a target with its whole mass from this pencil, so
-   // a receiver may store emits (the closing emit supersedes such a 0) or add them.
+   // each target this cell closes, with the target's whole mass from this pencil.  The receiver
+   // stores the emits: the first boundary is handled without a branch, and a cell that holds
+   // none emits the open target's mass so far, which the target's closing emit supersedes.
    template <class Emit>
    VACC_HD void cell(const float* c, float v, Emit&& emit) {
       const bool hit = t < 1.0f;
       float Fp = lean_integral<NC>(hit ? t : 0.0f, c);   // F(0) = 0
-      emit(g - 1, hit ? acc + Fp : 0.0f);
+      emit(o, acc + Fp);
       acc = hit ? 0.0f : acc;
-      g += hit ? 1 : 0;
+      o += hit ? 1 : 0;
       t += hit ? r : 0.0f;
       while (t < 1.0f) {   // more boundaries in this cell (dk < dv)
          const float Ft = lean_integral<NC>(t, c);
-         emit(g - 1, Ft - Fp);
+         emit(o, Ft - Fp);
          Fp = Ft;
-         ++g;
+         ++o;
          t += r;
       }
       acc += v - Fp;
@@ -280,7 +279,7 @@ struct Flux {
    // the open target after the pencil's last source cell
    template <class Emit>
    VACC_HD void end(Emit&& emit) {
-      emit(g - 1, acc);
+      emit(o, acc);
    }
 };
 
