@@ -124,10 +124,12 @@ struct Recon<VACC_PPM> {
    static constexpr int NC = 3;
    float fl_raw;      // h4 estimate of the left face of the current cell
    float s0, b;       // scaled v[k]; b = s0 - scaled v[k-1]
+   bool out;          // (v[-1] - fl_raw)(fl_raw - v[0]) < 0
    VACC_HD void start(const float* v, float scale) {
       fl_raw = face_h4(v);
       s0 = v[0] * scale;
       b = s0 - v[-1] * scale;
+      out = (v[-1] - fl_raw) * (fl_raw - v[0]) < 0.0f;
    }
    VACC_HD void coeffs(const float* v, float thr, float scale, float* c) {
       const float fr_raw = face_h4(v + 1);
@@ -137,12 +139,19 @@ struct Recon<VACC_PPM> {
       const float sabs = fabsf(sl) * thr;
       const float ssgn = (sl > 0.0f) ? 1.0f : -1.0f;
       float m = fl_raw, p = fr_raw;
-      if (sabs == 0.0f) {
+      // The face tests (v[-1] - m)(m - v[0]) < 0 and (v[1] - p)(p - v[0]) < 0 of a face shared
+      // by two cells are the same product (both factors negated, exactly), so the right face's
+      // test is carried to the next cell's left face.  A flat cell tests v[0] instead (product
+      // +-0, never below 0).
+      const bool flat = (sabs == 0.0f);
+      const bool lout = !flat && out;
+      out = (v[1] - fr_raw) * (fr_raw - v[0]) < 0.0f;
+      if (flat) {
          p = v[0];
          m = v[0];
       }
-      if ((v[-1] - m) * (m - v[0]) < 0.0f) m = v[0] - (0.5f * ssgn) * sabs;
-      if ((v[1] - p) * (p - v[0]) < 0.0f) p = v[0] + (0.5f * ssgn) * sabs;
+      if (lout) m = v[0] - (0.5f * ssgn) * sabs;
+      if (!flat && out) p = v[0] + (0.5f * ssgn) * sabs;
       const float one_sixth = (float)(1.0 / 6.0);
       {
          const float d = p - m;
@@ -169,11 +178,13 @@ struct Recon<VACC_PQM> {
    static constexpr int NC = 5;
    float fvl_raw, fdl_raw;   // h8 / h7 estimates at the left face of the current cell
    float s0, b;
+   bool out;                 // (v[-1] - fvl_raw)(fvl_raw - v[0]) < 0
    VACC_HD void start(const float* v, float scale) {
       fvl_raw = face_h8(v);
       fdl_raw = deriv_h7(v);
       s0 = v[0] * scale;
       b = s0 - v[-1] * scale;
+      out = (v[-1] - fvl_raw) * (fvl_raw - v[0]) < 0.0f;
    }
    VACC_HD void coeffs(const float* v, float thr, float scale, float* c) {
       const float fvr_raw = face_h8(v + 1), fdr_raw = deriv_h7(v + 1);
@@ -188,17 +199,21 @@ struct Recon<VACC_PQM> {
       fdl_raw = fdr_raw;
       s0 = s1;
       b = a;
-      if (sabs == 0.0f) {
+      // the face tests are carried from cell to cell as in Recon<VACC_PPM>
+      const bool flat = (sabs == 0.0f);
+      const bool lout = !flat && out;
+      out = (v[1] - fvr_raw) * (fvr_raw - vk) < 0.0f;
+      if (flat) {
          fvr = vk;
          fvl = vk;
          fdl = 0.0f;
          fdr = 0.0f;
       }
-      if ((v[-1] - fvl) * (fvl - vk) < 0.0f || ssgn * fdl < 0.0f) {
+      if (lout || ssgn * fdl < 0.0f) {
          fvl = vk - (0.5f * ssgn) * sabs;
          fdl = ssgn * sabs;
       }
-      if ((v[1] - fvr) * (fvr - vk) < 0.0f || ssgn * fdr < 0.0f) {
+      if ((!flat && out) || ssgn * fdr < 0.0f) {
          fvr = vk + (0.5f * ssgn) * sabs;
          fdr = ssgn * sabs;
       }
