@@ -192,11 +192,13 @@ struct ReconLean<VACC_PQM> {
          // ssg F': the + root for ssg > 0, the - root for ssg < 0 (a wrong-signed local maximum
          // of ssg F' inside [0, 1] has a lower local minimum between it and an end).  A
          // negative discriminant leaves F' monotone between its two well-signed ends, and the
-         // reference's fallback root then finds nothing either.
+         // reference's fallback root then finds nothing either.  A point outside [0, 1] is
+         // moved to the nearer end, where F' has the side of ssg, in place of the range test.
          const float disc = fmaf(4.0f * c3, c3, -10.0f * (c2 * c4));
-         const float x = (copysignf(lean_sqrt(fmaxf(disc, 0.0f)), ssg) - 2.0f * c3) * lean_rcp(10.0f * c4);
+         const float x0 = (copysignf(lean_sqrt(fmaxf(disc, 0.0f)), ssg) - 2.0f * c3) * lean_rcp(10.0f * c4);
+         const float x = fminf(fmaxf(x0, 0.0f), 1.0f);
          const float d = fmaf(x, fmaf(x, fmaf(x, 20.0f * c4, 12.0f * c3), 6.0f * c2), dl);
-         bad = disc >= 0.0f && x >= 0.0f && x <= 1.0f && d * ssg < 0.0f;
+         bad = disc >= 0.0f && d * ssg < 0.0f;
       }
       if (bad) {
          // repair (cpu_1d_pqm.hpp:79-145) on the side of the smaller one-sided slope
