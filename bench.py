@@ -49,13 +49,14 @@ def log(msg):
 
 FAST = True   # tolerance-form remap (fast_arith=1); --exact selects the exact form
 AUTOTUNE = True
+SUMS_FROM_F = False   # --sums-from-f: moments and removed mass summed from f, not from block sums
 
 
 def make_context(w: W.Workload, storage: str = "f32", graph: bool = False, fast: bool | None = None):
     mesh = V.make_mesh(w.grid_length, w.vmin, w.vmax)
     sp = V.make_species(w.mass, w.charge, w.min_value, w.add_width)
     op = V.make_options(w.method, fast_arith=FAST if fast is None else fast, storage=STORES[storage],
-                        use_graph=graph)
+                        use_graph=graph, sums_from_f=SUMS_FROM_F)
     return V.Context(mesh, sp, op)
 
 
@@ -417,12 +418,16 @@ def main():
                     help="keep the default remap grid (128 workgroups per CU) instead of timing candidates")
     ap.add_argument("--no-forms", dest="forms", action="store_false",
                     help="skip the short runs of the other storage forms")
+    ap.add_argument("--sums-from-f", action="store_true",
+                    help="tolerance form: take the moments and the adjust's removed mass from f (vacc_options."
+                         "sums_from_f=1) instead of from the block sums of each step's last remap pass")
     ap.add_argument("--translation-only", action="store_true",
                     help="print only the spatial-translation measurement of the workload (one JSON line)")
     args = ap.parse_args()
-    global FAST, AUTOTUNE
+    global FAST, AUTOTUNE, SUMS_FROM_F
     FAST = args.fast
     AUTOTUNE = args.autotune
+    SUMS_FROM_F = args.sums_from_f
 
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
@@ -462,6 +467,7 @@ def main():
                       "populations": len(pops), "grid_length": list(w.grid_length),
                       "method": ["PLM", "PPM", "PQM"][w.method], "storage": args.storage,
                       "arith": "tolerance" if FAST else "exact",
+                      "moment_sums": "f" if (SUMS_FROM_F or not FAST) else "block sums of the last remap pass",
                       "adjust_every_step": w.adjust_every_step, "blocks_at_start": r["blocks_at_start"],
                       "rotation_deg_per_step": w.rotation_deg,
                       "remap_wg_per_cu": [t[0] for t in r["remap_wg_per_cu"]],

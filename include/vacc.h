@@ -117,6 +117,13 @@ typedef struct vacc_options {
                                  variant (map order, list/pool parity, options) on first use
                                  and replays it afterwards; results are identical.  Not used
                                  while kernel profiling is on. */
+   int32_t sums_from_f;       /* tolerance form only.  0: the last remap pass of an acceleration
+                                 (vacc_step, vacc_accelerate_cells, vacc_accelerate_populations)
+                                 takes FP32 sums of each block as it stores it (sum f and the
+                                 in-block index sums), and the density/velocity moments and the
+                                 block adjust's removed mass that follow are formed from them
+                                 (relative error of order 1e-7) instead of re-reading f; 1: both
+                                 are always summed from f.  The exact form always sums f. */
 } vacc_options;
 
 /* Per-cell field input of the transform (cpu_acc_transform.cpp:71-115). */

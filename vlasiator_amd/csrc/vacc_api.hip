@@ -52,6 +52,10 @@ struct vacc_ctx {
    uint64_t nblocks = 0;       // total blocks as of the last synchronisation point
    bool content_valid = false;
    bool moments_valid = false; // `moments` holds the moments of the current list
+   // blk_stats holds the block sums of pool F[fcur] (written by the last remap pass of a
+   // tolerance-form step, vacc_options.sums_from_f = 0); the moments and the adjust's removed
+   // mass are then formed from them
+   bool stats_valid = false;
    bool fields_valid = false;  // `fields` holds the caller's transform inputs
    // lists and pools
    uint32_t* off[2] = {nullptr, nullptr};
@@ -60,6 +64,7 @@ struct vacc_ctx {
    void* f[2] = {nullptr, nullptr};      // block data: FP32, or 16-bit codes
    float* scl[2] = {nullptr, nullptr};   // per-block scales of the 16-bit forms
    uint8_t* content = nullptr;
+   float4* blk_stats = nullptr;          // per pool block (S, sum i f, sum j f, sum k f), or null
    // pass scratch
    uint32_t* sorted_src = nullptr;
    uint4* unit_rec = nullptr;
@@ -105,10 +110,10 @@ struct vacc_ctx {
       uint64_t gen;
       uint32_t map_order;
       int use_moments, adjust, lcur, fcur;
-      bool moments_in, content_in;
+      bool moments_in, content_in, stats_in;
       hipGraphExec_t exec;
       int lcur_out, fcur_out;
-      bool moments_out, content_out, sorted_out;
+      bool moments_out, content_out, sorted_out, stats_out;
    };
    std::vector<StepGraph> graphs;
    uint64_t gen = 0;
@@ -203,6 +208,9 @@ static void free_blocks(vacc_ctx* c) {
       c->f[i] = nullptr;
    }
    if (c->content) (void)hipFree(c->content);
+   if (c->blk_stats) (void)hipFree(c->blk_stats);
+   c->blk_stats = nullptr;
+   c->stats_valid = false;
    if (c->sorted_src) (void)hipFree(c->sorted_src);
    if (c->unit_rec) (void)hipFree(c->unit_rec);
    if (c->col_rec) (void)hipFree(c->col_rec);
@@ -283,6 +291,8 @@ static Pool pool(const vacc_ctx* c, int i) { return Pool{c->f[i], c->scl[i], c->
 
 // (Re)allocate block storage for `want` blocks, preserving the current list and pool.  On
 // failure the new buffers are freed and the old ones are kept.
+static bool use_block_sums(const vacc_ctx* c) { return c->opt.fast_arith && !c->opt.sums_from_f; }
+
 static int ensure_blocks(vacc_ctx* c, size_t want) {
    if (want <= c->cap && c->gid[0]) return 0;
    if (want > 0xF0000000ull) return set_err(VACC_E_NOMEM, "block count exceeds 32-bit list indices");
@@ -293,8 +303,9 @@ static int ensure_blocks(vacc_ctx* c, size_t want) {
    uint8_t* ct = nullptr;
    uint32_t *ss = nullptr, *pm = nullptr, *pc = nullptr;
    uint4 *sr = nullptr, *cr = nullptr;
+   float4* bs = nullptr;
    auto release = [&]() {
-      void* all[] = {g[0], g[1], s[0], s[1], f[0], f[1], sc[0], sc[1], ct, ss, pm, pc, sr, cr};
+      void* all[] = {g[0], g[1], s[0], s[1], f[0], f[1], sc[0], sc[1], ct, ss, pm, pc, sr, cr, bs};
       for (void* p : all)
          if (p) (void)hipFree(p);
    };
@@ -311,6 +322,7 @@ static int ensure_blocks(vacc_ctx* c, size_t want) {
    if (e == hipSuccess) e = hipMalloc(&cr, ncap * sizeof(uint4));
    if (e == hipSuccess) e = hipMalloc(&pm, ncap * sizeof(uint32_t));
    if (e == hipSuccess) e = hipMalloc(&pc, ncap * sizeof(uint32_t));
+   if (e == hipSuccess && use_block_sums(c)) e = hipMalloc(&bs, ncap * sizeof(float4));
    if (e == hipSuccess && c->cap && c->nblocks) {
       // carry the current state: list entries and the whole current pool extent
       const size_t n = c->nblocks;
@@ -322,6 +334,8 @@ static int ensure_blocks(vacc_ctx* c, size_t want) {
       if (e == hipSuccess && sc[c->fcur])
          e = hipMemcpyAsync(sc[c->fcur], c->scl[c->fcur], c->cap * sizeof(float), hipMemcpyDeviceToDevice, c->stream);
       if (e == hipSuccess) e = hipMemcpyAsync(ct, c->content, n, hipMemcpyDeviceToDevice, c->stream);
+      if (e == hipSuccess && bs && c->blk_stats && c->stats_valid)
+         e = hipMemcpyAsync(bs, c->blk_stats, c->cap * sizeof(float4), hipMemcpyDeviceToDevice, c->stream);
       const hipError_t e2 = hipStreamSynchronize(c->stream);
       if (e == hipSuccess) e = e2;
    }
@@ -331,6 +345,7 @@ static int ensure_blocks(vacc_ctx* c, size_t want) {
       return set_err(e == hipErrorOutOfMemory ? VACC_E_NOMEM : VACC_E_HIP,
                      std::string("block storage for ") + std::to_string(ncap) + " blocks: " + hipGetErrorString(e));
    }
+   const bool keep_stats = bs && c->blk_stats && c->stats_valid && c->nblocks;
    free_blocks(c);
    for (int i = 0; i < 2; ++i) {
       c->gid[i] = g[i];
@@ -344,6 +359,8 @@ static int ensure_blocks(vacc_ctx* c, size_t want) {
    c->col_rec = cr;
    c->perm = pm;
    c->perm_cell = pc;
+   c->blk_stats = bs;
+   c->stats_valid = keep_stats;   // carried with the pool
    c->cap = ncap;
    return 0;
 }
@@ -371,7 +388,8 @@ static float cell_f(const vacc_ctx* c, int d) { return (float)c->mi.cell[d]; }
 // caller flips the parity.  grid_mult: persistent remap workgroups per CU (0: the tuned or
 // default value).
 // grid_mult / wgs: the remap launch shape, or 0 for the context's tuned shape
-static int enqueue_pass(vacc_ctx* c, int dim, int grid_mult, int wgs = 0) {
+// stats: the pass also writes the block sums of its output (blk_stats)
+static int enqueue_pass(vacc_ctx* c, int dim, int grid_mult, int wgs = 0, bool stats = false) {
    const int li = c->lcur, lo = 1 - c->lcur, fi = c->fcur, fo = 1 - c->fcur;
    ColumnArgs ca{};
    ca.cell_off = c->off[li];
@@ -446,6 +464,7 @@ static int enqueue_pass(vacc_ctx* c, int dim, int grid_mult, int wgs = 0) {
    ra.len_hist = c->len_hist;
    ra.dv_d = c->mi.cell[dim];
    ra.vmin_d = c->mi.vmin[dim];
+   ra.blk_stats = (stats && c->blk_stats) ? c->blk_stats : nullptr;
    {
       ProfScope ps(c, 0);
       launch_remap(dim, c->opt.method, c->opt.fast_arith != 0, ra, grid_mult ? grid_mult : c->remap_mult[dim],
@@ -456,9 +475,11 @@ static int enqueue_pass(vacc_ctx* c, int dim, int grid_mult, int wgs = 0) {
    return 0;
 }
 
-static int do_pass(vacc_ctx* c, int dim) {
+// stats: the last pass of an acceleration (the block sums feed the adjust and the moments)
+static int do_pass(vacc_ctx* c, int dim, bool stats = false) {
    const int li = c->lcur, lo = 1 - c->lcur, fo = 1 - c->fcur;
-   const int rc = enqueue_pass(c, dim, 0);
+   stats = stats && c->blk_stats;
+   const int rc = enqueue_pass(c, dim, 0, 0, stats);
    if (rc) return rc;
    if (c->prof && c->nlog < PROF_LOG) {
       HIPCHK(hipMemcpyAsync(c->prof_counts + 2 * c->nlog, c->off[li] + c->ncells, sizeof(uint32_t),
@@ -471,6 +492,7 @@ static int do_pass(vacc_ctx* c, int dim) {
    c->fcur = fo;
    c->content_valid = true;
    c->moments_valid = false;
+   c->stats_valid = stats;
    c->sorted = false;   // column order of the pass
    return 0;
 }
@@ -514,6 +536,7 @@ static int do_relist(vacc_ctx* c, int add_width, bool want_loss, const SpatialIn
    aa.pool = pool(c, c->fcur);
    aa.n_keep = c->n_tgt;
    aa.rho_loss = want_loss ? c->rho_loss : nullptr;
+   aa.blk_stats = c->stats_valid ? c->blk_stats : nullptr;
    aa.ncells = c->ncells;
    for (int d = 0; d < 3; ++d) aa.len[d] = c->mi.len[d];
    aa.gs = make_gid_split(c->mi.len);
@@ -557,6 +580,7 @@ static int do_relist(vacc_ctx* c, int add_width, bool want_loss, const SpatialIn
    if (conserve) {   // block_adjust_cpu.cpp:96-113
       launch_cell_sums(c->off[lo], c->src[lo], pool(c, c->fcur), c->csum_post, c->ncells, c->stream);
       launch_cell_scale(c->off[lo], c->src[lo], pool(c, c->fcur), c->csum_pre, c->csum_post, c->ncells, c->stream);
+      c->stats_valid = false;   // f was rescaled
    }
    HIPCHK(hipGetLastError());
    c->lcur = lo;
@@ -572,6 +596,7 @@ static int do_compact(vacc_ctx* c) {
    launch_iota(c->src[c->lcur], (uint32_t)c->nblocks, c->stream);
    HIPCHK(hipGetLastError());
    c->fcur = 1 - c->fcur;
+   c->stats_valid = false;   // the pool was gathered
    return 0;
 }
 
@@ -655,7 +680,7 @@ static void abort_load(vacc_ctx* c) {
       (void)hipMemsetAsync(c->off[0], 0, sizeof(uint32_t), c->stream);
       (void)hipStreamSynchronize(c->stream);
    }
-   c->content_valid = c->moments_valid = c->fields_valid = c->sub_valid = false;
+   c->content_valid = c->moments_valid = c->fields_valid = c->sub_valid = c->stats_valid = false;
 }
 
 // First half of a load of new cells: checks the lists, sizes the buffers and copies the
@@ -668,7 +693,7 @@ static int begin_load(vacc_ctx* c, uint32_t ncells, const uint64_t* offsets, con
    std::vector<uint32_t> off32(ncells + 1);
    for (uint32_t i = 0; i <= ncells; ++i) off32[i] = (uint32_t)offsets[i];
    // from here on a failure empties the context
-   c->content_valid = c->moments_valid = c->fields_valid = c->sub_valid = false;
+   c->content_valid = c->moments_valid = c->fields_valid = c->sub_valid = c->stats_valid = false;
    rc = alloc_cells(c, ncells);
    if (!rc) {
       const size_t want = load_capacity(total, ncells);
@@ -714,6 +739,7 @@ static int end_load(vacc_ctx* c, uint64_t total) {
    c->nblocks = total;
    c->content_valid = false;
    c->moments_valid = false;
+   c->stats_valid = false;
    c->fields_valid = false;
    c->sub_valid = false;
    return 0;
@@ -1168,6 +1194,7 @@ static int init_peaks_load(vacc_ctx* c, uint32_t ncells, uint32_t max_peaks, con
    c->cell_epoch++;
    c->content_valid = false;
    c->moments_valid = false;
+   c->stats_valid = false;
    c->fields_valid = false;
    c->sub_valid = false;
    if (!ncells) return 0;
@@ -1471,6 +1498,7 @@ static void enqueue_moments(vacc_ctx* c) {
       ma.cell[d] = c->mi.cell[d];
    }
    ma.gs = make_gid_split(c->mi.len);
+   ma.blk_stats = c->stats_valid ? c->blk_stats : nullptr;
    launch_moments(ma, c->stream);
    c->moments_valid = true;
 }
@@ -1555,7 +1583,7 @@ int vacc_accelerate_cells(vacc_ctx* c, uint32_t map_order) {
    int rc = reserve_for_step(c);
    if (rc) return rc;
    for (int p = 0; p < 3; ++p) {
-      rc = do_pass(c, pass_dim((int)map_order, p));
+      rc = do_pass(c, pass_dim((int)map_order, p), p == 2);
       if (rc) return rc;
    }
    return finish(c);
@@ -1706,7 +1734,7 @@ int vacc_get_remap_shape(vacc_ctx* c, uint32_t* wg_per_cu, uint32_t* lanes_per_w
 static int enqueue_step(vacc_ctx* c, uint32_t map_order, int32_t use_moments, int32_t adjust) {
    enqueue_transform(c, map_order, use_moments ? 1 : 0);
    for (int p = 0; p < 3; ++p) {
-      const int rc = do_pass(c, pass_dim((int)map_order, p));
+      const int rc = do_pass(c, pass_dim((int)map_order, p), p == 2);
       if (rc) return rc;
    }
    if (adjust) {
@@ -1723,12 +1751,14 @@ static int graph_step(vacc_ctx* c, uint32_t map_order, int32_t use_moments, int3
    const int um = use_moments ? 1 : 0, ad = adjust ? 1 : 0;
    for (const auto& g : c->graphs)
       if (g.gen == c->gen && g.map_order == map_order && g.use_moments == um && g.adjust == ad && g.lcur == c->lcur &&
-          g.fcur == c->fcur && g.moments_in == c->moments_valid && g.content_in == c->content_valid) {
+          g.fcur == c->fcur && g.moments_in == c->moments_valid && g.content_in == c->content_valid &&
+          g.stats_in == c->stats_valid) {
          HIPCHK(hipGraphLaunch(g.exec, c->stream));
          c->lcur = g.lcur_out;
          c->fcur = g.fcur_out;
          c->moments_valid = g.moments_out;
          c->content_valid = g.content_out;
+         c->stats_valid = g.stats_out;
          c->sorted = g.sorted_out;
          return 0;
       }
@@ -1741,6 +1771,7 @@ static int graph_step(vacc_ctx* c, uint32_t map_order, int32_t use_moments, int3
    g.fcur = c->fcur;
    g.moments_in = c->moments_valid;
    g.content_in = c->content_valid;
+   g.stats_in = c->stats_valid;
    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
    const int rc = enqueue_step(c, map_order, use_moments, adjust);
    hipGraph_t graph = nullptr;
@@ -1758,6 +1789,7 @@ static int graph_step(vacc_ctx* c, uint32_t map_order, int32_t use_moments, int3
    g.moments_out = c->moments_valid;
    g.content_out = c->content_valid;
    g.sorted_out = c->sorted;
+   g.stats_out = c->stats_valid;
    c->graphs.push_back(g);
    HIPCHK(hipGraphLaunch(g.exec, c->stream));
    return 0;
@@ -1881,7 +1913,7 @@ int vacc_accelerate_populations(vacc_ctx* const* ctxs, uint32_t npop, uint32_t m
          enqueue_transform(c, map_order, 0);
       }
       for (int d = 0; d < 3; ++d) {
-         const int rc = do_pass(c, pass_dim((int)map_order, d));
+         const int rc = do_pass(c, pass_dim((int)map_order, d), d == 2);
          if (rc) return rc;
       }
       if (adjust) {
@@ -2023,7 +2055,7 @@ int vacc_init_peaks(vacc_ctx* c, uint32_t ncells, uint32_t max_peaks, const vacc
       }
       bm_off[i + 1] = bm_off[i] + words;
    }
-   c->content_valid = c->moments_valid = c->fields_valid = c->sub_valid = false;
+   c->content_valid = c->moments_valid = c->fields_valid = c->sub_valid = c->stats_valid = false;
    const int rc = init_peaks_load(c, ncells, max_peaks, peaks, npeaks, bbox, bm_off);
    if (rc) abort_load(c);
    return rc;
@@ -2207,6 +2239,7 @@ int vacc_trans_map_1d(vacc_ctx* c, uint32_t dim, double dt) {
    c->fcur = 1 - c->fcur;
    c->content_valid = false;
    c->moments_valid = false;
+   c->stats_valid = false;
    HIPCHK(hipStreamSynchronize(c->stream));
    return 0;
 }

@@ -88,6 +88,31 @@ __device__ inline float row_sum16(float s) {
    s = s + row_ror<2>(s);
    return s + row_ror<1>(s);
 }
+// DPP quad permutation within each group of 4 lanes (quad_perm [1,0,3,2]: lane ^ 1,
+// [2,3,0,1]: lane ^ 2); as with the rotations, all lanes of the row are active.
+template <int CTRL>
+__device__ inline float quad_perm(float x) {
+   return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), CTRL, 0xF, 0xF, true));
+}
+
+// Totals of four quantities a, b, c, d over the 16 lanes of a DPP row: lane L returns the
+// total of quantity L & 3 (a, b, c, d).  Lanes L and L ^ 2 first exchange the pair the other
+// keeps ((a, b) for bit 1 clear), lanes L and L ^ 1 then the value the other keeps, and the
+// four quads are summed by rotations of 8 and 4: 5 additions instead of the 16 of four
+// separate butterflies (summation order not canonical).
+__device__ inline float row_sum4x16(float a, float b, float c, float d, int l16) {
+   const bool b1 = (l16 & 2) != 0, b0 = (l16 & 1) != 0;
+   float k0 = b1 ? c : a, k1 = b1 ? d : b;
+   const float s0 = b1 ? a : c, s1 = b1 ? b : d;
+   k0 = k0 + quad_perm<0x4E>(s0);
+   k1 = k1 + quad_perm<0x4E>(s1);
+   float m = b0 ? k1 : k0;
+   const float s = b0 ? k0 : k1;
+   m = m + quad_perm<0xB1>(s);
+   m = m + row_ror<8>(m);
+   return m + row_ror<4>(m);
+}
+
 // maximum of non-negative finite values: their bit patterns order as unsigned integers
 __device__ inline float row_maxf16(float m) {
    uint32_t u = __builtin_bit_cast(uint32_t, m);

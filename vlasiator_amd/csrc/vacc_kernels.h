@@ -116,6 +116,9 @@ struct RemapArgs {
                                 // RANGE_EXACT (EXACT_BIN's units), RANGE_WIDE (WIDE_BIN's units)
    const uint32_t* len_hist;    // [LEN_BINS] units per length bin; [EXACT_BIN] = EXACT_FLAG units,
                                 // [WIDE_BIN] = WIDE_FLAG units
+   float4* blk_stats;           // null, or per output block (at its slot) the sums of its stored
+                                // values: S = sum f, and sum i f, sum j f, sum k f over the
+                                // in-block indices (i, j, k) = 0..3
 };
 
 struct AdjustArgs {
@@ -129,6 +132,8 @@ struct AdjustArgs {
    uint32_t* gid_out;          // (write phase)
    uint32_t* src_out;          // (write phase)
    double* rho_loss;           // [ncells] (count phase, may be null)
+   const float4* blk_stats;    // null, or per pool block its sums (the remap's blk_stats): the
+                               // count phase then takes a removed block's mass from them
    const uint32_t* flags;      // global flags (capacity overflow -> the write phase is skipped)
    uint32_t ncells;
    uint32_t len[3];
@@ -155,6 +160,8 @@ struct MomentArgs {
    GidSplit gs;                // block coordinates of a GID (from len)
    double vmin[3];
    double cell[3];
+   const float4* blk_stats;    // null, or per pool block its sums (the remap's blk_stats): the
+                               // moments are then formed from them instead of from f
 };
 
 // synthetic initial state (vacc_init.hip)

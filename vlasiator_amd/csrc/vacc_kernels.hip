@@ -657,13 +657,47 @@ __device__ inline void set_content(uint8_t* content, size_t slot, const float (&
    set_content_flag(content, slot, (w[0] >= thc) || (w[1] >= thc) || (w[2] >= thc) || (w[3] >= thc), setlane0, l16);
 }
 
+// Block sums of a stored block held by the 16 lanes of a DPP row (all active): lane values
+// v[e] at in-block index e along axis AX and the lane's fixed indices (cx, cy, cz) on the
+// other two axes, times `scale`.  Writes (S, sum i f, sum j f, sum k f) to st[slot], lanes 0..3
+// one component each.  FP32 sums of the 64 stored values, for the tolerance form's moments and
+// removed mass.
+template <int AX>
+__device__ inline void put_block_stats(float4* st, size_t slot, int l16, const float (&v)[4], int cx, int cy, int cz,
+                                       float scale) {
+   const float s = (v[0] + v[1]) + (v[2] + v[3]);
+   const float e = fmaf(3.0f, v[3], fmaf(2.0f, v[2], v[1]));
+   const float wx = (AX == 0) ? e : (float)cx * s;
+   const float wy = (AX == 1) ? e : (float)cy * s;
+   const float wz = (AX == 2) ? e : (float)cz * s;
+   const float r = row_sum4x16(s, wx, wy, wz, l16);
+   if (l16 < 4) reinterpret_cast<float*>(st + slot)[l16] = r * scale;
+}
+
+// Block sums of a lane's pencil (the FP32 store layout of pass DIM: lane (ip, jp) = (l16 & 3,
+// l16 >> 2) holds i = 0..3 at (j, k) = (jp, ip) for DIM 0, j = 0..3 at (i, k) = (ip, jp) for
+// DIM 1, k = 0..3 at (i, j) = (ip, jp) for DIM 2).
+template <int DIM>
+__device__ inline void put_pencil_stats(float4* st, size_t slot, int l16, const float (&v)[4]) {
+   const int ip = l16 & 3, jp = l16 >> 2;
+   if (DIM == 0) put_block_stats<0>(st, slot, l16, v, 0, jp, ip, 1.0f);
+   else if (DIM == 1) put_block_stats<1>(st, slot, l16, v, ip, 0, jp, 1.0f);
+   else put_block_stats<2>(st, slot, l16, v, ip, jp, 0, 1.0f);
+}
+
+// Block sums of the canonical rows of a 16-bit block (lane L: codes q of i = 0..3 at
+// (j, k) = (L & 3, L >> 2)), the stored values being scale * q.
+__device__ inline void put_row_stats(float4* st, size_t slot, int l16, const float (&q)[4], float scale) {
+   put_block_stats<0>(st, slot, l16, q, 0, l16 & 3, l16 >> 2, scale);
+}
+
 // Store one finished target block from the ring (all 16 lanes of the unit active).  FP32: each
 // lane stores its own pencil.  16-bit forms: lane L reads the canonical row (j,k) = (L&3, L>>2)
 // of the block from the ring columns of the unit's lanes, encodes the block and stores its 4
 // codes (8 contiguous bytes) and, lane 0, the scale.  The content flag is taken on the stored
 // (decoded) values.  FAST: the block scale's quotient from the hardware reciprocal (the
-// tolerance form).
-template <int DIM, int ST, int RN, int WGS, bool FAST = false>
+// tolerance form).  STATS: also the block sums, when a.blk_stats is set.
+template <int DIM, int ST, int RN, int WGS, bool FAST = false, bool STATS = true>
 __device__ inline void flush_block(const RemapArgs& a, float (*ring)[WGS], int tid, int l16, int setlane0, int o,
                                    int blk, size_t slot, float thc) {
    if constexpr (ST == ST_F32) {
@@ -672,6 +706,7 @@ __device__ inline void flush_block(const RemapArgs& a, float (*ring)[WGS], int t
       for (int e = 0; e < 4; ++e) q[e] = ring[(WID * blk + e) & (RN - 1)][tid];
       store_pencil<DIM>(static_cast<float*>(a.out.f), slot, o, q);
       set_content(a.content_out, slot, q, thc, setlane0, l16);
+      if (STATS && a.blk_stats) put_pencil_stats<DIM>(a.blk_stats, slot, l16, q);
    } else {
       const int L = l16, base = tid - l16;
       float x[4];
@@ -691,11 +726,15 @@ __device__ inline void flush_block(const RemapArgs& a, float (*ring)[WGS], int t
       *reinterpret_cast<uint2*>(p) = pk;
       if (L == 0) a.out.scale[slot] = sc;
       set_content_flag(a.content_out, slot, sc * qmax >= thc, setlane0, l16);
+      if (STATS && a.blk_stats) {
+         const float d[4] = {Pack2<ST>::lo(pk.x), Pack2<ST>::hi(pk.x), Pack2<ST>::lo(pk.y), Pack2<ST>::hi(pk.y)};
+         put_row_stats(a.blk_stats, slot, l16, d, sc);
+      }
    }
 }
 
 // Store an all-zero target block (a target block that received no contribution).
-template <int DIM, int ST>
+template <int DIM, int ST, bool STATS = true>
 __device__ inline void zero_block(const RemapArgs& a, int l16, int setlane0, int o, size_t slot, float thc) {
    const float q[4] = {0.0f, 0.0f, 0.0f, 0.0f};
    if constexpr (ST == ST_F32) {
@@ -706,10 +745,11 @@ __device__ inline void zero_block(const RemapArgs& a, int l16, int setlane0, int
       if (l16 == 0) a.out.scale[slot] = 0.0f;
    }
    set_content(a.content_out, slot, q, thc, setlane0, l16);
+   if (STATS && a.blk_stats && l16 < 4) reinterpret_cast<float*>(a.blk_stats + slot)[l16] = 0.0f;
 }
 
 // Pass-through copy of a source block (cells in copy mode keep their stored representation).
-template <int DIM, int ST>
+template <int DIM, int ST, bool STATS = true>
 __device__ inline void copy_block(const RemapArgs& a, int l16, int setlane0, int o, uint32_t sidx, size_t slot,
                                   float thc) {
    if constexpr (ST == ST_F32) {
@@ -717,6 +757,7 @@ __device__ inline void copy_block(const RemapArgs& a, int l16, int setlane0, int
       load_pencil<DIM, ST>(a.in, sidx, o, w);
       store_pencil<DIM>(static_cast<float*>(a.out.f), slot, o, w);
       set_content(a.content_out, slot, w, thc, setlane0, l16);
+      if (STATS && a.blk_stats) put_pencil_stats<DIM>(a.blk_stats, slot, l16, w);
    } else {
       uint2 raw = make_uint2(0u, 0u);
       float sc = 0.0f;
@@ -727,6 +768,10 @@ __device__ inline void copy_block(const RemapArgs& a, int l16, int setlane0, int
       *reinterpret_cast<uint2*>(static_cast<uint16_t*>(a.out.f) + slot * WID3 + 4 * l16) = raw;
       if (l16 == 0) a.out.scale[slot] = sc;
       set_content_flag(a.content_out, slot, sc * max_code<ST>(raw) >= thc, setlane0, l16);
+      if (STATS && a.blk_stats) {
+         const float d[4] = {Pack2<ST>::lo(raw.x), Pack2<ST>::hi(raw.x), Pack2<ST>::lo(raw.y), Pack2<ST>::hi(raw.y)};
+         put_row_stats(a.blk_stats, slot, l16, d, sc);
+      }
    }
 }
 
@@ -925,7 +970,7 @@ __device__ inline int row_min16(int x) {
 // the PQM workloads, six and eight are slower.  The 16-bit forms need 116 VGPRs for the block
 // codec: at five waves they would keep 50 to 90 bytes per lane in scratch, 1.6 to 2 times the
 // pass's HBM traffic for 1.7 % of time, so they run four waves without scratch.
-template <int DIM, int METHOD, int ST, int WGS>
+template <int DIM, int METHOD, int ST, int WGS, bool STATS>
 __global__ __launch_bounds__(WGS, (ST == ST_F32) ? 5 : 4) void k_remap_lean(RemapArgs a) {
    using R = ReconLean<METHOD>;
    constexpr int NC = R::NC;
@@ -971,7 +1016,7 @@ __global__ __launch_bounds__(WGS, (ST == ST_F32) ? 5 : 4) void k_remap_lean(Rema
             a.src_out[slot] = (uint32_t)slot;
          }
          if (copy) {
-            for (int b = 0; b < len; ++b) copy_block<DIM, ST>(a, l16, setlane0, o, srcs[b], slotb + b, thc);
+            for (int b = 0; b < len; ++b) copy_block<DIM, ST, STATS>(a, l16, setlane0, o, srcs[b], slotb + b, thc);
             prev_end = max(prev_end, lk);
             continue;
          }
@@ -1011,7 +1056,7 @@ __global__ __launch_bounds__(WGS, (ST == ST_F32) ? 5 : 4) void k_remap_lean(Rema
                if (next_open - next_flush >= LEAN_RING / WID) {   // ring full: not for LEAN_SPAN cells
                   atomicOr(&a.status[c], VACC_STATUS_DK);
                   atomicOr(&a.status[a.ncells], VACC_STATUS_DK);
-                  flush_block<DIM, ST, LEAN_RING, WGS, true>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), thc);
+                  flush_block<DIM, ST, LEAN_RING, WGS, true, STATS>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), thc);
                   ++next_flush;
                }
 #pragma unroll
@@ -1035,7 +1080,7 @@ __global__ __launch_bounds__(WGS, (ST == ST_F32) ? 5 : 4) void k_remap_lean(Rema
             load_pencil<DIM, ST>(a.in, s_next, o, w3);   // zeros past the column's end
             s_next = s_after;
             while (next_flush < next_open && next_flush < nfk && WID * next_flush + WID - 1 < flushLim) {
-               flush_block<DIM, ST, LEAN_RING, WGS, true>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), thc);
+               flush_block<DIM, ST, LEAN_RING, WGS, true, STATS>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), thc);
                ++next_flush;
             }
             __builtin_amdgcn_sched_barrier(0);
@@ -1054,10 +1099,10 @@ __global__ __launch_bounds__(WGS, (ST == ST_F32) ? 5 : 4) void k_remap_lean(Rema
          }
          fx.end(emit);
          while (next_flush < next_open && next_flush < nfk) {
-            flush_block<DIM, ST, LEAN_RING, WGS, true>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), thc);
+            flush_block<DIM, ST, LEAN_RING, WGS, true, STATS>(a, ring, tid, l16, setlane0, o, next_flush, slotb + (next_flush - fk), thc);
             ++next_flush;
          }
-         for (int tb = next_open; tb <= lk && tb < nfk; ++tb) zero_block<DIM, ST>(a, l16, setlane0, o, slotb + (tb - fk), thc);
+         for (int tb = next_open; tb <= lk && tb < nfk; ++tb) zero_block<DIM, ST, STATS>(a, l16, setlane0, o, slotb + (tb - fk), thc);
          carry_open = (next_open > nfk) ? next_open : -1;
          prev_end = max(prev_end, lk);
       }
@@ -1330,6 +1375,10 @@ __global__ __launch_bounds__(AWG) void k_adjust(AdjustArgs a) {
                   const int kx = (int)ux - mnx;
                   if (row >= 0 && row < nr && !(bm[row * W + (kx >> 5)] & (1u << (kx & 31)))) sidx = a.src[off + i];
                }
+               if (a.blk_stats) {   // the block's sum from the remap's store
+                  if (sidx != INVALID) lsum += (double)a.blk_stats[sidx].x;
+                  continue;
+               }
                unsigned long long bal = __ballot(sidx != INVALID);
                while (bal) {   // up to 8 removed blocks per batch, their loads in flight together
                   constexpr int B = 8;
@@ -1572,6 +1621,56 @@ __global__ __launch_bounds__(WG) void k_moments(MomentArgs a) {
    }
 }
 
+// The same moments from the block sums of the last remap pass (MomentArgs::blk_stats): one
+// workgroup per cell, one lane per block, FP64 sums of S, S times the block's corner velocity
+// and the in-block index sums; the 0.5-cell offsets and DV3 applied once per cell.
+__global__ __launch_bounds__(WG) void k_moments_sums(MomentArgs a) {
+   __shared__ double red[4][WG / 64];
+   const int t = threadIdx.x;
+   const uint32_t c = blockIdx.x;
+   const uint32_t off = a.cell_off[c];
+   const uint32_t n = a.cell_off[c + 1] - off;
+   const double bx = a.cell[0] * WID, by = a.cell[1] * WID, bz = a.cell[2] * WID;
+   double s0 = 0.0, sx = 0.0, sy = 0.0, sz = 0.0, wx = 0.0, wy = 0.0, wz = 0.0;
+   for (uint32_t b = t; b < n; b += WG) {
+      const uint32_t sidx = a.src[off + b];
+      if (sidx == INVALID) continue;   // a block the adjust created: all zero
+      const float4 st = a.blk_stats[sidx];
+      uint32_t x, y, z;
+      gid_xyz(a.gid[off + b], a.gs, x, y, z);
+      const double S = st.x;
+      s0 += S;
+      sx = fma(S, fma((double)x, bx, a.vmin[0]), sx);
+      sy = fma(S, fma((double)y, by, a.vmin[1]), sy);
+      sz = fma(S, fma((double)z, bz, a.vmin[2]), sz);
+      wx += (double)st.y;
+      wy += (double)st.z;
+      wz += (double)st.w;
+   }
+   double s1 = sx + (0.5 * a.cell[0]) * s0 + a.cell[0] * wx;
+   double s2 = sy + (0.5 * a.cell[1]) * s0 + a.cell[1] * wy;
+   double s3 = sz + (0.5 * a.cell[2]) * s0 + a.cell[2] * wz;
+   for (int o = 32; o > 0; o >>= 1) {
+      s0 += __shfl_down(s0, o, 64);
+      s1 += __shfl_down(s1, o, 64);
+      s2 += __shfl_down(s2, o, 64);
+      s3 += __shfl_down(s3, o, 64);
+   }
+   if ((t & 63) == 0) {
+      red[0][t >> 6] = s0;
+      red[1][t >> 6] = s1;
+      red[2][t >> 6] = s2;
+      red[3][t >> 6] = s3;
+   }
+   __syncthreads();
+   if (t < 4) {
+      const double DV3 = a.cell[0] * a.cell[1] * a.cell[2];
+      double sum = 0.0;
+      for (int w = 0; w < WG / 64; ++w) sum += red[t][w];
+      a.out[4 * (size_t)c + t] = sum * DV3;
+   }
+}
+
 __global__ void k_transform(MeshInfo m, const TransformIn* fields, const double* dt_override, const double* moments,
                             int use_moments, double mass, double charge, double hall_min_rhoq, int ohm_gradpe,
                             int map_order, double* isect_out, uint32_t ncells) {
@@ -1720,9 +1819,16 @@ static void remap_dms(RemapArgs a, uint32_t grid, double i_dv, bool fast, int wg
    } else {
       // tolerance form: the lean kernel (workgroups of wgs lanes, wgs/16 work units each), then
       // the exact kernel over the few units it leaves
-      if (wgs == 64) hipLaunchKernelGGL((k_remap_lean<DIM, METHOD, ST, 64>), dim3(grid), dim3(64), 0, s, a);
-      else if (wgs == 128) hipLaunchKernelGGL((k_remap_lean<DIM, METHOD, ST, 128>), dim3(grid), dim3(128), 0, s, a);
-      else hipLaunchKernelGGL((k_remap_lean<DIM, METHOD, ST, WG>), dim3(grid), dim3(WG), 0, s, a);
+      // (with block sums: their own instance, whose extra registers the other passes do not carry)
+      if (a.blk_stats) {
+         if (wgs == 64) hipLaunchKernelGGL((k_remap_lean<DIM, METHOD, ST, 64, true>), dim3(grid), dim3(64), 0, s, a);
+         else if (wgs == 128) hipLaunchKernelGGL((k_remap_lean<DIM, METHOD, ST, 128, true>), dim3(grid), dim3(128), 0, s, a);
+         else hipLaunchKernelGGL((k_remap_lean<DIM, METHOD, ST, WG, true>), dim3(grid), dim3(WG), 0, s, a);
+      } else {
+         if (wgs == 64) hipLaunchKernelGGL((k_remap_lean<DIM, METHOD, ST, 64, false>), dim3(grid), dim3(64), 0, s, a);
+         else if (wgs == 128) hipLaunchKernelGGL((k_remap_lean<DIM, METHOD, ST, 128, false>), dim3(grid), dim3(128), 0, s, a);
+         else hipLaunchKernelGGL((k_remap_lean<DIM, METHOD, ST, WG, false>), dim3(grid), dim3(WG), 0, s, a);
+      }
       a.range = RANGE_EXACT;
       hipLaunchKernelGGL((k_remap<DIM, METHOD, ST, RING, WG>), dim3((uint32_t)device_cus()), dim3(WG), 0, s, a, i_dv);
    }
@@ -1786,7 +1892,8 @@ void launch_content(const uint32_t* src, const Pool& pool, uint8_t* content, uin
 
 void launch_moments(const MomentArgs& a, hipStream_t s) {
    if (a.ncells == 0) return;
-   if (a.pool.storage == ST_F32) hipLaunchKernelGGL(k_moments<ST_F32>, dim3(a.ncells), dim3(WG), 0, s, a);
+   if (a.blk_stats) hipLaunchKernelGGL(k_moments_sums, dim3(a.ncells), dim3(WG), 0, s, a);
+   else if (a.pool.storage == ST_F32) hipLaunchKernelGGL(k_moments<ST_F32>, dim3(a.ncells), dim3(WG), 0, s, a);
    else if (a.pool.storage == ST_BF16) hipLaunchKernelGGL(k_moments<ST_BF16>, dim3(a.ncells), dim3(WG), 0, s, a);
    else hipLaunchKernelGGL(k_moments<ST_F16>, dim3(a.ncells), dim3(WG), 0, s, a);
 }

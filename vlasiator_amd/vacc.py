@@ -53,7 +53,8 @@ class Species(C.Structure):
 class Options(C.Structure):
     _fields_ = [("method", C.c_int32), ("storage", C.c_int32), ("wall_margin", C.c_int32),
                 ("ohm_gradpe_term", C.c_int32), ("hall_minimum_rhoq", C.c_double),
-                ("max_rotation_deg", C.c_double), ("fast_arith", C.c_int32), ("use_graph", C.c_int32)]
+                ("max_rotation_deg", C.c_double), ("fast_arith", C.c_int32), ("use_graph", C.c_int32),
+                ("sums_from_f", C.c_int32)]
 
 
 class CellFields(C.Structure):
@@ -157,10 +158,11 @@ def make_species(mass, charge, min_value=1e-15, add_width=1, dynamic=0, bulk=(0.
 
 
 def make_options(method=PQM, wall_margin=0, ohm_gradpe=0, hall_min_rhoq=1.60217653e-19, max_rotation_deg=25.0,
-                 storage=STORE_F32, fast_arith=False, use_graph=False) -> Options:
+                 storage=STORE_F32, fast_arith=False, use_graph=False, sums_from_f=False) -> Options:
     o = Options()
     o.fast_arith = 1 if fast_arith else 0
     o.use_graph = 1 if use_graph else 0
+    o.sums_from_f = 1 if sums_from_f else 0
     o.method, o.storage, o.wall_margin, o.ohm_gradpe_term = int(method), int(storage), int(wall_margin), int(ohm_gradpe)
     o.hall_minimum_rhoq, o.max_rotation_deg = float(hall_min_rhoq), float(max_rotation_deg)
     return o
