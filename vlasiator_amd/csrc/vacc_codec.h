@@ -12,6 +12,9 @@
 // csum(x): s_L = (x[4L] + x[4L+1]) + (x[4L+2] + x[4L+3]) for L = 0..15 (rows of fixed j,k),
 // then s_L <- s_L + s_{L^m} for m = 8, 4, 2, 1.  The order is fixed so that any lane layout
 // that holds the block produces the same bits.
+// The tolerance form's BF16 encoder takes p = 0 (codes RN(x_e), scale Sf/Sq near 1, else 1):
+// BF16 has the exponent range of FP32 and scaling by 2^-p commutes with the rounding, so
+// the decoded values are those of the normalised encoding, without the block maximum.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -144,6 +147,15 @@ template <int ST, bool FAST = false>
 // scale sc >= 0, the largest stored value max_e RN(sc * d_e) is RN(sc * qmax), rounding being
 // monotone, so one product gives the block's content test.
 __device__ inline float encode_rows16(const float (&x)[4], uint2& p, float& qmax) {
+   if constexpr (FAST && ST == ST_BF16) {   // p = 0 (above)
+      p.x = Pack2<ST>::enc(x[0], x[1]);
+      p.y = Pack2<ST>::enc(x[2], x[3]);
+      const float q0 = Pack2<ST>::lo(p.x), q1 = Pack2<ST>::hi(p.x), q2 = Pack2<ST>::lo(p.y), q3 = Pack2<ST>::hi(p.y);
+      qmax = fmaxf(fmaxf(q0, q1), fmaxf(q2, q3));
+      const float sf = row_sum16((x[0] + x[1]) + (x[2] + x[3]));
+      const float sq = row_sum16((q0 + q1) + (q2 + q3));
+      return pick_scale<FAST>(1.0f, sf, sq);
+   }
    float mx = fmaxf(fmaxf(fabsf(x[0]), fabsf(x[1])), fmaxf(fabsf(x[2]), fabsf(x[3])));
    mx = row_maxf16(mx);
    if (!(mx > 0.0f)) {   // the same in all lanes of the row

@@ -34,13 +34,13 @@ namespace vacc {
 constexpr int WG = 256;
 constexpr int CWG = 512;         // k_columns workgroup: 8 waves share one cell's LDS bitmap
 constexpr int AWG = 512;         // k_adjust workgroup, likewise
-// LDS bitmap words per chunk of rows, and rows per chunk of k_columns: 45 KiB of LDS per
-// k_columns workgroup and at most 40 KiB per k_adjust workgroup, so three of them share a
-// CU.  Larger chunks (28 KiB bitmaps, two workgroups per CU) made the column build 25 % and
-// the adjust 9 % slower on the 8192-cell 75^3 workload; most cells' boxes fit one chunk
-// either way.
+// LDS bitmap words per chunk of rows, and rows per chunk of k_columns: 37 KiB of LDS per
+// k_columns workgroup (four per CU, at 64 VGPRs: 7 % faster than three with 1536-row
+// chunks) and at most 40 KiB per k_adjust workgroup (three per CU).  Larger chunks (28 KiB
+// bitmaps, two workgroups per CU) made the column build 25 % and the adjust 9 % slower on the
+// 8192-cell 75^3 workload; most cells' boxes fit one chunk either way.
 constexpr int BM_WORDS = 5120;
-constexpr int ROW_MAX = 1536;
+constexpr int ROW_MAX = 1024;
 
 // ---------------------------------------------------------------------------
 // small helpers
@@ -264,7 +264,7 @@ __device__ inline void for_blocks4(const uint32_t* gid, uint32_t n, int t, F&& f
 }
 
 template <int DIM>
-__global__ __launch_bounds__(CWG) void k_columns(ColumnArgs a) {
+__global__ __launch_bounds__(CWG, 8) void k_columns(ColumnArgs a) {
    __shared__ uint32_t bm[BM_WORDS];
    __shared__ uint32_t rowv[4][ROW_MAX];
    __shared__ int sbox[6];
