@@ -964,14 +964,14 @@ __device__ inline int row_min16(int x) {
    return min(x, row_ror_i<1>(x));
 }
 
-// FP32 storage: at least five waves per SIMD (at most 96 VGPRs).  The few values that no
-// longer fit (12 to 20 bytes per lane) are kept in scratch across the per-unit setup and the
-// column ends, not in the block loop; five waves issue about 7 % more per pass than four on
-// the PQM workloads, six and eight are slower.  The 16-bit forms need 116 VGPRs for the block
-// codec: at five waves they would keep 50 to 90 bytes per lane in scratch, 1.6 to 2 times the
-// pass's HBM traffic for 1.7 % of time, so they run four waves without scratch.
+// At least five waves per SIMD (at most 96 VGPRs).  The values that no longer fit are kept in
+// scratch across the per-unit setup and the column ends, not in the block loop: 12 to 20
+// bytes per lane for FP32 storage (36 to 44 with block sums), 44 to 64 for the 16-bit forms
+// (their codec alone would need 111 to 116 VGPRs).  Five waves issue about 7 % more per pass
+// than four on the PQM workloads for FP32 and make the 16-bit forms' step 3 % faster; six and
+// eight are slower.
 template <int DIM, int METHOD, int ST, int WGS, bool STATS>
-__global__ __launch_bounds__(WGS, (ST == ST_F32) ? 5 : 4) void k_remap_lean(RemapArgs a) {
+__global__ __launch_bounds__(WGS, 5) void k_remap_lean(RemapArgs a) {
    using R = ReconLean<METHOD>;
    constexpr int NC = R::NC;
    // rows 0 .. LEAN_RING - 1: the target ring; row LEAN_RING: sink of deposits outside the
