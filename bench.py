@@ -362,7 +362,10 @@ def measure(pops, storage: str, steps: int, warmup: int, graph: bool = False, fa
     ms_per_step = 1e3 * elapsed / steps
     log(f"[{storage}] timed {steps} steps: {ms_per_step:.3f} ms/step, {value:.4g} cell-updates/s, status {status}")
 
-    remap_bytes = (prof["remap_src_blocks"] + prof["remap_tgt_blocks"]) * BYTES_PER_BLOCK[storage]
+    # every pass reads its source blocks and writes its target blocks; the step's last pass also
+    # writes 16 B of block sums per target block (tolerance form)
+    remap_bytes = ((prof["remap_src_blocks"] + prof["remap_tgt_blocks"]) * BYTES_PER_BLOCK[storage]
+                   + 16 * prof.get("remap_sum_blocks", 0))
     remap_s = prof["remap_ms"] / 1e3
     achieved = remap_bytes / remap_s / 1e9 if remap_s > 0 else 0.0
     launches = max(prof["remap_launches"], 1)

@@ -312,6 +312,7 @@ typedef struct vacc_profile {
    double transform_ms;       /* moments + transform/intersections */
    uint64_t trans_launches;
    double trans_ms;           /* spatial translation (include/vacc_trans.h), all its kernels */
+   uint64_t remap_sum_blocks; /* target blocks of the remaps that also stored block sums */
 } vacc_profile;
 int vacc_profile_begin(vacc_ctx* ctx);
 int vacc_profile_read(vacc_ctx* ctx, vacc_profile* out);

@@ -155,6 +155,7 @@ struct vacc_ctx {
    std::vector<hipEvent_t> ev_free;
    uint32_t* prof_counts = nullptr;   // [2*PROF_LOG] input/output totals of each remap
    uint32_t nlog = 0;
+   std::vector<uint8_t> prof_sums;    // [PROF_LOG] the logged remap stored block sums (16 B per target block)
 };
 
 static constexpr uint32_t PROF_LOG = 16384;
@@ -486,6 +487,7 @@ static int do_pass(vacc_ctx* c, int dim, bool stats = false) {
                             hipMemcpyDeviceToDevice, c->stream));
       HIPCHK(hipMemcpyAsync(c->prof_counts + 2 * c->nlog + 1, c->off[lo] + c->ncells, sizeof(uint32_t),
                             hipMemcpyDeviceToDevice, c->stream));
+      c->prof_sums[c->nlog] = stats ? 1 : 0;
       c->nlog++;
    }
    c->lcur = lo;
@@ -2065,6 +2067,7 @@ int vacc_profile_begin(vacc_ctx* c) {
    if (!c) return set_err(VACC_E_ARG, "null context");
    HIPCHK(hipStreamSynchronize(c->stream));
    if (!c->prof_counts) HIPCHK(hipMalloc(&c->prof_counts, 2 * PROF_LOG * sizeof(uint32_t)));
+   c->prof_sums.assign(PROF_LOG, 0);
    for (auto& r : c->recs) {
       c->ev_free.push_back(r.a);
       c->ev_free.push_back(r.b);
@@ -2094,6 +2097,7 @@ int vacc_profile_read(vacc_ctx* c, vacc_profile* out) {
       for (uint32_t i = 0; i < c->nlog; ++i) {
          out->remap_src_blocks += cnt[2 * i];
          out->remap_tgt_blocks += cnt[2 * i + 1];
+         if (c->prof_sums[i]) out->remap_sum_blocks += cnt[2 * i + 1];
       }
    }
    return 0;

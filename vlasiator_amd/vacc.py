@@ -66,7 +66,8 @@ class Profile(C.Structure):
     _fields_ = [("remap_launches", C.c_uint64), ("remap_ms", C.c_double), ("remap_src_blocks", C.c_uint64),
                 ("remap_tgt_blocks", C.c_uint64), ("build_launches", C.c_uint64), ("build_ms", C.c_double),
                 ("adjust_launches", C.c_uint64), ("adjust_ms", C.c_double), ("transform_launches", C.c_uint64),
-                ("transform_ms", C.c_double), ("trans_launches", C.c_uint64), ("trans_ms", C.c_double)]
+                ("transform_ms", C.c_double), ("trans_launches", C.c_uint64), ("trans_ms", C.c_double),
+                ("remap_sum_blocks", C.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
