@@ -119,6 +119,8 @@ struct vacc_ctx {
    uint64_t gen = 0;
    int remap_mult[3] = {128, 128, 128};   // remap workgroups per CU, per pass dimension (vacc_autotune)
    int remap_wgs[3] = {256, 256, 256};    // lanes per workgroup of the tolerance-form remap (vacc_autotune)
+   int remap_mult_s[3] = {128, 128, 128}; // the same for the passes that store block sums (their own
+   int remap_wgs_s[3] = {256, 256, 256};  // kernel instance, with more registers in scratch)
    uint32_t* h_sum = nullptr;        // pinned host words: capacity flag, block total, status OR
    // streamed block transfer (vacc_blocks_io.h): two pinned chunks, their copy events and,
    // for the 16-bit forms, a device FP32 stage of two chunks; allocated on first use
@@ -468,9 +470,9 @@ static int enqueue_pass(vacc_ctx* c, int dim, int grid_mult, int wgs = 0, bool s
    ra.blk_stats = (stats && c->blk_stats) ? c->blk_stats : nullptr;
    {
       ProfScope ps(c, 0);
-      launch_remap(dim, c->opt.method, c->opt.fast_arith != 0, ra, grid_mult ? grid_mult : c->remap_mult[dim],
-                   wgs ? wgs : c->remap_wgs[dim],
-                   c->stream);
+      launch_remap(dim, c->opt.method, c->opt.fast_arith != 0, ra,
+                   grid_mult ? grid_mult : (ra.blk_stats ? c->remap_mult_s[dim] : c->remap_mult[dim]),
+                   wgs ? wgs : (ra.blk_stats ? c->remap_wgs_s[dim] : c->remap_wgs[dim]), c->stream);
    }
    HIPCHK(hipGetLastError());
    return 0;
@@ -1684,14 +1686,19 @@ int vacc_autotune(vacc_ctx* c, uint32_t* mult_out) {
    hipEvent_t e0, e1;
    HIPCHK(hipEventCreate(&e0));
    HIPCHK(hipEventCreate(&e1));
+   // the tolerance form with block sums tunes the passes that store them separately
+   const int nvar = (fast && c->blk_stats) ? 2 : 1;
+   for (int v = 0; v < nvar && rc == 0; ++v)
    for (int dim = 0; dim < 3 && rc == 0; ++dim) {
       float best = 1e30f;
-      int arg = c->remap_mult[dim], argw = c->remap_wgs[dim];
+      int& mult = v ? c->remap_mult_s[dim] : c->remap_mult[dim];
+      int& wgs = v ? c->remap_wgs_s[dim] : c->remap_wgs[dim];
+      int arg = mult, argw = wgs;
       for (int rep = 0; rep < 2 && rc == 0; ++rep)
          for (int wi = 0; wi < nwg && rc == 0; ++wi)
             for (int m : cand) {
                (void)hipEventRecord(e0, c->stream);
-               rc = enqueue_pass(c, dim, m, wgc[wi]);
+               rc = enqueue_pass(c, dim, m, wgc[wi], v == 1);
                (void)hipEventRecord(e1, c->stream);
                if (rc) break;
                HIPCHK(hipEventSynchronize(e1));
@@ -1703,8 +1710,8 @@ int vacc_autotune(vacc_ctx* c, uint32_t* mult_out) {
                   argw = wgc[wi];
                }
             }
-      c->remap_mult[dim] = arg;
-      c->remap_wgs[dim] = argw;
+      mult = arg;
+      wgs = argw;
    }
    (void)hipEventDestroy(e0);
    (void)hipEventDestroy(e1);
@@ -1715,6 +1722,7 @@ int vacc_autotune(vacc_ctx* c, uint32_t* mult_out) {
    HIPCHK(hipFree(saved));
    if (rc) return rc;
    c->content_valid = false;   // the trial passes wrote content flags of the scratch list
+   if (nvar > 1) c->stats_valid = false;   // and block sums
    drop_graphs(c);             // captured steps hold the old launch shape
    const int fl = finish(c);
    if (fl < 0) return fl;
