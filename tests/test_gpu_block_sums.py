@@ -118,3 +118,44 @@ def test_block_sums_in_graph_replays():
     cases.assert_cells_equal(graph.download_cells(), eager.download_cells(), "graph vs eager")
     eager.close()
     graph.close()
+
+
+@pytest.mark.parametrize("storage", [V.STORE_F32, V.STORE_BF16])
+def test_block_sums_of_wide_and_pass_through_cells(storage):
+    """The sums are also stored by the exact kernel's instances that take the units of wide
+    deposit spans (22 to 118 cells) and by pass-through copies (span above 118 cells): an
+    acceleration in map order Y-Z-X whose last (x) pass has its cross shears scaled so that
+    the four cells have spans of about 30, 70, 1 and 150 cells.  Moments and removed mass
+    from the sums against those from f, as above."""
+    w = W.config(0, ncells=4)
+    cells = cases.build_cells(w)
+    n = len(cells)
+    nx, ny = w.grid_length[0], w.grid_length[1]
+    dv = (w.vmax[0] - w.vmin[0]) / (4 * nx)
+    isx = cases.oracle_intersections(w, 1)
+    spans = []
+    for i, factor in enumerate([6.0, 14.0, 1.0, 30.0]):
+        g = cells[i][0]
+        ic, jc = 4 * (g // (nx * ny)).mean() + 2, 4 * (g // nx % ny).mean() + 2
+        I, dk, dj, di = isx[i, 0:4]
+        di2, dj2 = di * factor + 2.0 * dv * factor, dj * factor
+        isx[i, 0] = I + ic * di + jc * dj - (ic * di2 + jc * dj2)
+        isx[i, 2], isx[i, 3] = dj2, di2
+        spans.append((dv + 3 * abs(di2) + 3 * abs(dj2)) / dk)
+    assert 22 < spans[0] < 118 and 22 < spans[1] < 118 and spans[2] < 22 and spans[3] > 118, spans
+    _say(f"block sums, wide and pass-through cells, spans {[round(x, 1) for x in spans]}, storage={storage}")
+    a, b = _ctx(w, storage, False), _ctx(w, storage, True)
+    for c in (a, b):
+        c.upload_cells(cells)
+        c.set_intersections(isx)
+        c.accelerate(1)
+    assert bool(a.status(n)[3] & V.STATUS_DK)
+    ma, mb = a.moments(n), b.moments(n)
+    assert not np.array_equal(ma, mb)
+    tol = 1e-6 * np.abs(mb[:, :1]) * np.array([[1.0, _vmax(w), _vmax(w), _vmax(w)]])
+    assert np.all(np.abs(ma - mb) <= tol), np.max(np.abs(ma - mb) / tol)
+    la, lb = a.adjust(n), b.adjust(n)
+    assert np.all(np.abs(la - lb) <= 1e-5 * np.abs(lb) + 1e-7 * mb[:, 0]), (la, lb)
+    cases.assert_cells_equal(a.download_cells(), b.download_cells(), "wide and pass-through")
+    a.close()
+    b.close()
