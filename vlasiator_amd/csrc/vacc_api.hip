@@ -93,6 +93,9 @@ struct vacc_ctx {
    double* minval = nullptr;
    double* csum_pre = nullptr;
    double* csum_post = nullptr;
+   // the adjust count phase's keep bitmap and box per cell, read by its write phase
+   uint32_t* keep_bm = nullptr;
+   int* keep_box = nullptr;
    bool sub_valid = false;           // sub_maxdt / sub_n hold the counts of the last vacc_subcycle_count
    bool subcycling = false;          // launches inside vacc_subcycle_step use sub_dt and active
    double sub_dt_total = 0.0;
@@ -242,9 +245,12 @@ static void free_cells(vacc_ctx* c) {
    if (c->moments) (void)hipFree(c->moments);
    if (c->pivot) (void)hipFree(c->pivot);
    if (c->rho_loss) (void)hipFree(c->rho_loss);
-   void* subs[] = {c->sub_maxdt, c->sub_dt, c->sub_n, c->active, c->sub_nmax, c->minval, c->csum_pre, c->csum_post};
+   void* subs[] = {c->sub_maxdt, c->sub_dt, c->sub_n, c->active, c->sub_nmax, c->minval, c->csum_pre, c->csum_post,
+                   c->keep_bm, c->keep_box};
    for (void* p : subs)
       if (p) (void)hipFree(p);
+   c->keep_bm = nullptr;
+   c->keep_box = nullptr;
    c->sub_maxdt = c->sub_dt = nullptr;
    c->sub_n = c->sub_nmax = nullptr;
    c->minval = c->csum_pre = c->csum_post = nullptr;
@@ -284,6 +290,8 @@ static int alloc_cells(vacc_ctx* c, uint32_t n) {
    HIPCHK(hipMalloc(&c->minval, m * sizeof(double)));
    HIPCHK(hipMalloc(&c->csum_pre, m * sizeof(double)));
    HIPCHK(hipMalloc(&c->csum_post, m * sizeof(double)));
+   HIPCHK(hipMalloc(&c->keep_bm, (size_t)n * ADJ_BM_WORDS * sizeof(uint32_t)));
+   HIPCHK(hipMalloc(&c->keep_box, m * 8 * sizeof(int)));
    c->cell_cap = n;
    return 0;
 }
@@ -292,10 +300,11 @@ static size_t elem_bytes(const vacc_ctx* c) { return c->opt.storage == VACC_STOR
 
 static Pool pool(const vacc_ctx* c, int i) { return Pool{c->f[i], c->scl[i], c->opt.storage}; }
 
-// (Re)allocate block storage for `want` blocks, preserving the current list and pool.  On
-// failure the new buffers are freed and the old ones are kept.
+// the tolerance form's block sums (blk_stats) are allocated with the block storage
 static bool use_block_sums(const vacc_ctx* c) { return c->opt.fast_arith && !c->opt.sums_from_f; }
 
+// (Re)allocate block storage for `want` blocks, preserving the current list and pool.  On
+// failure the new buffers are freed and the old ones are kept.
 static int ensure_blocks(vacc_ctx* c, size_t want) {
    if (want <= c->cap && c->gid[0]) return 0;
    if (want > 0xF0000000ull) return set_err(VACC_E_NOMEM, "block count exceeds 32-bit list indices");
@@ -548,6 +557,8 @@ static int do_relist(vacc_ctx* c, int add_width, bool want_loss, const SpatialIn
    aa.dv3 = c->mi.cell[0] * c->mi.cell[1] * c->mi.cell[2];
    aa.add_width = add_width;
    aa.flags = c->flags;
+   aa.keep_bm = c->keep_bm;
+   aa.keep_box = c->keep_box;
    if (sp) {
       aa.nbr_off = sp->nbr_off;
       aa.nbr = sp->nbr;

@@ -121,6 +121,9 @@ struct RemapArgs {
                                 // in-block indices (i, j, k) = 0..3
 };
 
+// LDS bitmap words per chunk of rows (k_columns, k_adjust)
+constexpr int ADJ_BM_WORDS = 5120;
+
 struct AdjustArgs {
    const uint32_t* cell_off;   // input list
    const uint32_t* gid;
@@ -147,6 +150,11 @@ struct AdjustArgs {
    const uint32_t* nbr;
    const uint64_t* ext_off;
    const uint32_t* ext_gid;
+   // per cell (null: not kept): the count phase's keep bitmap (ADJ_BM_WORDS words) and box
+   // (8 ints: the block box lo x, y, z, hi x, y, z, and 1 when the bitmap was stored) for a
+   // cell whose box fits one LDS chunk; the write phase reads them instead of building them
+   uint32_t* keep_bm;
+   int* keep_box;
 };
 
 struct MomentArgs {

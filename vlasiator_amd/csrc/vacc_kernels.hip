@@ -39,7 +39,7 @@ constexpr int AWG = 512;         // k_adjust workgroup, likewise
 // chunks) and at most 40 KiB per k_adjust workgroup (three per CU).  Larger chunks (28 KiB
 // bitmaps, two workgroups per CU) made the column build 25 % and the adjust 9 % slower on the
 // 8192-cell 75^3 workload; most cells' boxes fit one chunk either way.
-constexpr int BM_WORDS = 5120;
+constexpr int BM_WORDS = ADJ_BM_WORDS;
 constexpr int ROW_MAX = 1024;
 
 // ---------------------------------------------------------------------------
@@ -1207,16 +1207,21 @@ __global__ __launch_bounds__(AWG) void k_adjust(AdjustArgs a) {
       if (!WRITE && t == 0) {
          a.n_keep[c] = 0;
          if (a.rho_loss) a.rho_loss[c] = 0.0;
+         if (a.keep_box) a.keep_box[8 * (size_t)c + 6] = 0;
       }
       return;
    }
+   // the count phase leaves the keep bitmap and box of a cell that it built in one chunk for
+   // the write phase (keep_box[8c + 6] = 1), which then reads them instead of building them again
+   int* const kbox = a.keep_box ? a.keep_box + 8 * (size_t)c : nullptr;
+   const bool reuse = WRITE && kbox && kbox[6] != 0;
    if (t < 3) {
-      sbox[t] = INT_MAX;
-      sbox[t + 3] = INT_MIN;
+      sbox[t] = reuse ? kbox[t] : INT_MAX;
+      sbox[t + 3] = reuse ? kbox[t + 3] : INT_MIN;
    }
    if (t == 0) carry = 0;
    __syncthreads();
-   {
+   if (!reuse) {
       int lx = INT_MAX, ly = INT_MAX, lz = INT_MAX, hx = INT_MIN, hy = INT_MIN, hz = INT_MIN;
       auto grow = [&](uint32_t g) {
          uint32_t ux, uy, uz;
@@ -1236,6 +1241,7 @@ __global__ __launch_bounds__(AWG) void k_adjust(AdjustArgs a) {
       if (!WRITE && t == 0) {
          a.n_keep[c] = 0;
          if (a.rho_loss) a.rho_loss[c] = 0.0;
+         if (kbox) kbox[6] = 0;
       }
       return;
    }
@@ -1250,6 +1256,11 @@ __global__ __launch_bounds__(AWG) void k_adjust(AdjustArgs a) {
    double lsum = 0.0;      // removed f, summed per lane (count phase)
    for (int r0 = 0; r0 < rows; r0 += rpc) {
       const int nr = min(rpc, rows - r0);
+      if (reuse) {   // one chunk (the count phase found the box separable)
+         const uint32_t* src = a.keep_bm + (size_t)c * BM_WORDS;
+         for (int i = t; i < nr * W; i += AWG) bm[i] = src[i];
+         __syncthreads();
+      } else {
       for (int i = t; i < nr * W; i += AWG) bm[i] = 0;
       __syncthreads();
       if (separable) {
@@ -1271,6 +1282,10 @@ __global__ __launch_bounds__(AWG) void k_adjust(AdjustArgs a) {
             atomicOr(&bm[((z - mnz) * Ry + (y - mny)) * W + (kx >> 5)], 1u << (kx & 31));
          });
          __syncthreads();
+         if (!WRITE && kbox) {   // the finished keep bitmap of the whole box, for the write phase
+            uint32_t* dst = a.keep_bm + (size_t)c * BM_WORDS;
+            for (int i = t; i < nr * W; i += AWG) dst[i] = bm[i];
+         }
       } else {
       for_spatial([&](uint32_t g) {
          uint32_t ux, uy, uz;
@@ -1310,6 +1325,7 @@ __global__ __launch_bounds__(AWG) void k_adjust(AdjustArgs a) {
                }
             }
          }
+      }
       }
       }
       __syncthreads();
@@ -1419,6 +1435,10 @@ __global__ __launch_bounds__(AWG) void k_adjust(AdjustArgs a) {
          for (int q = 0; q < AWG / 64; ++q) sum += wloss[q];
          a.n_keep[c] = carry;
          if (a.rho_loss) a.rho_loss[c] = sum;
+         if (kbox) {
+            for (int q = 0; q < 6; ++q) kbox[q] = sbox[q];
+            kbox[6] = separable ? 1 : 0;
+         }
       }
    }
 }
