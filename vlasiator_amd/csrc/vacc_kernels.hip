@@ -494,27 +494,42 @@ __global__ __launch_bounds__(CWG, 8) void k_columns(ColumnArgs a) {
 // ---------------------------------------------------------------------------
 
 __global__ __launch_bounds__(1024) void k_scan2(ScanArgs a) {
+   // tiles of 1024 x K entries, K contiguous entries per thread (one workgroup scan per tile)
+   constexpr uint32_t K = 8;
    __shared__ uint32_t sh[16 * 2];
-   __shared__ uint32_t carry[2];
-   if (threadIdx.x < 2) carry[threadIdx.x] = 0;
-   __syncthreads();
-   for (uint32_t base = 0; base < a.n; base += 1024) {
-      const uint32_t i = base + threadIdx.x;
-      uint32_t v[2] = {i < a.n ? a.a[i] : 0u, (a.b && i < a.n) ? a.b[i] : 0u};
+   uint32_t carry0 = 0, carry1 = 0;   // running totals, the same in every thread
+   for (uint32_t base = 0; base < a.n; base += 1024 * K) {
+      const uint32_t i0 = base + threadIdx.x * K;
+      uint32_t va[K], vb[K];
+      uint32_t v[2] = {0u, 0u};
+#pragma unroll
+      for (uint32_t k = 0; k < K; ++k) {
+         const uint32_t i = i0 + k;
+         va[k] = (i < a.n) ? a.a[i] : 0u;
+         vb[k] = (a.b && i < a.n) ? a.b[i] : 0u;
+         v[0] += va[k];
+         v[1] += vb[k];
+      }
       uint32_t ex[2], tot[2];
       block_scan<2>(v, ex, tot, sh);
-      if (i < a.n) {
-         a.a_off[i] = carry[0] + ex[0];
-         if (a.b) a.b_off[i] = carry[1] + ex[1];
+      uint32_t r0 = carry0 + ex[0], r1 = carry1 + ex[1];
+#pragma unroll
+      for (uint32_t k = 0; k < K; ++k) {
+         const uint32_t i = i0 + k;
+         if (i < a.n) {
+            a.a_off[i] = r0;
+            if (a.b) a.b_off[i] = r1;
+         }
+         r0 += va[k];
+         r1 += vb[k];
       }
-      __syncthreads();
-      if (threadIdx.x < 2) carry[threadIdx.x] += tot[threadIdx.x];
-      __syncthreads();
+      carry0 += tot[0];
+      carry1 += tot[1];
    }
    if (threadIdx.x == 0) {
-      a.a_off[a.n] = carry[0];
-      if (a.b) a.b_off[a.n] = carry[1];
-      if (a.cap_a && carry[0] > a.cap_a && a.flags) atomicOr(a.flags, VACC_STATUS_CAPACITY);
+      a.a_off[a.n] = carry0;
+      if (a.b) a.b_off[a.n] = carry1;
+      if (a.cap_a && carry0 > a.cap_a && a.flags) atomicOr(a.flags, VACC_STATUS_CAPACITY);
    }
    if (a.hist) {
       // exclusive scan of the work-unit length histogram -> insertion cursors of k_order
