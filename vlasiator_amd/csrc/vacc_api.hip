@@ -96,6 +96,7 @@ struct vacc_ctx {
    // the adjust count phase's keep bitmap and box per cell, read by its write phase
    uint32_t* keep_bm = nullptr;
    int* keep_box = nullptr;
+   uint32_t* cell_hist = nullptr;   // [LEN_BINS * ncells] work units per (length bin, cell)
    bool sub_valid = false;           // sub_maxdt / sub_n hold the counts of the last vacc_subcycle_count
    bool subcycling = false;          // launches inside vacc_subcycle_step use sub_dt and active
    double sub_dt_total = 0.0;
@@ -246,11 +247,12 @@ static void free_cells(vacc_ctx* c) {
    if (c->pivot) (void)hipFree(c->pivot);
    if (c->rho_loss) (void)hipFree(c->rho_loss);
    void* subs[] = {c->sub_maxdt, c->sub_dt, c->sub_n, c->active, c->sub_nmax, c->minval, c->csum_pre, c->csum_post,
-                   c->keep_bm, c->keep_box};
+                   c->keep_bm, c->keep_box, c->cell_hist};
    for (void* p : subs)
       if (p) (void)hipFree(p);
    c->keep_bm = nullptr;
    c->keep_box = nullptr;
+   c->cell_hist = nullptr;
    c->sub_maxdt = c->sub_dt = nullptr;
    c->sub_n = c->sub_nmax = nullptr;
    c->minval = c->csum_pre = c->csum_post = nullptr;
@@ -292,6 +294,7 @@ static int alloc_cells(vacc_ctx* c, uint32_t n) {
    HIPCHK(hipMalloc(&c->csum_post, m * sizeof(double)));
    HIPCHK(hipMalloc(&c->keep_bm, (size_t)n * ADJ_BM_WORDS * sizeof(uint32_t)));
    HIPCHK(hipMalloc(&c->keep_box, m * 8 * sizeof(int)));
+   HIPCHK(hipMalloc(&c->cell_hist, (size_t)LEN_BINS * n * sizeof(uint32_t)));
    c->cell_cap = n;
    return 0;
 }
@@ -423,6 +426,7 @@ static int enqueue_pass(vacc_ctx* c, int dim, int grid_mult, int wgs = 0, bool s
    ca.wall_margin = c->opt.wall_margin;
    ca.active = c->subcycling ? c->active : nullptr;
    ca.fast = c->opt.fast_arith != 0;
+   ca.cell_hist = c->cell_hist;
    {
       ProfScope ps(c, 1);
       launch_fill_u32(c->len_hist, 0u, LEN_BINS, c->stream);
@@ -438,11 +442,13 @@ static int enqueue_pass(vacc_ctx* c, int dim, int grid_mult, int wgs = 0, bool s
       sa.hist = c->len_hist;
       sa.hist_cursor = c->len_hist + LEN_BINS;
       launch_scan(sa, c->stream);
+      launch_bin_scan(c->cell_hist, c->ncells, c->stream);
       OrderArgs oa{};
       oa.cell_off = c->off[li];
       oa.unit_base = c->unit_base;
       oa.unit_rec = c->unit_rec;
       oa.hist_cursor = c->len_hist + LEN_BINS;
+      oa.cell_hist = c->cell_hist;
       oa.perm = c->perm;
       oa.perm_cell = c->perm_cell;
       oa.flags = c->flags;

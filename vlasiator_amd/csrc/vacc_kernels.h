@@ -64,6 +64,7 @@ struct ColumnArgs {
    int wall_margin;
    const uint8_t* active;      // [ncells] or null: cells with 0 pass through unchanged (subcycling)
    int fast;                   // tolerance form: mark the units the exact kernel must take
+   uint32_t* cell_hist;        // [LEN_BINS * ncells] the cell's work units per length bin (bin-major)
 };
 
 struct ScanArgs {
@@ -82,7 +83,8 @@ struct OrderArgs {
    const uint32_t* cell_off;
    const uint32_t* unit_base;   // [ncells+1]
    const uint4* unit_rec;
-   uint32_t* hist_cursor;       // [LEN_BINS] insertion cursors
+   const uint32_t* hist_cursor; // [LEN_BINS] first list position of each length bin
+   const uint32_t* cell_hist;   // [LEN_BINS * ncells] exclusive prefix over cells of the bin's units
    uint32_t* perm;              // [total units] unit record indices (unit_rec), longest first
    uint32_t* perm_cell;         // [total units] the cell of each perm entry
    const uint32_t* flags;
@@ -201,6 +203,7 @@ void launch_content(const uint32_t* src, const Pool& pool, uint8_t* content, uin
 void launch_columns(int dim, const ColumnArgs& a, hipStream_t s);
 void launch_scan(const ScanArgs& a, hipStream_t s);
 void launch_order(const OrderArgs& a, hipStream_t s);
+void launch_bin_scan(uint32_t* cell_hist, uint32_t ncells, hipStream_t s);
 // grid: grid_mult persistent workgroups per CU
 // grid_mult workgroups per compute unit; wgs: lanes per workgroup of the tolerance-form kernel
 // (64, 128 or 256: 4, 8 or 16 work units); the exact kernel always runs 256-lane workgroups

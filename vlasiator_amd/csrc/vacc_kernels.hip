@@ -281,6 +281,7 @@ __global__ __launch_bounds__(CWG, 8) void k_columns(ColumnArgs a) {
          a.n_units[c] = 0;
          a.n_targets[c] = 0;
       }
+      for (int i = t; i < LEN_BINS; i += CWG) a.cell_hist[(size_t)i * a.ncells + c] = 0u;
       return;
    }
    for (int i = t; i < LEN_BINS; i += CWG) hist[i] = 0;
@@ -485,8 +486,39 @@ __global__ __launch_bounds__(CWG, 8) void k_columns(ColumnArgs a) {
          atomicOr(&a.status[a.ncells], sstat);   // OR over all cells
       }
    }
-   for (int i = t; i < LEN_BINS; i += CWG)
+   for (int i = t; i < LEN_BINS; i += CWG) {
+      a.cell_hist[(size_t)i * a.ncells + c] = hist[i];
       if (hist[i]) atomicAdd(&a.len_hist[i], hist[i]);
+   }
+}
+
+// Per length bin (one workgroup each), the exclusive prefix over cells of the cells' unit
+// counts, in place: cell c's units of bin b take list positions from hist_cursor[b] +
+// cell_hist[b][c] on.
+__global__ __launch_bounds__(1024) void k_bin_scan(uint32_t* cell_hist, uint32_t ncells) {
+   constexpr uint32_t K = 8;
+   __shared__ uint32_t sh[16];
+   uint32_t* row = cell_hist + (size_t)blockIdx.x * ncells;
+   uint32_t carry = 0;
+   for (uint32_t base = 0; base < ncells; base += 1024 * K) {
+      const uint32_t i0 = base + threadIdx.x * K;
+      uint32_t v[K];
+      uint32_t sum[1] = {0u};
+#pragma unroll
+      for (uint32_t k = 0; k < K; ++k) {
+         v[k] = (i0 + k < ncells) ? row[i0 + k] : 0u;
+         sum[0] += v[k];
+      }
+      uint32_t ex[1], tot[1];
+      block_scan<1>(sum, ex, tot, sh);
+      uint32_t r = carry + ex[0];
+#pragma unroll
+      for (uint32_t k = 0; k < K; ++k) {
+         if (i0 + k < ncells) row[i0 + k] = r;
+         r += v[k];
+      }
+      carry += tot[0];
+   }
 }
 
 // ---------------------------------------------------------------------------
@@ -541,12 +573,11 @@ __global__ __launch_bounds__(1024) void k_scan2(ScanArgs a) {
 }
 
 // Global work list of the remap: every unit's global index, ordered by source length
-// (longest first; order within a length is arbitrary, units are independent).
+// (longest first; within a length bin by cell, units are independent).
 __global__ __launch_bounds__(WG) void k_order(OrderArgs a) {
-   // one workgroup per cell: count the cell's units per length bin in LDS, reserve each bin's
-   // range with one global atomic, then place the units
+   // one workgroup per cell: the cell's range in each bin is known (hist_cursor + the bin's
+   // prefix over cells); units take positions in it through an LDS counter per bin
    __shared__ uint32_t cnt[LEN_BINS];
-   __shared__ uint32_t base_of[LEN_BINS];
    if (*a.flags & VACC_STATUS_CAPACITY) return;
    const uint32_t c = blockIdx.x;
    const uint32_t base = a.unit_base[c];
@@ -555,16 +586,9 @@ __global__ __launch_bounds__(WG) void k_order(OrderArgs a) {
    const uint32_t off = a.cell_off[c];
    for (int i = threadIdx.x; i < LEN_BINS; i += WG) cnt[i] = 0;
    __syncthreads();
-   for (uint32_t u = threadIdx.x; u < nu; u += WG) atomicAdd(&cnt[unit_bin(a.unit_rec[off + u])], 1u);
-   __syncthreads();
-   for (int i = threadIdx.x; i < LEN_BINS; i += WG) {
-      base_of[i] = cnt[i] ? atomicAdd(&a.hist_cursor[i], cnt[i]) : 0u;
-      cnt[i] = 0;
-   }
-   __syncthreads();
    for (uint32_t u = threadIdx.x; u < nu; u += WG) {
       const uint32_t b = unit_bin(a.unit_rec[off + u]);
-      const uint32_t pos = base_of[b] + atomicAdd(&cnt[b], 1u);
+      const uint32_t pos = a.hist_cursor[b] + a.cell_hist[(size_t)b * a.ncells + c] + atomicAdd(&cnt[b], 1u);
       a.perm[pos] = off + u;   // the unit's record
       a.perm_cell[pos] = c;
    }
@@ -1829,6 +1853,11 @@ void launch_columns(int dim, const ColumnArgs& a, hipStream_t s) {
 void launch_order(const OrderArgs& a, hipStream_t s) {
    if (a.ncells == 0) return;
    hipLaunchKernelGGL(k_order, dim3(a.ncells), dim3(WG), 0, s, a);
+}
+
+void launch_bin_scan(uint32_t* cell_hist, uint32_t ncells, hipStream_t s) {
+   if (ncells == 0) return;
+   hipLaunchKernelGGL(k_bin_scan, dim3(LEN_BINS), dim3(1024), 0, s, cell_hist, ncells);
 }
 
 void launch_scan(const ScanArgs& a, hipStream_t s) {
