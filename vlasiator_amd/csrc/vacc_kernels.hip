@@ -1691,20 +1691,32 @@ __global__ __launch_bounds__(WG) void k_moments_sums(MomentArgs a) {
    const uint32_t n = a.cell_off[c + 1] - off;
    const double bx = a.cell[0] * WID, by = a.cell[1] * WID, bz = a.cell[2] * WID;
    double s0 = 0.0, sx = 0.0, sy = 0.0, sz = 0.0, wx = 0.0, wy = 0.0, wz = 0.0;
-   for (uint32_t b = t; b < n; b += WG) {
-      const uint32_t sidx = a.src[off + b];
-      if (sidx == INVALID) continue;   // a block the adjust created: all zero
-      const float4 st = a.blk_stats[sidx];
-      uint32_t x, y, z;
-      gid_xyz(a.gid[off + b], a.gs, x, y, z);
-      const double S = st.x;
-      s0 += S;
-      sx = fma(S, fma((double)x, bx, a.vmin[0]), sx);
-      sy = fma(S, fma((double)y, by, a.vmin[1]), sy);
-      sz = fma(S, fma((double)z, bz, a.vmin[2]), sz);
-      wx += (double)st.y;
-      wy += (double)st.z;
-      wz += (double)st.w;
+   constexpr int UNR = 4;   // blocks per lane in flight
+   for (uint32_t b0 = t; b0 < n; b0 += UNR * WG) {
+      uint32_t sidx[UNR], g[UNR];
+#pragma unroll
+      for (int h = 0; h < UNR; ++h) {
+         const uint32_t b = b0 + h * WG;
+         sidx[h] = (b < n) ? a.src[off + b] : INVALID;
+         g[h] = (b < n) ? a.gid[off + b] : 0u;
+      }
+      float4 st[UNR];
+#pragma unroll
+      for (int h = 0; h < UNR; ++h)   // a block the adjust created (INVALID) is all zero
+         st[h] = (sidx[h] != INVALID) ? a.blk_stats[sidx[h]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+      for (int h = 0; h < UNR; ++h) {
+         uint32_t x, y, z;
+         gid_xyz(g[h], a.gs, x, y, z);
+         const double S = st[h].x;
+         s0 += S;
+         sx = fma(S, fma((double)x, bx, a.vmin[0]), sx);
+         sy = fma(S, fma((double)y, by, a.vmin[1]), sy);
+         sz = fma(S, fma((double)z, bz, a.vmin[2]), sz);
+         wx += (double)st[h].y;
+         wy += (double)st[h].z;
+         wz += (double)st[h].w;
+      }
    }
    double s1 = sx + (0.5 * a.cell[0]) * s0 + a.cell[0] * wx;
    double s2 = sy + (0.5 * a.cell[1]) * s0 + a.cell[1] * wy;
